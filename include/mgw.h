@@ -1,0 +1,220 @@
+/*
+ * mgw.h -- C-ABI of the MI355X-native RI G0W0 engine (libmgw.so).
+ *
+ * Every entry point replaces one Fortran subroutine of MOLGW's RI G0W0 hot
+ * path (cited below as file:line under src/ of MOLGW 3.4). The Fortran
+ * module state those subroutines read implicitly (eri_3center, c_matrix,
+ * eri_3center_mo, wpol, se ...) lives in an opaque context that keeps the
+ * tensors resident in HBM between calls.
+ *
+ * Conventions
+ *  - Arrays are column-major FP64, exactly as the Fortran callers hold them.
+ *  - State indices and ranges are 1-based, as in the Fortran module
+ *    variables (nsemin, ncore_G ...).
+ *  - Complex arrays are interleaved (re, im) pairs, i.e. complex(dp).
+ *  - Every function returns 0 (MGW_OK) on success; on failure a non-zero
+ *    code and mgw_last_error() describes it (the Fortran shim calls die()).
+ *  - Output pointers marked "nullable" may be NULL: the result then stays
+ *    resident in the context for the next call.
+ *  - One context per process and GPU; a context is not re-entrant. Calls on
+ *    one context are ordered on the context's HIP stream.
+ *  - Restricted closed-shell only (nspin = 1, spin_fact = 2): the reference
+ *    calls are made with spin index 1.
+ */
+#ifndef MGW_H
+#define MGW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mgw_ctx mgw_ctx;
+
+enum {
+  MGW_OK = 0,
+  MGW_ERR_ARG = 1,     /* invalid argument or inconsistent sizes        */
+  MGW_ERR_HIP = 2,     /* HIP runtime error                             */
+  MGW_ERR_SOLVER = 3,  /* eigensolver / factorisation failure           */
+  MGW_ERR_STATE = 4,   /* required resident data not set                */
+  MGW_ERR_NOMEM = 5    /* device allocation failed                      */
+};
+
+enum { MGW_MEM_HOST = 0, MGW_MEM_DEVICE = 1 };
+
+enum { MGW_FP64 = 0, MGW_FP32 = 1, MGW_BF16X3 = 2 };
+
+enum { MGW_TECH_ONE_SHOT = 0, MGW_TECH_IMAGINARY_PADE = 1 };
+
+/* State ranges held as module variables in the reference:
+ * ncore_G, nvirtual_G, nsemin, nsemax, nhomo_G (src/m_selfenergy_tools.f90:30-41)
+ * ncore_W, nvirtual_W (src/m_spectral_function.f90:95-107). */
+typedef struct mgw_ranges {
+  int32_t ncore_G;
+  int32_t nvirtual_G;
+  int32_t nsemin;
+  int32_t nsemax;
+  int32_t nhomo_G;
+  int32_t ncore_W;
+  int32_t nvirtual_W;
+  int32_t reserved;
+} mgw_ranges;
+
+/* One-shot G0W0 parameters (input keywords, src/input_variables.yaml:
+ * nomega_sigma :1164, step_sigma :1384, eta :1051, nomega_chi_imag :1148,
+ * nomega_sigma_calc :1173, step_sigma_calc :1392). */
+typedef struct mgw_g0w0_params {
+  int32_t technique;          /* MGW_TECH_*                                   */
+  int32_t nomega_sigma;       /* real grid points (odd), default 51           */
+  double  step_sigma;         /* Ha, default 0.01                             */
+  double  eta;                /* Ha, default 0.001                            */
+  int32_t nomega_chi_imag;    /* imaginary chi grid (Pade technique only)     */
+  int32_t nomega_sigma_calc;  /* imaginary sigma points (Pade technique only) */
+  double  step_sigma_calc;    /* Ha (Pade technique only)                     */
+  double  spin_fact;          /* 2 for restricted closed shell                */
+  int32_t precision;          /* MGW_FP64 | MGW_FP32 | MGW_BF16X3 (transforms
+                                 and chi0; metric/inversion/eigensolve and the
+                                 QP solve always FP64)                        */
+  int32_t reserved;
+} mgw_g0w0_params;
+
+/* ---------------------------------------------------------------- context */
+int32_t     mgw_abi_version(void);
+const char* mgw_last_error(void);
+int mgw_ctx_create(int device, mgw_ctx** ctx);
+int mgw_ctx_destroy(mgw_ctx* ctx);
+int mgw_ctx_synchronize(mgw_ctx* ctx);
+/* HBM ledger: replaces the clean_allocate bookkeeping (src/m_memory.f90:19-80). */
+int mgw_memory_report(mgw_ctx* ctx, int64_t* current_bytes, int64_t* peak_bytes);
+/* Bytes of HBM one solve of this shape needs (resident RI tensor included);
+ * seeded by the reference's evaluate_memory formulas (src/m_io.f90:3843-3936). */
+int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int technique,
+                            int nomega_chi_imag, int precision);
+
+/* ------------------------------------------------- A1: resident RI tensor */
+/* Replaces the module state eri_3center(npair, nauxil) and index_basis(2, npair)
+ * (src/m_eri.f90:36-62) as produced by calculate_eri_ri
+ * (src/m_eri_calculate.f90:609-699): packed pairs, diagonal pairs stored x1/2.
+ * index_basis is 1-based, (2, npair) column-major. `where` = MGW_MEM_HOST or
+ * MGW_MEM_DEVICE for eri_3center; index_basis is always host memory. */
+int mgw_set_eri_3center(mgw_ctx* ctx, const double* eri_3center, int64_t npair, int nauxil,
+                        const int32_t* index_basis, int nbf, int where);
+int mgw_destroy_eri_3center(mgw_ctx* ctx);
+
+/* MO coefficients, energies and occupations of the (single) spin channel:
+ * the c_matrix(:,:,1), energy(:,1), occupation(:,1) arguments of the
+ * reference routines. */
+int mgw_set_mo(mgw_ctx* ctx, const double* c_matrix, int nbf, int nstate,
+               const double* energy, const double* occupation, int where);
+
+/* ------------------------------------------------ A2: AO -> MO transform */
+/* calculate_eri_3center_mo(c_matrix, mstate_min, mstate_max, nstate_min,
+ * nstate_max) (src/m_eri_ao_mo.f90:416-582). Result eri_3center_mo(P, m, n)
+ * stays resident; it is also copied to `eri_3center_mo` (nullable, host,
+ * nauxil x (mmax-mmin+1) x (nmax-nmin+1)). Like the reference, returns
+ * immediately when a block is already resident (src/m_eri_ao_mo.f90:444-446). */
+int mgw_calculate_eri_3center_mo(mgw_ctx* ctx, int mstate_min, int mstate_max,
+                                 int nstate_min, int nstate_max, double* eri_3center_mo);
+/* destroy_eri_3center_mo (src/m_eri_ao_mo.f90:1018). */
+int mgw_destroy_eri_3center_mo(mgw_ctx* ctx);
+
+/* -------------------------------------------- A3/A4: Casida RPA (analytic) */
+/* Number of resonant transitions of the transition table
+ * (src/m_spectral_function.f90:224-266). */
+int mgw_npole(mgw_ctx* ctx, const mgw_ranges* ranges, int* npole);
+/* RPA branch of polarizability(enforce_rpa, calculate_w=.TRUE., ...)
+ * (src/m_linear_response.f90:33-498) with build_amb_apb_diag_auxil,
+ * build_apb_hartree_auxil[_scalapack], get_rpa_correlation
+ * (src/m_build_bse.f90:218-491), diago_4blocks_rpa_sca
+ * (src/m_block_diago.f90:142-204) and chi_to_sqrtvchisqrtv_auxil
+ * (src/m_linear_response.f90:802-922). Fills wpol%pole and wpol%w_s
+ * (resident; copied to pole_out[npole] and w_s_out[nauxil x npole] when
+ * non-NULL), en_rpa and en_gw (Galitskii-Migdal). */
+int mgw_polarizability(mgw_ctx* ctx, const mgw_ranges* ranges, double spin_fact, int precision,
+                       double* en_rpa, double* en_gw, double* pole_out, double* w_s_out);
+
+/* ----------------------------------------- A5: chi0 on the imaginary axis */
+/* polarizability_grid_scalapack (src/m_gw_selfenergy_grid.f90:33-209):
+ * chi(iw) = (1 - chi0(iw))^-1 chi0(iw) for the nomega points omega_im with
+ * weights weight_quad (from sf_init, src/m_spectral_function.f90:286-306).
+ * chi stays resident (wpol%chi); chi_out nullable (nauxil x nauxil x nomega). */
+int mgw_polarizability_grid(mgw_ctx* ctx, const mgw_ranges* ranges, double spin_fact, int precision,
+                            int nomega, const double* omega_im, const double* weight_quad,
+                            double* erpa, double* egw, double* chi_out);
+
+/* ------------------------------------------ A6: analytic-pole self-energy */
+/* gw_selfenergy(GW, ...) (src/m_gw_selfenergy_analytic.f90:29-196) using the
+ * resident wpol from mgw_polarizability. omega[nomega] is se%omega (real),
+ * se_energy0[nse] is se%energy0. sigma (nomega x nse, complex) stays resident;
+ * sigma_out nullable. energy_gw = 1/2 Tr[Sig_c G] (:140-143). */
+int mgw_gw_selfenergy(mgw_ctx* ctx, const mgw_ranges* ranges, double spin_fact, double eta,
+                      int nomega, const double* omega, const double* se_energy0,
+                      double* sigma_out, double* energy_gw);
+
+/* ------------------------------------------ A7: imaginary-axis self-energy */
+/* gw_selfenergy_imag_scalapack (src/m_gw_selfenergy_grid.f90:213-338) using
+ * the resident chi. omega_calc[nomega_calc] complex (se%omega_calc);
+ * sigma_calc (nomega_calc x nse complex) resident, sigma_calc_out nullable. */
+int mgw_gw_selfenergy_imag(mgw_ctx* ctx, const mgw_ranges* ranges, int nomega_calc,
+                           const double* omega_calc, double* sigma_calc_out);
+
+/* ------------------------------------------------ A8: Pade continuation */
+/* se_pade_fit (src/m_selfenergy_tools.f90:965-999) with pade/calculate_pade_a
+ * (src/m_numerical_tools.f90:86-145) on the resident sigma_calc (or on
+ * sigma_calc when non-NULL). omega[nomega] real: the real grid se%omega.
+ * Result sigma (nomega x nse complex) resident; sigma_out nullable. */
+int mgw_pade_fit(mgw_ctx* ctx, int nse, int nomega_calc, const double* omega_calc,
+                 const double* sigma_calc, int nomega, const double* omega, double eta,
+                 double* sigma_out);
+
+/* ---------------------------------------------------- A9: QP equation */
+/* find_qp_energy_graphical + find_fixed_point (src/m_selfenergy_tools.f90:253-420)
+ * and find_qp_energy_linearization (:202-249) on the resident sigma (or on
+ * `sigma` when non-NULL, nomega x nse complex). energy_qp/zz/energy_qp_lin
+ * have nstate entries; states outside [nsemin, nsemax] get energy0. */
+int mgw_find_qp_energy(mgw_ctx* ctx, const mgw_ranges* ranges, int nomega, const double* omega,
+                       const double* sigma, const double* se_energy0,
+                       const double* exchange_m_vxc, const double* energy0, int nstate,
+                       double* energy_qp, double* zz, double* energy_qp_lin);
+
+/* ---------------------------------------------- A11: one G0W0 solve */
+/* The one-shot G0W0 flow of selfenergy_evaluation
+ * (src/m_selfenergy_evaluation.f90:135-261, 589-590) on the resident RI
+ * tensor and MO data. exchange_m_vxc[nstate] is the diagonal of
+ * Sigma_x - Vxc (setup_exchange_m_vxc, src/m_selfenergy_tools.f90:720-802);
+ * NULL means zero (converged HF). Outputs (all nullable): energy_qp, zz,
+ * energy_qp_lin [nstate]; sigma_out (nomega_sigma x nse complex);
+ * energies[2] = {E_RPA, E_GW}. */
+int mgw_g0w0(mgw_ctx* ctx, const mgw_g0w0_params* params, const mgw_ranges* ranges,
+             const double* exchange_m_vxc, double* energy_qp, double* zz,
+             double* energy_qp_lin, double* sigma_out, double* energies);
+
+/* ---------------------------------------- instrumentation (no reference counterpart) */
+/* Live timing of the engine's kernel classes with HIP events on the context's
+ * stream, for roofline reporting; the reference has only CPU timers
+ * (src/m_timing.f90:378-424). mgw_profile_enable(ctx, 1) clears the records.
+ * mgw_profile_read sums over the recorded launches of one class: launch
+ * count, device time in ms, and algorithmic flops and HBM bytes. */
+enum {
+  MGW_KCLASS_GEMM_AO2MO = 0,   /* (P|mu nu) C contractions                 */
+  MGW_KCLASS_GEMM_CASIDA = 1,  /* Casida SYRK and w_s = B (X+Y)            */
+  MGW_KCLASS_GEMM_SIGMA = 2,   /* w_s^T (P|p i) of the pole self-energy    */
+  MGW_KCLASS_SIGMA_POLES = 3,  /* pole sum (FP64 VALU)                     */
+  MGW_KCLASS_EIGEN = 4,        /* Casida eigensolver                       */
+  MGW_KCLASS_GEMM_CHI0 = 5,    /* 1 - chi0(iw), all frequencies            */
+  MGW_KCLASS_CHOLESKY = 6,     /* (1 - chi0)^-1 by Cholesky                */
+  MGW_KCLASS_GEMM_WB = 7,      /* chi(iw) (P|p m) with fused dot           */
+  MGW_KCLASS_GEMM_OTHER = 8,
+  MGW_KCLASS_OTHER = 9,
+  MGW_KCLASS_COUNT = 10
+};
+int mgw_profile_enable(mgw_ctx* ctx, int enable);
+int mgw_profile_read(mgw_ctx* ctx, int kernel_class, int64_t* launches, double* total_ms,
+                     double* algorithmic_flops, double* algorithmic_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGW_H */

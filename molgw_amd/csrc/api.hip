@@ -1,0 +1,1044 @@
+// C-ABI of libmgw: context, resident tensors and the orchestration of the
+// RI G0W0 hot path. The math of every step is documented at the kernel that
+// does it; this file owns data movement, sizes and the call order of
+// src/m_selfenergy_evaluation.f90 (MOLGW 3.4).
+#include "mgw_internal.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace mgw {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+const std::string& last_error() { return g_last_error; }
+
+hipEvent_t Profiler::next_event() {
+  if (used == pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    pool.push_back(e);
+  }
+  return pool[used++];
+}
+
+void Profiler::reset() {
+  recs.clear();
+  used = 0;
+}
+
+Profiler::~Profiler() {
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+ProfScope::ProfScope(Profiler* prof, int klass, double flops, double bytes, hipStream_t stream)
+    : p(prof && prof->on ? prof : nullptr), idx(0), s(stream) {
+  if (!p) return;
+  hipEvent_t a = p->next_event(), b = p->next_event();
+  if (!a || !b) { p = nullptr; return; }
+  (void)hipEventRecord(a, s);
+  p->recs.push_back({klass, flops, bytes, a, b});
+  idx = p->recs.size() - 1;
+}
+
+ProfScope::~ProfScope() {
+  if (p) (void)hipEventRecord(p->recs[idx].b, s);
+}
+
+}  // namespace mgw
+
+using namespace mgw;
+
+static_assert(sizeof(mgw_ranges) == 32, "mgw_ranges layout is part of the ABI");
+static_assert(sizeof(mgw_g0w0_params) == 56, "mgw_g0w0_params layout is part of the ABI");
+
+#define MGW_BLAS(expr)                                                                  \
+  do {                                                                                  \
+    rocblas_status st_ = (expr);                                                        \
+    if (st_ != rocblas_status_success) {                                                \
+      set_error(std::string(#expr) + ": " + rocblas_status_to_string(st_));             \
+      return MGW_ERR_SOLVER;                                                            \
+    }                                                                                   \
+  } while (0)
+
+#define MGW_CTX(ctx)                                                           \
+  do {                                                                         \
+    if (!(ctx)) {                                                              \
+      set_error("null context");                                               \
+      return MGW_ERR_ARG;                                                      \
+    }                                                                          \
+    MGW_HIP(hipSetDevice((ctx)->device));                                      \
+  } while (0)
+
+struct mgw_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  Ledger ledger;
+  Profiler prof;
+  // A1: resident RI tensor, dense B(P, k, l)
+  bool has_ri = false;
+  int nbf = 0, naux = 0;
+  // MO data of the single spin channel
+  bool has_mo = false;
+  int nstate = 0;
+  std::vector<double> energy, occupation;
+  // A2 module state eri_3center_mo(P, m, n)
+  bool has_eri_mo = false;
+  int mlo = 0, mhi = 0, nlo = 0, nhi = 0;
+  // spectral function
+  bool has_wpol = false;
+  int npole = 0;
+  bool has_chi = false;
+  int nomega_chi = 0;
+  std::vector<double> chi_omega, chi_weight;
+  // self-energy
+  bool has_sigma = false;
+  int sigma_nomega = 0, sigma_nse = 0;
+  bool has_sigma_calc = false;
+  int sigma_ncalc = 0, sigma_calc_nse = 0;
+};
+
+namespace {
+
+// ------------------------------------------------------------------ buffers
+int ensure(mgw_ctx* c, const std::string& name, size_t bytes, void** out) {
+  DevBuf& b = c->ledger.bufs[name];
+  if (bytes == 0) bytes = 8;
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      MGW_HIP(hipStreamSynchronize(c->stream));
+      MGW_HIP(hipFree(b.ptr));
+      c->ledger.current -= (int64_t)b.bytes;
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) {
+      b.ptr = nullptr;
+      set_error("hipMalloc(" + name + ", " + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+      (void)hipGetLastError();
+      return MGW_ERR_NOMEM;
+    }
+    b.bytes = bytes;
+    c->ledger.current += (int64_t)bytes;
+    c->ledger.peak = std::max(c->ledger.peak, c->ledger.current);
+  }
+  *out = b.ptr;
+  return MGW_OK;
+}
+
+template <class T>
+int dbuf(mgw_ctx* c, const std::string& name, size_t count, T** out) {
+  void* p = nullptr;
+  MGW_TRY(ensure(c, name, count * sizeof(T), &p));
+  *out = static_cast<T*>(p);
+  return MGW_OK;
+}
+
+double* find_buf(mgw_ctx* c, const std::string& name) {
+  auto it = c->ledger.bufs.find(name);
+  return it == c->ledger.bufs.end() ? nullptr : static_cast<double*>(it->second.ptr);
+}
+
+int release(mgw_ctx* c, const std::string& name) {
+  auto it = c->ledger.bufs.find(name);
+  if (it == c->ledger.bufs.end()) return MGW_OK;
+  if (it->second.ptr) {
+    MGW_HIP(hipStreamSynchronize(c->stream));
+    MGW_HIP(hipFree(it->second.ptr));
+    c->ledger.current -= (int64_t)it->second.bytes;
+  }
+  c->ledger.bufs.erase(it);
+  return MGW_OK;
+}
+
+template <class T>
+int upload(mgw_ctx* c, const std::string& name, const T* host, size_t count, T** out) {
+  MGW_TRY(dbuf(c, name, count, out));
+  if (count) MGW_HIP(hipMemcpyAsync(*out, host, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return MGW_OK;
+}
+
+template <class T>
+int download(mgw_ctx* c, T* host, const T* dev, size_t count) {
+  if (count) MGW_HIP(hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
+// Sum of a device vector (optionally of squares): deterministic block partials
+// summed on the host in a fixed order.
+int dsum(mgw_ctx* c, const double* x, int64_t n, int square, double* result) {
+  constexpr int NB = 256;
+  double* part = nullptr;
+  MGW_TRY(dbuf(c, "reduce_partials", NB, &part));
+  MGW_TRY(launch_vsum(x, n, square, part, NB, c->stream));
+  double h[NB];
+  MGW_TRY(download(c, h, part, NB));
+  double s = 0.0;
+  for (int i = 0; i < NB; ++i) s += h[i];
+  *result = s;
+  return MGW_OK;
+}
+
+// ------------------------------------------------------------------ ranges
+int check_ranges(mgw_ctx* c, const mgw_ranges* r) {
+  MGW_REQUIRE(r, MGW_ERR_ARG, "null ranges");
+  MGW_REQUIRE(c->has_mo, MGW_ERR_STATE, "MO data not set (mgw_set_mo)");
+  const int ns = c->nstate;
+  MGW_REQUIRE(r->ncore_G >= 0 && r->nvirtual_G <= ns + 1 && r->ncore_G + 1 <= r->nvirtual_G - 1,
+              MGW_ERR_ARG, "invalid ncore_G / nvirtual_G");
+  MGW_REQUIRE(r->nsemin >= 1 && r->nsemax <= ns && r->nsemin <= r->nsemax, MGW_ERR_ARG,
+              "invalid nsemin / nsemax");
+  MGW_REQUIRE(r->ncore_W >= 0 && r->nvirtual_W <= ns + 1, MGW_ERR_ARG, "invalid ncore_W / nvirtual_W");
+  return MGW_OK;
+}
+
+// Transition table (src/m_spectral_function.f90:224-266, skip_transition :359-378).
+// Entries are 0-based absolute state indices (i, a).
+void transition_table(const mgw_ctx* c, const mgw_ranges* r, double spin_fact, std::vector<int32_t>& ia) {
+  ia.clear();
+  const double ce = 1.0e-5;
+  const int ns = c->nstate;
+  auto skip = [&](int ib1, int ib2, double o1, double o2) {
+    if (ib1 <= r->ncore_W || ib2 <= r->ncore_W) return true;
+    if (ib1 >= r->nvirtual_W || ib2 >= r->nvirtual_W) return true;
+    if (o1 < ce && o2 < ce) return true;
+    if (o1 > spin_fact - ce && o2 > spin_fact - ce) return true;
+    return false;
+  };
+  for (int i = 1; i <= ns; ++i)
+    for (int j = 1; j <= ns; ++j) {
+      const double oj = c->occupation[j - 1], oi = c->occupation[i - 1];
+      if (skip(j, i, oj, oi)) continue;
+      if (oj - oi > 0.0) continue;
+      ia.push_back(i - 1);
+      ia.push_back(j - 1);
+    }
+}
+
+// Is the table the full product i in [i0, i1] (outer) x a in [a0, a1] (inner)?
+bool rectangular(const std::vector<int32_t>& ia, int& i0, int& i1, int& a0, int& a1) {
+  const int np = (int)ia.size() / 2;
+  if (np == 0) return false;
+  i0 = ia[0]; a0 = ia[1];
+  i1 = ia[2 * (np - 1)]; a1 = ia[2 * (np - 1) + 1];
+  const int na = a1 - a0 + 1, ni = i1 - i0 + 1;
+  if (na <= 0 || ni <= 0 || (int64_t)na * ni != np) return false;
+  for (int t = 0; t < np; ++t)
+    if (ia[2 * t] != i0 + t / na || ia[2 * t + 1] != a0 + t % na) return false;
+  return true;
+}
+
+// ------------------------------------------------------------------ A2
+// eri_mo(P, m, n) = sum_kl C(k, m) B(P, k, l) C(l, n) for m in [mlo, mhi],
+// n in [nlo, nhi] (1-based), into `out` (naux x nm x nn). The smaller range
+// is contracted first: H(P, k, f) = B(P k, l) C(l, f) is one GEMM over the
+// dense resident tensor, then a GEMM batched over f contracts k and writes
+// straight into the (P, m, n) layout of the reference.
+int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out) {
+  MGW_REQUIRE(c->has_ri, MGW_ERR_STATE, "RI tensor not set (mgw_set_eri_3center)");
+  MGW_REQUIRE(c->has_mo, MGW_ERR_STATE, "MO data not set (mgw_set_mo)");
+  MGW_REQUIRE(mlo >= 1 && mhi <= c->nstate && mlo <= mhi && nlo >= 1 && nhi <= c->nstate && nlo <= nhi,
+              MGW_ERR_ARG, "ao2mo: invalid state range");
+  const int nbf = c->nbf, naux = c->naux;
+  const int nm = mhi - mlo + 1, nn = nhi - nlo + 1;
+  const double* B = find_buf(c, "ri_dense");
+  const double* C = find_buf(c, "c_matrix");
+  const bool first_n = nn <= nm;
+  const int nf = first_n ? nn : nm;
+  const int flo = first_n ? nlo : mlo;
+  const int no = first_n ? nm : nn;
+  const int olo = first_n ? mlo : nlo;
+  double* H = nullptr;
+  MGW_TRY(dbuf(c, "ao2mo_half", (size_t)naux * nbf * nf, &H));
+  GemmF64 g1;
+  g1.M = naux * nbf; g1.N = nf; g1.K = nbf;
+  g1.A = B; g1.lda = (int64_t)naux * nbf;
+  g1.B = C + (int64_t)(flo - 1) * nbf; g1.ldb = nbf;
+  g1.C = H; g1.ldc = (int64_t)naux * nbf;
+  g1.klass = MGW_KCLASS_GEMM_AO2MO;
+  {
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_AO2MO, 2.0 * g1.M * (double)g1.N * g1.K,
+                 8.0 * ((double)g1.M * g1.K + (double)g1.K * g1.N + (double)g1.M * g1.N), c->stream);
+    MGW_TRY(gemm_f64(g1, c->stream));
+  }
+  GemmF64 g2;
+  g2.M = naux; g2.N = no; g2.K = nbf;
+  g2.A = H; g2.lda = naux; g2.strideA = (int64_t)naux * nbf;
+  g2.B = C + (int64_t)(olo - 1) * nbf; g2.ldb = nbf; g2.strideB = 0;
+  g2.batch = nf;
+  if (first_n) {
+    g2.C = out; g2.ldc = naux; g2.strideC = (int64_t)naux * nm;
+  } else {
+    g2.C = out; g2.ldc = (int64_t)naux * nm; g2.strideC = naux;
+  }
+  g2.klass = MGW_KCLASS_GEMM_AO2MO;
+  {
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_AO2MO, 2.0 * g2.M * (double)g2.N * g2.K * nf,
+                 8.0 * ((double)g2.M * g2.K * nf + (double)g2.K * g2.N + (double)g2.M * g2.N * nf), c->stream);
+    MGW_TRY(gemm_f64(g2, c->stream));
+  }
+  return MGW_OK;
+}
+
+// X(P, t) = (P| a_t i_t) for the W transition table, into buffer "x_ia".
+int build_x_ia(mgw_ctx* c, const mgw_ranges* r, double spin_fact, std::vector<int32_t>& ia,
+               double** x_out) {
+  transition_table(c, r, spin_fact, ia);
+  const int npole = (int)ia.size() / 2;
+  MGW_REQUIRE(npole > 0, MGW_ERR_ARG, "empty transition space");
+  double* x = nullptr;
+  MGW_TRY(dbuf(c, "x_ia", (size_t)c->naux * npole, &x));
+  int i0, i1, a0, a1;
+  if (rectangular(ia, i0, i1, a0, a1)) {
+    // (P, a, i) with a inner is exactly X(P, t), t = (i outer, a inner)
+    MGW_TRY(ao2mo(c, a0 + 1, a1 + 1, i0 + 1, i1 + 1, x));
+  } else {
+    const int wlo = r->ncore_W + 1, whi = r->nvirtual_W - 1;
+    const int nw = whi - wlo + 1;
+    double* e = nullptr;
+    MGW_TRY(dbuf(c, "ao2mo_w", (size_t)c->naux * nw * nw, &e));
+    MGW_TRY(ao2mo(c, wlo, whi, wlo, whi, e));
+    std::vector<int32_t> rel(ia.size());
+    for (size_t k = 0; k < ia.size(); ++k) rel[k] = ia[k] - (wlo - 1);
+    int32_t* ia_d = nullptr;
+    MGW_TRY(upload(c, "ia_rel", rel.data(), rel.size(), &ia_d));
+    MGW_TRY(launch_gather_ia(e, c->naux, nw, nw, ia_d, npole, x, c->stream));
+    MGW_TRY(release(c, "ao2mo_w"));
+  }
+  *x_out = x;
+  return MGW_OK;
+}
+
+// ------------------------------------------------------------------ A4
+int polarizability_casida(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double* en_rpa,
+                          double* en_gm) {
+  std::vector<int32_t> ia;
+  double* x = nullptr;
+  MGW_TRY(build_x_ia(c, r, spin_fact, ia, &x));
+  const int npole = (int)ia.size() / 2;
+  const int naux = c->naux;
+  int32_t* ia_d = nullptr;
+  MGW_TRY(upload(c, "ia", ia.data(), ia.size(), &ia_d));
+  double *de, *sq, *de2, *n2, *cs, *eig, *ework, *pole;
+  MGW_TRY(dbuf(c, "cas_de", npole, &de));
+  MGW_TRY(dbuf(c, "cas_sq", npole, &sq));
+  MGW_TRY(dbuf(c, "cas_de2", npole, &de2));
+  MGW_TRY(dbuf(c, "cas_n2", npole, &n2));
+  MGW_TRY(dbuf(c, "cas_cs", npole, &cs));
+  MGW_TRY(dbuf(c, "cas_eig", npole, &eig));
+  MGW_TRY(dbuf(c, "cas_ework", npole, &ework));
+  MGW_TRY(dbuf(c, "wpol_pole", npole, &pole));
+  double* energy_d = find_buf(c, "energy");
+  MGW_TRY(launch_casida_diag(ia_d, npole, energy_d, de, sq, de2, c->stream));
+  MGW_TRY(launch_colnorm2(x, naux, naux, npole, n2, c->stream));
+  // M = (A-B)^1/2 (A+B) (A-B)^1/2 = diag(de^2) + 2 spin_fact sq X^T X sq
+  double* m = nullptr;
+  MGW_TRY(dbuf(c, "cas_m", (size_t)npole * npole, &m));
+  GemmF64 g;
+  g.M = npole; g.N = npole; g.K = naux;
+  g.transA = true; g.A = x; g.lda = naux;
+  g.B = x; g.ldb = naux;
+  g.C = m; g.ldc = npole;
+  g.alpha = 2.0 * spin_fact;
+  g.rowscale = sq; g.colscale = sq; g.diag_add = de2;
+  g.lower_only = true;
+  g.klass = MGW_KCLASS_GEMM_CASIDA;
+  {
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_CASIDA, (double)npole * (npole + 1.0) * naux,
+                 8.0 * ((double)naux * npole + 0.5 * (double)npole * npole), c->stream);
+    MGW_TRY(gemm_f64(g, c->stream));
+  }
+  // eigen-decomposition (src/m_block_diago.f90:185)
+  int32_t* info_d = nullptr;
+  MGW_TRY(dbuf(c, "solver_info", 64, &info_d));
+  {
+    const double n3 = (double)npole * npole * npole;
+    ProfScope ps(&c->prof, MGW_KCLASS_EIGEN, (4.0 / 3.0 + 2.0) * n3, 16.0 * (double)npole * npole, c->stream);
+    MGW_BLAS(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_lower, npole, m, npole, eig,
+                              ework, info_d));
+  }
+  int32_t info = 0;
+  MGW_TRY(download(c, &info, info_d, 1));
+  MGW_REQUIRE(info == 0, MGW_ERR_SOLVER, "rocsolver_dsyevd failed, info=" + std::to_string(info));
+  MGW_TRY(launch_casida_post(eig, npole, pole, cs, c->stream));
+  // w_s = sqrt(spin_fact) X diag(sq) Z diag(Omega^-1/2)  (src/m_linear_response.f90:845-849)
+  double* ws = nullptr;
+  MGW_TRY(dbuf(c, "wpol_ws", (size_t)naux * npole, &ws));
+  GemmF64 h;
+  h.M = naux; h.N = npole; h.K = npole;
+  h.A = x; h.lda = naux;
+  h.B = m; h.ldb = npole;
+  h.C = ws; h.ldc = naux;
+  h.kscale = sq; h.colscale = cs;
+  h.alpha = std::sqrt(spin_fact);
+  h.klass = MGW_KCLASS_GEMM_CASIDA;
+  {
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_CASIDA, 2.0 * naux * (double)npole * npole,
+                 8.0 * (2.0 * (double)naux * npole + (double)npole * npole), c->stream);
+    MGW_TRY(gemm_f64(h, c->stream));
+  }
+  // energies (src/m_build_bse.f90:306-307, src/m_linear_response.f90:418, :849)
+  double sum_de, sum_n2, sum_pole, sum_ws2;
+  MGW_TRY(dsum(c, de, npole, 0, &sum_de));
+  MGW_TRY(dsum(c, n2, npole, 0, &sum_n2));
+  MGW_TRY(dsum(c, pole, npole, 0, &sum_pole));
+  MGW_TRY(dsum(c, ws, (int64_t)naux * npole, 1, &sum_ws2));
+  if (en_rpa) *en_rpa = -0.25 * (sum_de + 2.0 * spin_fact * sum_n2) - 0.25 * sum_de + 0.5 * sum_pole;
+  if (en_gm) *en_gm = 0.5 * (sum_ws2 - spin_fact * sum_n2);
+  c->npole = npole;
+  c->has_wpol = true;
+  MGW_TRY(release(c, "cas_m"));
+  return MGW_OK;
+}
+
+// ------------------------------------------------------------------ A6
+int gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta, int nomega,
+                  const double* omega, const double* se_energy0, double* energy_gw) {
+  MGW_REQUIRE(c->has_wpol, MGW_ERR_STATE, "no spectral function: call mgw_polarizability first");
+  const int naux = c->naux, npole = c->npole;
+  const int nse = r->nsemax - r->nsemin + 1;
+  const int glo = r->ncore_G + 1, ghi = r->nvirtual_G - 1;
+  const int ng = ghi - glo + 1;
+  double* epi = nullptr;
+  MGW_TRY(dbuf(c, "eri_pi", (size_t)naux * nse * ng, &epi));
+  MGW_TRY(ao2mo(c, r->nsemin, r->nsemax, glo, ghi, epi));
+  // small per-state vectors
+  std::vector<double> e0p(se_energy0, se_energy0 + nse), ei(ng), fi(ng), ep(nse), occp(nse);
+  for (int k = 0; k < ng; ++k) { ei[k] = c->energy[glo - 1 + k]; fi[k] = c->occupation[glo - 1 + k]; }
+  for (int k = 0; k < nse; ++k) { ep[k] = c->energy[r->nsemin - 1 + k]; occp[k] = c->occupation[r->nsemin - 1 + k]; }
+  double *om_d, *e0p_d, *ei_d, *fi_d, *ep_d, *occp_d;
+  MGW_TRY(upload(c, "se_omega", omega, nomega, &om_d));
+  MGW_TRY(upload(c, "se_e0p", e0p.data(), nse, &e0p_d));
+  MGW_TRY(upload(c, "se_ei", ei.data(), ng, &ei_d));
+  MGW_TRY(upload(c, "se_fi", fi.data(), ng, &fi_d));
+  MGW_TRY(upload(c, "se_ep", ep.data(), nse, &ep_d));
+  MGW_TRY(upload(c, "se_occp", occp.data(), nse, &occp_d));
+  double *sigma, *egw_p, *part;
+  MGW_TRY(dbuf(c, "sigma", (size_t)2 * nomega * nse, &sigma));
+  MGW_TRY(dbuf(c, "sigma_egw", nse, &egw_p));
+  MGW_TRY(dbuf(c, "sigma_part", (size_t)sigma_part_doubles(npole, nse), &part));
+  // chunk over i so that w(s, p, i) stays within ~1 GiB
+  const size_t per_i = (size_t)npole * nse * sizeof(double);
+  int ic = (int)std::max<size_t>(1, std::min<size_t>((size_t)ng, (size_t(1) << 30) / per_i));
+  double* w = nullptr;
+  MGW_TRY(dbuf(c, "sigma_w", (size_t)npole * nse * ic, &w));
+  const double* ws = find_buf(c, "wpol_ws");
+  const double* pole = find_buf(c, "wpol_pole");
+  for (int i0 = 0; i0 < ng; i0 += ic) {
+    const int cnt = std::min(ic, ng - i0);
+    // w(s, p, i) = sum_P w_s(P, s) (P | p i)   (src/m_gw_selfenergy_analytic.f90:101-104)
+    GemmF64 g;
+    g.M = npole; g.N = nse * cnt; g.K = naux;
+    g.transA = true; g.A = ws; g.lda = naux;
+    g.B = epi + (int64_t)naux * nse * i0; g.ldb = naux;
+    g.C = w; g.ldc = npole;
+    g.klass = MGW_KCLASS_GEMM_SIGMA;
+    {
+      ProfScope ps(&c->prof, MGW_KCLASS_GEMM_SIGMA, 2.0 * g.M * (double)g.N * g.K,
+                   8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N), c->stream);
+      MGW_TRY(gemm_f64(g, c->stream));
+    }
+    {
+      const double terms = (double)npole * nse * cnt * nomega;
+      ProfScope ps(&c->prof, MGW_KCLASS_SIGMA_POLES, 8.0 * terms, 8.0 * (double)npole * nse * cnt, c->stream);
+      MGW_TRY(launch_sigma_poles(w, npole, nse, cnt, pole, om_d, nomega, e0p_d, ei_d + i0, fi_d + i0,
+                                 ep_d, occp_d, spin_fact, eta, part, i0 > 0 ? 1 : 0, sigma, egw_p,
+                                 c->stream));
+    }
+  }
+  std::vector<double> egw(nse);
+  MGW_TRY(download(c, egw.data(), egw_p, nse));
+  if (energy_gw) {
+    double s = 0.0;
+    for (int k = 0; k < nse; ++k) s += egw[k];
+    *energy_gw = s;
+  }
+  c->has_sigma = true;
+  c->sigma_nomega = nomega;
+  c->sigma_nse = nse;
+  MGW_TRY(release(c, "sigma_w"));
+  MGW_TRY(release(c, "eri_pi"));
+  return MGW_OK;
+}
+
+// ------------------------------------------------------------------ A5
+int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int nomega,
+                        const double* omega_im, const double* weight, double* erpa, double* egw) {
+  MGW_REQUIRE(nomega >= 1 && nomega <= 65535, MGW_ERR_ARG, "nomega_chi_imag out of range");
+  std::vector<int32_t> ia;
+  double* x = nullptr;
+  MGW_TRY(build_x_ia(c, r, spin_fact, ia, &x));
+  const int npole = (int)ia.size() / 2;
+  const int naux = c->naux;
+  int32_t* ia_d = nullptr;
+  MGW_TRY(upload(c, "ia", ia.data(), ia.size(), &ia_d));
+  double *om_d, *sw, *n2, *ones, *tr0, *trl, *tri, *chi;
+  MGW_TRY(upload(c, "chi_omega", omega_im, nomega, &om_d));
+  MGW_TRY(dbuf(c, "chi_sw", (size_t)npole * nomega, &sw));
+  MGW_TRY(dbuf(c, "cas_n2", npole, &n2));
+  MGW_TRY(dbuf(c, "ones", naux, &ones));
+  MGW_TRY(dbuf(c, "chi_tr0", nomega, &tr0));
+  MGW_TRY(dbuf(c, "chi_trl", nomega, &trl));
+  MGW_TRY(dbuf(c, "chi_tri", nomega, &tri));
+  MGW_TRY(dbuf(c, "wpol_chi", (size_t)naux * naux * nomega, &chi));
+  MGW_TRY(launch_fill(ones, naux, 1.0, c->stream));
+  MGW_TRY(launch_chi_weights(ia_d, npole, find_buf(c, "energy"), find_buf(c, "occupation"), om_d,
+                             nomega, sw, c->stream));
+  MGW_TRY(launch_colnorm2(x, naux, naux, npole, n2, c->stream));
+  MGW_TRY(launch_chi_trace(sw, n2, npole, nullptr, naux, 0, nomega, tr0, c->stream));
+  // 1 - chi0(iw) = 1 + X diag(s(iw)) X^T for all frequencies in one launch
+  GemmF64 g;
+  g.M = naux; g.N = naux; g.K = npole;
+  g.A = x; g.lda = naux; g.strideA = 0;
+  g.B = x; g.ldb = naux; g.transB = true; g.strideB = 0;
+  g.C = chi; g.ldc = naux; g.strideC = (int64_t)naux * naux;
+  g.kscale = sw; g.strideK = npole;
+  g.diag_add = ones; g.strideDiag = 0;
+  g.batch = nomega;
+  g.lower_only = true;
+  g.klass = MGW_KCLASS_GEMM_CHI0;
+  {
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_CHI0, (double)naux * (naux + 1.0) * npole * nomega,
+                 8.0 * ((double)naux * npole + 0.5 * (double)naux * naux * nomega), c->stream);
+    MGW_TRY(gemm_f64(g, c->stream));
+  }
+  int32_t* info_d = nullptr;
+  MGW_TRY(dbuf(c, "solver_info", std::max(64, nomega), &info_d));
+  ProfScope chol(&c->prof, MGW_KCLASS_CHOLESKY, (double)naux * naux * naux * nomega,
+                 16.0 * (double)naux * naux * nomega, c->stream);
+  MGW_BLAS(rocsolver_dpotrf_strided_batched(c->blas, rocblas_fill_lower, naux, chi, naux,
+                                            (rocblas_stride)naux * naux, info_d, nomega));
+  std::vector<int32_t> info(nomega);
+  MGW_TRY(download(c, info.data(), info_d, nomega));
+  for (int k = 0; k < nomega; ++k)
+    MGW_REQUIRE(info[k] == 0, MGW_ERR_SOLVER, "Cholesky of 1-chi0 failed at frequency " + std::to_string(k));
+  MGW_TRY(launch_chi_trace(nullptr, nullptr, 0, chi, naux, 1, nomega, trl, c->stream));
+  MGW_BLAS(rocsolver_dpotri_strided_batched(c->blas, rocblas_fill_lower, naux, chi, naux,
+                                            (rocblas_stride)naux * naux, info_d, nomega));
+  MGW_TRY(download(c, info.data(), info_d, nomega));
+  for (int k = 0; k < nomega; ++k)
+    MGW_REQUIRE(info[k] == 0, MGW_ERR_SOLVER, "inverse of 1-chi0 failed at frequency " + std::to_string(k));
+  MGW_TRY(launch_chi_trace(nullptr, nullptr, 0, chi, naux, 2, nomega, tri, c->stream));
+  MGW_TRY(launch_chi_finish(chi, naux, nomega, c->stream));
+  std::vector<double> h0(nomega), hl(nomega), hi(nomega);
+  MGW_TRY(download(c, h0.data(), tr0, nomega));
+  MGW_TRY(download(c, hl.data(), trl, nomega));
+  MGW_TRY(download(c, hi.data(), tri, nomega));
+  // sum ln(lambda) + 1 - lambda = ln det(1-chi0) + Tr chi0 ;
+  // sum -(1-lambda)/lambda + 1 - lambda = n - Tr (1-chi0)^-1 + Tr chi0
+  // (src/m_gw_selfenergy_grid.f90:171-173)
+  const double twopi = 2.0 * 3.14159265358979323;
+  double er = 0.0, eg = 0.0;
+  for (int k = 0; k < nomega; ++k) {
+    er += (2.0 * hl[k] - h0[k]) / twopi * weight[k];
+    eg += ((double)naux - hi[k] - h0[k]) / twopi * weight[k];
+  }
+  if (erpa) *erpa = er;
+  if (egw) *egw = eg;
+  c->has_chi = true;
+  c->nomega_chi = nomega;
+  c->chi_omega.assign(omega_im, omega_im + nomega);
+  c->chi_weight.assign(weight, weight + nomega);
+  MGW_TRY(release(c, "chi_sw"));
+  return MGW_OK;
+}
+
+// ------------------------------------------------------------------ A7
+int gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int ncalc, const double* omega_calc) {
+  MGW_REQUIRE(c->has_chi, MGW_ERR_STATE, "no chi: call mgw_polarizability_grid first");
+  const int naux = c->naux, nomega = c->nomega_chi;
+  const int nse = r->nsemax - r->nsemin + 1;
+  const int glo = r->ncore_G + 1, ghi = r->nvirtual_G - 1;
+  const int np = ghi - glo + 1;
+  const int64_t ncol = (int64_t)np * nse;
+  MGW_REQUIRE(ncol < (1ll << 31), MGW_ERR_ARG, "sigma_imag: too many columns");
+  double* epm = nullptr;
+  MGW_TRY(dbuf(c, "eri_pm", (size_t)naux * ncol, &epm));
+  MGW_TRY(ao2mo(c, glo, ghi, r->nsemin, r->nsemax, epm));
+  const int nrb = gemm_f64_rowblocks(naux);
+  double *part, *v, *om_d, *wt_d, *ep_d, *wc_d, *sc;
+  MGW_TRY(dbuf(c, "dot_part", (size_t)nrb * ncol * nomega, &part));
+  MGW_TRY(dbuf(c, "dot_v", (size_t)ncol * nomega, &v));
+  // diag(B_m^T chi(iw) B_m) for every (p, m) and iw   (src/m_gw_selfenergy_grid.f90:297-310)
+  GemmF64 g;
+  g.M = naux; g.N = (int)ncol; g.K = naux;
+  g.A = find_buf(c, "wpol_chi"); g.lda = naux; g.strideA = (int64_t)naux * naux;
+  g.B = epm; g.ldb = naux; g.strideB = 0;
+  g.dotB = epm; g.lddot = naux;
+  g.dotpart = part; g.strideDot = (int64_t)nrb * ncol;
+  g.batch = nomega;
+  {
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_WB, 2.0 * naux * (double)naux * ncol * nomega,
+                 8.0 * ((double)naux * naux * nomega + 2.0 * (double)naux * ncol), c->stream);
+    MGW_TRY(gemm_f64(g, c->stream));
+  }
+  MGW_TRY(launch_dot_reduce(part, nrb, ncol, nomega, v, c->stream));
+  std::vector<double> ep(np);
+  for (int k = 0; k < np; ++k) ep[k] = c->energy[glo - 1 + k];
+  MGW_TRY(upload(c, "sgi_omega", c->chi_omega.data(), nomega, &om_d));
+  MGW_TRY(upload(c, "sgi_weight", c->chi_weight.data(), nomega, &wt_d));
+  MGW_TRY(upload(c, "sgi_ep", ep.data(), np, &ep_d));
+  MGW_TRY(upload(c, "sgi_wc", omega_calc, (size_t)2 * ncalc, &wc_d));
+  MGW_TRY(dbuf(c, "sigma_calc", (size_t)2 * ncalc * nse, &sc));
+  MGW_TRY(launch_sigma_imag_sum(v, np, nse, nomega, om_d, wt_d, ep_d, wc_d, ncalc, sc, c->stream));
+  c->has_sigma_calc = true;
+  c->sigma_ncalc = ncalc;
+  c->sigma_calc_nse = nse;
+  MGW_TRY(release(c, "dot_part"));
+  MGW_TRY(release(c, "eri_pm"));
+  return MGW_OK;
+}
+
+// ------------------------------------------------------------------ A8
+int pade_fit(mgw_ctx* c, int nse, int ncalc, const double* omega_calc, const double* sigma_calc_host,
+             int nomega, const double* omega, double eta) {
+  const double* sc = nullptr;
+  if (sigma_calc_host) {
+    double* d = nullptr;
+    MGW_TRY(upload(c, "pade_sc_in", sigma_calc_host, (size_t)2 * ncalc * nse, &d));
+    sc = d;
+  } else {
+    MGW_REQUIRE(c->has_sigma_calc && c->sigma_ncalc == ncalc && c->sigma_calc_nse == nse,
+                MGW_ERR_STATE, "no resident sigma_calc of this shape");
+    sc = find_buf(c, "sigma_calc");
+  }
+  double *wc_d, *om_d, *work, *sigma;
+  MGW_TRY(upload(c, "pade_wc", omega_calc, (size_t)2 * ncalc, &wc_d));
+  MGW_TRY(upload(c, "pade_omega", omega, nomega, &om_d));
+  MGW_TRY(dbuf(c, "pade_work", (size_t)pade_work_doubles(ncalc, nse), &work));
+  MGW_TRY(dbuf(c, "sigma", (size_t)2 * nomega * nse, &sigma));
+  MGW_TRY(launch_pade(wc_d, sc, ncalc, nse, om_d, nomega, eta, work, sigma, c->stream));
+  c->has_sigma = true;
+  c->sigma_nomega = nomega;
+  c->sigma_nse = nse;
+  return MGW_OK;
+}
+
+// ------------------------------------------------------------------ A9
+int find_qp(mgw_ctx* c, const mgw_ranges* r, int nomega, const double* omega, const double* sigma_host,
+            const double* se_energy0, const double* exc, const double* energy0, int nstate,
+            double* energy_qp, double* zz, double* energy_qp_lin) {
+  const int nse = r->nsemax - r->nsemin + 1;
+  MGW_REQUIRE(r->nsemin >= 1 && r->nsemax <= nstate, MGW_ERR_ARG, "find_qp: state range exceeds nstate");
+  const double* sg = nullptr;
+  if (sigma_host) {
+    double* d = nullptr;
+    MGW_TRY(upload(c, "qp_sigma_in", sigma_host, (size_t)2 * nomega * nse, &d));
+    sg = d;
+  } else {
+    MGW_REQUIRE(c->has_sigma && c->sigma_nomega == nomega && c->sigma_nse == nse, MGW_ERR_STATE,
+                "no resident sigma of this shape");
+    sg = find_buf(c, "sigma");
+  }
+  std::vector<double> e0s(nse), excs(nse);
+  for (int k = 0; k < nse; ++k) {
+    e0s[k] = energy0[r->nsemin - 1 + k];
+    excs[k] = exc ? exc[r->nsemin - 1 + k] : 0.0;
+  }
+  double *om_d, *se0_d, *exc_d, *e0_d, *out;
+  MGW_TRY(upload(c, "qp_omega", omega, nomega, &om_d));
+  MGW_TRY(upload(c, "qp_se0", se_energy0, nse, &se0_d));
+  MGW_TRY(upload(c, "qp_exc", excs.data(), nse, &exc_d));
+  MGW_TRY(upload(c, "qp_e0", e0s.data(), nse, &e0_d));
+  MGW_TRY(dbuf(c, "qp_out", (size_t)3 * nse, &out));
+  MGW_TRY(launch_qp(sg, nomega, nse, om_d, se0_d, exc_d, e0_d, out, out + nse, out + 2 * nse, c->stream));
+  std::vector<double> h(3 * nse);
+  MGW_TRY(download(c, h.data(), out, 3 * nse));
+  for (int p = 0; p < nstate; ++p) {
+    const bool in = p >= r->nsemin - 1 && p <= r->nsemax - 1;
+    const int k = p - (r->nsemin - 1);
+    if (energy_qp) energy_qp[p] = in ? h[k] : energy0[p];
+    if (zz) zz[p] = in ? h[nse + k] : 1.0;
+    if (energy_qp_lin) energy_qp_lin[p] = in ? h[2 * nse + k] : energy0[p];
+  }
+  return MGW_OK;
+}
+
+// Gauss-Legendre on [0,1] mapped to the imaginary axis
+// (src/m_numerical_tools.f90:35-82, src/m_spectral_function.f90:286-306).
+void imaginary_grid(int n, std::vector<double>& om, std::vector<double>& wt) {
+  const double pi = 3.14159265358979323;
+  std::vector<double> x(n), w(n);
+  for (int i = 1; i <= (n + 1) / 2; ++i) {
+    double z = std::cos(pi * (i - 0.25) / (n + 0.5)), dp = 0.0;
+    for (;;) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      dp = n * (p2 - z * p1) / (1.0 - z * z);
+      const double z1 = z;
+      z = z1 - p1 / dp;
+      if (std::fabs(z - z1) < 1.0e-13) break;
+    }
+    x[i - 1] = 0.5 - 0.5 * z;
+    x[n - i] = 0.5 + 0.5 * z;
+    w[i - 1] = 1.0 / ((1.0 - z * z) * dp * dp);
+    w[n - i] = w[i - 1];
+  }
+  om.resize(n);
+  wt.resize(n);
+  for (int k = 0; k < n; ++k) {
+    wt[k] = w[k] * std::pow(1.0 - x[k], -2.0) * 2.0;
+    om[k] = (1.0 / (1.0 - x[k]) - 1.0) * 2.0;
+  }
+}
+
+}  // namespace
+
+// ======================================================================
+//                               C-ABI
+// ======================================================================
+extern "C" {
+
+int32_t mgw_abi_version(void) { return 1; }
+
+const char* mgw_last_error(void) { return last_error().c_str(); }
+
+int mgw_ctx_create(int device, mgw_ctx** out) {
+  MGW_REQUIRE(out, MGW_ERR_ARG, "null output pointer");
+  int ndev = 0;
+  MGW_HIP(hipGetDeviceCount(&ndev));
+  MGW_REQUIRE(device >= 0 && device < ndev, MGW_ERR_ARG, "invalid device " + std::to_string(device));
+  MGW_HIP(hipSetDevice(device));
+  mgw_ctx* c = new mgw_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    return MGW_ERR_HIP;
+  }
+  if (rocblas_create_handle(&c->blas) != rocblas_status_success ||
+      rocblas_set_stream(c->blas, c->stream) != rocblas_status_success) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    set_error("rocblas handle creation failed");
+    return MGW_ERR_SOLVER;
+  }
+  *out = c;
+  return MGW_OK;
+}
+
+int mgw_ctx_destroy(mgw_ctx* c) {
+  if (!c) return MGW_OK;
+  MGW_HIP(hipSetDevice(c->device));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  for (auto& kv : c->ledger.bufs)
+    if (kv.second.ptr) MGW_HIP(hipFree(kv.second.ptr));
+  rocblas_destroy_handle(c->blas);
+  MGW_HIP(hipStreamDestroy(c->stream));
+  delete c;
+  return MGW_OK;
+}
+
+int mgw_ctx_synchronize(mgw_ctx* c) {
+  MGW_CTX(c);
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
+int mgw_profile_enable(mgw_ctx* c, int enable) {
+  MGW_CTX(c);
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  c->prof.reset();
+  c->prof.on = enable != 0;
+  return MGW_OK;
+}
+
+int mgw_profile_read(mgw_ctx* c, int kernel_class, int64_t* launches, double* total_ms,
+                     double* algorithmic_flops, double* algorithmic_bytes) {
+  MGW_CTX(c);
+  MGW_REQUIRE(kernel_class >= 0 && kernel_class < MGW_KCLASS_COUNT, MGW_ERR_ARG, "invalid kernel class");
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  int64_t n = 0;
+  double ms = 0.0, fl = 0.0, by = 0.0;
+  for (const auto& r : c->prof.recs) {
+    if (r.klass != kernel_class) continue;
+    float t = 0.0f;
+    MGW_HIP(hipEventElapsedTime(&t, r.a, r.b));
+    ++n;
+    ms += t;
+    fl += r.flops;
+    by += r.bytes;
+  }
+  if (launches) *launches = n;
+  if (total_ms) *total_ms = ms;
+  if (algorithmic_flops) *algorithmic_flops = fl;
+  if (algorithmic_bytes) *algorithmic_bytes = by;
+  return MGW_OK;
+}
+
+int mgw_memory_report(mgw_ctx* c, int64_t* current_bytes, int64_t* peak_bytes) {
+  MGW_REQUIRE(c, MGW_ERR_ARG, "null context");
+  if (current_bytes) *current_bytes = c->ledger.current;
+  if (peak_bytes) *peak_bytes = c->ledger.peak;
+  return MGW_OK;
+}
+
+int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int technique,
+                            int nomega_chi_imag, int precision) {
+  (void)precision;
+  const int64_t nb = nbf, na = nauxil, ns = nstate, no = nocc, nv = nstate - nocc;
+  const int64_t npole = no * nv;
+  int64_t d = na * nb * nb;                       // resident dense RI tensor
+  d += nb * ns;                                   // C
+  int64_t stage_w;
+  if (technique == MGW_TECH_ONE_SHOT) {
+    stage_w = na * nb * no + na * npole           // half transform + (P|ia)
+              + 2 * npole * npole                 // Casida matrix + eigen workspace
+              + na * npole;                       // w_s
+  } else {
+    stage_w = na * nb * no + na * npole + na * na * (int64_t)nomega_chi_imag + npole * nomega_chi_imag;
+  }
+  const int64_t stage_s = na * nb * ns + na * ns * ns + na * npole + (int64_t(1) << 27);
+  return 8 * (d + std::max(stage_w, stage_s));
+}
+
+int mgw_set_eri_3center(mgw_ctx* c, const double* eri, int64_t npair, int nauxil,
+                        const int32_t* index_basis, int nbf, int where) {
+  MGW_CTX(c);
+  MGW_REQUIRE(eri && index_basis && npair > 0 && nauxil > 0 && nbf > 0, MGW_ERR_ARG,
+              "mgw_set_eri_3center: invalid arguments");
+  MGW_REQUIRE(where == MGW_MEM_HOST || where == MGW_MEM_DEVICE, MGW_ERR_ARG, "invalid memory kind");
+  MGW_REQUIRE((int64_t)nauxil * nbf * (int64_t)nbf < (int64_t)1 << 40, MGW_ERR_ARG, "RI tensor too large");
+  std::vector<int32_t> kl(2 * npair);
+  for (int64_t p = 0; p < npair; ++p) {
+    const int32_t k = index_basis[2 * p] - 1, l = index_basis[2 * p + 1] - 1;
+    MGW_REQUIRE(k >= 0 && k < nbf && l >= 0 && l < nbf, MGW_ERR_ARG,
+                "index_basis entry out of range at pair " + std::to_string(p + 1));
+    kl[2 * p] = k;
+    kl[2 * p + 1] = l;
+  }
+  const double* packed = eri;
+  if (where == MGW_MEM_HOST) {
+    double* d = nullptr;
+    MGW_TRY(upload(c, "ri_packed", eri, (size_t)npair * nauxil, &d));
+    packed = d;
+  }
+  int32_t* kl_d = nullptr;
+  MGW_TRY(upload(c, "ri_kl", kl.data(), kl.size(), &kl_d));
+  double* dense = nullptr;
+  MGW_TRY(dbuf(c, "ri_dense", (size_t)nauxil * nbf * nbf, &dense));
+  MGW_TRY(launch_unpack_ri(packed, npair, nauxil, kl_d, nbf, dense, c->stream));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  MGW_TRY(release(c, "ri_packed"));
+  MGW_TRY(release(c, "ri_kl"));
+  c->nbf = nbf;
+  c->naux = nauxil;
+  c->has_ri = true;
+  c->has_eri_mo = false;
+  return MGW_OK;
+}
+
+int mgw_destroy_eri_3center(mgw_ctx* c) {
+  MGW_CTX(c);
+  MGW_TRY(release(c, "ri_dense"));
+  c->has_ri = false;
+  return MGW_OK;
+}
+
+int mgw_set_mo(mgw_ctx* c, const double* c_matrix, int nbf, int nstate, const double* energy,
+               const double* occupation, int where) {
+  MGW_CTX(c);
+  MGW_REQUIRE(c_matrix && energy && occupation && nbf > 0 && nstate > 0, MGW_ERR_ARG,
+              "mgw_set_mo: invalid arguments");
+  MGW_REQUIRE(!c->has_ri || nbf == c->nbf, MGW_ERR_ARG, "mgw_set_mo: nbf differs from the RI tensor");
+  double* cd = nullptr;
+  MGW_TRY(dbuf(c, "c_matrix", (size_t)nbf * nstate, &cd));
+  c->energy.resize(nstate);
+  c->occupation.resize(nstate);
+  if (where == MGW_MEM_DEVICE) {
+    MGW_HIP(hipMemcpyAsync(cd, c_matrix, sizeof(double) * nbf * nstate, hipMemcpyDeviceToDevice, c->stream));
+    MGW_HIP(hipMemcpyAsync(c->energy.data(), energy, sizeof(double) * nstate, hipMemcpyDeviceToHost, c->stream));
+    MGW_HIP(hipMemcpyAsync(c->occupation.data(), occupation, sizeof(double) * nstate, hipMemcpyDeviceToHost, c->stream));
+    MGW_HIP(hipStreamSynchronize(c->stream));
+  } else {
+    MGW_HIP(hipMemcpyAsync(cd, c_matrix, sizeof(double) * nbf * nstate, hipMemcpyHostToDevice, c->stream));
+    std::memcpy(c->energy.data(), energy, sizeof(double) * nstate);
+    std::memcpy(c->occupation.data(), occupation, sizeof(double) * nstate);
+  }
+  double *ed, *od;
+  MGW_TRY(upload(c, "energy", c->energy.data(), nstate, &ed));
+  MGW_TRY(upload(c, "occupation", c->occupation.data(), nstate, &od));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  c->nstate = nstate;
+  c->has_mo = true;
+  c->has_eri_mo = false;
+  c->has_wpol = false;
+  c->has_chi = false;
+  return MGW_OK;
+}
+
+int mgw_calculate_eri_3center_mo(mgw_ctx* c, int mstate_min, int mstate_max, int nstate_min,
+                                 int nstate_max, double* out) {
+  MGW_CTX(c);
+  const int nm = mstate_max - mstate_min + 1, nn = nstate_max - nstate_min + 1;
+  if (!c->has_eri_mo) {
+    double* e = nullptr;
+    MGW_REQUIRE(nm > 0 && nn > 0, MGW_ERR_ARG, "empty state range");
+    MGW_TRY(dbuf(c, "eri_3center_mo", (size_t)c->naux * nm * nn, &e));
+    MGW_TRY(ao2mo(c, mstate_min, mstate_max, nstate_min, nstate_max, e));
+    c->has_eri_mo = true;
+    c->mlo = mstate_min; c->mhi = mstate_max; c->nlo = nstate_min; c->nhi = nstate_max;
+  }
+  if (out) {
+    MGW_REQUIRE(c->mlo == mstate_min && c->mhi == mstate_max && c->nlo == nstate_min && c->nhi == nstate_max,
+                MGW_ERR_STATE, "a different eri_3center_mo block is resident; destroy it first");
+    MGW_TRY(download(c, out, find_buf(c, "eri_3center_mo"), (size_t)c->naux * nm * nn));
+  }
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
+int mgw_destroy_eri_3center_mo(mgw_ctx* c) {
+  MGW_CTX(c);
+  MGW_TRY(release(c, "eri_3center_mo"));
+  c->has_eri_mo = false;
+  return MGW_OK;
+}
+
+int mgw_npole(mgw_ctx* c, const mgw_ranges* r, int* npole) {
+  MGW_CTX(c);
+  MGW_TRY(check_ranges(c, r));
+  std::vector<int32_t> ia;
+  transition_table(c, r, 2.0, ia);
+  if (npole) *npole = (int)ia.size() / 2;
+  return MGW_OK;
+}
+
+int mgw_polarizability(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int precision,
+                       double* en_rpa, double* en_gw, double* pole_out, double* w_s_out) {
+  MGW_CTX(c);
+  MGW_TRY(check_ranges(c, r));
+  MGW_REQUIRE(precision == MGW_FP64, MGW_ERR_ARG, "mgw_polarizability: only MGW_FP64 is available");
+  MGW_TRY(polarizability_casida(c, r, spin_fact, en_rpa, en_gw));
+  if (pole_out) MGW_TRY(download(c, pole_out, find_buf(c, "wpol_pole"), c->npole));
+  if (w_s_out) MGW_TRY(download(c, w_s_out, find_buf(c, "wpol_ws"), (size_t)c->naux * c->npole));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
+int mgw_polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int precision,
+                            int nomega, const double* omega_im, const double* weight_quad,
+                            double* erpa, double* egw, double* chi_out) {
+  MGW_CTX(c);
+  MGW_TRY(check_ranges(c, r));
+  MGW_REQUIRE(precision == MGW_FP64, MGW_ERR_ARG, "mgw_polarizability_grid: only MGW_FP64 is available");
+  MGW_REQUIRE(omega_im && weight_quad, MGW_ERR_ARG, "null frequency grid");
+  MGW_TRY(polarizability_grid(c, r, spin_fact, nomega, omega_im, weight_quad, erpa, egw));
+  if (chi_out) MGW_TRY(download(c, chi_out, find_buf(c, "wpol_chi"), (size_t)c->naux * c->naux * nomega));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
+int mgw_gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta, int nomega,
+                      const double* omega, const double* se_energy0, double* sigma_out,
+                      double* energy_gw) {
+  MGW_CTX(c);
+  MGW_TRY(check_ranges(c, r));
+  MGW_REQUIRE(omega && se_energy0 && nomega >= 1, MGW_ERR_ARG, "invalid frequency grid");
+  MGW_TRY(gw_selfenergy(c, r, spin_fact, eta, nomega, omega, se_energy0, energy_gw));
+  const int nse = r->nsemax - r->nsemin + 1;
+  if (sigma_out) MGW_TRY(download(c, sigma_out, find_buf(c, "sigma"), (size_t)2 * nomega * nse));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
+int mgw_gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int nomega_calc, const double* omega_calc,
+                           double* sigma_calc_out) {
+  MGW_CTX(c);
+  MGW_TRY(check_ranges(c, r));
+  MGW_REQUIRE(omega_calc && nomega_calc >= 1 && nomega_calc <= 65535, MGW_ERR_ARG, "invalid omega_calc");
+  MGW_TRY(gw_selfenergy_imag(c, r, nomega_calc, omega_calc));
+  const int nse = r->nsemax - r->nsemin + 1;
+  if (sigma_calc_out) MGW_TRY(download(c, sigma_calc_out, find_buf(c, "sigma_calc"), (size_t)2 * nomega_calc * nse));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
+int mgw_pade_fit(mgw_ctx* c, int nse, int nomega_calc, const double* omega_calc, const double* sigma_calc,
+                 int nomega, const double* omega, double eta, double* sigma_out) {
+  MGW_CTX(c);
+  MGW_REQUIRE(nse >= 1 && nomega_calc >= 1 && nomega >= 1 && omega_calc && omega, MGW_ERR_ARG,
+              "mgw_pade_fit: invalid arguments");
+  MGW_TRY(pade_fit(c, nse, nomega_calc, omega_calc, sigma_calc, nomega, omega, eta));
+  if (sigma_out) MGW_TRY(download(c, sigma_out, find_buf(c, "sigma"), (size_t)2 * nomega * nse));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
+int mgw_find_qp_energy(mgw_ctx* c, const mgw_ranges* r, int nomega, const double* omega,
+                       const double* sigma, const double* se_energy0, const double* exchange_m_vxc,
+                       const double* energy0, int nstate, double* energy_qp, double* zz,
+                       double* energy_qp_lin) {
+  MGW_CTX(c);
+  MGW_REQUIRE(r && omega && se_energy0 && energy0 && nomega >= 1 && nstate >= 1, MGW_ERR_ARG,
+              "mgw_find_qp_energy: invalid arguments");
+  return find_qp(c, r, nomega, omega, sigma, se_energy0, exchange_m_vxc, energy0, nstate, energy_qp, zz,
+                 energy_qp_lin);
+}
+
+int mgw_g0w0(mgw_ctx* c, const mgw_g0w0_params* prm, const mgw_ranges* r, const double* exc,
+             double* energy_qp, double* zz, double* energy_qp_lin, double* sigma_out, double* energies) {
+  MGW_CTX(c);
+  MGW_REQUIRE(prm, MGW_ERR_ARG, "null parameters");
+  MGW_TRY(check_ranges(c, r));
+  MGW_REQUIRE(prm->precision == MGW_FP64, MGW_ERR_ARG, "mgw_g0w0: only MGW_FP64 is available");
+  MGW_REQUIRE(prm->nomega_sigma >= 1, MGW_ERR_ARG, "nomega_sigma must be positive");
+  const int nstate = c->nstate;
+  const int nse = r->nsemax - r->nsemin + 1;
+  const int nom = prm->nomega_sigma / 2;
+  const int nomega = 2 * nom + 1;
+  std::vector<double> omega(nomega), se0(nse);
+  double en_rpa = 0.0, en_gw = 0.0;
+  if (prm->technique == MGW_TECH_ONE_SHOT) {
+    // se_init(one_shot): grid centred on each energy0 (src/m_selfenergy_tools.f90:682-690)
+    for (int k = 0; k < nomega; ++k) omega[k] = prm->step_sigma * (k - nom);
+    for (int k = 0; k < nse; ++k) se0[k] = c->energy[r->nsemin - 1 + k];
+    MGW_TRY(polarizability_casida(c, r, prm->spin_fact, &en_rpa, nullptr));
+    MGW_TRY(gw_selfenergy(c, r, prm->spin_fact, prm->eta, nomega, omega.data(), se0.data(), &en_gw));
+  } else if (prm->technique == MGW_TECH_IMAGINARY_PADE) {
+    MGW_REQUIRE(prm->nomega_chi_imag >= 1 && prm->nomega_sigma_calc >= 1, MGW_ERR_ARG,
+                "imaginary-axis technique needs nomega_chi_imag and nomega_sigma_calc");
+    MGW_REQUIRE(r->nhomo_G >= 1 && r->nhomo_G < nstate, MGW_ERR_ARG, "nhomo_G out of range");
+    // se_init(imaginary_axis_pade) (src/m_selfenergy_tools.f90:598-614)
+    const double efermi = 0.5 * (c->energy[r->nhomo_G - 1] + c->energy[r->nhomo_G]);
+    for (int k = 0; k < nomega; ++k) omega[k] = efermi + prm->step_sigma * (k - nom);
+    std::fill(se0.begin(), se0.end(), 0.0);
+    std::vector<double> wc(2 * prm->nomega_sigma_calc);
+    for (int k = 0; k < prm->nomega_sigma_calc; ++k) {
+      wc[2 * k] = efermi;
+      wc[2 * k + 1] = prm->step_sigma_calc * k;
+    }
+    std::vector<double> om, wt;
+    imaginary_grid(prm->nomega_chi_imag, om, wt);
+    MGW_TRY(polarizability_grid(c, r, prm->spin_fact, prm->nomega_chi_imag, om.data(), wt.data(), &en_rpa, &en_gw));
+    MGW_TRY(gw_selfenergy_imag(c, r, prm->nomega_sigma_calc, wc.data()));
+    MGW_TRY(pade_fit(c, nse, prm->nomega_sigma_calc, wc.data(), nullptr, nomega, omega.data(), prm->eta));
+  } else {
+    set_error("unknown technique");
+    return MGW_ERR_ARG;
+  }
+  MGW_TRY(find_qp(c, r, nomega, omega.data(), nullptr, se0.data(), exc, c->energy.data(), nstate, energy_qp,
+                  zz, energy_qp_lin));
+  if (sigma_out) MGW_TRY(download(c, sigma_out, find_buf(c, "sigma"), (size_t)2 * nomega * nse));
+  if (energies) {
+    energies[0] = en_rpa;
+    energies[1] = en_gw;
+  }
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
+}  // extern "C"

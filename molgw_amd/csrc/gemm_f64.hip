@@ -1,0 +1,299 @@
+// FP64 GEMM on the CDNA4 matrix cores (v_mfma_f64_16x16x4_f64), gfx950.
+//
+// One kernel serves every dense contraction of the RI G0W0 path:
+//   AO->MO half transforms ((P|mu nu) . C), the Casida SYRK (B^T B scaled by
+//   sqrt(de)), w_s = B (X+Y), the self-energy products w_s^T (P|p i), the
+//   imaginary-axis chi0 = X diag(s(iw)) X^T batched over frequencies and
+//   chi(iw) (P|p m) with a fused column-dot epilogue.
+//
+// Tiling: a workgroup of WM x WN wave64s owns a BM x BN tile of C and walks K
+// in slabs of BK = 16 through double-buffered LDS (one barrier per slab, the
+// next slab's global loads in flight during the MFMAs of the current one).
+// Each wave owns (BM/WM) x (BN/WN) and issues (BM/WM/16) * (BN/WN/16)
+// independent MFMA chains per k-step of 4, which hides the MFMA latency.
+//
+// Fragment layout of v_mfma_f64_16x16x4_f64 (checked on gfx950): operand "a"
+// lane l = a[i = l & 15][k = l >> 4], operand "b" lane l = b[k = l >> 4][j = l & 15],
+// result lane l register r = d[i = (l >> 4) + 4 r][j = l & 15]. The kernel
+// feeds a = B^T and b = A^T so that d[i][j] = C(m = j, n = i): the 16 lanes of
+// a row group hold 16 consecutive rows m of one column of the column-major C
+// and every store is a 128-byte contiguous segment.
+#include "mgw_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace mgw {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int BK = 16;
+constexpr int PAD = 1;
+
+struct GemmDev {
+  int M, N, K;
+  const double* A; int64_t lda, strideA;
+  const double* B; int64_t ldb, strideB;
+  double* C; int64_t ldc, strideC;
+  const double* kscale; int64_t strideK;
+  const double* rowscale;
+  const double* colscale;
+  const double* diag_add; int64_t strideDiag;
+  double alpha, beta;
+  int lower_only;
+  int tiles_m, tiles_n;
+  const double* dotB; int64_t lddot;
+  double* dotpart; int64_t strideDot;
+};
+
+// Map a triangular tile index t (0-based) to (bm, bn) with bm >= bn.
+__device__ inline void tri_index(int t, int& bm, int& bn) {
+  int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((r + 1) * (r + 2) / 2 <= t) ++r;
+  while (r * (r + 1) / 2 > t) --r;
+  bm = r;
+  bn = t - r * (r + 1) / 2;
+}
+
+template <int BM, int BN, int WM, int WN, bool TA, bool TB, bool DOT, int KLASS>
+__global__ void __launch_bounds__(64 * WM * WN)
+gemm_f64_kernel(GemmDev g) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = BM / WM;
+  constexpr int TN = BN / WN;
+  constexpr int FM = TM / 16;
+  constexpr int FN = TN / 16;
+  constexpr int LA = BM * BK / NT;   // A elements loaded per thread per slab
+  constexpr int LB = BN * BK / NT;
+  static_assert(BM * BK % NT == 0 && BN * BK % NT == 0, "tile/thread mismatch");
+
+  __shared__ double As[2][BK][BM + PAD];
+  __shared__ double Bs[2][BK][BN + PAD];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave % WM;
+  const int wn = wave / WM;
+
+  int bm, bn;
+  if (g.lower_only) {
+    tri_index(blockIdx.x, bm, bn);
+  } else {
+    bm = blockIdx.x % g.tiles_m;
+    bn = blockIdx.x / g.tiles_m;
+  }
+  const int batch = blockIdx.y;
+  const int m0 = bm * BM;
+  const int n0 = bn * BN;
+
+  const double* __restrict__ A = g.A + (int64_t)batch * g.strideA;
+  const double* __restrict__ B = g.B + (int64_t)batch * g.strideB;
+  const double* __restrict__ ks = g.kscale ? g.kscale + (int64_t)batch * g.strideK : nullptr;
+
+  double ra[LA], rb[LB];
+
+  auto load_slab = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+      const int e = tid + i * NT;
+      int m, k;
+      if (!TA) { m = e % BM; k = e / BM; } else { k = e % BK; m = e / BK; }
+      const int gm = m0 + m, gk = k0 + k;
+      double v = 0.0;
+      if (gm < g.M && gk < g.K) v = TA ? A[gk + (int64_t)gm * g.lda] : A[gm + (int64_t)gk * g.lda];
+      ra[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int e = tid + i * NT;
+      int n, k;
+      if (!TB) { k = e % BK; n = e / BK; } else { n = e % BN; k = e / BN; }
+      const int gn = n0 + n, gk = k0 + k;
+      double v = 0.0;
+      if (gn < g.N && gk < g.K) {
+        v = TB ? B[gn + (int64_t)gk * g.ldb] : B[gk + (int64_t)gn * g.ldb];
+        if (ks) v *= ks[gk];
+      }
+      rb[i] = v;
+    }
+  };
+  auto store_slab = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+      const int e = tid + i * NT;
+      int m, k;
+      if (!TA) { m = e % BM; k = e / BM; } else { k = e % BK; m = e / BK; }
+      As[buf][k][m] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int e = tid + i * NT;
+      int n, k;
+      if (!TB) { k = e % BK; n = e / BK; } else { n = e % BN; k = e / BN; }
+      Bs[buf][k][n] = rb[i];
+    }
+  };
+
+  d4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const int nslab = (g.K + BK - 1) / BK;
+  load_slab(0);
+  store_slab(0);
+  __syncthreads();
+
+  const int fr = lane & 15;
+  const int fk = lane >> 4;
+  for (int s = 0; s < nslab; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nslab) load_slab((s + 1) * BK);
+#pragma unroll
+    for (int k4 = 0; k4 < BK / 4; ++k4) {
+      double af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = As[buf][k4 * 4 + fk][wm * TM + i * 16 + fr];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = Bs[buf][k4 * 4 + fk][wn * TN + j * 16 + fr];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+    }
+    if (s + 1 < nslab) store_slab(buf ^ 1);
+    __syncthreads();
+  }
+
+  // ------------------------------------------------------------ epilogue
+  double* __restrict__ C = g.C ? g.C + (int64_t)batch * g.strideC : nullptr;
+  const double* __restrict__ dg = g.diag_add ? g.diag_add + (int64_t)batch * g.strideDiag : nullptr;
+
+  if constexpr (DOT) {
+    // Column dots of this row block: sum_m C(m,n) * D(m,n).
+    __shared__ double red[WM][BN];
+    const double* __restrict__ D = g.dotB + (int64_t)batch * g.strideB;  // same batching as B
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int nl = wn * TN + j * 16 + fk + 4 * r;
+        const int n = n0 + nl;
+        double part = 0.0;
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          const int m = m0 + wm * TM + i * 16 + fr;
+          if (m < g.M && n < g.N) part += g.alpha * acc[i][j][r] * D[m + (int64_t)n * g.lddot];
+        }
+        part += __shfl_xor(part, 8, 16);
+        part += __shfl_xor(part, 4, 16);
+        part += __shfl_xor(part, 2, 16);
+        part += __shfl_xor(part, 1, 16);
+        if (fr == 0) red[wm][nl] = part;
+      }
+    }
+    __syncthreads();
+    double* __restrict__ out = g.dotpart + (int64_t)batch * g.strideDot;
+    for (int nl = tid; nl < BN; nl += NT) {
+      const int n = n0 + nl;
+      if (n < g.N) {
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < WM; ++w) sum += red[w][nl];
+        out[(int64_t)bm * g.N + n] = sum;
+      }
+    }
+    return;
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int m = m0 + wm * TM + i * 16 + fr;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn * TN + j * 16 + fk + 4 * r;
+        if (m >= g.M || n >= g.N) continue;
+        if (g.lower_only && m < n) continue;
+        double v = g.alpha * acc[i][j][r];
+        if (g.rowscale) v *= g.rowscale[m];
+        if (g.colscale) v *= g.colscale[n];
+        if (dg && m == n) v += dg[m];
+        const int64_t idx = m + (int64_t)n * g.ldc;
+        if (g.beta != 0.0) v += g.beta * C[idx];
+        C[idx] = v;
+        if (g.lower_only && m > n) C[n + (int64_t)m * g.ldc] = v;
+      }
+    }
+  }
+}
+
+template <int BM, int BN, int WM, int WN, bool DOT, int KLASS>
+int launch_cfg(const GemmF64& h, hipStream_t stream) {
+  GemmDev g;
+  g.M = h.M; g.N = h.N; g.K = h.K;
+  g.A = h.A; g.lda = h.lda; g.strideA = h.strideA;
+  g.B = h.B; g.ldb = h.ldb; g.strideB = h.strideB;
+  g.C = h.C; g.ldc = h.ldc; g.strideC = h.strideC;
+  g.kscale = h.kscale; g.strideK = h.strideK;
+  g.rowscale = h.rowscale; g.colscale = h.colscale;
+  g.diag_add = h.diag_add; g.strideDiag = h.strideDiag;
+  g.alpha = h.alpha; g.beta = h.beta;
+  g.lower_only = h.lower_only ? 1 : 0;
+  g.tiles_m = (h.M + BM - 1) / BM;
+  g.tiles_n = (h.N + BN - 1) / BN;
+  g.dotB = h.dotB; g.lddot = h.lddot; g.dotpart = h.dotpart; g.strideDot = h.strideDot;
+  int64_t tiles;
+  if (h.lower_only) {
+    tiles = (int64_t)g.tiles_m * (g.tiles_m + 1) / 2;
+  } else {
+    tiles = (int64_t)g.tiles_m * g.tiles_n;
+  }
+  MGW_REQUIRE(tiles < (1ll << 31), MGW_ERR_ARG, "gemm_f64: too many tiles");
+  MGW_REQUIRE(h.batch >= 1 && h.batch <= 65535, MGW_ERR_ARG, "gemm_f64: batch out of range");
+  dim3 grid((unsigned)tiles, (unsigned)h.batch, 1);
+  dim3 block(64 * WM * WN, 1, 1);
+  const int ta = h.transA ? 1 : 0, tb = h.transB ? 1 : 0;
+#define MGW_GEMM_CASE(TA_, TB_)                                                                  \
+  if (ta == TA_ && tb == TB_) {                                                                  \
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, WM, WN, TA_, TB_, DOT, KLASS>), grid, block, 0,  \
+                       stream, g);                                                               \
+  }
+  MGW_GEMM_CASE(0, 0)
+  MGW_GEMM_CASE(0, 1)
+  MGW_GEMM_CASE(1, 0)
+  MGW_GEMM_CASE(1, 1)
+#undef MGW_GEMM_CASE
+  MGW_HIP(hipGetLastError());
+  return MGW_OK;
+}
+
+}  // namespace
+
+int gemm_f64_rowblocks(int M) { return (M + 127) / 128; }
+
+int gemm_f64(const GemmF64& h, hipStream_t stream) {
+  if (h.M <= 0 || h.N <= 0 || h.batch <= 0) return MGW_OK;
+  MGW_REQUIRE(h.K >= 0, MGW_ERR_ARG, "gemm_f64: negative K");
+  MGW_REQUIRE(!h.lower_only || h.M == h.N, MGW_ERR_ARG, "gemm_f64: lower_only needs M == N");
+  MGW_REQUIRE(h.dotpart == nullptr || (!h.lower_only && h.dotB), MGW_ERR_ARG,
+              "gemm_f64: dot epilogue needs dotB and a full tile grid");
+  MGW_REQUIRE(h.dotpart != nullptr || h.C != nullptr, MGW_ERR_ARG, "gemm_f64: no output");
+  if (h.dotpart) return launch_cfg<128, 128, 2, 2, true, MGW_KCLASS_GEMM_WB>(h, stream);
+  // Skinny N: a 256 x 32 tile keeps the matrix cores busy when N is small.
+  if (h.N <= 32 && !h.lower_only) return launch_cfg<256, 32, 4, 1, false, MGW_KCLASS_GEMM_AO2MO>(h, stream);
+  switch (h.klass) {
+    case MGW_KCLASS_GEMM_AO2MO: return launch_cfg<128, 128, 2, 2, false, MGW_KCLASS_GEMM_AO2MO>(h, stream);
+    case MGW_KCLASS_GEMM_CASIDA: return launch_cfg<128, 128, 2, 2, false, MGW_KCLASS_GEMM_CASIDA>(h, stream);
+    case MGW_KCLASS_GEMM_SIGMA: return launch_cfg<128, 128, 2, 2, false, MGW_KCLASS_GEMM_SIGMA>(h, stream);
+    case MGW_KCLASS_GEMM_CHI0: return launch_cfg<128, 128, 2, 2, false, MGW_KCLASS_GEMM_CHI0>(h, stream);
+    default: return launch_cfg<128, 128, 2, 2, false, MGW_KCLASS_GEMM_OTHER>(h, stream);
+  }
+}
+
+}  // namespace mgw

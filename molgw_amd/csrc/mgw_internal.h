@@ -1,0 +1,153 @@
+// Internal declarations of libmgw: context, device buffers, kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mgw.h"
+
+namespace mgw {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+const std::string& last_error();
+
+struct Status {
+  int code = MGW_OK;
+  static Status ok() { return Status{}; }
+};
+
+#define MGW_HIP(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      ::mgw::set_error(std::string(#expr) + ": " + hipGetErrorString(e_) + " (" +  \
+                       __FILE__ + ":" + std::to_string(__LINE__) + ")");           \
+      return MGW_ERR_HIP;                                                          \
+    }                                                                              \
+  } while (0)
+
+#define MGW_TRY(expr)             \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_ != MGW_OK) return rc_; \
+  } while (0)
+
+#define MGW_REQUIRE(cond, code, msg) \
+  do {                               \
+    if (!(cond)) {                   \
+      ::mgw::set_error(msg);         \
+      return (code);                 \
+    }                                \
+  } while (0)
+
+// ---------------------------------------------------------------- HBM ledger
+// A named device buffer owned by a context. Buffers grow on demand and are
+// reused across calls, so a solve loop does no hipMalloc after warm-up.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct Ledger {
+  std::map<std::string, DevBuf> bufs;
+  int64_t current = 0;
+  int64_t peak = 0;
+};
+
+// ---------------------------------------------------------------- profiling
+// Live per-kernel-class timing with HIP events recorded on the stream the
+// kernels are launched on; classes are the MGW_KCLASS_* of mgw.h.
+struct Profiler {
+  bool on = false;
+  struct Rec {
+    int klass;
+    double flops, bytes;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  hipEvent_t next_event();
+  void reset();
+  ~Profiler();
+};
+
+// RAII scope: records a start event now and an end event at destruction.
+struct ProfScope {
+  Profiler* p;
+  size_t idx;
+  hipStream_t s;
+  ProfScope(Profiler* prof, int klass, double flops, double bytes, hipStream_t stream);
+  ~ProfScope();
+};
+
+// ---------------------------------------------------------------- GEMM
+// C(MxN) = alpha * opA(A) * diag(kscale) * opB(B) scaled by rowscale/colscale,
+// plus diag_add on the diagonal, plus beta * C. Column-major; batch strides in
+// elements. lower_only: M == N, only tiles on/below the diagonal are computed
+// and mirrored to the upper triangle (symmetric rank-k update).
+struct GemmF64 {
+  int M = 0, N = 0, K = 0;
+  bool transA = false, transB = false;
+  const double* A = nullptr; int64_t lda = 0, strideA = 0;
+  const double* B = nullptr; int64_t ldb = 0, strideB = 0;
+  double* C = nullptr; int64_t ldc = 0, strideC = 0;
+  const double* kscale = nullptr; int64_t strideK = 0;
+  const double* rowscale = nullptr;
+  const double* colscale = nullptr;
+  const double* diag_add = nullptr; int64_t strideDiag = 0;
+  double alpha = 1.0, beta = 0.0;
+  int batch = 1;
+  bool lower_only = false;
+  // Fused column dot: instead of storing C, accumulate
+  // dotpart[(rowblock * N + n) + batch * strideDot] = sum_{m in rowblock} C(m,n) * D(m,n)
+  // with D(m, n) = dotB[m + n * lddot]. Used for diag(B^T chi B).
+  const double* dotB = nullptr; int64_t lddot = 0;
+  double* dotpart = nullptr; int64_t strideDot = 0;
+  // kernel class: the hot classes get their own kernel symbol so that
+  // profiler per-kernel statistics separate them
+  int klass = MGW_KCLASS_GEMM_OTHER;
+};
+int gemm_f64(const GemmF64& g, hipStream_t stream);
+int gemm_f64_rowblocks(int M);   // number of row blocks of the dot epilogue
+
+// ---------------------------------------------------------------- kernels
+int launch_unpack_ri(const double* packed, int64_t npair, int naux, const int32_t* kl,
+                     int nbf, double* dense, hipStream_t s);
+int launch_fill(double* p, int64_t n, double v, hipStream_t s);
+int launch_gather_ia(const double* eri, int naux, int nm, int nn, const int32_t* ia_dev, int npole,
+                     double* x, hipStream_t s);
+int launch_colnorm2(const double* x, int64_t ld, int rows, int cols, double* out, hipStream_t s);
+int launch_vsum(const double* x, int64_t n, int square, double* partial, int nblk, hipStream_t s);
+int launch_casida_diag(const int32_t* ia, int npole, const double* energy, double* de, double* sq,
+                       double* de2, hipStream_t s);
+int launch_casida_post(const double* eig, int npole, double* pole, double* cs, hipStream_t s);
+int sigma_part_doubles(int npole, int nse);
+int launch_sigma_poles(const double* w, int npole, int nse, int ni, const double* pole,
+                       const double* omega, int nomega, const double* e0p, const double* ei,
+                       const double* fi, const double* ep, const double* occp, double spin_fact,
+                       double eta, double* part, int accumulate, double* sigma, double* egw_p,
+                       hipStream_t s);
+int launch_chi_weights(const int32_t* ia, int npole, const double* energy, const double* occ,
+                       const double* omega, int nomega, double* sw, hipStream_t s);
+int launch_chi_trace(const double* sw, const double* n2, int npole, const double* mat, int naux,
+                     int mode, int nomega, double* out, hipStream_t s);
+int launch_chi_finish(double* m, int n, int nomega, hipStream_t s);
+int launch_dot_reduce(const double* part, int nrb, int64_t ncol, int nomega, double* v, hipStream_t s);
+int launch_sigma_imag_sum(const double* v, int np, int nse, int nomega, const double* omega,
+                          const double* weight, const double* ep, const double* wc, int ncalc,
+                          double* sigma_calc, hipStream_t s);
+int pade_work_doubles(int ncalc, int nse);
+int launch_pade(const double* wc, const double* sc, int ncalc, int nse, const double* omega,
+                int nomega, double eta, double* work, double* sigma, hipStream_t s);
+int launch_qp(const double* sigma, int nomega, int nse, const double* omega, const double* se_e0,
+              const double* exc, const double* e0, double* eqp, double* zz, double* elin,
+              hipStream_t s);
+
+}  // namespace mgw

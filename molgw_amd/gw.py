@@ -1,0 +1,324 @@
+"""Host-side mirror of MOLGW's RI G0W0 interfaces, executed on the GPU.
+
+``Engine`` holds one libmgw context (one GPU) and exposes the reference's
+subroutines under their own names with the same argument meaning:
+
+================================  ==============================================
+reference (src/ of MOLGW 3.4)     here
+================================  ==============================================
+calculate_eri_ri result           ``Engine.set_eri_3center``
+  (m_eri.f90:36-62)
+calculate_eri_3center_mo          ``Engine.calculate_eri_3center_mo``
+  (m_eri_ao_mo.f90:416-582)
+polarizability (RPA, Casida)      ``Engine.polarizability``
+  (m_linear_response.f90:33-498)
+polarizability_grid_scalapack     ``Engine.polarizability_grid``
+  (m_gw_selfenergy_grid.f90:33-209)
+gw_selfenergy                     ``Engine.gw_selfenergy``
+  (m_gw_selfenergy_analytic.f90:29-196)
+gw_selfenergy_imag_scalapack      ``Engine.gw_selfenergy_imag``
+  (m_gw_selfenergy_grid.f90:213-338)
+se_pade_fit (m_selfenergy_tools.  ``Engine.pade_fit``
+  f90:965-999)
+find_qp_energy_graphical /        ``Engine.find_qp_energy``
+  _linearization (:202-420)
+selfenergy_evaluation one-shot    ``Engine.g0w0``
+  (m_selfenergy_evaluation.f90)
+selfenergy_set_state_range        ``selfenergy_set_state_range``
+  (m_selfenergy_tools.f90:67-122)
+================================  ==============================================
+
+Arrays use the reference's column-major meaning: ``eri_3center`` is
+(npair, nauxil), ``c_matrix`` is (nbf, nstate), ``eri_3center_mo`` is
+(nauxil, m, n) and ``sigma`` is (nomega, nse) complex. Errors raise
+``MgwError`` (the reference calls ``die``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import G0W0Params, Ranges, check
+
+COMPLETELY_EMPTY = 1.0e-5   # src/m_definitions.f90:66
+
+
+def _f64(a) -> np.ndarray:
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@dataclass
+class StateRanges:
+    """1-based state ranges of the reference (nsemin, ncore_G ...)."""
+    ncore_G: int
+    nvirtual_G: int
+    nhomo_G: int
+    nlumo_G: int
+    nsemin: int
+    nsemax: int
+    ncore_W: int
+    nvirtual_W: int
+
+    def to_c(self) -> Ranges:
+        return Ranges(self.ncore_G, self.nvirtual_G, self.nsemin, self.nsemax, self.nhomo_G,
+                      self.ncore_W, self.nvirtual_W, 0)
+
+    @property
+    def nse(self) -> int:
+        return self.nsemax - self.nsemin + 1
+
+
+def selfenergy_set_state_range(occupation, ncoreg=0, nvirtualg=100000, ncorew=0, nvirtualw=100000,
+                               selfenergy_state_min=1, selfenergy_state_max=100000,
+                               selfenergy_state_range=100000) -> StateRanges:
+    """State ranges as set by selfenergy_set_state_range
+    (src/m_selfenergy_tools.f90:67-122) and sf_init (src/m_spectral_function.f90:183-184),
+    for one spin channel. ``selfenergy_state_range`` counts from the HOMO."""
+    occ = np.asarray(occupation, dtype=np.float64).reshape(-1)
+    nstate = len(occ)
+    ncore_G = ncoreg
+    nvirtual_G = min(nvirtualg, nstate + 1)
+    filled = np.nonzero(~(occ < COMPLETELY_EMPTY))[0]
+    nhomo_G = max(1, int(filled.max()) + 1) if len(filled) else 1
+    empty = np.nonzero(~(occ > COMPLETELY_EMPTY))[0]
+    nlumo_G = int(empty.min()) + 1 if len(empty) else 100000
+    nsemin = max(ncore_G + 1, selfenergy_state_min, 1, nhomo_G - selfenergy_state_range)
+    nsemax = min(nvirtual_G - 1, selfenergy_state_max, nstate, nhomo_G + selfenergy_state_range)
+    return StateRanges(ncore_G, nvirtual_G, nhomo_G, nlumo_G, nsemin, nsemax, ncorew,
+                       min(nvirtualw, nstate + 1))
+
+
+def imaginary_quad_grid(nomega: int):
+    """Gauss-Legendre grid of chi on the imaginary axis (alpha=1, beta=2):
+    src/m_spectral_function.f90:286-306, nodes from numpy's leggauss."""
+    x, w = np.polynomial.legendre.leggauss(nomega)
+    x = 0.5 * (x + 1.0)
+    w = 0.5 * w
+    return 2.0 * x / (1.0 - x), 2.0 * w / (1.0 - x) ** 2
+
+
+@dataclass
+class G0W0Output:
+    energy_qp: np.ndarray
+    zz: np.ndarray
+    energy_qp_lin: np.ndarray
+    sigma: np.ndarray
+    omega: np.ndarray
+    en_rpa: float
+    en_gw: float
+    extra: dict = field(default_factory=dict)
+
+
+class Engine:
+    """One libmgw context on one GPU (the reference's module state, resident in HBM)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        check(self._lib.mgw_ctx_create(int(device), C.byref(h)))
+        self._h = h
+        self.nbf = self.naux = self.nstate = 0
+        self.energy = self.occupation = None
+
+    # ------------------------------------------------------------ lifetime
+    def close(self) -> None:
+        if getattr(self, '_h', None) is not None and self._h.value:
+            check(self._lib.mgw_ctx_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self) -> None:
+        check(self._lib.mgw_ctx_synchronize(self._h))
+
+    def profile(self, enable: bool) -> None:
+        """Start (clearing earlier records) or stop live per-kernel-class timing."""
+        check(self._lib.mgw_profile_enable(self._h, 1 if enable else 0))
+
+    def profile_read(self) -> dict:
+        """{class name: {launches, ms, flops, bytes}} over the recorded launches."""
+        out = {}
+        for k, name in enumerate(_lib.KCLASS):
+            n, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+            check(self._lib.mgw_profile_read(self._h, k, C.byref(n), C.byref(ms), C.byref(fl), C.byref(by)))
+            if n.value:
+                out[name] = {'launches': n.value, 'ms': ms.value, 'flops': fl.value, 'bytes': by.value}
+        return out
+
+    def memory(self):
+        cur, peak = C.c_int64(), C.c_int64()
+        check(self._lib.mgw_memory_report(self._h, C.byref(cur), C.byref(peak)))
+        return cur.value, peak.value
+
+    # ------------------------------------------------------------ A1 / MO
+    def set_eri_3center(self, eri_3center, index_basis, nbf: int) -> None:
+        """Upload eri_3center(npair, nauxil) (host, packed, diagonal pairs x1/2)."""
+        b = _f64(eri_3center)
+        ib = np.asfortranarray(np.asarray(index_basis, dtype=np.int32))
+        npair, naux = b.shape
+        if ib.shape != (2, npair):
+            raise ValueError(f'index_basis must be (2, {npair}), got {ib.shape}')
+        check(self._lib.mgw_set_eri_3center(self._h, b.ctypes.data, npair, naux,
+                                            ib.ctypes.data_as(C.POINTER(C.c_int32)), int(nbf),
+                                            _lib.MGW_MEM_HOST))
+        self.nbf, self.naux = int(nbf), int(naux)
+
+    def set_eri_3center_device(self, ptr: int, npair: int, naux: int, index_basis, nbf: int) -> None:
+        """Same as set_eri_3center for a device buffer (column-major npair x nauxil)."""
+        ib = np.asfortranarray(np.asarray(index_basis, dtype=np.int32))
+        check(self._lib.mgw_set_eri_3center(self._h, C.c_void_p(ptr), int(npair), int(naux),
+                                            ib.ctypes.data_as(C.POINTER(C.c_int32)), int(nbf),
+                                            _lib.MGW_MEM_DEVICE))
+        self.nbf, self.naux = int(nbf), int(naux)
+
+    def set_mo(self, c_matrix, energy, occupation) -> None:
+        c = _f64(c_matrix)
+        e = _f64(energy).reshape(-1)
+        o = _f64(occupation).reshape(-1)
+        nbf, nstate = c.shape
+        if len(e) != nstate or len(o) != nstate:
+            raise ValueError('energy/occupation length must equal nstate')
+        check(self._lib.mgw_set_mo(self._h, c.ctypes.data, nbf, nstate, e.ctypes.data, o.ctypes.data,
+                                   _lib.MGW_MEM_HOST))
+        self.nstate = nstate
+        self.energy, self.occupation = e.copy(), o.copy()
+
+    # ------------------------------------------------------------ A2
+    def calculate_eri_3center_mo(self, mstate_min, mstate_max, nstate_min, nstate_max, copy=True):
+        out = None
+        if copy:
+            out = np.zeros((self.naux, mstate_max - mstate_min + 1, nstate_max - nstate_min + 1), order='F')
+        check(self._lib.mgw_calculate_eri_3center_mo(self._h, mstate_min, mstate_max, nstate_min, nstate_max,
+                                                     _ptr(out) if copy else None))
+        return out
+
+    def destroy_eri_3center_mo(self) -> None:
+        check(self._lib.mgw_destroy_eri_3center_mo(self._h))
+
+    # ------------------------------------------------------------ A4
+    def npole(self, rng: StateRanges) -> int:
+        n = C.c_int()
+        r = rng.to_c()
+        check(self._lib.mgw_npole(self._h, C.byref(r), C.byref(n)))
+        return n.value
+
+    def polarizability(self, rng: StateRanges, spin_fact=2.0, return_ws=False):
+        """Returns (en_rpa, en_gw, pole, w_s or None)."""
+        r = rng.to_c()
+        npole = self.npole(rng)
+        pole = np.zeros(npole)
+        ws = np.zeros((self.naux, npole), order='F') if return_ws else None
+        erpa, egw = C.c_double(), C.c_double()
+        check(self._lib.mgw_polarizability(self._h, C.byref(r), spin_fact, _lib.MGW_FP64, C.byref(erpa),
+                                           C.byref(egw), _ptr(pole), _ptr(ws) if return_ws else None))
+        return erpa.value, egw.value, pole, ws
+
+    # ------------------------------------------------------------ A5
+    def polarizability_grid(self, rng: StateRanges, omega_im, weight, spin_fact=2.0, return_chi=False):
+        """Returns (erpa, egw, chi[naux, naux, nomega] or None)."""
+        r = rng.to_c()
+        om = _f64(omega_im)
+        wt = _f64(weight)
+        n = len(om)
+        chi = np.zeros((self.naux, self.naux, n), order='F') if return_chi else None
+        erpa, egw = C.c_double(), C.c_double()
+        check(self._lib.mgw_polarizability_grid(self._h, C.byref(r), spin_fact, _lib.MGW_FP64, n, _ptr(om),
+                                                _ptr(wt), C.byref(erpa), C.byref(egw),
+                                                _ptr(chi) if return_chi else None))
+        return erpa.value, egw.value, chi
+
+    # ------------------------------------------------------------ A6
+    def gw_selfenergy(self, rng: StateRanges, omega, se_energy0, eta=0.001, spin_fact=2.0):
+        """Returns (sigma[nomega, nse] complex, energy_gw)."""
+        r = rng.to_c()
+        om = _f64(omega)
+        e0 = _f64(se_energy0)
+        sig = np.zeros((len(om), rng.nse), dtype=np.complex128, order='F')
+        egw = C.c_double()
+        check(self._lib.mgw_gw_selfenergy(self._h, C.byref(r), spin_fact, eta, len(om), _ptr(om), _ptr(e0),
+                                          sig.ctypes.data_as(C.POINTER(C.c_double)), C.byref(egw)))
+        return sig, egw.value
+
+    # ------------------------------------------------------------ A7
+    def gw_selfenergy_imag(self, rng: StateRanges, omega_calc):
+        r = rng.to_c()
+        wc = np.asfortranarray(np.asarray(omega_calc, dtype=np.complex128))
+        out = np.zeros((len(wc), rng.nse), dtype=np.complex128, order='F')
+        check(self._lib.mgw_gw_selfenergy_imag(self._h, C.byref(r), len(wc),
+                                               wc.ctypes.data_as(C.POINTER(C.c_double)),
+                                               out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    # ------------------------------------------------------------ A8
+    def pade_fit(self, omega_calc, sigma_calc, omega, eta=0.001):
+        wc = np.asfortranarray(np.asarray(omega_calc, dtype=np.complex128))
+        sc = np.asfortranarray(np.asarray(sigma_calc, dtype=np.complex128))
+        om = _f64(omega)
+        nse = sc.shape[1]
+        out = np.zeros((len(om), nse), dtype=np.complex128, order='F')
+        check(self._lib.mgw_pade_fit(self._h, nse, len(wc), wc.ctypes.data_as(C.POINTER(C.c_double)),
+                                     sc.ctypes.data_as(C.POINTER(C.c_double)), len(om), _ptr(om), eta,
+                                     out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    # ------------------------------------------------------------ A9
+    def find_qp_energy(self, rng: StateRanges, omega, sigma, se_energy0, exchange_m_vxc, energy0):
+        """Returns (energy_qp_graphical, zz, energy_qp_linearized), each (nstate,)."""
+        r = rng.to_c()
+        om = _f64(omega)
+        sg = np.asfortranarray(np.asarray(sigma, dtype=np.complex128))
+        e0s = _f64(se_energy0)
+        exc = _f64(exchange_m_vxc)
+        e0 = _f64(energy0)
+        n = len(e0)
+        eqp, zz, elin = np.zeros(n), np.zeros(n), np.zeros(n)
+        check(self._lib.mgw_find_qp_energy(self._h, C.byref(r), len(om), _ptr(om),
+                                           sg.ctypes.data_as(C.POINTER(C.c_double)), _ptr(e0s), _ptr(exc),
+                                           _ptr(e0), n, _ptr(eqp), _ptr(zz), _ptr(elin)))
+        return eqp, zz, elin
+
+    # ------------------------------------------------------------ A11
+    def g0w0(self, rng: StateRanges | None = None, technique='one_shot', nomega_sigma=51,
+             step_sigma=0.01, eta=0.001, nomega_chi_imag=0, nomega_sigma_calc=1,
+             step_sigma_calc=0.03, exchange_m_vxc=None, spin_fact=2.0, return_sigma=True) -> G0W0Output:
+        """One-shot G0W0 (postscf='GW' for technique='one_shot', 'G0W0_PADE' for
+        technique='imaginary_pade') on the resident tensors."""
+        if rng is None:
+            rng = selfenergy_set_state_range(self.occupation)
+        tech = {'one_shot': _lib.MGW_TECH_ONE_SHOT, 'imaginary_pade': _lib.MGW_TECH_IMAGINARY_PADE}[technique]
+        prm = G0W0Params(tech, nomega_sigma, step_sigma, eta, nomega_chi_imag, nomega_sigma_calc,
+                         step_sigma_calc, spin_fact, _lib.MGW_FP64, 0)
+        r = rng.to_c()
+        n = self.nstate
+        nom = nomega_sigma // 2
+        eqp, zz, elin = np.zeros(n), np.zeros(n), np.zeros(n)
+        sig = np.zeros((2 * nom + 1, rng.nse), dtype=np.complex128, order='F') if return_sigma else None
+        en = np.zeros(2)
+        exc = None if exchange_m_vxc is None else _f64(exchange_m_vxc)
+        check(self._lib.mgw_g0w0(self._h, C.byref(prm), C.byref(r), _ptr(exc) if exc is not None else None,
+                                 _ptr(eqp), _ptr(zz), _ptr(elin),
+                                 sig.ctypes.data_as(C.POINTER(C.c_double)) if return_sigma else None,
+                                 _ptr(en)))
+        if technique == 'one_shot':
+            omega = step_sigma * np.arange(-nom, nom + 1)
+        else:
+            efermi = 0.5 * (self.energy[rng.nhomo_G - 1] + self.energy[rng.nhomo_G])
+            omega = efermi + step_sigma * np.arange(-nom, nom + 1)
+        return G0W0Output(eqp, zz, elin, sig, omega, float(en[0]), float(en[1]))

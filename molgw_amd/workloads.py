@@ -1,0 +1,119 @@
+"""Synthetic RI G0W0 inputs.
+
+The recipe is the synthetic imaginary-frequency kernel of the benchmark
+definition, usable at any size:
+
+* C: random orthonormal FP64 matrix, QR of an N(0,1) matrix from the seed;
+* energies: occupied uniform in [-1.2, -0.3] Ha, virtual uniform in
+  [0.05, 3.0] Ha, each sorted;
+* B(P, mu nu) = sum_k L(P, k) G(k, mu nu) with G ~ N(0,1) symmetric in
+  (mu, nu) and L = U diag(1/(1+k)^1.5) V^T (seeded orthogonal U, V);
+* B is handed over in MOLGW's packed layout eri_3center(npair, nauxil) with
+  diagonal pairs stored x1/2 (src/m_eri_calculate.f90:1840-1841) and the
+  upper-triangle pair list (k <= l, column by column).
+
+Molecular shapes (nbf, nauxil, nocc) of the cc-pVDZ / cc-pVTZ systems used
+by the benchmark are given by ``alkane_shape`` and ``water_cluster_shape``
+(basis-function counts of basis/{H,C,O}_cc-pV{D,T}Z{,-RI} of the reference).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class System:
+    name: str
+    nbf: int
+    naux: int
+    nocc: int
+    eri_3center: np.ndarray     # (npair, naux), Fortran order
+    index_basis: np.ndarray     # (2, npair), 1-based
+    c_matrix: np.ndarray        # (nbf, nstate)
+    energy: np.ndarray          # (nstate,)
+    occupation: np.ndarray      # (nstate,)
+
+    @property
+    def nstate(self) -> int:
+        return self.c_matrix.shape[1]
+
+
+# basis-function counts per element (pure functions): (orbital, RI) for cc-pVDZ, cc-pVTZ
+_BASIS = {
+    'cc-pVDZ': {'H': (5, 14), 'C': (14, 56), 'O': (14, 56)},
+    'cc-pVTZ': {'H': (14, 30), 'C': (30, 81), 'O': (30, 81)},
+}
+_ELECTRONS = {'H': 1, 'C': 6, 'O': 8}
+
+
+def shape_of(counts: dict, basis: str):
+    nbf = sum(n * _BASIS[basis][el][0] for el, n in counts.items())
+    naux = sum(n * _BASIS[basis][el][1] for el, n in counts.items())
+    nocc = sum(n * _ELECTRONS[el] for el, n in counts.items()) // 2
+    return nbf, naux, nocc
+
+
+def alkane_shape(n: int, basis: str = 'cc-pVDZ'):
+    """(nbf, naux, nocc) of CnH(2n+2)."""
+    return shape_of({'C': n, 'H': 2 * n + 2}, basis)
+
+
+def water_cluster_shape(m: int, basis: str = 'cc-pVDZ'):
+    """(nbf, naux, nocc) of (H2O)m."""
+    return shape_of({'O': m, 'H': 2 * m}, basis)
+
+
+def upper_pairs(nbf: int) -> np.ndarray:
+    """index_basis (2, npair), 1-based, pairs k <= l ordered column by column."""
+    l, k = np.triu_indices(nbf)[1], np.triu_indices(nbf)[0]
+    order = np.lexsort((k, l))
+    return np.vstack([k[order] + 1, l[order] + 1]).astype(np.int32)
+
+
+def orthogonal(rng: np.random.Generator, n: int) -> np.ndarray:
+    q, r = np.linalg.qr(rng.standard_normal((n, n)))
+    return q * np.sign(np.diag(r))[None, :]
+
+
+def synthetic_system(nbf: int, naux: int, nocc: int, seed: int = 0, name: str = 'synthetic',
+                     scale: float = 1.0) -> System:
+    """Seeded synthetic system in MOLGW's packed RI layout (host numpy).
+
+    ``scale`` multiplies B: 1.0 is the benchmark recipe (which gives
+    (ia|ia) ~ 2 Ha, an unphysically strong coupling); ~0.3 gives molecular
+    magnitudes ((ia|ia) ~ 0.2 Ha, QP roots inside the +-0.25 Ha window)."""
+    rng = np.random.default_rng(seed)
+    c = orthogonal(rng, nbf)
+    e_occ = np.sort(rng.uniform(-1.2, -0.3, nocc))
+    e_virt = np.sort(rng.uniform(0.05, 3.0, nbf - nocc))
+    energy = np.concatenate([e_occ, e_virt])
+    occupation = np.concatenate([np.full(nocc, 2.0), np.zeros(nbf - nocc)])
+    u = orthogonal(rng, naux)
+    v = orthogonal(rng, naux)
+    sv = 1.0 / (1.0 + np.arange(naux)) ** 1.5
+    lmat = (u * sv[None, :]) @ v.T
+    ib = upper_pairs(nbf)
+    npair = ib.shape[1]
+    g = rng.standard_normal((naux, npair))
+    # G symmetric in (mu, nu): one value per unordered pair, the full matrix
+    # element; packed storage halves the diagonal pairs
+    b = (lmat @ g).T
+    if scale != 1.0:
+        b *= scale
+    diag = ib[0] == ib[1]
+    b[diag, :] *= 0.5
+    return System(name, nbf, naux, nocc, np.asfortranarray(b), ib, np.asfortranarray(c), energy, occupation)
+
+
+def dense_ri(sys: System) -> np.ndarray:
+    """Dense symmetric B(P, mu, nu) of a System (for checks at small sizes)."""
+    k = sys.index_basis[0] - 1
+    l = sys.index_basis[1] - 1
+    out = np.zeros((sys.naux, sys.nbf, sys.nbf))
+    out[:, k, l] = sys.eri_3center.T
+    out[:, l, k] = sys.eri_3center.T
+    d = k == l
+    out[:, k[d], l[d]] = 2.0 * sys.eri_3center[d].T
+    return out

@@ -1,0 +1,283 @@
+"""Pins of the CPU oracle (oracle/gw_oracle.py) against the reference's golden
+values, closed forms and independent routes. CPU only.
+
+Golden values from the reference's test database (tests/inputs/testsuite):
+  * lih_bulk08_abinit_rpa  RPA correlation          -0.1152301335 +- 1e-7  (:1206-1207)
+  * lih_bulk08_abinit_mp2  MP2 Energy               -0.0772686571 +- 1e-7  (:1203-1204)
+  * lih_bulk08_*           Testing integral (11|11)  2.92396387E-01 +- 1e-7 (:1211)
+The fixture files are the reference's own tests/data_for_tests/lih_bulk08_*.
+"""
+import hashlib
+import math
+
+import numpy as np
+import pytest
+
+from oracle import gw_oracle as O
+from molgw_amd import cc4s
+from tests.conftest import LIH_ROOT
+
+
+@pytest.fixture(scope='module')
+def lih():
+    e, occ, ef = cc4s.read_eigenenergies(LIH_ROOT)
+    v = cc4s.read_coulomb_vertex(LIH_ROOT)
+    return e, occ, ef, v
+
+
+@pytest.fixture(scope='module')
+def lih_packed(lih):
+    e, occ, ef, v = lih
+    n = len(e)
+    k, l = np.triu_indices(n)
+    ib = np.vstack([k + 1, l + 1])
+    return O.pack_eri_3center(v, ib), ib, np.eye(n)
+
+
+def test_fixture_checksums():
+    h = hashlib.sha256(open(LIH_ROOT + '_CoulombVertex.elements', 'rb').read()).hexdigest()
+    assert len(h) == 64
+    assert open(LIH_ROOT + '_CoulombVertex.elements', 'rb').read(16) != b'\0' * 16
+
+
+# ---------------------------------------------------------------- grids
+@pytest.mark.parametrize('n', [1, 2, 3, 7, 16, 32, 40, 64])
+def test_gausslegendre_matches_numpy(n):
+    x, w = O.coeffs_gausslegint(0.0, 1.0, n)
+    xr, wr = np.polynomial.legendre.leggauss(n)
+    np.testing.assert_allclose(x, 0.5 * (xr + 1.0), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(w, 0.5 * wr, rtol=0, atol=1e-13)  # weights use the last Newton derivative
+
+
+def test_imaginary_grid_integrates_lorentzian():
+    # int_0^inf 2a / (w^2 + a^2) dw = pi for any a > 0
+    om, wt = O.imaginary_quad_grid(64)
+    assert np.all(np.diff(om) > 0) and om[0] > 0
+    for a in (0.1, 0.5, 2.0, 10.0):
+        val = np.sum(wt * 2 * a / (om ** 2 + a ** 2))
+        assert abs(val - math.pi) < 1e-6 * math.pi
+
+
+# ---------------------------------------------------------------- transition table
+def test_transition_table_order_and_frozen_core():
+    occ = np.array([2, 2, 2, 0, 0.0])
+    tt = O.transition_table(occ[:, None], 2.0, 0, 6)
+    assert [tuple(r[:2]) for r in tt] == [(i, a) for i in (1, 2, 3) for a in (4, 5)]
+    tt = O.transition_table(occ[:, None], 2.0, 1, 5)      # ncore_W = 1, nvirtual_W = 5
+    assert [tuple(r[:2]) for r in tt] == [(i, a) for i in (2, 3) for a in (4,)]
+    # fractional occupation: state 3 is both an origin and a destination, and
+    # skip_transition keeps the (3 -> 3) pair (occ difference 0 is not > 0)
+    occ = np.array([2, 2, 1, 0.0])
+    tt = [tuple(int(v) for v in r[:2]) for r in O.transition_table(occ[:, None], 2.0, 0, 5)]
+    assert tt == [(1, 3), (1, 4), (2, 3), (2, 4), (3, 3), (3, 4)]
+
+
+# ---------------------------------------------------------------- A2
+def test_ao2mo_matches_dense_einsum():
+    rng = np.random.default_rng(3)
+    nbf, naux = 11, 7
+    b = rng.standard_normal((naux, nbf, nbf))
+    b = b + b.transpose(0, 2, 1)
+    # shell-pair-like order with unordered pairs (k > l allowed), as setup_basispair
+    pairs = [(k, l) for l in range(nbf) for k in range(nbf) if k <= l]
+    rng.shuffle(pairs)
+    pairs = [(l, k) if (i % 3 == 0) else (k, l) for i, (k, l) in enumerate(pairs)]
+    ib = np.array(pairs).T + 1
+    packed = O.pack_eri_3center(b, ib)
+    c = rng.standard_normal((nbf, nbf))
+    ref = np.einsum('km,Pkl,ln->Pmn', c[:, 2:9], b, c[:, 1:5])
+    got = O.calculate_eri_3center_mo(packed, ib, c, 3, 9, 2, 5)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_cc4s_reader_testing_integral(lih):
+    e, occ, ef, v = lih
+    assert v.shape == (138, 43, 43)
+    assert abs(np.dot(v[:, 0, 0], v[:, 0, 0]) - 2.92396387e-01) < 1e-7
+    assert occ.sum() == 16.0
+
+
+def test_cc4s_mp2_golden(lih):
+    e, occ, ef, v = lih
+    no = int(occ.sum() / 2)
+    iajb = np.einsum('Pia,Pjb->iajb', v[:, :no, no:], v[:, :no, no:])
+    eo, ev = e[:no], e[no:]
+    den = eo[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] - ev[None, None, None, :]
+    emp2 = np.sum(iajb * (2 * iajb - iajb.transpose(0, 3, 2, 1)) / den)
+    assert abs(emp2 - (-0.0772686571)) < 1e-7
+
+
+# ---------------------------------------------------------------- A4
+def test_casida_lih_golden(lih):
+    e, occ, ef, v = lih
+    rng = O.selfenergy_set_state_range(occ[:, None])
+    tt = O.transition_table(occ[:, None], 2.0, rng.ncore_W, rng.nvirtual_W)
+    x = O.casida_ia_matrix(v, tt, 1)
+    pole, ws, erpa, egm = O.polarizability_casida(x, e, tt)
+    assert abs(erpa - (-0.1152301335)) < 1e-7
+    assert np.all(pole > 0)
+
+
+def test_casida_single_transition_closed_form():
+    # one occupied, one virtual: Omega^2 = de^2 + 2 spin_fact de v, v = sum_P B^2
+    e = np.array([-0.5, 0.3])
+    x = np.array([[0.2], [0.1], [-0.05]])
+    tt = np.array([[1, 2, 1]])
+    pole, ws, erpa, egm = O.polarizability_casida(x, e, tt)
+    de, v = 0.8, float(np.sum(x ** 2))
+    om = math.sqrt(de ** 2 + 4.0 * de * v)
+    assert abs(pole[0] - om) < 1e-14
+    assert abs(erpa - (0.5 * om - 0.5 * (de + 4 * v) * 0.5 - 0.25 * de)) < 1e-14
+    # w_s = sqrt(2) x sqrt(de / om)
+    np.testing.assert_allclose(ws[:, 0] ** 2, 2.0 * x[:, 0] ** 2 * de / om, rtol=1e-13)
+
+
+# ---------------------------------------------------------------- A5 / A6 / A7 / A8 cross-pins
+def test_gm_identity_and_imag_axis_lih(lih_packed, lih):
+    e, occ, ef, v = lih
+    b, ib, c = lih_packed
+    r = O.g0w0_casida(b, ib, c, e, occ)
+    assert abs(r.en_rpa - (-0.1152301335)) < 1e-7
+    # 1/2 Tr[Sigma_c G] from the pole sum equals E_GM from w_s (same W)
+    assert abs(r.en_gw - r.en_gm) < 1e-10 * abs(r.en_gm)
+    rp = O.g0w0_pade(b, ib, c, e, occ, 32, nomega_sigma_calc=16)
+    # imaginary-axis E_RPA / E_GW reproduce the Casida numbers (quadrature)
+    assert abs(rp.en_rpa - r.en_rpa) < 1e-8
+    assert abs(rp.en_gw - r.en_gm) < 1e-8
+    # HOMO/LUMO from Pade agree with the analytic route to the Pade error
+    assert abs(rp.energy_qp[7] - r.energy_qp[7]) < 1e-5
+    assert abs(rp.energy_qp[8] - r.energy_qp[8]) < 1e-5
+
+
+def test_chi_grid_equals_casida_poles():
+    # chi(iw) = -sum_s w_s w_s^T 2 Omega_s / (w^2 + Omega_s^2) (m_spectral_function.f90:656-672)
+    from molgw_amd.workloads import synthetic_system
+    s = synthetic_system(14, 30, 3, seed=5)
+    rng = O.selfenergy_set_state_range(s.occupation[:, None])
+    tt = O.transition_table(s.occupation[:, None], 2.0, 0, s.nstate + 1)
+    eri_ai = O.calculate_eri_3center_mo(s.eri_3center, s.index_basis, s.c_matrix, 4, s.nstate, 1, 3)
+    om, wt = O.imaginary_quad_grid(6)
+    chi, erpa, egw = O.polarizability_grid(eri_ai, s.energy, s.occupation, tt, om, wt, 4, 1)
+    eri_w = O.calculate_eri_3center_mo(s.eri_3center, s.index_basis, s.c_matrix, 1, s.nstate, 1, s.nstate)
+    x = O.casida_ia_matrix(eri_w, tt, 1)
+    pole, ws, _, _ = O.polarizability_casida(x, s.energy, tt)
+    for k, w in enumerate(om):
+        ref = -(ws * (2 * pole / (w ** 2 + pole ** 2))[None, :]) @ ws.T
+        np.testing.assert_allclose(chi[k], ref, rtol=1e-9, atol=1e-11 * np.abs(ref).max())
+
+
+def test_sigma_imag_equals_pole_sum_at_complex_frequency():
+    from molgw_amd.workloads import synthetic_system
+    s = synthetic_system(12, 26, 3, seed=7)
+    occ, e = s.occupation, s.energy
+    rng = O.selfenergy_set_state_range(occ[:, None])
+    tt = O.transition_table(occ[:, None], 2.0, 0, s.nstate + 1)
+    eri_w = O.calculate_eri_3center_mo(s.eri_3center, s.index_basis, s.c_matrix, 1, s.nstate, 1, s.nstate)
+    x = O.casida_ia_matrix(eri_w, tt, 1)
+    pole, ws, _, _ = O.polarizability_casida(x, e, tt)
+    efermi = O.fermi_level(e, rng)
+    wc = efermi + 0.05 * np.arange(4) * 1j
+    # analytic Sigma_c(z) = sum_i sum_s w^2 [f/(z - e_i + Om) + (1-f)/(z - e_i - Om)], eta -> 0
+    ref = np.zeros((len(wc), s.nstate), dtype=complex)
+    for i in range(s.nstate):
+        w = ws.T @ eri_w[:, :, i]                      # (npole, p)
+        f = occ[i] / 2
+        g = f / (wc[:, None] - e[i] + pole[None, :]) + (1 - f) / (wc[:, None] - e[i] - pole[None, :])
+        ref += g @ (w ** 2)
+    om, wt = O.imaginary_quad_grid(128)
+    eri_ai = O.calculate_eri_3center_mo(s.eri_3center, s.index_basis, s.c_matrix, 4, s.nstate, 1, 3)
+    chi, _, _ = O.polarizability_grid(eri_ai, e, occ, tt, om, wt, 4, 1)
+    eri_pm = O.calculate_eri_3center_mo(s.eri_3center, s.index_basis, s.c_matrix, 1, s.nstate, 1, s.nstate)
+    got = O.gw_selfenergy_imag(eri_pm, chi, om, wt, wc, e, rng)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=2e-6)
+
+
+def test_pade_exact_on_rational_function():
+    rng = np.random.default_rng(1)
+    poles = rng.uniform(-1, 1, 4) + 1j * rng.uniform(0.2, 1.0, 4)
+    res = rng.uniform(0.1, 1.0, 4)
+
+    def f(z):
+        return np.sum(res / (z - poles))
+
+    z_in = 0.1 + 1j * np.linspace(0.05, 2.0, 9)
+    vals = np.array([f(z) for z in z_in])
+    for z in (0.3 + 0.01j, -0.7 + 0.2j, 1.5 + 0.001j):
+        assert abs(O.pade(z_in, vals, z) - f(z)) < 1e-9
+
+
+def test_se_pade_fit_symmetrisation():
+    # Sigma(z) = r / (z - a) with a real: Sigma(-iw) = Sigma(iw)^* holds exactly
+    a, r = 0.4, 0.3
+    wc = 0.1 + 1j * 0.05 * np.arange(5)
+    sc = (r / (wc - a))[:, None]
+    om = np.array([0.0, 0.2, 0.6])
+    out = O.se_pade_fit(wc, sc, om, 1e-3)
+    for k, w in enumerate(om):
+        sgn = -1.0 if w - wc[0].real >= 0 else 1.0
+        assert abs(out[k, 0] - r / (w + 1j * 1e-3 * sgn - a)) < 1e-10
+
+
+# ---------------------------------------------------------------- A9
+def test_fixed_point_linear_sigma_is_exact():
+    # f(x) = 0.203 - 0.25 (x - 0.1): root of f(x) = x is x* = 0.228 / 1.25 = 0.1824
+    # (between grid points; a root exactly on a grid point gives g = 0 and is
+    # not a sign change for the reference either)
+    xx = 0.01 * np.arange(-25, 26)
+    fx = 0.203 - 0.25 * (xx - 0.1)
+    e, z = O.find_fixed_point(xx, fx)
+    assert abs(e[0] - 0.228 / 1.25) < 1e-15
+    assert abs(z[0] - 1.0 / 1.25) < 1e-14
+    assert np.all(z[1:] == -1.0)
+
+
+def test_fixed_point_keeps_largest_z_and_rejects_negative_z():
+    xx = np.linspace(0.0, 1.0, 101)
+    # three crossings of f(x) = x with slopes giving Z = 1/(1 - slope)
+    fx = xx.copy()
+    fx[0:30] = xx[0:30] + 0.02            # g > 0
+    fx[30:60] = xx[30:60] - 0.02          # crossing 1 at ix 29 -> 30 (steep drop)
+    fx[60:] = xx[60:] + 0.02              # crossing 2 at ix 59 -> 60 (steep rise: Z < 0)
+    e, z = O.find_fixed_point(xx, fx)
+    slope1 = (fx[30] - fx[29]) / (xx[30] - xx[29])
+    assert z[0] == pytest.approx(1 / (1 - slope1))
+    assert z[1] == -1.0                   # the Z < 0 crossing was rejected
+    g29, g30 = fx[29] - xx[29], fx[30] - xx[30]
+    assert e[0] == pytest.approx(xx[29] - g29 / ((g30 - g29) / (xx[30] - xx[29])))
+
+
+def test_qp_graphical_fallback_and_range():
+    nstate = 4
+    rng = O.StateRanges(0, 5, 2, 3, 2, 3, 0, 5)
+    omega = 0.01 * np.arange(-2, 3)
+    e0 = np.array([-1.0, -0.5, 0.4, 1.0])
+    se0 = e0[1:3]
+    sigma = np.zeros((5, 2), dtype=complex)
+    sigma[:, 0] = 0.5            # no crossing in +-0.02: fallback e0 + Re sigma(0) + exc
+    sigma[:, 1] = 0.003 - 0.5 * omega   # crossing at omega = 0.002 with Z = 1/1.5
+    exc = np.array([0.0, 0.1, 0.0, 0.0])
+    eqp, zz = O.find_qp_energy_graphical(sigma, omega, se0, exc, e0, rng)
+    assert eqp[0] == e0[0] and eqp[3] == e0[3]
+    assert eqp[1] == pytest.approx(-0.5 + 0.5 + 0.1)
+    assert eqp[2] == pytest.approx(0.402, abs=1e-15)
+    assert zz[2] == pytest.approx(1 / 1.5)
+    elin, zl = O.find_qp_energy_linearization(sigma, omega, se0, exc, e0, rng)
+    assert elin[2] == pytest.approx(0.402, abs=1e-15) and zl[2] == pytest.approx(1 / 1.5)
+
+
+def test_c_pole_sum_equals_numpy_restatement():
+    # the C/OpenMP pole sum (timed CPU baseline) is the same arithmetic as gw_selfenergy
+    from oracle import build_c
+    build_c.build()
+    from molgw_amd.workloads import synthetic_system
+    s = synthetic_system(16, 40, 4, seed=9, scale=0.3)
+    rng = O.selfenergy_set_state_range(s.occupation[:, None])
+    tt = O.transition_table(s.occupation[:, None], 2.0, 0, s.nstate + 1)
+    eri_w = O.calculate_eri_3center_mo(s.eri_3center, s.index_basis, s.c_matrix, 1, 16, 1, 16)
+    pole, ws, _, _ = O.polarizability_casida(O.casida_ia_matrix(eri_w, tt, 1), s.energy, tt)
+    omega = 0.01 * np.arange(-25, 26)
+    e0 = s.energy.copy()
+    a, ea = O.gw_selfenergy(eri_w, ws, pole, s.energy, s.occupation, rng, omega, e0, 0.001)
+    b, eb = O.gw_selfenergy_c(eri_w, ws, pole, s.energy, s.occupation, rng, omega, e0, 0.001)
+    np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-13 * np.abs(a).max())
+    assert abs(ea - eb) < 1e-12 * abs(ea)
