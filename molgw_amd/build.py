@@ -20,7 +20,7 @@ ROCM = os.environ.get('ROCM_PATH', '/opt/rocm')
 HIPCC = os.path.join(ROCM, 'bin', 'hipcc')
 ARCH = 'gfx950'
 
-SOURCES = ['gemm_f64.hip', 'kernels.hip', 'api.hip']
+SOURCES = ['gemm_f64.hip', 'gemm_lowp.hip', 'kernels.hip', 'api.hip']
 
 CFLAGS = [
     f'--offload-arch={ARCH}', '-O3', '-std=c++17', '-fPIC', '-Wall', '-Wno-unused-result',
@@ -58,7 +58,32 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print('built', LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FLANG = os.path.join(ROCM, 'lib', 'llvm', 'bin', 'amdflang')
+FDIR = os.path.join(HERE, 'fortran')
+FBUILD = os.path.join(FDIR, '_build')
+FDRIVER = os.path.join(FBUILD, 'mgw_fdriver')
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the iso_c_binding module m_mgw and the Fortran driver that runs
+    the one-shot flow through it (linked against libmgw.so)."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print('amdflang not found: Fortran interface not built')
+        return None
+    os.makedirs(FBUILD, exist_ok=True)
+    obj = os.path.join(FBUILD, 'm_mgw.o')
+    _run([FLANG, '-O2', '-c', os.path.join(FDIR, 'm_mgw.f90'), '-module-dir', FBUILD, '-o', obj])
+    _run([FLANG, '-O2', '-I', FBUILD, os.path.join(FDIR, 'mgw_fdriver.f90'), obj, f'-L{HERE}', '-lmgw',
+          '-Wl,-rpath,$ORIGIN/../..', '-o', FDRIVER + '.tmp'])
+    os.replace(FDRIVER + '.tmp', FDRIVER)
+    if verbose:
+        print('built', FDRIVER)
+    return FDRIVER
 
 
 if __name__ == '__main__':

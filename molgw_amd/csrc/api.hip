@@ -242,7 +242,7 @@ bool rectangular(const std::vector<int32_t>& ia, int& i0, int& i1, int& a0, int&
 // is contracted first: H(P, k, f) = B(P k, l) C(l, f) is one GEMM over the
 // dense resident tensor, then a GEMM batched over f contracts k and writes
 // straight into the (P, m, n) layout of the reference.
-int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out) {
+int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int precision = MGW_FP64) {
   MGW_REQUIRE(c->has_ri, MGW_ERR_STATE, "RI tensor not set (mgw_set_eri_3center)");
   MGW_REQUIRE(c->has_mo, MGW_ERR_STATE, "MO data not set (mgw_set_mo)");
   MGW_REQUIRE(mlo >= 1 && mhi <= c->nstate && mlo <= mhi && nlo >= 1 && nhi <= c->nstate && nlo <= nhi,
@@ -264,6 +264,7 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out) {
   g1.B = C + (int64_t)(flo - 1) * nbf; g1.ldb = nbf;
   g1.C = H; g1.ldc = (int64_t)naux * nbf;
   g1.klass = MGW_KCLASS_GEMM_AO2MO;
+  g1.precision = precision;
   {
     ProfScope ps(&c->prof, MGW_KCLASS_GEMM_AO2MO, 2.0 * g1.M * (double)g1.N * g1.K,
                  8.0 * ((double)g1.M * g1.K + (double)g1.K * g1.N + (double)g1.M * g1.N), c->stream);
@@ -280,6 +281,7 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out) {
     g2.C = out; g2.ldc = (int64_t)naux * nm; g2.strideC = naux;
   }
   g2.klass = MGW_KCLASS_GEMM_AO2MO;
+  g2.precision = precision;
   {
     ProfScope ps(&c->prof, MGW_KCLASS_GEMM_AO2MO, 2.0 * g2.M * (double)g2.N * g2.K * nf,
                  8.0 * ((double)g2.M * g2.K * nf + (double)g2.K * g2.N + (double)g2.M * g2.N * nf), c->stream);
@@ -290,7 +292,7 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out) {
 
 // X(P, t) = (P| a_t i_t) for the W transition table, into buffer "x_ia".
 int build_x_ia(mgw_ctx* c, const mgw_ranges* r, double spin_fact, std::vector<int32_t>& ia,
-               double** x_out) {
+               double** x_out, int precision) {
   transition_table(c, r, spin_fact, ia);
   const int npole = (int)ia.size() / 2;
   MGW_REQUIRE(npole > 0, MGW_ERR_ARG, "empty transition space");
@@ -299,13 +301,13 @@ int build_x_ia(mgw_ctx* c, const mgw_ranges* r, double spin_fact, std::vector<in
   int i0, i1, a0, a1;
   if (rectangular(ia, i0, i1, a0, a1)) {
     // (P, a, i) with a inner is exactly X(P, t), t = (i outer, a inner)
-    MGW_TRY(ao2mo(c, a0 + 1, a1 + 1, i0 + 1, i1 + 1, x));
+    MGW_TRY(ao2mo(c, a0 + 1, a1 + 1, i0 + 1, i1 + 1, x, precision));
   } else {
     const int wlo = r->ncore_W + 1, whi = r->nvirtual_W - 1;
     const int nw = whi - wlo + 1;
     double* e = nullptr;
     MGW_TRY(dbuf(c, "ao2mo_w", (size_t)c->naux * nw * nw, &e));
-    MGW_TRY(ao2mo(c, wlo, whi, wlo, whi, e));
+    MGW_TRY(ao2mo(c, wlo, whi, wlo, whi, e, precision));
     std::vector<int32_t> rel(ia.size());
     for (size_t k = 0; k < ia.size(); ++k) rel[k] = ia[k] - (wlo - 1);
     int32_t* ia_d = nullptr;
@@ -319,10 +321,10 @@ int build_x_ia(mgw_ctx* c, const mgw_ranges* r, double spin_fact, std::vector<in
 
 // ------------------------------------------------------------------ A4
 int polarizability_casida(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double* en_rpa,
-                          double* en_gm) {
+                          double* en_gm, int precision) {
   std::vector<int32_t> ia;
   double* x = nullptr;
-  MGW_TRY(build_x_ia(c, r, spin_fact, ia, &x));
+  MGW_TRY(build_x_ia(c, r, spin_fact, ia, &x, precision));
   const int npole = (int)ia.size() / 2;
   const int naux = c->naux;
   int32_t* ia_d = nullptr;
@@ -351,6 +353,7 @@ int polarizability_casida(mgw_ctx* c, const mgw_ranges* r, double spin_fact, dou
   g.rowscale = sq; g.colscale = sq; g.diag_add = de2;
   g.lower_only = true;
   g.klass = MGW_KCLASS_GEMM_CASIDA;
+  g.precision = precision;
   {
     ProfScope ps(&c->prof, MGW_KCLASS_GEMM_CASIDA, (double)npole * (npole + 1.0) * naux,
                  8.0 * ((double)naux * npole + 0.5 * (double)npole * npole), c->stream);
@@ -401,7 +404,7 @@ int polarizability_casida(mgw_ctx* c, const mgw_ranges* r, double spin_fact, dou
 
 // ------------------------------------------------------------------ A6
 int gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta, int nomega,
-                  const double* omega, const double* se_energy0, double* energy_gw) {
+                  const double* omega, const double* se_energy0, double* energy_gw, int precision) {
   MGW_REQUIRE(c->has_wpol, MGW_ERR_STATE, "no spectral function: call mgw_polarizability first");
   const int naux = c->naux, npole = c->npole;
   const int nse = r->nsemax - r->nsemin + 1;
@@ -409,7 +412,7 @@ int gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta,
   const int ng = ghi - glo + 1;
   double* epi = nullptr;
   MGW_TRY(dbuf(c, "eri_pi", (size_t)naux * nse * ng, &epi));
-  MGW_TRY(ao2mo(c, r->nsemin, r->nsemax, glo, ghi, epi));
+  MGW_TRY(ao2mo(c, r->nsemin, r->nsemax, glo, ghi, epi, precision));
   // small per-state vectors
   std::vector<double> e0p(se_energy0, se_energy0 + nse), ei(ng), fi(ng), ep(nse), occp(nse);
   for (int k = 0; k < ng; ++k) { ei[k] = c->energy[glo - 1 + k]; fi[k] = c->occupation[glo - 1 + k]; }
@@ -471,11 +474,12 @@ int gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta,
 
 // ------------------------------------------------------------------ A5
 int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int nomega,
-                        const double* omega_im, const double* weight, double* erpa, double* egw) {
+                        const double* omega_im, const double* weight, double* erpa, double* egw,
+                        int precision) {
   MGW_REQUIRE(nomega >= 1 && nomega <= 65535, MGW_ERR_ARG, "nomega_chi_imag out of range");
   std::vector<int32_t> ia;
   double* x = nullptr;
-  MGW_TRY(build_x_ia(c, r, spin_fact, ia, &x));
+  MGW_TRY(build_x_ia(c, r, spin_fact, ia, &x, precision));
   const int npole = (int)ia.size() / 2;
   const int naux = c->naux;
   int32_t* ia_d = nullptr;
@@ -505,6 +509,7 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
   g.batch = nomega;
   g.lower_only = true;
   g.klass = MGW_KCLASS_GEMM_CHI0;
+  g.precision = precision;
   {
     ProfScope ps(&c->prof, MGW_KCLASS_GEMM_CHI0, (double)naux * (naux + 1.0) * npole * nomega,
                  8.0 * ((double)naux * npole + 0.5 * (double)naux * naux * nomega), c->stream);
@@ -552,7 +557,7 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
 }
 
 // ------------------------------------------------------------------ A7
-int gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int ncalc, const double* omega_calc) {
+int gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int ncalc, const double* omega_calc, int precision) {
   MGW_REQUIRE(c->has_chi, MGW_ERR_STATE, "no chi: call mgw_polarizability_grid first");
   const int naux = c->naux, nomega = c->nomega_chi;
   const int nse = r->nsemax - r->nsemin + 1;
@@ -562,7 +567,7 @@ int gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int ncalc, const double*
   MGW_REQUIRE(ncol < (1ll << 31), MGW_ERR_ARG, "sigma_imag: too many columns");
   double* epm = nullptr;
   MGW_TRY(dbuf(c, "eri_pm", (size_t)naux * ncol, &epm));
-  MGW_TRY(ao2mo(c, glo, ghi, r->nsemin, r->nsemax, epm));
+  MGW_TRY(ao2mo(c, glo, ghi, r->nsemin, r->nsemax, epm, precision));
   const int nrb = gemm_f64_rowblocks(naux);
   double *part, *v, *om_d, *wt_d, *ep_d, *wc_d, *sc;
   MGW_TRY(dbuf(c, "dot_part", (size_t)nrb * ncol * nomega, &part));
@@ -921,8 +926,9 @@ int mgw_polarizability(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int pr
                        double* en_rpa, double* en_gw, double* pole_out, double* w_s_out) {
   MGW_CTX(c);
   MGW_TRY(check_ranges(c, r));
-  MGW_REQUIRE(precision == MGW_FP64, MGW_ERR_ARG, "mgw_polarizability: only MGW_FP64 is available");
-  MGW_TRY(polarizability_casida(c, r, spin_fact, en_rpa, en_gw));
+  MGW_REQUIRE(precision == MGW_FP64 || precision == MGW_FP32 || precision == MGW_BF16X3, MGW_ERR_ARG,
+              "mgw_polarizability: unknown precision");
+  MGW_TRY(polarizability_casida(c, r, spin_fact, en_rpa, en_gw, precision));
   if (pole_out) MGW_TRY(download(c, pole_out, find_buf(c, "wpol_pole"), c->npole));
   if (w_s_out) MGW_TRY(download(c, w_s_out, find_buf(c, "wpol_ws"), (size_t)c->naux * c->npole));
   MGW_HIP(hipStreamSynchronize(c->stream));
@@ -934,9 +940,10 @@ int mgw_polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, i
                             double* erpa, double* egw, double* chi_out) {
   MGW_CTX(c);
   MGW_TRY(check_ranges(c, r));
-  MGW_REQUIRE(precision == MGW_FP64, MGW_ERR_ARG, "mgw_polarizability_grid: only MGW_FP64 is available");
+  MGW_REQUIRE(precision == MGW_FP64 || precision == MGW_FP32 || precision == MGW_BF16X3, MGW_ERR_ARG,
+              "mgw_polarizability_grid: unknown precision");
   MGW_REQUIRE(omega_im && weight_quad, MGW_ERR_ARG, "null frequency grid");
-  MGW_TRY(polarizability_grid(c, r, spin_fact, nomega, omega_im, weight_quad, erpa, egw));
+  MGW_TRY(polarizability_grid(c, r, spin_fact, nomega, omega_im, weight_quad, erpa, egw, precision));
   if (chi_out) MGW_TRY(download(c, chi_out, find_buf(c, "wpol_chi"), (size_t)c->naux * c->naux * nomega));
   MGW_HIP(hipStreamSynchronize(c->stream));
   return MGW_OK;
@@ -948,7 +955,7 @@ int mgw_gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double 
   MGW_CTX(c);
   MGW_TRY(check_ranges(c, r));
   MGW_REQUIRE(omega && se_energy0 && nomega >= 1, MGW_ERR_ARG, "invalid frequency grid");
-  MGW_TRY(gw_selfenergy(c, r, spin_fact, eta, nomega, omega, se_energy0, energy_gw));
+  MGW_TRY(gw_selfenergy(c, r, spin_fact, eta, nomega, omega, se_energy0, energy_gw, MGW_FP64));
   const int nse = r->nsemax - r->nsemin + 1;
   if (sigma_out) MGW_TRY(download(c, sigma_out, find_buf(c, "sigma"), (size_t)2 * nomega * nse));
   MGW_HIP(hipStreamSynchronize(c->stream));
@@ -960,7 +967,7 @@ int mgw_gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int nomega_calc, con
   MGW_CTX(c);
   MGW_TRY(check_ranges(c, r));
   MGW_REQUIRE(omega_calc && nomega_calc >= 1 && nomega_calc <= 65535, MGW_ERR_ARG, "invalid omega_calc");
-  MGW_TRY(gw_selfenergy_imag(c, r, nomega_calc, omega_calc));
+  MGW_TRY(gw_selfenergy_imag(c, r, nomega_calc, omega_calc, MGW_FP64));
   const int nse = r->nsemax - r->nsemin + 1;
   if (sigma_calc_out) MGW_TRY(download(c, sigma_calc_out, find_buf(c, "sigma_calc"), (size_t)2 * nomega_calc * nse));
   MGW_HIP(hipStreamSynchronize(c->stream));
@@ -994,7 +1001,8 @@ int mgw_g0w0(mgw_ctx* c, const mgw_g0w0_params* prm, const mgw_ranges* r, const 
   MGW_CTX(c);
   MGW_REQUIRE(prm, MGW_ERR_ARG, "null parameters");
   MGW_TRY(check_ranges(c, r));
-  MGW_REQUIRE(prm->precision == MGW_FP64, MGW_ERR_ARG, "mgw_g0w0: only MGW_FP64 is available");
+  const int prec = prm->precision;
+  MGW_REQUIRE(prec == MGW_FP64 || prec == MGW_FP32 || prec == MGW_BF16X3, MGW_ERR_ARG, "mgw_g0w0: unknown precision");
   MGW_REQUIRE(prm->nomega_sigma >= 1, MGW_ERR_ARG, "nomega_sigma must be positive");
   const int nstate = c->nstate;
   const int nse = r->nsemax - r->nsemin + 1;
@@ -1006,8 +1014,8 @@ int mgw_g0w0(mgw_ctx* c, const mgw_g0w0_params* prm, const mgw_ranges* r, const 
     // se_init(one_shot): grid centred on each energy0 (src/m_selfenergy_tools.f90:682-690)
     for (int k = 0; k < nomega; ++k) omega[k] = prm->step_sigma * (k - nom);
     for (int k = 0; k < nse; ++k) se0[k] = c->energy[r->nsemin - 1 + k];
-    MGW_TRY(polarizability_casida(c, r, prm->spin_fact, &en_rpa, nullptr));
-    MGW_TRY(gw_selfenergy(c, r, prm->spin_fact, prm->eta, nomega, omega.data(), se0.data(), &en_gw));
+    MGW_TRY(polarizability_casida(c, r, prm->spin_fact, &en_rpa, nullptr, prec));
+    MGW_TRY(gw_selfenergy(c, r, prm->spin_fact, prm->eta, nomega, omega.data(), se0.data(), &en_gw, prec));
   } else if (prm->technique == MGW_TECH_IMAGINARY_PADE) {
     MGW_REQUIRE(prm->nomega_chi_imag >= 1 && prm->nomega_sigma_calc >= 1, MGW_ERR_ARG,
                 "imaginary-axis technique needs nomega_chi_imag and nomega_sigma_calc");
@@ -1023,8 +1031,9 @@ int mgw_g0w0(mgw_ctx* c, const mgw_g0w0_params* prm, const mgw_ranges* r, const 
     }
     std::vector<double> om, wt;
     imaginary_grid(prm->nomega_chi_imag, om, wt);
-    MGW_TRY(polarizability_grid(c, r, prm->spin_fact, prm->nomega_chi_imag, om.data(), wt.data(), &en_rpa, &en_gw));
-    MGW_TRY(gw_selfenergy_imag(c, r, prm->nomega_sigma_calc, wc.data()));
+    MGW_TRY(polarizability_grid(c, r, prm->spin_fact, prm->nomega_chi_imag, om.data(), wt.data(), &en_rpa, &en_gw,
+                                prec));
+    MGW_TRY(gw_selfenergy_imag(c, r, prm->nomega_sigma_calc, wc.data(), prec));
     MGW_TRY(pade_fit(c, nse, prm->nomega_sigma_calc, wc.data(), nullptr, nomega, omega.data(), prm->eta));
   } else {
     set_error("unknown technique");

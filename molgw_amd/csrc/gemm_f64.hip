@@ -28,8 +28,9 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int BK = 16;
 constexpr int PAD = 1;
+constexpr int NXCD = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int GROUP_M = 8;     // tile rows per group of the grouped ordering
 
 struct GemmDev {
   int M, N, K;
@@ -42,7 +43,8 @@ struct GemmDev {
   const double* diag_add; int64_t strideDiag;
   double alpha, beta;
   int lower_only;
-  int tiles_m, tiles_n;
+  int tiles_m, tiles_n, batch;
+  int64_t nblocks;
   const double* dotB; int64_t lddot;
   double* dotpart; int64_t strideDot;
 };
@@ -56,7 +58,15 @@ __device__ inline void tri_index(int t, int& bm, int& bn) {
   bn = t - r * (r + 1) / 2;
 }
 
-template <int BM, int BN, int WM, int WN, bool TA, bool TB, bool DOT, int KLASS>
+// Workgroup w is dispatched to XCD w % 8. Renumber so that consecutive
+// logical ids run on the same XCD (and share its L2): bijective on [0, G).
+__device__ inline int64_t xcd_logical(int64_t w, int64_t G) {
+  const int64_t q = G / NXCD, r = G % NXCD;
+  const int64_t x = w % NXCD, i = w / NXCD;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+template <int BM, int BN, int WM, int WN, int BK, bool TA, bool TB, bool DOT, int KLASS>
 __global__ void __launch_bounds__(64 * WM * WN)
 gemm_f64_kernel(GemmDev g) {
   constexpr int NT = 64 * WM * WN;
@@ -67,6 +77,9 @@ gemm_f64_kernel(GemmDev g) {
   constexpr int LA = BM * BK / NT;   // A elements loaded per thread per slab
   constexpr int LB = BN * BK / NT;
   static_assert(BM * BK % NT == 0 && BN * BK % NT == 0, "tile/thread mismatch");
+  // contiguous-dimension strides of the per-thread element walk
+  static_assert(TA ? (NT % BK == 0) : (NT % BM == 0), "A walk");
+  static_assert(TB ? (NT % BN == 0) : (NT % BK == 0), "B walk");
 
   __shared__ double As[2][BK][BM + PAD];
   __shared__ double Bs[2][BK][BN + PAD];
@@ -77,62 +90,77 @@ gemm_f64_kernel(GemmDev g) {
   const int wm = wave % WM;
   const int wn = wave / WM;
 
+  // logical block: batch fastest (the frequencies of one tile read the same
+  // panels), then the tile in grouped order
+  const int64_t L = xcd_logical(blockIdx.x, g.nblocks);
+  const int batch = (int)(L % g.batch);
+  const int tile = (int)(L / g.batch);
   int bm, bn;
-  if (g.lower_only) {
-    tri_index(blockIdx.x, bm, bn);
+  if (g.lower_only && BM == BN) {
+    tri_index(tile, bm, bn);
   } else {
-    bm = blockIdx.x % g.tiles_m;
-    bn = blockIdx.x / g.tiles_m;
+    const int per_group = GROUP_M * g.tiles_n;
+    const int group = tile / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(g.tiles_m - first_m, GROUP_M);
+    const int in_group = tile - group * per_group;
+    bm = first_m + in_group % gsize;
+    bn = in_group / gsize;
   }
-  const int batch = blockIdx.y;
   const int m0 = bm * BM;
   const int n0 = bn * BN;
+  // symmetric rank-k update: tiles strictly above the diagonal have nothing to do
+  if (g.lower_only && m0 + BM - 1 < n0) return;
 
   const double* __restrict__ A = g.A + (int64_t)batch * g.strideA;
   const double* __restrict__ B = g.B + (int64_t)batch * g.strideB;
   const double* __restrict__ ks = g.kscale ? g.kscale + (int64_t)batch * g.strideK : nullptr;
+
+  // Per-thread walk over a slab: element e = tid + i * NT.
+  //  A non-trans: m = tid % BM fixed, k = tid / BM + i * (NT / BM)
+  //  A trans:     k = tid % BK fixed, m = tid / BK + i * (NT / BK)
+  const int a_fix = TA ? tid % BK : tid % BM;
+  const int a_var = TA ? tid / BK : tid / BM;
+  constexpr int a_step = TA ? NT / BK : NT / BM;
+  const int b_fix = TB ? tid % BN : tid % BK;
+  const int b_var = TB ? tid / BN : tid / BK;
+  constexpr int b_step = TB ? NT / BN : NT / BK;
 
   double ra[LA], rb[LB];
 
   auto load_slab = [&](int k0) {
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
-      const int e = tid + i * NT;
-      int m, k;
-      if (!TA) { m = e % BM; k = e / BM; } else { k = e % BK; m = e / BK; }
-      const int gm = m0 + m, gk = k0 + k;
-      double v = 0.0;
-      if (gm < g.M && gk < g.K) v = TA ? A[gk + (int64_t)gm * g.lda] : A[gm + (int64_t)gk * g.lda];
-      ra[i] = v;
+      const int v = a_var + i * a_step;
+      const int gm = m0 + (TA ? v : a_fix);
+      const int gk = k0 + (TA ? a_fix : v);
+      double x = 0.0;
+      if (gm < g.M && gk < g.K) x = TA ? A[gk + (int64_t)gm * g.lda] : A[gm + (int64_t)gk * g.lda];
+      ra[i] = x;
     }
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
-      const int e = tid + i * NT;
-      int n, k;
-      if (!TB) { k = e % BK; n = e / BK; } else { n = e % BN; k = e / BN; }
-      const int gn = n0 + n, gk = k0 + k;
-      double v = 0.0;
+      const int v = b_var + i * b_step;
+      const int gn = n0 + (TB ? b_fix : v);
+      const int gk = k0 + (TB ? v : b_fix);
+      double x = 0.0;
       if (gn < g.N && gk < g.K) {
-        v = TB ? B[gn + (int64_t)gk * g.ldb] : B[gk + (int64_t)gn * g.ldb];
-        if (ks) v *= ks[gk];
+        x = TB ? B[gn + (int64_t)gk * g.ldb] : B[gk + (int64_t)gn * g.ldb];
+        if (ks) x *= ks[gk];
       }
-      rb[i] = v;
+      rb[i] = x;
     }
   };
   auto store_slab = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
-      const int e = tid + i * NT;
-      int m, k;
-      if (!TA) { m = e % BM; k = e / BM; } else { k = e % BK; m = e / BK; }
-      As[buf][k][m] = ra[i];
+      const int v = a_var + i * a_step;
+      if (TA) As[buf][a_fix][v] = ra[i]; else As[buf][v][a_fix] = ra[i];
     }
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
-      const int e = tid + i * NT;
-      int n, k;
-      if (!TB) { k = e % BK; n = e / BK; } else { n = e % BN; k = e / BN; }
-      Bs[buf][k][n] = rb[i];
+      const int v = b_var + i * b_step;
+      if (TB) Bs[buf][v][b_fix] = rb[i]; else Bs[buf][b_fix][v] = rb[i];
     }
   };
 
@@ -233,7 +261,7 @@ gemm_f64_kernel(GemmDev g) {
   }
 }
 
-template <int BM, int BN, int WM, int WN, bool DOT, int KLASS>
+template <int BM, int BN, int WM, int WN, int BK, bool DOT, int KLASS>
 int launch_cfg(const GemmF64& h, hipStream_t stream) {
   GemmDev g;
   g.M = h.M; g.N = h.N; g.K = h.K;
@@ -247,22 +275,22 @@ int launch_cfg(const GemmF64& h, hipStream_t stream) {
   g.lower_only = h.lower_only ? 1 : 0;
   g.tiles_m = (h.M + BM - 1) / BM;
   g.tiles_n = (h.N + BN - 1) / BN;
+  g.batch = h.batch;
   g.dotB = h.dotB; g.lddot = h.lddot; g.dotpart = h.dotpart; g.strideDot = h.strideDot;
-  int64_t tiles;
-  if (h.lower_only) {
-    tiles = (int64_t)g.tiles_m * (g.tiles_m + 1) / 2;
-  } else {
-    tiles = (int64_t)g.tiles_m * g.tiles_n;
-  }
-  MGW_REQUIRE(tiles < (1ll << 31), MGW_ERR_ARG, "gemm_f64: too many tiles");
-  MGW_REQUIRE(h.batch >= 1 && h.batch <= 65535, MGW_ERR_ARG, "gemm_f64: batch out of range");
-  dim3 grid((unsigned)tiles, (unsigned)h.batch, 1);
+  // square-tile symmetric updates enumerate the lower triangle of tiles only;
+  // other lower-only shapes launch the full grid and skip upper tiles
+  const int64_t tiles = (h.lower_only && BM == BN) ? (int64_t)g.tiles_m * (g.tiles_m + 1) / 2
+                                                   : (int64_t)g.tiles_m * g.tiles_n;
+  g.nblocks = tiles * h.batch;
+  MGW_REQUIRE(tiles < (1ll << 31) && g.nblocks < (1ll << 31), MGW_ERR_ARG, "gemm_f64: too many tiles");
+  MGW_REQUIRE(h.batch >= 1, MGW_ERR_ARG, "gemm_f64: batch out of range");
+  dim3 grid((unsigned)g.nblocks, 1, 1);
   dim3 block(64 * WM * WN, 1, 1);
   const int ta = h.transA ? 1 : 0, tb = h.transB ? 1 : 0;
-#define MGW_GEMM_CASE(TA_, TB_)                                                                  \
-  if (ta == TA_ && tb == TB_) {                                                                  \
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, WM, WN, TA_, TB_, DOT, KLASS>), grid, block, 0,  \
-                       stream, g);                                                               \
+#define MGW_GEMM_CASE(TA_, TB_)                                                                        \
+  if (ta == TA_ && tb == TB_) {                                                                        \
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, WM, WN, BK, TA_, TB_, DOT, KLASS>), grid, block, 0,    \
+                       stream, g);                                                                     \
   }
   MGW_GEMM_CASE(0, 0)
   MGW_GEMM_CASE(0, 1)
@@ -275,24 +303,30 @@ int launch_cfg(const GemmF64& h, hipStream_t stream) {
 
 }  // namespace
 
-int gemm_f64_rowblocks(int M) { return (M + 127) / 128; }
+int gemm_f64_rowblocks(int M) { return (M + 127) / 128; }   // row tile of the dot epilogue
 
 int gemm_f64(const GemmF64& h, hipStream_t stream) {
   if (h.M <= 0 || h.N <= 0 || h.batch <= 0) return MGW_OK;
+  if (h.precision != MGW_FP64) return gemm_lowp(h, h.precision, stream);
   MGW_REQUIRE(h.K >= 0, MGW_ERR_ARG, "gemm_f64: negative K");
   MGW_REQUIRE(!h.lower_only || h.M == h.N, MGW_ERR_ARG, "gemm_f64: lower_only needs M == N");
   MGW_REQUIRE(h.dotpart == nullptr || (!h.lower_only && h.dotB), MGW_ERR_ARG,
               "gemm_f64: dot epilogue needs dotB and a full tile grid");
   MGW_REQUIRE(h.dotpart != nullptr || h.C != nullptr, MGW_ERR_ARG, "gemm_f64: no output");
-  if (h.dotpart) return launch_cfg<128, 128, 2, 2, true, MGW_KCLASS_GEMM_WB>(h, stream);
-  // Skinny N: a 256 x 32 tile keeps the matrix cores busy when N is small.
-  if (h.N <= 32 && !h.lower_only) return launch_cfg<256, 32, 4, 1, false, MGW_KCLASS_GEMM_AO2MO>(h, stream);
+  // Tile shapes measured on MI355X (FP64 MFMA): 256x128 with 16 waves for the
+  // large products (57-62 TF/s), 128x128 with 8 waves for the long-K symmetric
+  // chi0 update (latency-bound on panels streamed from HBM), 256x32 for skinny N.
+  if (h.dotpart) return launch_cfg<128, 128, 4, 2, 16, true, MGW_KCLASS_GEMM_WB>(h, stream);
+  if (h.N <= 32 && !h.lower_only) return launch_cfg<256, 32, 4, 1, 16, false, MGW_KCLASS_GEMM_AO2MO>(h, stream);
+  if (h.lower_only) {
+    if (h.klass == MGW_KCLASS_GEMM_CHI0) return launch_cfg<128, 128, 4, 2, 16, false, MGW_KCLASS_GEMM_CHI0>(h, stream);
+    return launch_cfg<128, 128, 4, 2, 16, false, MGW_KCLASS_GEMM_CASIDA>(h, stream);
+  }
   switch (h.klass) {
-    case MGW_KCLASS_GEMM_AO2MO: return launch_cfg<128, 128, 2, 2, false, MGW_KCLASS_GEMM_AO2MO>(h, stream);
-    case MGW_KCLASS_GEMM_CASIDA: return launch_cfg<128, 128, 2, 2, false, MGW_KCLASS_GEMM_CASIDA>(h, stream);
-    case MGW_KCLASS_GEMM_SIGMA: return launch_cfg<128, 128, 2, 2, false, MGW_KCLASS_GEMM_SIGMA>(h, stream);
-    case MGW_KCLASS_GEMM_CHI0: return launch_cfg<128, 128, 2, 2, false, MGW_KCLASS_GEMM_CHI0>(h, stream);
-    default: return launch_cfg<128, 128, 2, 2, false, MGW_KCLASS_GEMM_OTHER>(h, stream);
+    case MGW_KCLASS_GEMM_AO2MO: return launch_cfg<256, 128, 4, 4, 16, false, MGW_KCLASS_GEMM_AO2MO>(h, stream);
+    case MGW_KCLASS_GEMM_CASIDA: return launch_cfg<256, 128, 4, 4, 16, false, MGW_KCLASS_GEMM_CASIDA>(h, stream);
+    case MGW_KCLASS_GEMM_SIGMA: return launch_cfg<256, 128, 4, 4, 16, false, MGW_KCLASS_GEMM_SIGMA>(h, stream);
+    default: return launch_cfg<256, 128, 4, 4, 16, false, MGW_KCLASS_GEMM_OTHER>(h, stream);
   }
 }
 

@@ -113,8 +113,12 @@ struct GemmF64 {
   // kernel class: the hot classes get their own kernel symbol so that
   // profiler per-kernel statistics separate them
   int klass = MGW_KCLASS_GEMM_OTHER;
+  // arithmetic of the product: MGW_FP64 (default), MGW_FP32 or MGW_BF16X3
+  // (FP64 operands rounded on the way to LDS, FP64 accumulation)
+  int precision = MGW_FP64;
 };
 int gemm_f64(const GemmF64& g, hipStream_t stream);
+int gemm_lowp(const GemmF64& g, int precision, hipStream_t stream);
 int gemm_f64_rowblocks(int M);   // number of row blocks of the dot epilogue
 
 // ---------------------------------------------------------------- kernels

@@ -1,0 +1,83 @@
+!=========================================================================
+! mgw_fdriver: runs the one-shot G0W0 flow of MOLGW's selfenergy_evaluation
+! (src/m_selfenergy_evaluation.f90:135-261, 589-590) through the Fortran
+! interface module m_mgw, on inputs read from a binary stream file, and
+! writes the results to a second file.
+!
+! usage: mgw_fdriver input.bin output.bin
+! input  (stream): int32 nbf, nauxil, nstate; int64 npair;
+!                  real64 eri_3center(npair, nauxil); int32 index_basis(2, npair);
+!                  real64 c_matrix(nbf, nstate), energy(nstate), occupation(nstate)
+! output (stream): real64 energy_qp(nstate), zz(nstate), energy_qp_lin(nstate),
+!                  en_rpa, en_gw, energy_qp_one_shot(nstate)
+!=========================================================================
+program mgw_fdriver
+  use m_mgw
+  implicit none
+  integer, parameter :: dp = kind(1.0d0)
+  integer :: nbf, nauxil, nstate, iu, istate, nhomo, k
+  integer(8) :: npair
+  real(dp), allocatable :: eri(:, :), c_matrix(:, :), energy(:), occupation(:)
+  integer, allocatable :: index_basis(:, :)
+  real(dp), allocatable :: pole(:), omega(:), se_energy0(:), exc(:)
+  real(dp), allocatable :: energy_qp(:), zz(:), energy_qp_lin(:), eqp2(:), zz2(:), elin2(:)
+  complex(dp), allocatable :: sigma(:, :), sigma2(:, :)
+  real(dp) :: en_rpa, en_gw, energies(2)
+  type(mgw_ranges) :: ranges
+  type(mgw_g0w0_params) :: params
+  character(len=512) :: fin, fout
+  integer, parameter :: nomega_sigma = 51
+  real(dp), parameter :: step_sigma = 0.01_dp, eta = 0.001_dp
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(newunit=iu, file=TRIM(fin), access='stream', form='unformatted', status='old', action='read')
+  read(iu) nbf, nauxil, nstate
+  read(iu) npair
+  allocate(eri(npair, nauxil), index_basis(2, npair), c_matrix(nbf, nstate), energy(nstate), occupation(nstate))
+  read(iu) eri
+  read(iu) index_basis
+  read(iu) c_matrix
+  read(iu) energy
+  read(iu) occupation
+  close(iu)
+
+  ! selfenergy_set_state_range defaults: all states, no frozen core
+  nhomo = 1
+  do istate = 1, nstate
+    if( occupation(istate) > 1.0e-5_dp ) nhomo = istate
+  end do
+  ranges%ncore_G = 0
+  ranges%nvirtual_G = nstate + 1
+  ranges%nsemin = 1
+  ranges%nsemax = nstate
+  ranges%nhomo_G = nhomo
+  ranges%ncore_W = 0
+  ranges%nvirtual_W = nstate + 1
+
+  call mgw_init(0)
+  call mgw_upload_eri_3center(eri, index_basis, nbf)
+  call mgw_upload_mo(c_matrix, energy, occupation)
+
+  ! W by Casida, then Sigma_c on se%omega = step_sigma * k centred on energy0
+  call mgw_polarizability_rpa(ranges, en_rpa, en_gw, pole)
+  allocate(omega(nomega_sigma), se_energy0(nstate), exc(nstate), sigma(nomega_sigma, nstate))
+  do k = 1, nomega_sigma
+    omega(k) = step_sigma * (k - 1 - nomega_sigma / 2)
+  end do
+  se_energy0(:) = energy(:)
+  exc(:) = 0.0_dp
+  call mgw_selfenergy_poles(ranges, eta, omega, se_energy0, sigma, en_gw)
+  allocate(energy_qp(nstate), zz(nstate), energy_qp_lin(nstate))
+  call mgw_qp_energy(ranges, omega, sigma, se_energy0, exc, energy, energy_qp, zz, energy_qp_lin)
+
+  ! the same solve in one call
+  allocate(eqp2(nstate), zz2(nstate), elin2(nstate), sigma2(nomega_sigma, nstate))
+  call mgw_one_shot(params, ranges, exc, eqp2, zz2, elin2, sigma2, energies)
+  call mgw_finalize()
+
+  open(newunit=iu, file=TRIM(fout), access='stream', form='unformatted', status='replace', action='write')
+  write(iu) energy_qp, zz, energy_qp_lin, en_rpa, en_gw, eqp2
+  close(iu)
+  write(*, '(a,i0,a,f16.10)') ' mgw_fdriver: nstate=', nstate, ' E_RPA=', en_rpa
+end program mgw_fdriver

@@ -44,6 +44,9 @@ from . import _lib
 from ._lib import G0W0Params, Ranges, check
 
 COMPLETELY_EMPTY = 1.0e-5   # src/m_definitions.f90:66
+HA_EV = 27.21138505         # src/m_definitions.f90:43
+
+PRECISION = {'fp64': _lib.MGW_FP64, 'fp32': _lib.MGW_FP32, 'bf16x3': _lib.MGW_BF16X3}
 
 
 def _f64(a) -> np.ndarray:
@@ -219,19 +222,20 @@ class Engine:
         check(self._lib.mgw_npole(self._h, C.byref(r), C.byref(n)))
         return n.value
 
-    def polarizability(self, rng: StateRanges, spin_fact=2.0, return_ws=False):
+    def polarizability(self, rng: StateRanges, spin_fact=2.0, return_ws=False, precision='fp64'):
         """Returns (en_rpa, en_gw, pole, w_s or None)."""
         r = rng.to_c()
         npole = self.npole(rng)
         pole = np.zeros(npole)
         ws = np.zeros((self.naux, npole), order='F') if return_ws else None
         erpa, egw = C.c_double(), C.c_double()
-        check(self._lib.mgw_polarizability(self._h, C.byref(r), spin_fact, _lib.MGW_FP64, C.byref(erpa),
+        check(self._lib.mgw_polarizability(self._h, C.byref(r), spin_fact, PRECISION[precision], C.byref(erpa),
                                            C.byref(egw), _ptr(pole), _ptr(ws) if return_ws else None))
         return erpa.value, egw.value, pole, ws
 
     # ------------------------------------------------------------ A5
-    def polarizability_grid(self, rng: StateRanges, omega_im, weight, spin_fact=2.0, return_chi=False):
+    def polarizability_grid(self, rng: StateRanges, omega_im, weight, spin_fact=2.0, return_chi=False,
+                            precision='fp64'):
         """Returns (erpa, egw, chi[naux, naux, nomega] or None)."""
         r = rng.to_c()
         om = _f64(omega_im)
@@ -239,7 +243,7 @@ class Engine:
         n = len(om)
         chi = np.zeros((self.naux, self.naux, n), order='F') if return_chi else None
         erpa, egw = C.c_double(), C.c_double()
-        check(self._lib.mgw_polarizability_grid(self._h, C.byref(r), spin_fact, _lib.MGW_FP64, n, _ptr(om),
+        check(self._lib.mgw_polarizability_grid(self._h, C.byref(r), spin_fact, PRECISION[precision], n, _ptr(om),
                                                 _ptr(wt), C.byref(erpa), C.byref(egw),
                                                 _ptr(chi) if return_chi else None))
         return erpa.value, egw.value, chi
@@ -297,14 +301,18 @@ class Engine:
     # ------------------------------------------------------------ A11
     def g0w0(self, rng: StateRanges | None = None, technique='one_shot', nomega_sigma=51,
              step_sigma=0.01, eta=0.001, nomega_chi_imag=0, nomega_sigma_calc=1,
-             step_sigma_calc=0.03, exchange_m_vxc=None, spin_fact=2.0, return_sigma=True) -> G0W0Output:
+             step_sigma_calc=0.03, exchange_m_vxc=None, spin_fact=2.0, return_sigma=True,
+             precision='fp64') -> G0W0Output:
         """One-shot G0W0 (postscf='GW' for technique='one_shot', 'G0W0_PADE' for
-        technique='imaginary_pade') on the resident tensors."""
+        technique='imaginary_pade') on the resident tensors. ``precision``
+        ('fp64', 'fp32', 'bf16x3') applies to the AO->MO transforms and the
+        chi0 / Casida-matrix build; the eigensolve, the (1 - chi0) inversion,
+        the self-energy products and the QP solve are FP64."""
         if rng is None:
             rng = selfenergy_set_state_range(self.occupation)
         tech = {'one_shot': _lib.MGW_TECH_ONE_SHOT, 'imaginary_pade': _lib.MGW_TECH_IMAGINARY_PADE}[technique]
         prm = G0W0Params(tech, nomega_sigma, step_sigma, eta, nomega_chi_imag, nomega_sigma_calc,
-                         step_sigma_calc, spin_fact, _lib.MGW_FP64, 0)
+                         step_sigma_calc, spin_fact, PRECISION[precision], 0)
         r = rng.to_c()
         n = self.nstate
         nom = nomega_sigma // 2

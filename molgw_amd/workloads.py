@@ -117,3 +117,62 @@ def dense_ri(sys: System) -> np.ndarray:
     d = k == l
     out[:, k[d], l[d]] = 2.0 * sys.eri_3center[d].T
     return out
+
+
+@dataclass
+class DeviceSystem:
+    """A synthetic system whose packed RI tensor lives in device memory (torch
+    tensor of shape (naux, npair), i.e. column-major (npair, naux))."""
+    name: str
+    nbf: int
+    naux: int
+    nocc: int
+    eri_3center_dev: object      # torch.Tensor on cuda, (naux, npair) contiguous
+    index_basis: np.ndarray
+    c_matrix: np.ndarray
+    energy: np.ndarray
+    occupation: np.ndarray
+
+    @property
+    def nstate(self) -> int:
+        return self.c_matrix.shape[1]
+
+    @property
+    def npair(self) -> int:
+        return self.index_basis.shape[1]
+
+
+def synthetic_system_device(nbf: int, naux: int, nocc: int, seed: int = 0, name: str = 'synthetic',
+                            scale: float = 1.0, chunk: int = 65536) -> DeviceSystem:
+    """Same recipe as ``synthetic_system`` with the large factor G and the product
+    B = L G drawn and formed on the GPU (torch is used as device-memory plumbing),
+    for sizes whose RI tensor does not fit the host comfortably. C, energies and
+    L come from the same numpy stream as ``synthetic_system``; G comes from a
+    seeded torch generator, so values differ from the host recipe."""
+    import torch
+    rng = np.random.default_rng(seed)
+    c = orthogonal(rng, nbf)
+    e_occ = np.sort(rng.uniform(-1.2, -0.3, nocc))
+    e_virt = np.sort(rng.uniform(0.05, 3.0, nbf - nocc))
+    energy = np.concatenate([e_occ, e_virt])
+    occupation = np.concatenate([np.full(nocc, 2.0), np.zeros(nbf - nocc)])
+    u = orthogonal(rng, naux)
+    v = orthogonal(rng, naux)
+    sv = 1.0 / (1.0 + np.arange(naux)) ** 1.5
+    lmat = torch.from_numpy((u * sv[None, :]) @ v.T).cuda()
+    ib = upper_pairs(nbf)
+    npair = ib.shape[1]
+    gen = torch.Generator(device='cuda')
+    gen.manual_seed(seed)
+    b = torch.empty((naux, npair), dtype=torch.float64, device='cuda')
+    diag = torch.from_numpy(ib[0] == ib[1]).cuda()
+    for p0 in range(0, npair, chunk):
+        p1 = min(npair, p0 + chunk)
+        g = torch.randn((naux, p1 - p0), generator=gen, dtype=torch.float64, device='cuda')
+        blk = lmat @ g
+        if scale != 1.0:
+            blk *= scale
+        blk[:, diag[p0:p1]] *= 0.5
+        b[:, p0:p1] = blk
+    torch.cuda.synchronize()
+    return DeviceSystem(name, nbf, naux, nocc, b, ib, np.asfortranarray(c), energy, occupation)

@@ -1,0 +1,64 @@
+"""Reduced-precision variants (FP32, BF16x3 split products with FP64
+accumulation) of the AO->MO transforms and the chi0 / Casida-matrix build,
+against the FP64 path on the GPU. Acceptance: every QP energy within 1 meV
+of FP64 (the benchmark's accuracy condition); the measured errors are printed.
+"""
+import numpy as np
+import pytest
+
+from molgw_amd.gw import HA_EV, imaginary_quad_grid, selfenergy_set_state_range
+from molgw_amd.workloads import synthetic_system
+
+pytestmark = pytest.mark.gpu
+
+MEV = 1.0e-3 / HA_EV     # 1 meV in Ha
+
+
+def load(engine, s):
+    engine.set_eri_3center(s.eri_3center, s.index_basis, s.nbf)
+    engine.set_mo(s.c_matrix, s.energy, s.occupation)
+
+
+@pytest.fixture(scope='module')
+def mol():
+    return synthetic_system(64, 190, 9, seed=31, name='molecular-scale', scale=0.3)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize('prec,tol_chi', [('fp32', 1e-6), ('bf16x3', 1e-4)])
+def test_chi_precision(engine, mol, prec, tol_chi):
+    load(engine, mol)
+    rng = selfenergy_set_state_range(mol.occupation)
+    om, wt = imaginary_quad_grid(8)
+    e64, g64, chi64 = engine.polarizability_grid(rng, om, wt, return_chi=True)
+    e, g, chi = engine.polarizability_grid(rng, om, wt, return_chi=True, precision=prec)
+    err = rel(chi, chi64)
+    print(f'{prec}: chi rel err {err:.2e}, E_RPA err {abs(e - e64):.2e} Ha')
+    assert err < tol_chi
+
+
+@pytest.mark.parametrize('prec', ['fp32', 'bf16x3'])
+@pytest.mark.parametrize('technique', ['one_shot', 'imaginary_pade'])
+def test_qp_error_within_1_mev(engine, mol, prec, technique):
+    load(engine, mol)
+    kw = dict(nomega_chi_imag=24, nomega_sigma_calc=12) if technique == 'imaginary_pade' else {}
+    ref = engine.g0w0(technique=technique, **kw)
+    out = engine.g0w0(technique=technique, precision=prec, **kw)
+    err = np.abs(out.energy_qp - ref.energy_qp)
+    print(f'{technique} {prec}: max |dE_QP| = {err.max() / MEV:.4f} meV, '
+          f'HOMO {err[mol.nocc - 1] / MEV:.4f} meV, LUMO {err[mol.nocc] / MEV:.4f} meV')
+    assert err.max() < 1.0 * MEV
+
+
+def test_bf16x3_benchmark_recipe(engine):
+    # the unscaled benchmark recipe (Sigma ~ 10 Ha): the error stays below 1 meV
+    s = synthetic_system(48, 140, 8, seed=32)
+    load(engine, s)
+    ref = engine.g0w0()
+    out = engine.g0w0(precision='bf16x3')
+    err = np.abs(out.energy_qp - ref.energy_qp).max()
+    print(f'benchmark recipe bf16x3: max |dE_QP| = {err / MEV:.4f} meV')
+    assert err < 1.0 * MEV

@@ -97,3 +97,19 @@ def test_product_does_not_import_oracle():
             if f.endswith(('.py', '.hip', '.h', '.cpp', '.f90')):
                 text = open(os.path.join(dirpath, f), errors='ignore').read()
                 assert 'import oracle' not in text and 'from oracle' not in text, f
+
+
+def test_fortran_interface_compiles(tmp_path):
+    """m_mgw (the iso_c_binding module MOLGW would use) compiles and a program
+    using it links against libmgw.so."""
+    import subprocess
+    from molgw_amd import build as b
+    if not os.path.exists(b.FLANG):
+        pytest.skip('amdflang not available')
+    obj = tmp_path / 'm_mgw.o'
+    subprocess.run([b.FLANG, '-c', os.path.join(b.FDIR, 'm_mgw.f90'), '-module-dir', str(tmp_path), '-o', str(obj)],
+                   check=True)
+    exe = tmp_path / 'drv'
+    subprocess.run([b.FLANG, '-I', str(tmp_path), os.path.join(b.FDIR, 'mgw_fdriver.f90'), str(obj), f'-L{b.HERE}',
+                    '-lmgw', f'-Wl,-rpath,{b.HERE}', '-o', str(exe)], check=True)
+    assert exe.exists()
