@@ -49,11 +49,58 @@ WORKLOADS = {
 }
 
 
+def stream_bench(workload, args):
+    """Mixed-size request streams (configs 4 and 5): an FP64 pass that serves
+    one request at a time (the FP64 drop-in, also the QP reference), then the
+    serving pass (mixed precision, concurrent slots, continuous batching)."""
+    import torch
+    from molgw_amd.serving import Request, Server, summarize, poisson_stream, alkane_batch
+    from molgw_amd.workloads import synthetic_system_device, alkane_shape, water_cluster_shape
+    if workload == 'serving':
+        spec = poisson_stream(args.requests, args.rate, seed=args.seed)
+        realtime = True
+    else:
+        spec = [(0.0, 'alkane', n) for n in alkane_batch(args.requests, seed=args.seed)]
+        realtime = False
+    shapes = {}
+    for _, kind, size in spec:
+        key = (kind, size)
+        if key not in shapes:
+            nbf, naux, nocc = alkane_shape(size) if kind == 'alkane' else water_cluster_shape(size)
+            shapes[key] = synthetic_system_device(nbf, naux, nocc, seed=1000 + 97 * size + (kind == 'water'),
+                                                  name=f'{kind}{size}', scale=0.3)
+    torch.cuda.synchronize()
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(0.85 * free)
+    print(f'# {workload}: {len(spec)} requests, {len(shapes)} shapes resident, HBM budget {budget / 2**30:.0f} GiB',
+          file=sys.stderr, flush=True)
+
+    def requests(precision):
+        return [Request(k, shapes[(kind, size)], arrival=t, technique='imaginary_pade', precision=precision,
+                        nomega_chi_imag=32, nomega_sigma_calc=16) for k, (t, kind, size) in enumerate(spec)]
+
+    with Server(nslots=1, hbm_budget=budget) as srv:
+        srv.run(requests('fp64')[:1], realtime=False)                 # warm-up
+        ref = srv.run(requests('fp64'), realtime=False)
+    ref_sum = summarize(ref)
+    print(f'# fp64 one at a time: {ref_sum}', file=sys.stderr, flush=True)
+    with Server(nslots=args.slots, hbm_budget=budget) as srv:
+        srv.run(requests(args.precision)[:args.slots], realtime=False)  # warm-up
+        res = srv.run(requests(args.precision), realtime=realtime)
+    sm = summarize(res)
+    err = 0.0
+    for a, b in zip(res, ref):
+        sl = slice(a.nsemin - 1, a.nsemax)
+        err = max(err, float(np.max(np.abs(a.energy_qp[sl] - b.energy_qp[sl]))))
+    print(f'# serving: {sm}, max |dE_QP| = {err * 27211.38505:.4f} meV', file=sys.stderr, flush=True)
+    return sm, ref_sum, err * 27211.38505, spec
+
+
 def _percentile(v, q):
     return float(np.percentile(np.asarray(v), q)) if len(v) else None
 
 
-def gpu_bench(system, technique, steps, warmup, kw):
+def gpu_bench(system, technique, steps, warmup, kw, precision='fp64'):
     from molgw_amd.gw import Engine
     eng = Engine(0)
     if hasattr(system, 'eri_3center_dev'):
@@ -65,39 +112,56 @@ def gpu_bench(system, technique, steps, warmup, kw):
     else:
         eng.set_eri_3center(system.eri_3center, system.index_basis, system.nbf)
     eng.set_mo(system.c_matrix, system.energy, system.occupation)
+    ref = None
+    if precision != 'fp64':
+        # accuracy condition of the metric: QP energies against the FP64 solve
+        ref = eng.g0w0(technique=technique, return_sigma=False, **kw)
     for _ in range(warmup):
-        eng.g0w0(technique=technique, return_sigma=False, **kw)
+        eng.g0w0(technique=technique, return_sigma=False, precision=precision, **kw)
     eng.synchronize()
     eng.profile(True)
     lat = []
     t0 = time.perf_counter()
     for _ in range(steps):
         ts = time.perf_counter()
-        out = eng.g0w0(technique=technique, return_sigma=False, **kw)   # returns after the stream is drained
-        lat.append(time.perf_counter() - ts)
+        out = eng.g0w0(technique=technique, return_sigma=False, precision=precision, **kw)
+        lat.append(time.perf_counter() - ts)   # g0w0 returns after the stream is drained
     eng.synchronize()
     t1 = time.perf_counter()
     prof = eng.profile_read()
     eng.profile(False)
     mem = eng.memory()
     eng.close()
-    return t1 - t0, lat, prof, out, mem
+    err_mev = None
+    if ref is not None:
+        lo, hi = kw['rng'].nsemin - 1 if 'rng' in kw else 0, kw['rng'].nsemax if 'rng' in kw else len(out.energy_qp)
+        err_mev = float(np.max(np.abs(out.energy_qp[lo:hi] - ref.energy_qp[lo:hi]))) * 27.21138505e3
+    return t1 - t0, lat, prof, out, mem, err_mev
 
 
-def roofline_of(prof):
-    """Dominant hand-written kernel class by device time (library solver calls excluded)."""
+LOWP_CLASSES = ('gemm_ao2mo', 'gemm_chi0')
+PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0 / 3.0}
+
+
+def roofline_of(prof, precision='fp64'):
+    """Dominant hand-written kernel class by device time (library solver calls excluded).
+
+    achieved = algorithmic flops per launch / average launch time. For the
+    BF16x3 products the ceiling is the published dense BF16 MFMA peak
+    (2.5 PFLOP/s) divided by the 3 products each algorithmic product costs."""
     own = {k: v for k, v in prof.items() if k not in ('eigensolver', 'cholesky')}
     if not own:
         return None, None
     name = max(own, key=lambda k: own[k]['ms'])
     v = own[name]
     t_avg = v['ms'] / v['launches'] * 1e-3
+    achieved = v['flops'] / v['launches'] / t_avg / 1e12
     if name == 'sigma_poles':
-        achieved = v['flops'] / v['launches'] / t_avg / 1e12
         r = {'bound': 'valu', 'achieved': achieved, 'peak': VALU_F64_PEAK_TFLOPS, 'unit': 'TFLOP/s'}
     else:
-        achieved = v['flops'] / v['launches'] / t_avg / 1e12
-        r = {'bound': 'mfma', 'achieved': achieved, 'peak': MFMA_F64_PEAK_TFLOPS, 'unit': 'TFLOP/s'}
+        prec = precision if name in LOWP_CLASSES else 'fp64'
+        r = {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_TFLOPS[prec], 'unit': 'TFLOP/s',
+             'arith': prec}
     r['frac'] = r['achieved'] / r['peak']
     r['traffic'] = _measured_traffic(name)
     r['kernel'] = name
@@ -196,15 +260,46 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--workload', default='octane', choices=sorted(WORKLOADS))
+    ap.add_argument('--workload', default='octane', choices=sorted(WORKLOADS) + ['serving', 'batch64'])
+    ap.add_argument('--requests', type=int, default=64, help='stream / batch length (serving, batch64)')
+    ap.add_argument('--rate', type=float, default=20.0, help='Poisson arrival rate, requests/s (serving)')
+    ap.add_argument('--slots', type=int, default=8, help='concurrent solve slots (serving, batch64)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--precision', default='fp64', choices=['fp64', 'fp32', 'bf16x3'],
+                    help='arithmetic of the AO->MO transforms and the chi0/Casida build')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py: this engine runs one G0W0 solve per GPU on a single MI355X; multi-GPU runs are '
               'not part of this build (each solve is independent: shard requests across GPUs).',
               file=sys.stderr)
         sys.exit(2)
+    if args.workload in ('serving', 'batch64'):
+        prec = args.precision if args.precision != 'fp64' else 'bf16x3'
+        args.precision = prec
+        sm, ref, err, spec = stream_bench(args.workload, args)
+        desc = ('Poisson stream of G0W0 requests: alkanes CnH2n+2 (n in 2..24) and water clusters (H2O)m '
+                f'(m in 1..32), cc-pVDZ/cc-pVDZ-RI shapes, {args.requests} requests at {args.rate}/s'
+                if args.workload == 'serving' else
+                f'batch of {args.requests} alkanes CnH2n+2, n uniform in 1..16, cc-pVDZ/cc-pVDZ-RI shapes')
+        line = {
+            'metric': METRIC, 'value': sm['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
+            'steps': sm['solves'], 'warmup': args.slots, 'ms_per_step': 1e3 / sm['solves_per_s'],
+            'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+            'dtype': f'{prec} products, f64 accumulation; f64 inversion and QP',
+            'qp_error_vs_fp64_mev': err,
+            'data': 'synthetic RI tensors at the molecules\' shapes (seeded, molecular magnitudes)',
+            'config': {'workload': desc, 'technique': 'imaginary_pade', 'nomega_chi_imag': 32,
+                       'nomega_sigma_calc': 16, 'states': 'HOMO-7..LUMO+8', 'slots': args.slots,
+                       'precision': prec},
+            'latency_ms': {'p50': sm['p50_ms'], 'p99': sm['p99_ms']},
+            'fp64_one_at_a_time': {'solves_per_s': ref['solves_per_s'], 'p50_ms': ref['p50_ms'],
+                                   'p99_ms': ref['p99_ms']},
+            'speedup_vs_fp64_one_at_a_time': sm['solves_per_s'] / ref['solves_per_s'],
+            'roofline': None, 'cpu_baseline': None,
+        }
+        print(json.dumps(line), flush=True)
+        return
     from molgw_amd.workloads import synthetic_system, synthetic_system_device
     nbf, naux, nocc, technique, desc = WORKLOADS[args.workload]
     t0 = time.perf_counter()
@@ -220,12 +315,13 @@ def main():
         rng = selfenergy_set_state_range(system.occupation)
         rng.nsemin, rng.nsemax = rng.nhomo_G - 7, rng.nhomo_G + 8
         kw = dict(rng=rng, nomega_chi_imag=32, nomega_sigma_calc=16)
-    elapsed, lat, prof, out, mem = gpu_bench(system, technique, args.steps, args.warmup, kw)
+    elapsed, lat, prof, out, mem, err_mev = gpu_bench(system, technique, args.steps, args.warmup, kw,
+                                                      args.precision)
     print(f'# gpu: {args.steps} solves in {elapsed:.3f}s; per-class device ms: '
           + ', '.join(f'{k}={v["ms"]:.1f}' for k, v in prof.items()), file=sys.stderr, flush=True)
-    kname, roof = roofline_of(prof)
+    kname, roof = roofline_of(prof, args.precision)
     cpu = None
-    if not args.no_cpu_baseline and technique == 'one_shot':
+    if not args.no_cpu_baseline and technique == 'one_shot' and hasattr(system, 'eri_3center'):
         print('# cpu baseline (stage-sampled oracle) ...', file=sys.stderr, flush=True)
         cpu = cpu_baseline(system)
         print(f'# cpu: {cpu["seconds_per_solve"]:.1f} s per solve, stages {cpu["stage_seconds"]}',
@@ -242,10 +338,12 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'f64',
+        'dtype': {'fp64': 'f64', 'fp32': 'f32 products, f64 accumulation',
+                  'bf16x3': 'bf16x3 products, f64 accumulation'}[args.precision],
+        'qp_error_vs_fp64_mev': err_mev,
         'data': 'synthetic (seeded RI tensor, orthonormal C and energies at the molecule\'s shape; '
                 'no integral engine in the image)',
-        'config': {'workload': desc, 'technique': technique, 'precision': 'fp64',
+        'config': {'workload': desc, 'technique': technique, 'precision': args.precision,
                    'nbf': nbf, 'naux': naux, 'nocc': nocc, 'nstate': nbf},
         'latency_ms': {'p50': 1e3 * _percentile(lat, 50), 'p99': 1e3 * _percentile(lat, 99)},
         'roofline': roof,

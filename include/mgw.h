@@ -85,6 +85,13 @@ const char* mgw_last_error(void);
 int mgw_ctx_create(int device, mgw_ctx** ctx);
 int mgw_ctx_destroy(mgw_ctx* ctx);
 int mgw_ctx_synchronize(mgw_ctx* ctx);
+/* Free the context's cached temporaries (everything but the resident RI
+ * tensor and MO data); with eager != 0, later stages also free their
+ * temporaries as they finish (the reference's destroy_* calls after each
+ * stage). Temporaries are cached by default because freeing device memory
+ * synchronises the whole GPU. No reference counterpart beyond
+ * clean_deallocate (src/m_memory.f90:238-274). */
+int mgw_ctx_trim(mgw_ctx* ctx, int eager);
 /* HBM ledger: replaces the clean_allocate bookkeeping (src/m_memory.f90:19-80). */
 int mgw_memory_report(mgw_ctx* ctx, int64_t* current_bytes, int64_t* peak_bytes);
 /* Bytes of HBM one solve of this shape needs (resident RI tensor included);

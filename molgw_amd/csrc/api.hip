@@ -81,6 +81,7 @@ struct mgw_ctx {
   rocblas_handle blas = nullptr;
   Ledger ledger;
   Profiler prof;
+  bool trim_eagerly = false;
   // A1: resident RI tensor, dense B(P, k, l)
   bool has_ri = false;
   int nbf = 0, naux = 0;
@@ -157,6 +158,21 @@ int release(mgw_ctx* c, const std::string& name) {
   c->ledger.bufs.erase(it);
   return MGW_OK;
 }
+
+int scratch_done(mgw_ctx* c, const std::string& name) {
+  return c->trim_eagerly ? release(c, name) : MGW_OK;
+}
+
+// Buffers that hold the caller's resident inputs survive a trim.
+bool is_resident_input(const std::string& name) {
+  return name == "ri_dense" || name == "c_matrix" || name == "energy" || name == "occupation";
+}
+
+// End of a stage's use of a temporary. Temporaries stay cached for the next
+// solve (hipFree synchronises the whole device, which would serialise
+// contexts that share the GPU); mgw_ctx_trim returns them. When the context
+// is set to trim eagerly, they are freed here.
+int scratch_done(mgw_ctx* c, const std::string& name);
 
 template <class T>
 int upload(mgw_ctx* c, const std::string& name, const T* host, size_t count, T** out) {
@@ -313,7 +329,7 @@ int build_x_ia(mgw_ctx* c, const mgw_ranges* r, double spin_fact, std::vector<in
     int32_t* ia_d = nullptr;
     MGW_TRY(upload(c, "ia_rel", rel.data(), rel.size(), &ia_d));
     MGW_TRY(launch_gather_ia(e, c->naux, nw, nw, ia_d, npole, x, c->stream));
-    MGW_TRY(release(c, "ao2mo_w"));
+    MGW_TRY(scratch_done(c, "ao2mo_w"));
   }
   *x_out = x;
   return MGW_OK;
@@ -398,7 +414,7 @@ int polarizability_casida(mgw_ctx* c, const mgw_ranges* r, double spin_fact, dou
   if (en_gm) *en_gm = 0.5 * (sum_ws2 - spin_fact * sum_n2);
   c->npole = npole;
   c->has_wpol = true;
-  MGW_TRY(release(c, "cas_m"));
+  MGW_TRY(scratch_done(c, "cas_m"));
   return MGW_OK;
 }
 
@@ -467,8 +483,8 @@ int gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta,
   c->has_sigma = true;
   c->sigma_nomega = nomega;
   c->sigma_nse = nse;
-  MGW_TRY(release(c, "sigma_w"));
-  MGW_TRY(release(c, "eri_pi"));
+  MGW_TRY(scratch_done(c, "sigma_w"));
+  MGW_TRY(scratch_done(c, "eri_pi"));
   return MGW_OK;
 }
 
@@ -552,7 +568,7 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
   c->nomega_chi = nomega;
   c->chi_omega.assign(omega_im, omega_im + nomega);
   c->chi_weight.assign(weight, weight + nomega);
-  MGW_TRY(release(c, "chi_sw"));
+  MGW_TRY(scratch_done(c, "chi_sw"));
   return MGW_OK;
 }
 
@@ -597,8 +613,8 @@ int gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int ncalc, const double*
   c->has_sigma_calc = true;
   c->sigma_ncalc = ncalc;
   c->sigma_calc_nse = nse;
-  MGW_TRY(release(c, "dot_part"));
-  MGW_TRY(release(c, "eri_pm"));
+  MGW_TRY(scratch_done(c, "dot_part"));
+  MGW_TRY(scratch_done(c, "eri_pm"));
   return MGW_OK;
 }
 
@@ -750,6 +766,22 @@ int mgw_ctx_destroy(mgw_ctx* c) {
 int mgw_ctx_synchronize(mgw_ctx* c) {
   MGW_CTX(c);
   MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
+int mgw_ctx_trim(mgw_ctx* c, int eager) {
+  MGW_CTX(c);
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  std::vector<std::string> names;
+  for (auto& kv : c->ledger.bufs)
+    if (!is_resident_input(kv.first)) names.push_back(kv.first);
+  for (auto& n : names) MGW_TRY(release(c, n));
+  c->has_eri_mo = false;
+  c->has_wpol = false;
+  c->has_chi = false;
+  c->has_sigma = false;
+  c->has_sigma_calc = false;
+  c->trim_eagerly = eager != 0;
   return MGW_OK;
 }
 

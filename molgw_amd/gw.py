@@ -151,6 +151,10 @@ class Engine:
     def synchronize(self) -> None:
         check(self._lib.mgw_ctx_synchronize(self._h))
 
+    def trim(self, eager: bool = False) -> None:
+        """Free cached temporaries (keeps the resident RI tensor and MO data)."""
+        check(self._lib.mgw_ctx_trim(self._h, 1 if eager else 0))
+
     def profile(self, enable: bool) -> None:
         """Start (clearing earlier records) or stop live per-kernel-class timing."""
         check(self._lib.mgw_profile_enable(self._h, 1 if enable else 0))

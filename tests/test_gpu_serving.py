@@ -1,0 +1,58 @@
+"""Continuous batching: solves overlap on the GPU, each result is the same
+alone and in the mix (tolerance 1e-10 Ha; the kernels are deterministic),
+and admission respects the HBM budget."""
+import numpy as np
+import pytest
+
+from molgw_amd.serving import Request, Server, summarize
+from molgw_amd.workloads import synthetic_system, water_cluster_shape, alkane_shape
+
+pytestmark = pytest.mark.gpu
+
+
+def small_requests(precision='bf16x3', technique='imaginary_pade'):
+    shapes = [alkane_shape(1), water_cluster_shape(1), alkane_shape(2), water_cluster_shape(2),
+              alkane_shape(3), water_cluster_shape(3)]
+    reqs = []
+    for k, (nbf, naux, nocc) in enumerate(shapes):
+        s = synthetic_system(nbf, naux, nocc, seed=100 + k, scale=0.3)
+        reqs.append(Request(k, s, arrival=0.002 * k, technique=technique, precision=precision,
+                            nomega_chi_imag=16, nomega_sigma_calc=8))
+    return reqs
+
+
+@pytest.mark.parametrize('technique', ['imaginary_pade', 'one_shot'])
+def test_each_result_same_alone_and_in_the_mix(technique):
+    reqs = small_requests(technique=technique)
+    with Server(nslots=3, hbm_budget=8 << 30) as srv:
+        mixed = srv.run(reqs, realtime=True)
+        # overlap happened: some solve started before another ended
+        spans = sorted((r.start, r.end) for r in mixed)
+        assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+    with Server(nslots=1, hbm_budget=8 << 30) as srv:
+        alone = [srv.run([r], realtime=False)[0] for r in reqs]
+    for a, b in zip(mixed, alone):
+        assert a.rid == b.rid
+        assert np.max(np.abs(a.energy_qp - b.energy_qp)) <= 1e-10
+        assert abs(a.en_rpa - b.en_rpa) <= 1e-10 * max(1.0, abs(b.en_rpa))
+    summary = summarize(mixed)
+    assert summary['solves'] == len(reqs) and summary['p99_ms'] >= summary['p50_ms']
+
+
+def test_admission_keeps_within_budget():
+    reqs = small_requests()
+    from molgw_amd.serving import footprint_bytes
+    biggest = max(footprint_bytes(r) for r in reqs)
+    # room for one request at a time only: solves must not overlap
+    with Server(nslots=3, hbm_budget=int(biggest * 1.2)) as srv:
+        res = srv.run(reqs, realtime=False)
+    spans = sorted((r.start, r.end) for r in res)
+    assert all(spans[i + 1][0] >= spans[i][1] for i in range(len(spans) - 1))
+    assert len(res) == len(reqs)
+
+
+def test_request_larger_than_budget_is_refused():
+    reqs = small_requests()[:1]
+    with Server(nslots=1, hbm_budget=1 << 20) as srv:
+        with pytest.raises(ValueError):
+            srv.run(reqs)

@@ -113,3 +113,20 @@ def test_fortran_interface_compiles(tmp_path):
     subprocess.run([b.FLANG, '-I', str(tmp_path), os.path.join(b.FDIR, 'mgw_fdriver.f90'), str(obj), f'-L{b.HERE}',
                     '-lmgw', f'-Wl,-rpath,{b.HERE}', '-o', str(exe)], check=True)
     assert exe.exists()
+
+
+def test_poisson_stream_and_batch_are_seeded():
+    from molgw_amd.serving import poisson_stream, alkane_batch
+    a = poisson_stream(200, rate=5.0, seed=3)
+    b = poisson_stream(200, rate=5.0, seed=3)
+    assert a == b
+    t = np.array([x[0] for x in a])
+    assert np.all(np.diff(t) > 0)
+    # mean inter-arrival 1/rate
+    assert abs(np.mean(np.diff(t)) - 0.2) < 0.05
+    kinds = {x[1] for x in a}
+    assert kinds == {'alkane', 'water'}
+    assert all(2 <= x[2] <= 24 for x in a if x[1] == 'alkane')
+    assert all(1 <= x[2] <= 32 for x in a if x[1] == 'water')
+    n = alkane_batch(64, seed=0)
+    assert len(n) == 64 and min(n) >= 1 and max(n) <= 16 and n == alkane_batch(64, seed=0)
