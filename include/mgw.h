@@ -92,6 +92,13 @@ int mgw_ctx_synchronize(mgw_ctx* ctx);
  * synchronises the whole GPU. No reference counterpart beyond
  * clean_deallocate (src/m_memory.f90:238-274). */
 int mgw_ctx_trim(mgw_ctx* ctx, int eager);
+/* GEMM tile autotuner (process-wide, no reference counterpart): mode 1 times
+ * the candidate tile configurations on the first product of each new shape
+ * class and keeps the fastest; 0 uses the table or the measured defaults;
+ * -1 only queries. The table is read from and appended to the file named by
+ * the environment variable MGW_TUNE_FILE, if set. `previous` (nullable)
+ * receives the mode before the call. */
+int mgw_autotune(int mode, int* previous);
 /* HBM ledger: replaces the clean_allocate bookkeeping (src/m_memory.f90:19-80). */
 int mgw_memory_report(mgw_ctx* ctx, int64_t* current_bytes, int64_t* peak_bytes);
 /* Bytes of HBM one solve of this shape needs (resident RI tensor included);

@@ -42,6 +42,7 @@ SIGNATURES = {
     'mgw_ctx_destroy': (C.c_int, [_vp]),
     'mgw_ctx_synchronize': (C.c_int, [_vp]),
     'mgw_ctx_trim': (C.c_int, [_vp, C.c_int]),
+    'mgw_autotune': (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
     'mgw_memory_report': (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'mgw_footprint_bytes': (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     'mgw_set_eri_3center': (C.c_int, [_vp, _vp, C.c_int64, C.c_int, _ip, C.c_int, C.c_int]),

@@ -769,6 +769,13 @@ int mgw_ctx_synchronize(mgw_ctx* c) {
   return MGW_OK;
 }
 
+int mgw_autotune(int mode, int* previous) {
+  MGW_REQUIRE(mode >= -1 && mode <= 1, MGW_ERR_ARG, "mgw_autotune: mode must be -1, 0 or 1");
+  const int old = gemm_autotune_mode(mode);
+  if (previous) *previous = old;
+  return MGW_OK;
+}
+
 int mgw_ctx_trim(mgw_ctx* c, int eager) {
   MGW_CTX(c);
   MGW_HIP(hipStreamSynchronize(c->stream));

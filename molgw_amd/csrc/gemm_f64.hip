@@ -22,6 +22,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 
 namespace mgw {
 namespace {
@@ -305,6 +311,109 @@ int launch_cfg(const GemmF64& h, hipStream_t stream) {
 
 int gemm_f64_rowblocks(int M) { return (M + 127) / 128; }   // row tile of the dot epilogue
 
+// ------------------------------------------------------------------ autotuner
+// Candidate tile configurations (all FP64 MFMA, BK = 16):
+//   0: 256x128, 16 waves   1: 128x128, 8 waves   2: 128x128, 4 waves   3: 64x64, 4 waves
+// The defaults were measured on MI355X (0 for large products, 1 for the
+// long-K symmetric update). With tuning on, the first product of a new
+// (class, transposes, lower, log2 M/N/K bucket) times every candidate on
+// the actual operands and keeps the fastest; the table can be persisted.
+namespace {
+
+constexpr int NCFG = 4;
+
+struct TuneKey {
+  int klass, ta, tb, lower, mb, nb, kb;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(klass, ta, tb, lower, mb, nb, kb) < std::tie(o.klass, o.ta, o.tb, o.lower, o.mb, o.nb, o.kb);
+  }
+};
+
+struct Tuner {
+  std::mutex mu;
+  std::map<TuneKey, int> table;
+  int mode = 0;           // 0 off, 1 tune unseen shapes
+  std::string file;       // persisted table (appended as entries are found)
+  bool loaded = false;
+};
+
+Tuner& tuner() {
+  static Tuner t;
+  return t;
+}
+
+int log2_bucket(int64_t v) {
+  int b = 0;
+  while (v > 1) { v >>= 1; ++b; }
+  return b;
+}
+
+void load_table(Tuner& t) {
+  if (t.loaded) return;
+  t.loaded = true;
+  const char* f = std::getenv("MGW_TUNE_FILE");
+  if (!f || !*f) return;
+  t.file = f;
+  FILE* fp = std::fopen(f, "r");
+  if (!fp) return;
+  TuneKey k;
+  int cfg;
+  while (std::fscanf(fp, "%d %d %d %d %d %d %d %d", &k.klass, &k.ta, &k.tb, &k.lower, &k.mb, &k.nb, &k.kb, &cfg) == 8)
+    if (cfg >= 0 && cfg < NCFG) t.table[k] = cfg;
+  std::fclose(fp);
+}
+
+template <int KLASS>
+int launch_candidate(int cfg, const GemmF64& h, hipStream_t stream) {
+  switch (cfg) {
+    case 0: return launch_cfg<256, 128, 4, 4, 16, false, KLASS>(h, stream);
+    case 1: return launch_cfg<128, 128, 4, 2, 16, false, KLASS>(h, stream);
+    case 2: return launch_cfg<128, 128, 2, 2, 16, false, KLASS>(h, stream);
+    default: return launch_cfg<64, 64, 2, 2, 16, false, KLASS>(h, stream);
+  }
+}
+
+int launch_by_class(int cfg, const GemmF64& h, hipStream_t stream) {
+  switch (h.klass) {
+    case MGW_KCLASS_GEMM_AO2MO: return launch_candidate<MGW_KCLASS_GEMM_AO2MO>(cfg, h, stream);
+    case MGW_KCLASS_GEMM_CASIDA: return launch_candidate<MGW_KCLASS_GEMM_CASIDA>(cfg, h, stream);
+    case MGW_KCLASS_GEMM_SIGMA: return launch_candidate<MGW_KCLASS_GEMM_SIGMA>(cfg, h, stream);
+    case MGW_KCLASS_GEMM_CHI0: return launch_candidate<MGW_KCLASS_GEMM_CHI0>(cfg, h, stream);
+    default: return launch_candidate<MGW_KCLASS_GEMM_OTHER>(cfg, h, stream);
+  }
+}
+
+int tune(const GemmF64& h, hipStream_t stream, int* best) {
+  hipEvent_t a, b;
+  MGW_HIP(hipEventCreate(&a));
+  MGW_HIP(hipEventCreate(&b));
+  float best_ms = 1e30f;
+  *best = -1;
+  for (int cfg = 0; cfg < NCFG; ++cfg) {
+    MGW_HIP(hipEventRecord(a, stream));
+    MGW_TRY(launch_by_class(cfg, h, stream));
+    MGW_HIP(hipEventRecord(b, stream));
+    MGW_HIP(hipEventSynchronize(b));
+    float ms = 0.0f;
+    MGW_HIP(hipEventElapsedTime(&ms, a, b));
+    if (ms < best_ms) { best_ms = ms; *best = cfg; }
+  }
+  MGW_HIP(hipEventDestroy(a));
+  MGW_HIP(hipEventDestroy(b));
+  return MGW_OK;
+}
+
+}  // namespace
+
+int gemm_autotune_mode(int mode) {
+  Tuner& t = tuner();
+  std::lock_guard<std::mutex> lk(t.mu);
+  load_table(t);
+  const int old = t.mode;
+  if (mode >= 0) t.mode = mode;
+  return old;
+}
+
 int gemm_f64(const GemmF64& h, hipStream_t stream) {
   if (h.M <= 0 || h.N <= 0 || h.batch <= 0) return MGW_OK;
   if (h.precision != MGW_FP64) return gemm_lowp(h, h.precision, stream);
@@ -313,21 +422,38 @@ int gemm_f64(const GemmF64& h, hipStream_t stream) {
   MGW_REQUIRE(h.dotpart == nullptr || (!h.lower_only && h.dotB), MGW_ERR_ARG,
               "gemm_f64: dot epilogue needs dotB and a full tile grid");
   MGW_REQUIRE(h.dotpart != nullptr || h.C != nullptr, MGW_ERR_ARG, "gemm_f64: no output");
-  // Tile shapes measured on MI355X (FP64 MFMA): 256x128 with 16 waves for the
-  // large products (57-62 TF/s), 128x128 with 8 waves for the long-K symmetric
-  // chi0 update (latency-bound on panels streamed from HBM), 256x32 for skinny N.
   if (h.dotpart) return launch_cfg<128, 128, 4, 2, 16, true, MGW_KCLASS_GEMM_WB>(h, stream);
+  // Skinny N: a 256 x 32 tile keeps the matrix cores busy when N is small.
   if (h.N <= 32 && !h.lower_only) return launch_cfg<256, 32, 4, 1, 16, false, MGW_KCLASS_GEMM_AO2MO>(h, stream);
-  if (h.lower_only) {
-    if (h.klass == MGW_KCLASS_GEMM_CHI0) return launch_cfg<128, 128, 4, 2, 16, false, MGW_KCLASS_GEMM_CHI0>(h, stream);
-    return launch_cfg<128, 128, 4, 2, 16, false, MGW_KCLASS_GEMM_CASIDA>(h, stream);
+  // Defaults measured on MI355X: 256x128 with 16 waves for the large products
+  // (57-62 TF/s), 128x128 with 8 waves for the long-K symmetric updates.
+  int cfg = h.lower_only ? 1 : 0;
+  Tuner& t = tuner();
+  const TuneKey key{h.klass, h.transA ? 1 : 0, h.transB ? 1 : 0, h.lower_only ? 1 : 0,
+                    log2_bucket(h.M), log2_bucket(h.N), log2_bucket((int64_t)h.K * h.batch)};
+  bool need_tune = false;
+  {
+    std::lock_guard<std::mutex> lk(t.mu);
+    load_table(t);
+    auto it = t.table.find(key);
+    if (it != t.table.end()) cfg = it->second;
+    else need_tune = t.mode == 1 && h.beta == 0.0;   // candidates overwrite C: only pure products
   }
-  switch (h.klass) {
-    case MGW_KCLASS_GEMM_AO2MO: return launch_cfg<256, 128, 4, 4, 16, false, MGW_KCLASS_GEMM_AO2MO>(h, stream);
-    case MGW_KCLASS_GEMM_CASIDA: return launch_cfg<256, 128, 4, 4, 16, false, MGW_KCLASS_GEMM_CASIDA>(h, stream);
-    case MGW_KCLASS_GEMM_SIGMA: return launch_cfg<256, 128, 4, 4, 16, false, MGW_KCLASS_GEMM_SIGMA>(h, stream);
-    default: return launch_cfg<256, 128, 4, 4, 16, false, MGW_KCLASS_GEMM_OTHER>(h, stream);
+  if (need_tune) {
+    int best = cfg;
+    MGW_TRY(tune(h, stream, &best));
+    std::lock_guard<std::mutex> lk(t.mu);
+    t.table[key] = best;
+    cfg = best;
+    if (!t.file.empty()) {
+      if (FILE* fp = std::fopen(t.file.c_str(), "a")) {
+        std::fprintf(fp, "%d %d %d %d %d %d %d %d\n", key.klass, key.ta, key.tb, key.lower, key.mb, key.nb, key.kb,
+                     best);
+        std::fclose(fp);
+      }
+    }
   }
+  return launch_by_class(cfg, h, stream);
 }
 
 }  // namespace mgw

@@ -120,6 +120,7 @@ struct GemmF64 {
 int gemm_f64(const GemmF64& g, hipStream_t stream);
 int gemm_lowp(const GemmF64& g, int precision, hipStream_t stream);
 int gemm_f64_rowblocks(int M);   // number of row blocks of the dot epilogue
+int gemm_autotune_mode(int mode); // -1 query; returns the previous mode
 
 // ---------------------------------------------------------------- kernels
 int launch_unpack_ri(const double* packed, int64_t npair, int naux, const int32_t* kl,

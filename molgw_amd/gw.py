@@ -98,6 +98,12 @@ def selfenergy_set_state_range(occupation, ncoreg=0, nvirtualg=100000, ncorew=0,
                        min(nvirtualw, nstate + 1))
 
 
+def autotune(enable: bool = True) -> None:
+    """Turn the GEMM tile autotuner on (tune each new shape class on first use)
+    or off (use the tuned table / measured defaults). Process-wide."""
+    check(_lib.load().mgw_autotune(1 if enable else 0, None))
+
+
 def imaginary_quad_grid(nomega: int):
     """Gauss-Legendre grid of chi on the imaginary axis (alpha=1, beta=2):
     src/m_spectral_function.f90:286-306, nodes from numpy's leggauss."""
