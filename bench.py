@@ -86,8 +86,16 @@ def stream_bench(workload, args):
     print(f'# fp64 one at a time: {ref_sum}', file=sys.stderr, flush=True)
     with Server(nslots=args.slots, hbm_budget=budget) as srv:
         srv.run(requests(args.precision)[:args.slots], realtime=False)  # warm-up
+        srv.profile(True)
         res = srv.run(requests(args.precision), realtime=realtime)
+        prof = srv.profile_read()
+        srv.profile(False)
+        esc = srv.escalations()
     sm = summarize(res)
+    sm['profile'] = prof
+    sm['escalations'] = esc
+    print('# serving per-class device ms (summed over slots): '
+          + ', '.join(f'{k}={v["ms"]:.1f}' for k, v in prof.items()), file=sys.stderr, flush=True)
     err = 0.0
     for a, b in zip(res, ref):
         sl = slice(a.nsemin - 1, a.nsemax)
@@ -140,15 +148,16 @@ def gpu_bench(system, technique, steps, warmup, kw, precision='fp64'):
 
 
 LOWP_CLASSES = ('gemm_ao2mo', 'gemm_chi0')
-PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0 / 3.0}
+PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0 / 3.0, 'bf16x6': 2500.0 / 6.0}
 
 
 def roofline_of(prof, precision='fp64'):
     """Dominant hand-written kernel class by device time (library solver calls excluded).
 
     achieved = algorithmic flops per launch / average launch time. For the
-    BF16x3 products the ceiling is the published dense BF16 MFMA peak
-    (2.5 PFLOP/s) divided by the 3 products each algorithmic product costs."""
+    BF16x3 (BF16x6) products the ceiling is the published dense BF16 MFMA
+    peak (2.5 PFLOP/s) divided by the 3 (6) products each algorithmic product
+    costs; BF16x6 keeps the transforms in FP64."""
     own = {k: v for k, v in prof.items() if k not in ('eigensolver', 'cholesky')}
     if not own:
         return None, None
@@ -160,6 +169,8 @@ def roofline_of(prof, precision='fp64'):
         r = {'bound': 'valu', 'achieved': achieved, 'peak': VALU_F64_PEAK_TFLOPS, 'unit': 'TFLOP/s'}
     else:
         prec = precision if name in LOWP_CLASSES else 'fp64'
+        if prec == 'bf16x6' and name != 'gemm_chi0':
+            prec = 'fp64'
         r = {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_TFLOPS[prec], 'unit': 'TFLOP/s',
              'arith': prec}
     r['frac'] = r['achieved'] / r['peak']
@@ -266,7 +277,7 @@ def main():
     ap.add_argument('--slots', type=int, default=8, help='concurrent solve slots (serving, batch64)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--seed', type=int, default=0)
-    ap.add_argument('--precision', default='fp64', choices=['fp64', 'fp32', 'bf16x3'],
+    ap.add_argument('--precision', default='fp64', choices=['fp64', 'fp32', 'bf16x3', 'bf16x6'],
                     help='arithmetic of the AO->MO transforms and the chi0/Casida build')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
@@ -296,7 +307,9 @@ def main():
             'fp64_one_at_a_time': {'solves_per_s': ref['solves_per_s'], 'p50_ms': ref['p50_ms'],
                                    'p99_ms': ref['p99_ms']},
             'speedup_vs_fp64_one_at_a_time': sm['solves_per_s'] / ref['solves_per_s'],
-            'roofline': None, 'cpu_baseline': None,
+            'roofline': roofline_of(sm['profile'], prec)[1], 'cpu_baseline': None,
+            'stages_device_ms_summed_over_slots': {k: v['ms'] for k, v in sm['profile'].items()},
+            'fp64_rebuilt_frequencies': sm['escalations'],
         }
         print(json.dumps(line), flush=True)
         return
@@ -339,7 +352,8 @@ def main():
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': {'fp64': 'f64', 'fp32': 'f32 products, f64 accumulation',
-                  'bf16x3': 'bf16x3 products, f64 accumulation'}[args.precision],
+                  'bf16x3': 'bf16x3 products, f64 accumulation',
+                  'bf16x6': 'bf16x6 products (chi0), f64 transforms, f64 accumulation'}[args.precision],
         'qp_error_vs_fp64_mev': err_mev,
         'data': 'synthetic (seeded RI tensor, orthonormal C and energies at the molecule\'s shape; '
                 'no integral engine in the image)',

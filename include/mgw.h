@@ -43,7 +43,14 @@ enum {
 
 enum { MGW_MEM_HOST = 0, MGW_MEM_DEVICE = 1 };
 
-enum { MGW_FP64 = 0, MGW_FP32 = 1, MGW_BF16X3 = 2 };
+/* Arithmetic of the transforms and of the chi0 / Casida-matrix build
+ * (the metric, the (1 - chi0) factorisation, the eigensolve and the QP
+ * solve are FP64 in every mode):
+ *   MGW_FP32   FP32 products on MFMA, FP64 accumulation
+ *   MGW_BF16X3 two-way bf16 split, 3 products (~2^-17 relative)
+ *   MGW_BF16X6 three-way bf16 split for chi0, 6 products (~2^-24, FP32
+ *              level); the transforms stay FP64                           */
+enum { MGW_FP64 = 0, MGW_FP32 = 1, MGW_BF16X3 = 2, MGW_BF16X6 = 3 };
 
 enum { MGW_TECH_ONE_SHOT = 0, MGW_TECH_IMAGINARY_PADE = 1 };
 
@@ -73,9 +80,9 @@ typedef struct mgw_g0w0_params {
   int32_t nomega_sigma_calc;  /* imaginary sigma points (Pade technique only) */
   double  step_sigma_calc;    /* Ha (Pade technique only)                     */
   double  spin_fact;          /* 2 for restricted closed shell                */
-  int32_t precision;          /* MGW_FP64 | MGW_FP32 | MGW_BF16X3 (transforms
-                                 and chi0; metric/inversion/eigensolve and the
-                                 QP solve always FP64)                        */
+  int32_t precision;          /* MGW_FP64 | MGW_FP32 | MGW_BF16X3 | MGW_BF16X6
+                                 (transforms and chi0; metric/inversion/
+                                 eigensolve and the QP solve always FP64)     */
   int32_t reserved;
 } mgw_g0w0_params;
 
@@ -101,6 +108,11 @@ int mgw_ctx_trim(mgw_ctx* ctx, int eager);
 int mgw_autotune(int mode, int* previous);
 /* HBM ledger: replaces the clean_allocate bookkeeping (src/m_memory.f90:19-80). */
 int mgw_memory_report(mgw_ctx* ctx, int64_t* current_bytes, int64_t* peak_bytes);
+/* Frequencies whose reduced-precision 1 - chi0 lost positive definiteness
+ * in the Cholesky factorisation and were rebuilt in FP64 on the device,
+ * counted since the context was created. No reference counterpart: the
+ * reference has no reduced-precision build. */
+int mgw_precision_escalations(mgw_ctx* ctx, int64_t* count);
 /* Bytes of HBM one solve of this shape needs (resident RI tensor included);
  * seeded by the reference's evaluate_memory formulas (src/m_io.f90:3843-3936). */
 int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int technique,

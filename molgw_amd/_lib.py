@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libmgw.so')
 
 MGW_OK = 0
 MGW_MEM_HOST, MGW_MEM_DEVICE = 0, 1
-MGW_FP64, MGW_FP32, MGW_BF16X3 = 0, 1, 2
+MGW_FP64, MGW_FP32, MGW_BF16X3, MGW_BF16X6 = 0, 1, 2, 3
 MGW_TECH_ONE_SHOT, MGW_TECH_IMAGINARY_PADE = 0, 1
 
 
@@ -44,6 +44,7 @@ SIGNATURES = {
     'mgw_ctx_trim': (C.c_int, [_vp, C.c_int]),
     'mgw_autotune': (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
     'mgw_memory_report': (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    'mgw_precision_escalations': (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     'mgw_footprint_bytes': (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     'mgw_set_eri_3center': (C.c_int, [_vp, _vp, C.c_int64, C.c_int, _ip, C.c_int, C.c_int]),
     'mgw_destroy_eri_3center': (C.c_int, [_vp]),

@@ -103,6 +103,8 @@ struct mgw_ctx {
   int sigma_nomega = 0, sigma_nse = 0;
   bool has_sigma_calc = false;
   int sigma_ncalc = 0, sigma_calc_nse = 0;
+  // frequencies of a reduced-precision chi0 rebuilt in FP64
+  int64_t escalations = 0;
 };
 
 namespace {
@@ -526,11 +528,27 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
   g.lower_only = true;
   g.klass = MGW_KCLASS_GEMM_CHI0;
   g.precision = precision;
+  // the split-operand kernels factor s = sqrt(s) sqrt(s): every weight must be >= 0
+  bool nonneg = true;
+  for (int t = 0; t < npole; ++t) {
+    const int i = ia[2 * t], a = ia[2 * t + 1];
+    if ((c->occupation[i] - c->occupation[a]) * (c->energy[a] - c->energy[i]) < 0.0) nonneg = false;
+  }
+  if ((precision == MGW_BF16X3 || precision == MGW_BF16X6) && !nonneg)
+    g.precision = MGW_FP32;   // per-element split kernels take signed weights at FP32
   {
     ProfScope ps(&c->prof, MGW_KCLASS_GEMM_CHI0, (double)naux * (naux + 1.0) * npole * nomega,
                  8.0 * ((double)naux * npole + 0.5 * (double)naux * naux * nomega), c->stream);
-    MGW_TRY(gemm_f64(g, c->stream));
+    if ((precision == MGW_BF16X3 || precision == MGW_BF16X6) && nonneg) {
+      const size_t bytes = chi0_split_workspace(naux, npole, nomega, precision);
+      double* work = nullptr;
+      MGW_TRY(dbuf(c, "chi_split", (bytes + 7) / 8, &work));
+      MGW_TRY(chi0_split_syrk(x, naux, npole, sw, nomega, precision, work, bytes, chi, c->stream));
+    } else {
+      MGW_TRY(gemm_f64(g, c->stream));
+    }
   }
+  if (precision != MGW_FP64) MGW_TRY(scratch_done(c, "chi_split"));
   int32_t* info_d = nullptr;
   MGW_TRY(dbuf(c, "solver_info", std::max(64, nomega), &info_d));
   ProfScope chol(&c->prof, MGW_KCLASS_CHOLESKY, (double)naux * naux * naux * nomega,
@@ -539,8 +557,37 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
                                             (rocblas_stride)naux * naux, info_d, nomega));
   std::vector<int32_t> info(nomega);
   MGW_TRY(download(c, info.data(), info_d, nomega));
-  for (int k = 0; k < nomega; ++k)
-    MGW_REQUIRE(info[k] == 0, MGW_ERR_SOLVER, "Cholesky of 1-chi0 failed at frequency " + std::to_string(k));
+  bool checked_x = false;
+  for (int k = 0; k < nomega; ++k) {
+    if (info[k] == 0) continue;
+    MGW_REQUIRE(precision != MGW_FP64, MGW_ERR_SOLVER,
+                "Cholesky of 1-chi0 failed at frequency " + std::to_string(k) + " (info " +
+                    std::to_string(info[k]) + ")");
+    if (!checked_x) {
+      // 1 + X diag(s) X^T with s >= 0 is positive definite for any finite X
+      std::vector<double> h(npole);
+      MGW_TRY(download(c, h.data(), n2, npole));
+      for (int t = 0; t < npole; ++t)
+        MGW_REQUIRE(std::isfinite(h[t]), MGW_ERR_SOLVER,
+                    "non-finite (P|ia) column " + std::to_string(t) + " before the chi0 build");
+      checked_x = true;
+    }
+    // a reduced-precision 1 - chi0 that is not positive definite: rebuild
+    // this frequency in FP64 and factor it again
+    GemmF64 g1 = g;
+    g1.precision = MGW_FP64;
+    g1.batch = 1;
+    g1.kscale = sw + (int64_t)npole * k;
+    g1.C = chi + (int64_t)naux * naux * k;
+    MGW_TRY(gemm_f64(g1, c->stream));
+    MGW_BLAS(rocsolver_dpotrf(c->blas, rocblas_fill_lower, naux, g1.C, naux, info_d));
+    int32_t one = 0;
+    MGW_TRY(download(c, &one, info_d, 1));
+    MGW_REQUIRE(one == 0, MGW_ERR_SOLVER,
+                "Cholesky of the FP64 rebuild of 1-chi0 failed at frequency " + std::to_string(k) + " (info " +
+                    std::to_string(one) + "; reduced-precision info " + std::to_string(info[k]) + ")");
+    ++c->escalations;
+  }
   MGW_TRY(launch_chi_trace(nullptr, nullptr, 0, chi, naux, 1, nomega, trl, c->stream));
   MGW_BLAS(rocsolver_dpotri_strided_batched(c->blas, rocblas_fill_lower, naux, chi, naux,
                                             (rocblas_stride)naux * naux, info_d, nomega));
@@ -823,6 +870,12 @@ int mgw_profile_read(mgw_ctx* c, int kernel_class, int64_t* launches, double* to
   return MGW_OK;
 }
 
+int mgw_precision_escalations(mgw_ctx* c, int64_t* count) {
+  MGW_REQUIRE(c && count, MGW_ERR_ARG, "mgw_precision_escalations: null argument");
+  *count = c->escalations;
+  return MGW_OK;
+}
+
 int mgw_memory_report(mgw_ctx* c, int64_t* current_bytes, int64_t* peak_bytes) {
   MGW_REQUIRE(c, MGW_ERR_ARG, "null context");
   if (current_bytes) *current_bytes = c->ledger.current;
@@ -832,7 +885,6 @@ int mgw_memory_report(mgw_ctx* c, int64_t* current_bytes, int64_t* peak_bytes) {
 
 int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int technique,
                             int nomega_chi_imag, int precision) {
-  (void)precision;
   const int64_t nb = nbf, na = nauxil, ns = nstate, no = nocc, nv = nstate - nocc;
   const int64_t npole = no * nv;
   int64_t d = na * nb * nb;                       // resident dense RI tensor
@@ -844,6 +896,8 @@ int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int techn
               + na * npole;                       // w_s
   } else {
     stage_w = na * nb * no + na * npole + na * na * (int64_t)nomega_chi_imag + npole * nomega_chi_imag;
+    if (precision == MGW_BF16X3 || precision == MGW_BF16X6)
+      stage_w += (int64_t)(mgw::chi0_split_workspace((int)na, (int)npole, nomega_chi_imag, precision) / 8);
   }
   const int64_t stage_s = na * nb * ns + na * ns * ns + na * npole + (int64_t(1) << 27);
   return 8 * (d + std::max(stage_w, stage_s));
@@ -965,7 +1019,7 @@ int mgw_polarizability(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int pr
                        double* en_rpa, double* en_gw, double* pole_out, double* w_s_out) {
   MGW_CTX(c);
   MGW_TRY(check_ranges(c, r));
-  MGW_REQUIRE(precision == MGW_FP64 || precision == MGW_FP32 || precision == MGW_BF16X3, MGW_ERR_ARG,
+  MGW_REQUIRE(precision >= MGW_FP64 && precision <= MGW_BF16X6, MGW_ERR_ARG,
               "mgw_polarizability: unknown precision");
   MGW_TRY(polarizability_casida(c, r, spin_fact, en_rpa, en_gw, precision));
   if (pole_out) MGW_TRY(download(c, pole_out, find_buf(c, "wpol_pole"), c->npole));
@@ -979,7 +1033,7 @@ int mgw_polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, i
                             double* erpa, double* egw, double* chi_out) {
   MGW_CTX(c);
   MGW_TRY(check_ranges(c, r));
-  MGW_REQUIRE(precision == MGW_FP64 || precision == MGW_FP32 || precision == MGW_BF16X3, MGW_ERR_ARG,
+  MGW_REQUIRE(precision >= MGW_FP64 && precision <= MGW_BF16X6, MGW_ERR_ARG,
               "mgw_polarizability_grid: unknown precision");
   MGW_REQUIRE(omega_im && weight_quad, MGW_ERR_ARG, "null frequency grid");
   MGW_TRY(polarizability_grid(c, r, spin_fact, nomega, omega_im, weight_quad, erpa, egw, precision));
@@ -1041,7 +1095,7 @@ int mgw_g0w0(mgw_ctx* c, const mgw_g0w0_params* prm, const mgw_ranges* r, const 
   MGW_REQUIRE(prm, MGW_ERR_ARG, "null parameters");
   MGW_TRY(check_ranges(c, r));
   const int prec = prm->precision;
-  MGW_REQUIRE(prec == MGW_FP64 || prec == MGW_FP32 || prec == MGW_BF16X3, MGW_ERR_ARG, "mgw_g0w0: unknown precision");
+  MGW_REQUIRE(prec >= MGW_FP64 && prec <= MGW_BF16X6, MGW_ERR_ARG, "mgw_g0w0: unknown precision");
   MGW_REQUIRE(prm->nomega_sigma >= 1, MGW_ERR_ARG, "nomega_sigma must be positive");
   const int nstate = c->nstate;
   const int nse = r->nsemax - r->nsemin + 1;

@@ -1,0 +1,327 @@
+// Reduced-precision 1 - chi0(iw) from pre-split operands.
+//
+// For every imaginary frequency w_f of the grid
+//   1 - chi0(iw_f) = I + Y_f Y_f^T,   Y_f = X diag(sqrt(s_f)),
+//   s_f(t) = 2 (f_i - f_a)(e_a - e_i) / (w_f^2 + (e_a - e_i)^2)   (>= 0; the caller checks)
+// (src/m_gw_selfenergy_grid.f90:110-150 forms X diag(s) X^T with DSYRK in FP64).
+//
+// A pre-pass writes Y_f once per frequency, split into NS bf16 planes
+// y = y0 + y1 (+ y2), with y0 = bf16(y), y1 = bf16(y - y0), y2 = bf16(y - y0 - y1).
+// Each plane is k-tiled, plane[f][kt][P][32] (P padded to a multiple of 128,
+// t padded to a multiple of 32, zeros in the padding), so the K slab of a
+// row block is one contiguous run and a lane's 8 consecutive k are one
+// 16-byte load. The SYRK kernel multiplies the planes on
+// v_mfma_f32_16x16x32_bf16:
+//   NS = 2 (BF16x3): y0 y0 + y0 y1 + y1 y0                  (~2^-17 relative per product)
+//   NS = 3 (BF16x6): + y1 y1 + y0 y2 + y2 y0                (~2^-24, FP32 level)
+// FP32 partials are flushed into FP64 accumulators every FLUSH slabs of 32,
+// so sums over K ~ 10^5 transitions keep FP64 accumulation error.
+//
+// Fragment layout (gfx950, v_mfma_f32_16x16x32_bf16): a[i = l & 15][k = 8 (l >> 4) + e],
+// b[k = 8 (l >> 4) + e][j = l & 15], d: lane l, register r -> d[4 (l >> 4) + r][l & 15].
+// The kernel feeds a = rows of the n block, b = rows of the m block, so lane
+// l, register r holds C(m = l & 15, n = 4 (l >> 4) + r) of the 16 x 16 tile.
+#include "mgw_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+
+namespace mgw {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+constexpr int NXCD = 8;
+constexpr int KT = 32;          // k per tile of the plane layout (one MFMA step)
+constexpr int PAD_ROWS = 128;   // P padding of the plane layout
+
+__device__ inline int64_t xcd_logical(int64_t w, int64_t G) {
+  const int64_t q = G / NXCD, r = G % NXCD;
+  const int64_t x = w % NXCD, i = w / NXCD;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+__device__ inline void tri_index(int t, int& bm, int& bn) {
+  int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((r + 1) * (r + 2) / 2 <= t) ++r;
+  while (r * (r + 1) / 2 > t) --r;
+  bm = r;
+  bn = t - r * (r + 1) / 2;
+}
+
+// ---------------------------------------------------------------- split
+// Block: 64 rows P x 32 transitions t (one k tile), all nf frequencies of
+// the chunk. X is read once through LDS (coalesced along P), each thread
+// then writes 8 consecutive t of one row per plane (one 16-byte store).
+template <int NS>
+__global__ void __launch_bounds__(256)
+chi_split_kernel(const double* __restrict__ x, int naux, int npole, const double* __restrict__ sw,
+                 int f0, int nf, int ppad, int nkt, __bf16* __restrict__ y, int64_t plane) {
+  __shared__ double tile[KT][65];
+  const int tid = threadIdx.x;
+  const int p0 = blockIdx.x * 64, kt = blockIdx.y, t0 = kt * KT;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int tl = (tid >> 6) + 4 * j, p = tid & 63;
+    const int P = p0 + p, t = t0 + tl;
+    tile[tl][p] = (P < naux && t < npole) ? x[P + (int64_t)t * naux] : 0.0;
+  }
+  __syncthreads();
+  const int p = tid >> 2, c8 = tid & 3;
+  const int64_t row = (int64_t)(p0 + p) * KT + 8 * c8;
+  for (int f = 0; f < nf; ++f) {
+    const double* __restrict__ s = sw + (int64_t)(f0 + f) * npole;
+    bf8 out[NS];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int t = t0 + 8 * c8 + e;
+      double v = (t < npole) ? tile[8 * c8 + e][p] * sqrt(s[t]) : 0.0;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const __bf16 h = (__bf16)(float)v;
+        out[q][e] = h;
+        v -= (double)(float)h;
+      }
+    }
+    const int64_t off = ((int64_t)f * nkt + kt) * (int64_t)ppad * KT + row;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) *reinterpret_cast<bf8*>(y + q * plane + off) = out[q];
+  }
+}
+
+// ---------------------------------------------------------------- SYRK
+struct SyrkArgs {
+  const __bf16* y;
+  int64_t plane;       // elements between planes
+  int64_t fstride;     // elements between frequencies
+  int ppad, nkt, naux, nf, tiles;
+  double* C;           // chi + f0 * naux^2
+  int64_t strideC;
+  int64_t nblocks;
+};
+
+// Lower-triangle tiles BM x BM, WM x WN waves, one 32-k slab per step.
+// DEPTH register sets prefetch the next DEPTH slabs from global memory while
+// the MFMAs consume the LDS copy of the current one, so each load has DEPTH
+// slabs of MFMA work to arrive.
+template <int NS, int BM, int WM, int WN, int FLUSH, int DEPTH>
+__global__ void __launch_bounds__(64 * WM * WN)
+chi_syrk_kernel(SyrkArgs g) {
+  constexpr int KP = KT + 8;              // padded LDS row: 80 bytes
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = BM / WM, TN = BM / WN;
+  constexpr int FM = TM / 16, FN = TN / 16;
+  constexpr int CH = BM * KT / 8;         // 16-byte chunks per plane per operand slab
+  constexpr int LPT = CH / NT;
+  static_assert(CH % NT == 0, "chunks per thread");
+  static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile");
+  static_assert(DEPTH == 1 || DEPTH == 2, "prefetch depth");
+
+  __shared__ __attribute__((aligned(16))) __bf16 As[NS][BM][KP];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[NS][BM][KP];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int64_t L = xcd_logical(blockIdx.x, g.nblocks);
+  const int f = (int)(L / g.tiles);
+  int bm, bn;
+  tri_index((int)(L % g.tiles), bm, bn);
+  const int m0 = bm * BM, n0 = bn * BM;
+
+  const __bf16* __restrict__ Ya = g.y + (int64_t)f * g.fstride + (int64_t)m0 * KT;
+  const __bf16* __restrict__ Yb = g.y + (int64_t)f * g.fstride + (int64_t)n0 * KT;
+  const int64_t slab = (int64_t)g.ppad * KT;
+
+  // prefetch registers (native vectors, so they stay in VGPRs)
+  u4 ra0[NS][LPT], rb0[NS][LPT], ra1[NS][LPT], rb1[NS][LPT];
+#define MGW_SYRK_LOAD(RA, RB, S_)                                                             \
+  _Pragma("unroll") for (int q = 0; q < NS; ++q)                                               \
+    _Pragma("unroll") for (int i = 0; i < LPT; ++i) {                                          \
+      const int c = tid + i * NT;                                                              \
+      RA[q][i] = *reinterpret_cast<const u4*>(Ya + q * g.plane + (int64_t)(S_) * slab + 8 * c); \
+      RB[q][i] = *reinterpret_cast<const u4*>(Yb + q * g.plane + (int64_t)(S_) * slab + 8 * c); \
+    }
+#define MGW_SYRK_STORE(RA, RB)                                             \
+  _Pragma("unroll") for (int q = 0; q < NS; ++q)                            \
+    _Pragma("unroll") for (int i = 0; i < LPT; ++i) {                       \
+      const int c = tid + i * NT;                                           \
+      *reinterpret_cast<u4*>(&As[q][c >> 2][8 * (c & 3)]) = RA[q][i];       \
+      *reinterpret_cast<u4*>(&Bs[q][c >> 2][8 * (c & 3)]) = RB[q][i];       \
+    }
+
+  double acc[FM][FN][4];
+  f4 part[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      part[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
+    }
+
+  const int fr = lane & 15, fq = lane >> 4;
+  const int nslab = g.nkt;
+  // MFMAs of the slab in LDS, then the FP64 flush every FLUSH slabs
+  auto compute = [&](int s) __attribute__((always_inline)) {
+    bf8 a[NS][FM], b[NS][FN];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i) a[q][i] = *reinterpret_cast<const bf8*>(&As[q][wm * TM + i * 16 + fr][8 * fq]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) b[q][j] = *reinterpret_cast<const bf8*>(&Bs[q][wn * TN + j * 16 + fr][8 * fq]);
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        f4 d = part[i][j];
+        if constexpr (NS == 3) {
+          d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0][j], a[2][i], d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[2][j], a[0][i], d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1][j], a[1][i], d, 0, 0, 0);
+        }
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0][j], a[1][i], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1][j], a[0][i], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0][j], a[0][i], d, 0, 0, 0);
+        part[i][j] = d;
+      }
+    if ((s + 1) % FLUSH == 0 || s + 1 == nslab) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[i][j][r] += (double)part[i][j][r];
+          part[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+  };
+
+  if constexpr (DEPTH == 1) {
+    MGW_SYRK_LOAD(ra0, rb0, 0)
+    for (int s = 0; s < nslab; ++s) {
+      __syncthreads();                   // previous slab's fragments consumed
+      MGW_SYRK_STORE(ra0, rb0)
+      __syncthreads();
+      if (s + 1 < nslab) { MGW_SYRK_LOAD(ra0, rb0, s + 1) }
+      compute(s);
+    }
+  } else {
+    MGW_SYRK_LOAD(ra0, rb0, 0)
+    if (nslab > 1) { MGW_SYRK_LOAD(ra1, rb1, 1) }
+    for (int s = 0; s < nslab; s += 2) {
+      __syncthreads();
+      MGW_SYRK_STORE(ra0, rb0)
+      __syncthreads();
+      if (s + 2 < nslab) { MGW_SYRK_LOAD(ra0, rb0, s + 2) }
+      compute(s);
+      if (s + 1 >= nslab) break;
+      __syncthreads();
+      MGW_SYRK_STORE(ra1, rb1)
+      __syncthreads();
+      if (s + 3 < nslab) { MGW_SYRK_LOAD(ra1, rb1, s + 3) }
+      compute(s + 1);
+    }
+  }
+
+  // epilogue: lower triangle plus its mirror, 1 on the diagonal
+  double* __restrict__ C = g.C + (int64_t)f * g.strideC;
+  const int n_ld = g.naux;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int m = m0 + wm * TM + i * 16 + fr;
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn * TN + j * 16 + 4 * fq + r;
+        if (m >= g.naux || n >= g.naux || m < n) continue;
+        double v = acc[i][j][r];
+        if (m == n) v += 1.0;
+        C[m + (int64_t)n * n_ld] = v;
+        if (m > n) C[n + (int64_t)m * n_ld] = v;
+      }
+  }
+}
+
+#undef MGW_SYRK_LOAD
+#undef MGW_SYRK_STORE
+
+// Prefetch depth of the SYRK (2 unless MGW_CHI_PREFETCH=1 in the environment).
+int chi_prefetch_depth() {
+  static const int depth = [] {
+    const char* v = std::getenv("MGW_CHI_PREFETCH");
+    return (v && std::atoi(v) == 1) ? 1 : 2;
+  }();
+  return depth;
+}
+
+template <int NS>
+int split_and_syrk(const double* x, int naux, int npole, const double* sw, int nomega, __bf16* work,
+                   size_t work_bytes, double* chi, hipStream_t stream) {
+  constexpr int BM = 128;
+  const int ppad = (naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
+  const int nkt = (npole + KT - 1) / KT;
+  const int64_t per_plane_f = (int64_t)nkt * ppad * KT;                 // elements
+  const size_t per_f = (size_t)per_plane_f * NS * sizeof(__bf16);
+  const int nf_max = (int)std::min<size_t>((size_t)nomega, work_bytes / per_f);
+  MGW_REQUIRE(nf_max >= 1, MGW_ERR_ARG, "chi0 split: workspace smaller than one frequency");
+  const int tm = (naux + BM - 1) / BM;
+  const int tiles = tm * (tm + 1) / 2;
+  for (int f0 = 0; f0 < nomega; f0 += nf_max) {
+    const int nf = std::min(nf_max, nomega - f0);
+    const int64_t plane = per_plane_f * nf;
+    hipLaunchKernelGGL((chi_split_kernel<NS>), dim3(ppad / 64, nkt), dim3(256), 0, stream, x, naux, npole, sw,
+                       f0, nf, ppad, nkt, work, plane);
+    MGW_HIP(hipGetLastError());
+    SyrkArgs g;
+    g.y = work; g.plane = plane; g.fstride = per_plane_f;
+    g.ppad = ppad; g.nkt = nkt; g.naux = naux; g.nf = nf; g.tiles = tiles;
+    g.C = chi + (int64_t)f0 * naux * naux; g.strideC = (int64_t)naux * naux;
+    g.nblocks = (int64_t)tiles * nf;
+    MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chi0 split: too many tiles");
+    if (chi_prefetch_depth() == 1)
+      hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 1>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
+    else
+      hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 2>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
+    MGW_HIP(hipGetLastError());
+  }
+  return MGW_OK;
+}
+
+}  // namespace
+
+size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision) {
+  const int ns = precision == MGW_BF16X6 ? 3 : 2;
+  const size_t ppad = (size_t)(naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
+  const size_t npad = (size_t)(npole + KT - 1) / KT * KT;
+  return ppad * npad * ns * sizeof(__bf16);
+}
+
+// Workspace of the split planes: all frequencies when they fit in
+// max(4 x the FP64 operand, 2 GiB), else as many as fit (at least one).
+size_t chi0_split_workspace(int naux, int npole, int nomega, int precision) {
+  const size_t per_f = chi0_split_bytes_per_frequency(naux, npole, precision);
+  const size_t budget = std::max<size_t>((size_t)32 * naux * npole, (size_t)2 << 30);
+  const size_t nf = std::max<size_t>(1, std::min<size_t>((size_t)nomega, budget / per_f));
+  return nf * per_f;
+}
+
+int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int nomega, int precision,
+                    void* work, size_t work_bytes, double* chi, hipStream_t stream) {
+  if (naux <= 0 || nomega <= 0) return MGW_OK;
+  MGW_REQUIRE(npole > 0, MGW_ERR_ARG, "chi0 split: empty transition space");
+  if (precision == MGW_BF16X3)
+    return split_and_syrk<2>(x, naux, npole, sw, nomega, static_cast<__bf16*>(work), work_bytes, chi, stream);
+  if (precision == MGW_BF16X6)
+    return split_and_syrk<3>(x, naux, npole, sw, nomega, static_cast<__bf16*>(work), work_bytes, chi, stream);
+  set_error("chi0 split: precision must be BF16X3 or BF16X6");
+  return MGW_ERR_ARG;
+}
+
+}  // namespace mgw

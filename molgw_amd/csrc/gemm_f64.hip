@@ -416,7 +416,9 @@ int gemm_autotune_mode(int mode) {
 
 int gemm_f64(const GemmF64& h, hipStream_t stream) {
   if (h.M <= 0 || h.N <= 0 || h.batch <= 0) return MGW_OK;
-  if (h.precision != MGW_FP64) return gemm_lowp(h, h.precision, stream);
+  // MGW_BF16X6 splits only the chi0 operands (chi0_split.hip); every other
+  // product of that mode stays FP64
+  if (h.precision == MGW_FP32 || h.precision == MGW_BF16X3) return gemm_lowp(h, h.precision, stream);
   MGW_REQUIRE(h.K >= 0, MGW_ERR_ARG, "gemm_f64: negative K");
   MGW_REQUIRE(!h.lower_only || h.M == h.N, MGW_ERR_ARG, "gemm_f64: lower_only needs M == N");
   MGW_REQUIRE(h.dotpart == nullptr || (!h.lower_only && h.dotB), MGW_ERR_ARG,

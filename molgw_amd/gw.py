@@ -46,7 +46,7 @@ from ._lib import G0W0Params, Ranges, check
 COMPLETELY_EMPTY = 1.0e-5   # src/m_definitions.f90:66
 HA_EV = 27.21138505         # src/m_definitions.f90:43
 
-PRECISION = {'fp64': _lib.MGW_FP64, 'fp32': _lib.MGW_FP32, 'bf16x3': _lib.MGW_BF16X3}
+PRECISION = {'fp64': _lib.MGW_FP64, 'fp32': _lib.MGW_FP32, 'bf16x3': _lib.MGW_BF16X3, 'bf16x6': _lib.MGW_BF16X6}
 
 
 def _f64(a) -> np.ndarray:
@@ -179,6 +179,13 @@ class Engine:
         cur, peak = C.c_int64(), C.c_int64()
         check(self._lib.mgw_memory_report(self._h, C.byref(cur), C.byref(peak)))
         return cur.value, peak.value
+
+    def precision_escalations(self) -> int:
+        """Frequencies whose reduced-precision 1 - chi0 was not positive
+        definite and were rebuilt in FP64 (since the engine was created)."""
+        n = C.c_int64()
+        check(self._lib.mgw_precision_escalations(self._h, C.byref(n)))
+        return n.value
 
     # ------------------------------------------------------------ A1 / MO
     def set_eri_3center(self, eri_3center, index_basis, nbf: int) -> None:
@@ -315,7 +322,7 @@ class Engine:
              precision='fp64') -> G0W0Output:
         """One-shot G0W0 (postscf='GW' for technique='one_shot', 'G0W0_PADE' for
         technique='imaginary_pade') on the resident tensors. ``precision``
-        ('fp64', 'fp32', 'bf16x3') applies to the AO->MO transforms and the
+        ('fp64', 'fp32', 'bf16x3', 'bf16x6') applies to the AO->MO transforms and the
         chi0 / Casida-matrix build; the eigensolve, the (1 - chi0) inversion,
         the self-energy products and the QP solve are FP64."""
         if rng is None:

@@ -184,6 +184,27 @@ class Server:
             results.append(res)
         return sorted(results, key=lambda r: r.rid)
 
+    # ------------------------------------------------------------ profiling
+    def profile(self, enable: bool = True) -> None:
+        """Per-kernel-class device timing on every slot (HIP events on the
+        slot's own stream); enabling resets the counters."""
+        for e in self.slots:
+            e.profile(enable)
+
+    def profile_read(self) -> dict:
+        """Per-class totals over all slots. Slots run concurrently, so the
+        summed device time can exceed the wall time."""
+        out: dict = {}
+        for e in self.slots:
+            for name, v in e.profile_read().items():
+                t = out.setdefault(name, {'launches': 0, 'ms': 0.0, 'flops': 0.0, 'bytes': 0.0})
+                for k in t:
+                    t[k] += v[k]
+        return out
+
+    def escalations(self) -> int:
+        return sum(e.precision_escalations() for e in self.slots)
+
     def close(self) -> None:
         for q in self._jobs:
             q.put(None)

@@ -130,3 +130,12 @@ def test_poisson_stream_and_batch_are_seeded():
     assert all(1 <= x[2] <= 32 for x in a if x[1] == 'water')
     n = alkane_batch(64, seed=0)
     assert len(n) == 64 and min(n) >= 1 and max(n) <= 16 and n == alkane_batch(64, seed=0)
+
+
+def test_entry_scripts_parse():
+    """bench.py and __graft_entry__.py are run as scripts on the GPU box:
+    they must at least compile."""
+    import ast
+    for name in ('bench.py', '__graft_entry__.py'):
+        with open(os.path.join(ROOT, name)) as f:
+            ast.parse(f.read(), name)
