@@ -502,15 +502,14 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
   const int naux = c->naux;
   int32_t* ia_d = nullptr;
   MGW_TRY(upload(c, "ia", ia.data(), ia.size(), &ia_d));
-  double *om_d, *sw, *n2, *ones, *tr0, *trl, *tri, *chi;
+  double *om_d, *sw, *n2, *ones, *tr0, *chi;
   MGW_TRY(upload(c, "chi_omega", omega_im, nomega, &om_d));
   MGW_TRY(dbuf(c, "chi_sw", (size_t)npole * nomega, &sw));
   MGW_TRY(dbuf(c, "cas_n2", npole, &n2));
   MGW_TRY(dbuf(c, "ones", naux, &ones));
   MGW_TRY(dbuf(c, "chi_tr0", nomega, &tr0));
-  MGW_TRY(dbuf(c, "chi_trl", nomega, &trl));
-  MGW_TRY(dbuf(c, "chi_tri", nomega, &tri));
-  MGW_TRY(dbuf(c, "wpol_chi", (size_t)naux * naux * nomega, &chi));
+  // 1 - chi0, then its inverse Cholesky factor, resident for the self-energy
+  MGW_TRY(dbuf(c, "wpol_linv", (size_t)naux * naux * nomega, &chi));
   MGW_TRY(launch_fill(ones, naux, 1.0, c->stream));
   MGW_TRY(launch_chi_weights(ia_d, npole, find_buf(c, "energy"), find_buf(c, "occupation"), om_d,
                              nomega, sw, c->stream));
@@ -549,19 +548,26 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
     }
   }
   if (precision != MGW_FP64) MGW_TRY(scratch_done(c, "chi_split"));
+  // 1 - chi0 = L L^T for every frequency, then L^-1 in place (chol.hip)
   int32_t* info_d = nullptr;
+  double *logdet, *work, *frob;
   MGW_TRY(dbuf(c, "solver_info", std::max(64, nomega), &info_d));
-  ProfScope chol(&c->prof, MGW_KCLASS_CHOLESKY, (double)naux * naux * naux * nomega,
-                 16.0 * (double)naux * naux * nomega, c->stream);
-  MGW_BLAS(rocsolver_dpotrf_strided_batched(c->blas, rocblas_fill_lower, naux, chi, naux,
-                                            (rocblas_stride)naux * naux, info_d, nomega));
+  MGW_TRY(dbuf(c, "chol_logdet", nomega, &logdet));
+  MGW_TRY(dbuf(c, "chol_work", (size_t)naux * CHOL_NB * nomega, &work));
+  MGW_TRY(dbuf(c, "chol_frob", (size_t)FROB_BLOCKS * nomega, &frob));
+  const double chol_flops = 2.0 / 3.0 * (double)naux * naux * naux;
+  {
+    ProfScope chol(&c->prof, MGW_KCLASS_CHOLESKY, chol_flops * nomega, 16.0 * (double)naux * naux * nomega,
+                   c->stream);
+    MGW_TRY(chol_inverse_batched(chi, naux, nomega, logdet, info_d, work, c->stream));
+  }
   std::vector<int32_t> info(nomega);
   MGW_TRY(download(c, info.data(), info_d, nomega));
   bool checked_x = false;
   for (int k = 0; k < nomega; ++k) {
     if (info[k] == 0) continue;
     MGW_REQUIRE(precision != MGW_FP64, MGW_ERR_SOLVER,
-                "Cholesky of 1-chi0 failed at frequency " + std::to_string(k) + " (info " +
+                "Cholesky of 1-chi0 failed at frequency " + std::to_string(k) + " (column " +
                     std::to_string(info[k]) + ")");
     if (!checked_x) {
       // 1 + X diag(s) X^T with s >= 0 is positive definite for any finite X
@@ -580,34 +586,29 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
     g1.kscale = sw + (int64_t)npole * k;
     g1.C = chi + (int64_t)naux * naux * k;
     MGW_TRY(gemm_f64(g1, c->stream));
-    MGW_BLAS(rocsolver_dpotrf(c->blas, rocblas_fill_lower, naux, g1.C, naux, info_d));
+    MGW_TRY(chol_inverse_batched(g1.C, naux, 1, logdet + k, info_d + k, work, c->stream));
     int32_t one = 0;
-    MGW_TRY(download(c, &one, info_d, 1));
+    MGW_TRY(download(c, &one, info_d + k, 1));
     MGW_REQUIRE(one == 0, MGW_ERR_SOLVER,
-                "Cholesky of the FP64 rebuild of 1-chi0 failed at frequency " + std::to_string(k) + " (info " +
-                    std::to_string(one) + "; reduced-precision info " + std::to_string(info[k]) + ")");
+                "Cholesky of the FP64 rebuild of 1-chi0 failed at frequency " + std::to_string(k) + " (column " +
+                    std::to_string(one) + "; reduced-precision column " + std::to_string(info[k]) + ")");
     ++c->escalations;
   }
-  MGW_TRY(launch_chi_trace(nullptr, nullptr, 0, chi, naux, 1, nomega, trl, c->stream));
-  MGW_BLAS(rocsolver_dpotri_strided_batched(c->blas, rocblas_fill_lower, naux, chi, naux,
-                                            (rocblas_stride)naux * naux, info_d, nomega));
-  MGW_TRY(download(c, info.data(), info_d, nomega));
-  for (int k = 0; k < nomega; ++k)
-    MGW_REQUIRE(info[k] == 0, MGW_ERR_SOLVER, "inverse of 1-chi0 failed at frequency " + std::to_string(k));
-  MGW_TRY(launch_chi_trace(nullptr, nullptr, 0, chi, naux, 2, nomega, tri, c->stream));
-  MGW_TRY(launch_chi_finish(chi, naux, nomega, c->stream));
-  std::vector<double> h0(nomega), hl(nomega), hi(nomega);
+  MGW_TRY(launch_lower_frob2(chi, naux, nomega, frob, c->stream));
+  std::vector<double> h0(nomega), hl(nomega), fr((size_t)FROB_BLOCKS * nomega);
   MGW_TRY(download(c, h0.data(), tr0, nomega));
-  MGW_TRY(download(c, hl.data(), trl, nomega));
-  MGW_TRY(download(c, hi.data(), tri, nomega));
+  MGW_TRY(download(c, hl.data(), logdet, nomega));
+  MGW_TRY(download(c, fr.data(), frob, (size_t)FROB_BLOCKS * nomega));
   // sum ln(lambda) + 1 - lambda = ln det(1-chi0) + Tr chi0 ;
   // sum -(1-lambda)/lambda + 1 - lambda = n - Tr (1-chi0)^-1 + Tr chi0
-  // (src/m_gw_selfenergy_grid.f90:171-173)
+  // (src/m_gw_selfenergy_grid.f90:171-173); Tr (1-chi0)^-1 = ||L^-1||_F^2
   const double twopi = 2.0 * 3.14159265358979323;
   double er = 0.0, eg = 0.0;
   for (int k = 0; k < nomega; ++k) {
+    double tri = 0.0;
+    for (int b = 0; b < FROB_BLOCKS; ++b) tri += fr[(size_t)k * FROB_BLOCKS + b];
     er += (2.0 * hl[k] - h0[k]) / twopi * weight[k];
-    eg += ((double)naux - hi[k] - h0[k]) / twopi * weight[k];
+    eg += ((double)naux - tri - h0[k]) / twopi * weight[k];
   }
   if (erpa) *erpa = er;
   if (egw) *egw = eg;
@@ -635,20 +636,24 @@ int gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int ncalc, const double*
   double *part, *v, *om_d, *wt_d, *ep_d, *wc_d, *sc;
   MGW_TRY(dbuf(c, "dot_part", (size_t)nrb * ncol * nomega, &part));
   MGW_TRY(dbuf(c, "dot_v", (size_t)ncol * nomega, &v));
-  // diag(B_m^T chi(iw) B_m) for every (p, m) and iw   (src/m_gw_selfenergy_grid.f90:297-310)
+  // diag(B_m^T chi(iw) B_m) for every (p, m) and iw (src/m_gw_selfenergy_grid.f90:297-310),
+  // as ||L^-1 B_m||^2 - ||B_m||^2 with chi = (L L^T)^-1 - 1: a triangular
+  // product with the squared column norms fused into its epilogue
+  double* bn2 = nullptr;
+  MGW_TRY(dbuf(c, "eri_pm_n2", (size_t)ncol, &bn2));
+  MGW_TRY(launch_colnorm2(epm, naux, naux, (int)ncol, bn2, c->stream));
   GemmF64 g;
   g.M = naux; g.N = (int)ncol; g.K = naux;
-  g.A = find_buf(c, "wpol_chi"); g.lda = naux; g.strideA = (int64_t)naux * naux;
+  g.A = find_buf(c, "wpol_linv"); g.lda = naux; g.strideA = (int64_t)naux * naux; g.tri_a = true;
   g.B = epm; g.ldb = naux; g.strideB = 0;
-  g.dotB = epm; g.lddot = naux;
   g.dotpart = part; g.strideDot = (int64_t)nrb * ncol;
   g.batch = nomega;
   {
-    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_WB, 2.0 * naux * (double)naux * ncol * nomega,
-                 8.0 * ((double)naux * naux * nomega + 2.0 * (double)naux * ncol), c->stream);
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_WB, (double)naux * (naux + 1.0) * ncol * nomega,
+                 8.0 * (0.5 * (double)naux * naux * nomega + (double)naux * ncol), c->stream);
     MGW_TRY(gemm_f64(g, c->stream));
   }
-  MGW_TRY(launch_dot_reduce(part, nrb, ncol, nomega, v, c->stream));
+  MGW_TRY(launch_dot_reduce(part, nrb, ncol, nomega, bn2, v, c->stream));
   std::vector<double> ep(np);
   for (int k = 0; k < np; ++k) ep[k] = c->energy[glo - 1 + k];
   MGW_TRY(upload(c, "sgi_omega", c->chi_omega.data(), nomega, &om_d));
@@ -895,7 +900,8 @@ int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int techn
               + 2 * npole * npole                 // Casida matrix + eigen workspace
               + na * npole;                       // w_s
   } else {
-    stage_w = na * nb * no + na * npole + na * na * (int64_t)nomega_chi_imag + npole * nomega_chi_imag;
+    stage_w = na * nb * no + na * npole + na * na * (int64_t)nomega_chi_imag + npole * nomega_chi_imag +
+              na * (int64_t)mgw::CHOL_NB * nomega_chi_imag;
     if (precision == MGW_BF16X3 || precision == MGW_BF16X6)
       stage_w += (int64_t)(mgw::chi0_split_workspace((int)na, (int)npole, nomega_chi_imag, precision) / 8);
   }
@@ -1037,7 +1043,27 @@ int mgw_polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, i
               "mgw_polarizability_grid: unknown precision");
   MGW_REQUIRE(omega_im && weight_quad, MGW_ERR_ARG, "null frequency grid");
   MGW_TRY(polarizability_grid(c, r, spin_fact, nomega, omega_im, weight_quad, erpa, egw, precision));
-  if (chi_out) MGW_TRY(download(c, chi_out, find_buf(c, "wpol_chi"), (size_t)c->naux * c->naux * nomega));
+  if (chi_out) {
+    // chi = (L L^T)^-1 - 1 = L^-T L^-1 - 1, formed only for callers that ask for it
+    const int n = c->naux;
+    double* linv = find_buf(c, "wpol_linv");
+    double *full, *mones;
+    MGW_TRY(dbuf(c, "chi_full", (size_t)n * n * nomega, &full));
+    MGW_TRY(dbuf(c, "minus_ones", n, &mones));
+    MGW_TRY(launch_fill(mones, n, -1.0, c->stream));
+    MGW_TRY(launch_zero_upper(linv, n, nomega, c->stream));
+    GemmF64 g;
+    g.M = n; g.N = n; g.K = n;
+    g.A = linv; g.lda = n; g.strideA = (int64_t)n * n; g.transA = true;
+    g.B = linv; g.ldb = n; g.strideB = (int64_t)n * n;
+    g.C = full; g.ldc = n; g.strideC = (int64_t)n * n;
+    g.diag_add = mones;
+    g.lower_only = true;
+    g.batch = nomega;
+    MGW_TRY(gemm_f64(g, c->stream));
+    MGW_TRY(download(c, chi_out, full, (size_t)n * n * nomega));
+    MGW_TRY(release(c, "chi_full"));
+  }
   MGW_HIP(hipStreamSynchronize(c->stream));
   return MGW_OK;
 }

@@ -104,9 +104,10 @@ struct SyrkArgs {
 };
 
 // Lower-triangle tiles BM x BM, WM x WN waves, one 32-k slab per step.
-// DEPTH register sets prefetch the next DEPTH slabs from global memory while
-// the MFMAs consume the LDS copy of the current one, so each load has DEPTH
-// slabs of MFMA work to arrive.
+// DEPTH 1, 2: that many register sets prefetch the next slabs from global
+// memory while the MFMAs consume the LDS copy of the current one (two
+// barriers per slab). DEPTH 3: two LDS buffers; the next slab is written to
+// the idle buffer while the MFMAs read the other (one barrier per slab).
 template <int NS, int BM, int WM, int WN, int FLUSH, int DEPTH>
 __global__ void __launch_bounds__(64 * WM * WN)
 chi_syrk_kernel(SyrkArgs g) {
@@ -118,10 +119,11 @@ chi_syrk_kernel(SyrkArgs g) {
   constexpr int LPT = CH / NT;
   static_assert(CH % NT == 0, "chunks per thread");
   static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile");
-  static_assert(DEPTH == 1 || DEPTH == 2, "prefetch depth");
+  static_assert(DEPTH >= 1 && DEPTH <= 3, "prefetch depth");
+  constexpr int NBUF = DEPTH == 3 ? 2 : 1;
 
-  __shared__ __attribute__((aligned(16))) __bf16 As[NS][BM][KP];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[NS][BM][KP];
+  __shared__ __attribute__((aligned(16))) __bf16 As[NBUF][NS][BM][KP];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[NBUF][NS][BM][KP];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
@@ -144,12 +146,12 @@ chi_syrk_kernel(SyrkArgs g) {
       RA[q][i] = *reinterpret_cast<const u4*>(Ya + q * g.plane + (int64_t)(S_) * slab + 8 * c); \
       RB[q][i] = *reinterpret_cast<const u4*>(Yb + q * g.plane + (int64_t)(S_) * slab + 8 * c); \
     }
-#define MGW_SYRK_STORE(RA, RB)                                             \
-  _Pragma("unroll") for (int q = 0; q < NS; ++q)                            \
-    _Pragma("unroll") for (int i = 0; i < LPT; ++i) {                       \
-      const int c = tid + i * NT;                                           \
-      *reinterpret_cast<u4*>(&As[q][c >> 2][8 * (c & 3)]) = RA[q][i];       \
-      *reinterpret_cast<u4*>(&Bs[q][c >> 2][8 * (c & 3)]) = RB[q][i];       \
+#define MGW_SYRK_STORE(RA, RB, BUF)                                              \
+  _Pragma("unroll") for (int q = 0; q < NS; ++q)                                  \
+    _Pragma("unroll") for (int i = 0; i < LPT; ++i) {                             \
+      const int c = tid + i * NT;                                                 \
+      *reinterpret_cast<u4*>(&As[BUF][q][c >> 2][8 * (c & 3)]) = RA[q][i];        \
+      *reinterpret_cast<u4*>(&Bs[BUF][q][c >> 2][8 * (c & 3)]) = RB[q][i];        \
     }
 
   double acc[FM][FN][4];
@@ -166,14 +168,16 @@ chi_syrk_kernel(SyrkArgs g) {
   const int fr = lane & 15, fq = lane >> 4;
   const int nslab = g.nkt;
   // MFMAs of the slab in LDS, then the FP64 flush every FLUSH slabs
-  auto compute = [&](int s) __attribute__((always_inline)) {
+  auto compute = [&](int s, int buf) __attribute__((always_inline)) {
     bf8 a[NS][FM], b[NS][FN];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
 #pragma unroll
-      for (int i = 0; i < FM; ++i) a[q][i] = *reinterpret_cast<const bf8*>(&As[q][wm * TM + i * 16 + fr][8 * fq]);
+      for (int i = 0; i < FM; ++i)
+        a[q][i] = *reinterpret_cast<const bf8*>(&As[buf][q][wm * TM + i * 16 + fr][8 * fq]);
 #pragma unroll
-      for (int j = 0; j < FN; ++j) b[q][j] = *reinterpret_cast<const bf8*>(&Bs[q][wn * TN + j * 16 + fr][8 * fq]);
+      for (int j = 0; j < FN; ++j)
+        b[q][j] = *reinterpret_cast<const bf8*>(&Bs[buf][q][wn * TN + j * 16 + fr][8 * fq]);
     }
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -206,26 +210,42 @@ chi_syrk_kernel(SyrkArgs g) {
     MGW_SYRK_LOAD(ra0, rb0, 0)
     for (int s = 0; s < nslab; ++s) {
       __syncthreads();                   // previous slab's fragments consumed
-      MGW_SYRK_STORE(ra0, rb0)
+      MGW_SYRK_STORE(ra0, rb0, 0)
       __syncthreads();
       if (s + 1 < nslab) { MGW_SYRK_LOAD(ra0, rb0, s + 1) }
-      compute(s);
+      compute(s, 0);
     }
-  } else {
+  } else if constexpr (DEPTH == 2) {
     MGW_SYRK_LOAD(ra0, rb0, 0)
     if (nslab > 1) { MGW_SYRK_LOAD(ra1, rb1, 1) }
     for (int s = 0; s < nslab; s += 2) {
       __syncthreads();
-      MGW_SYRK_STORE(ra0, rb0)
+      MGW_SYRK_STORE(ra0, rb0, 0)
       __syncthreads();
       if (s + 2 < nslab) { MGW_SYRK_LOAD(ra0, rb0, s + 2) }
-      compute(s);
+      compute(s, 0);
       if (s + 1 >= nslab) break;
       __syncthreads();
-      MGW_SYRK_STORE(ra1, rb1)
+      MGW_SYRK_STORE(ra1, rb1, 0)
       __syncthreads();
       if (s + 3 < nslab) { MGW_SYRK_LOAD(ra1, rb1, s + 3) }
-      compute(s + 1);
+      compute(s + 1, 0);
+    }
+  } else {
+    // buffer s & 1 holds slab s; slab s + 1 goes to the other buffer while
+    // slab s is multiplied, and the one barrier per slab orders both ways
+    MGW_SYRK_LOAD(ra0, rb0, 0)
+    MGW_SYRK_STORE(ra0, rb0, 0)
+    if (nslab > 1) { MGW_SYRK_LOAD(ra0, rb0, 1) }
+    __syncthreads();
+    for (int s = 0; s < nslab; ++s) {
+      const int cur = s & 1;
+      compute(s, cur);
+      if (s + 1 < nslab) {
+        MGW_SYRK_STORE(ra0, rb0, cur ^ 1)
+        if (s + 2 < nslab) { MGW_SYRK_LOAD(ra0, rb0, s + 2) }
+      }
+      __syncthreads();
     }
   }
 
@@ -252,11 +272,13 @@ chi_syrk_kernel(SyrkArgs g) {
 #undef MGW_SYRK_LOAD
 #undef MGW_SYRK_STORE
 
-// Prefetch depth of the SYRK (2 unless MGW_CHI_PREFETCH=1 in the environment).
+// Pipeline of the SYRK: 2 (two register sets) unless MGW_CHI_PREFETCH selects
+// 1 (one register set) or 3 (two LDS buffers).
 int chi_prefetch_depth() {
   static const int depth = [] {
     const char* v = std::getenv("MGW_CHI_PREFETCH");
-    return (v && std::atoi(v) == 1) ? 1 : 2;
+    const int d = v ? std::atoi(v) : 2;
+    return (d >= 1 && d <= 3) ? d : 2;
   }();
   return depth;
 }
@@ -285,10 +307,13 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
     g.C = chi + (int64_t)f0 * naux * naux; g.strideC = (int64_t)naux * naux;
     g.nblocks = (int64_t)tiles * nf;
     MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chi0 split: too many tiles");
-    if (chi_prefetch_depth() == 1)
+    const int depth = chi_prefetch_depth();
+    if (depth == 1)
       hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 1>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
-    else
+    else if (depth == 2)
       hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 2>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
+    else
+      hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 3>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
     MGW_HIP(hipGetLastError());
   }
   return MGW_OK;

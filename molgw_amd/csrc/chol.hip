@@ -1,0 +1,214 @@
+// Batched inverse Cholesky factor of 1 - chi0(iw) for all frequencies.
+//
+// The reference inverts 1 - chi0 per frequency with LU (DGETRF/DGETRI) and
+// forms chi = (1 - chi0)^-1 chi0 (src/m_gw_selfenergy_grid.f90:150-170);
+// its E_RPA / E_GW traces come from the eigenvalues of 1 - chi0 (:171-173).
+// Here 1 - chi0 = L L^T is factored for every frequency of the batch at
+// once, and L is inverted in place:
+//   ln det(1 - chi0) = 2 sum_i ln L_ii,   Tr (1 - chi0)^-1 = ||L^-1||_F^2,
+//   B^T chi B = ||L^-1 B||^2 - ||B||^2   (chi = (1 - chi0)^-1 - 1),
+// so the self-energy never needs the inverse itself.
+//
+// Blocked right-looking factorisation with 64-wide block columns: a
+// diagonal-block kernel factors L_kk in LDS and replaces it by its inverse
+// T_k = L_kk^-1; the panel L21 = A21 T_k^T and the trailing update
+// A22 -= L21 L21^T are batched MFMA GEMMs. The inverse is then built from
+// the last block column backwards: (L^-1)21 = -(L^-1)22 L21 T_k, two GEMMs,
+// the second with a triangular A whose K range stops at the diagonal.
+#include "mgw_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace mgw {
+namespace {
+
+constexpr int NB = CHOL_NB;
+
+// One workgroup per matrix: factor the kb x kb diagonal block at (k, k),
+// add sum ln L_jj to logdet, write L_kk^-1 (zeros above the diagonal) in place.
+// A non-positive pivot records its 1-based column in info (first failure
+// only) and is replaced by 1 so the batch stays finite.
+__global__ void __launch_bounds__(256)
+chol_diag_kernel(double* __restrict__ A, int n, int64_t strideA, int k, int kb, double* __restrict__ logdet,
+                 int32_t* __restrict__ info) {
+  __shared__ double L[NB][NB + 1];
+  __shared__ int failed;
+  const int f = blockIdx.x, tid = threadIdx.x;
+  double* __restrict__ a = A + (int64_t)f * strideA + k + (int64_t)k * n;
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e % NB, j = e / NB;
+    L[i][j] = (i < kb && j < kb && i >= j) ? a[i + (int64_t)j * n] : 0.0;
+  }
+  if (tid == 0) failed = 0;
+  __syncthreads();
+  for (int j = 0; j < kb; ++j) {
+    if (tid == 0) {
+      double d = L[j][j];
+      if (!(d > 0.0)) {
+        if (!failed) failed = k + j + 1;
+        d = 1.0;
+      }
+      L[j][j] = sqrt(d);
+    }
+    __syncthreads();
+    const double djj = L[j][j];
+    for (int i = j + 1 + tid; i < kb; i += 256) L[i][j] /= djj;
+    __syncthreads();
+    const int m = kb - j - 1;
+    for (int e = tid; e < m * m; e += 256) {
+      const int i = j + 1 + e % m, l = j + 1 + e / m;
+      if (l <= i) L[i][l] -= L[i][j] * L[l][j];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double s = 0.0;
+    for (int j = 0; j < kb; ++j) s += log(L[j][j]);
+    logdet[f] = (k == 0 ? 0.0 : logdet[f]) + s;
+    if (k == 0) info[f] = 0;
+    if (failed && info[f] == 0) info[f] = failed;
+  }
+  // in-place inverse, last column first: T(i, j) = -sum_{l=j+1..i} T(i, l) L(l, j) / L(j, j)
+  for (int j = kb - 1; j >= 0; --j) {
+    const double ljj = L[j][j];
+    const int i = j + 1 + tid;
+    double x = 0.0;
+    if (i < kb)
+      for (int l = j + 1; l <= i; ++l) x += L[i][l] * L[l][j];
+    __syncthreads();
+    if (i < kb) L[i][j] = -x / ljj;
+    if (tid == 0) L[j][j] = 1.0 / ljj;
+    __syncthreads();
+  }
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e % NB, j = e / NB;
+    if (i < kb && j < kb) a[i + (int64_t)j * n] = (i >= j) ? L[i][j] : 0.0;
+  }
+}
+
+// dst_f(rows x cols, ldd) = src_f(rows x cols, lds) for every matrix f.
+__global__ void copy_block_kernel(const double* __restrict__ src, int64_t lds, int64_t strideS,
+                                  double* __restrict__ dst, int64_t ldd, int64_t strideD, int rows, int cols) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (e >= (int64_t)rows * cols) return;
+  const int i = (int)(e % rows), j = (int)(e / rows);
+  dst[(int64_t)f * strideD + i + j * ldd] = src[(int64_t)f * strideS + i + j * lds];
+}
+
+int gemm_chol(const GemmF64& g0, hipStream_t stream) {
+  GemmF64 g = g0;
+  g.klass = MGW_KCLASS_CHOLESKY;
+  return gemm_f64(g, stream);
+}
+
+// ---------------------------------------------------------------- traces
+// partial[f][b] = sum over columns j = b, b + FROB_BLOCKS, ... of
+// sum_{i >= j} m_f(i, j)^2.
+__global__ void __launch_bounds__(256)
+lower_frob2_kernel(const double* __restrict__ m, int n, double* __restrict__ partial) {
+  __shared__ double red[4];
+  const int b = blockIdx.x, f = blockIdx.y;
+  const double* __restrict__ a = m + (int64_t)f * n * n;
+  double s = 0.0;
+  for (int j = b; j < n; j += FROB_BLOCKS)
+    for (int i = j + threadIdx.x; i < n; i += 256) {
+      const double v = a[i + (int64_t)j * n];
+      s += v * v;
+    }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[(int64_t)f * FROB_BLOCKS + b] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void zero_upper_kernel(double* __restrict__ m, int n) {
+  const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (i < n && j < n && i < j) m[(int64_t)blockIdx.z * n * n + i + (int64_t)j * n] = 0.0;
+}
+
+}  // namespace
+
+int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* info, double* work,
+                         hipStream_t stream) {
+  if (n <= 0 || nbatch <= 0) return MGW_OK;
+  MGW_REQUIRE(nbatch <= 65535, MGW_ERR_ARG, "chol_inverse_batched: batch too large");
+  const int64_t s2 = (int64_t)n * n;
+  const int64_t sw = (int64_t)n * NB;    // W(f): rest x kb, leading dimension n
+  // factorisation
+  for (int k = 0; k < n; k += NB) {
+    const int kb = std::min(NB, n - k);
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(nbatch), dim3(256), 0, stream, A, n, s2, k, kb, logdet, info);
+    MGW_HIP(hipGetLastError());
+    const int rest = n - k - kb;
+    if (rest <= 0) continue;
+    double* a21 = A + (k + kb) + (int64_t)k * n;
+    double* akk = A + k + (int64_t)k * n;
+    double* a22 = A + (k + kb) + (int64_t)(k + kb) * n;
+    // L21 = A21 T_k^T  (T_k = L_kk^-1 sits in the diagonal block)
+    GemmF64 p;
+    p.M = rest; p.N = kb; p.K = kb;
+    p.A = a21; p.lda = n; p.strideA = s2;
+    p.B = akk; p.ldb = n; p.strideB = s2; p.transB = true;
+    p.C = work; p.ldc = n; p.strideC = sw;
+    p.batch = nbatch;
+    MGW_TRY(gemm_chol(p, stream));
+    // A22 -= L21 L21^T (lower triangle only)
+    GemmF64 u;
+    u.M = rest; u.N = rest; u.K = kb;
+    u.A = work; u.lda = n; u.strideA = sw;
+    u.B = work; u.ldb = n; u.strideB = sw; u.transB = true;
+    u.C = a22; u.ldc = n; u.strideC = s2;
+    u.alpha = -1.0; u.beta = 1.0;
+    u.lower_only = true; u.no_mirror = true;
+    u.batch = nbatch;
+    MGW_TRY(gemm_chol(u, stream));
+    const int64_t cnt = (int64_t)rest * kb;
+    hipLaunchKernelGGL(copy_block_kernel, dim3((unsigned)((cnt + 255) / 256), nbatch), dim3(256), 0, stream,
+                       work, (int64_t)n, sw, a21, (int64_t)n, s2, rest, kb);
+    MGW_HIP(hipGetLastError());
+  }
+  // inverse, last block column first: (L^-1)21 = -(L^-1)22 (L21 T_k)
+  const int nblk = (n + NB - 1) / NB;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int k = b * NB;
+    const int kb = std::min(NB, n - k);
+    const int rest = n - k - kb;
+    if (rest <= 0) continue;
+    double* a21 = A + (k + kb) + (int64_t)k * n;
+    double* akk = A + k + (int64_t)k * n;
+    double* a22 = A + (k + kb) + (int64_t)(k + kb) * n;
+    GemmF64 p;
+    p.M = rest; p.N = kb; p.K = kb;
+    p.A = a21; p.lda = n; p.strideA = s2;
+    p.B = akk; p.ldb = n; p.strideB = s2;
+    p.C = work; p.ldc = n; p.strideC = sw;
+    p.batch = nbatch;
+    MGW_TRY(gemm_chol(p, stream));
+    GemmF64 t;
+    t.M = rest; t.N = kb; t.K = rest;
+    t.A = a22; t.lda = n; t.strideA = s2; t.tri_a = true;
+    t.B = work; t.ldb = n; t.strideB = sw;
+    t.C = a21; t.ldc = n; t.strideC = s2;
+    t.alpha = -1.0;
+    t.batch = nbatch;
+    MGW_TRY(gemm_chol(t, stream));
+  }
+  return MGW_OK;
+}
+
+int launch_lower_frob2(const double* m, int n, int nbatch, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(lower_frob2_kernel, dim3(FROB_BLOCKS, nbatch), dim3(256), 0, s, m, n, partial);
+  MGW_HIP(hipGetLastError());
+  return MGW_OK;
+}
+
+int launch_zero_upper(double* m, int n, int nbatch, hipStream_t s) {
+  hipLaunchKernelGGL(zero_upper_kernel, dim3((n + 15) / 16, (n + 15) / 16, nbatch), dim3(256), 0, s, m, n);
+  MGW_HIP(hipGetLastError());
+  return MGW_OK;
+}
+
+}  // namespace mgw

@@ -49,6 +49,8 @@ struct GemmDev {
   const double* diag_add; int64_t strideDiag;
   double alpha, beta;
   int lower_only;
+  int no_mirror;       // lower_only: leave the strict upper triangle untouched
+  int tri_a;           // A (not transposed) is lower triangular: A(m, k) = 0 for k > m
   int tiles_m, tiles_n, batch;
   int64_t nblocks;
   const double* dotB; int64_t lddot;
@@ -141,7 +143,8 @@ gemm_f64_kernel(GemmDev g) {
       const int gm = m0 + (TA ? v : a_fix);
       const int gk = k0 + (TA ? a_fix : v);
       double x = 0.0;
-      if (gm < g.M && gk < g.K) x = TA ? A[gk + (int64_t)gm * g.lda] : A[gm + (int64_t)gk * g.lda];
+      if (gm < g.M && gk < g.K && (!g.tri_a || gk <= gm))
+        x = TA ? A[gk + (int64_t)gm * g.lda] : A[gm + (int64_t)gk * g.lda];
       ra[i] = x;
     }
 #pragma unroll
@@ -176,7 +179,9 @@ gemm_f64_kernel(GemmDev g) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  const int nslab = (g.K + BK - 1) / BK;
+  // a lower-triangular A has no terms beyond the tile's last row
+  const int kend = g.tri_a ? min(g.K, m0 + BM) : g.K;
+  const int nslab = (kend + BK - 1) / BK;
   load_slab(0);
   store_slab(0);
   __syncthreads();
@@ -208,9 +213,10 @@ gemm_f64_kernel(GemmDev g) {
   const double* __restrict__ dg = g.diag_add ? g.diag_add + (int64_t)batch * g.strideDiag : nullptr;
 
   if constexpr (DOT) {
-    // Column dots of this row block: sum_m C(m,n) * D(m,n).
+    // Column dots of this row block: sum_m C(m,n) * D(m,n), or sum_m C(m,n)^2
+    // without D.
     __shared__ double red[WM][BN];
-    const double* __restrict__ D = g.dotB + (int64_t)batch * g.strideB;  // same batching as B
+    const double* __restrict__ D = g.dotB ? g.dotB + (int64_t)batch * g.strideB : nullptr;  // batched as B
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
 #pragma unroll
@@ -221,7 +227,10 @@ gemm_f64_kernel(GemmDev g) {
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
           const int m = m0 + wm * TM + i * 16 + fr;
-          if (m < g.M && n < g.N) part += g.alpha * acc[i][j][r] * D[m + (int64_t)n * g.lddot];
+          if (m < g.M && n < g.N) {
+            const double c = g.alpha * acc[i][j][r];
+            part += D ? c * D[m + (int64_t)n * g.lddot] : c * c;
+          }
         }
         part += __shfl_xor(part, 8, 16);
         part += __shfl_xor(part, 4, 16);
@@ -261,7 +270,7 @@ gemm_f64_kernel(GemmDev g) {
         const int64_t idx = m + (int64_t)n * g.ldc;
         if (g.beta != 0.0) v += g.beta * C[idx];
         C[idx] = v;
-        if (g.lower_only && m > n) C[n + (int64_t)m * g.ldc] = v;
+        if (g.lower_only && m > n && !g.no_mirror) C[n + (int64_t)m * g.ldc] = v;
       }
     }
   }
@@ -279,6 +288,8 @@ int launch_cfg(const GemmF64& h, hipStream_t stream) {
   g.diag_add = h.diag_add; g.strideDiag = h.strideDiag;
   g.alpha = h.alpha; g.beta = h.beta;
   g.lower_only = h.lower_only ? 1 : 0;
+  g.no_mirror = h.no_mirror ? 1 : 0;
+  g.tri_a = h.tri_a ? 1 : 0;
   g.tiles_m = (h.M + BM - 1) / BM;
   g.tiles_n = (h.N + BN - 1) / BN;
   g.batch = h.batch;
@@ -379,6 +390,7 @@ int launch_by_class(int cfg, const GemmF64& h, hipStream_t stream) {
     case MGW_KCLASS_GEMM_CASIDA: return launch_candidate<MGW_KCLASS_GEMM_CASIDA>(cfg, h, stream);
     case MGW_KCLASS_GEMM_SIGMA: return launch_candidate<MGW_KCLASS_GEMM_SIGMA>(cfg, h, stream);
     case MGW_KCLASS_GEMM_CHI0: return launch_candidate<MGW_KCLASS_GEMM_CHI0>(cfg, h, stream);
+    case MGW_KCLASS_CHOLESKY: return launch_candidate<MGW_KCLASS_CHOLESKY>(cfg, h, stream);
     default: return launch_candidate<MGW_KCLASS_GEMM_OTHER>(cfg, h, stream);
   }
 }
@@ -421,8 +433,9 @@ int gemm_f64(const GemmF64& h, hipStream_t stream) {
   if (h.precision == MGW_FP32 || h.precision == MGW_BF16X3) return gemm_lowp(h, h.precision, stream);
   MGW_REQUIRE(h.K >= 0, MGW_ERR_ARG, "gemm_f64: negative K");
   MGW_REQUIRE(!h.lower_only || h.M == h.N, MGW_ERR_ARG, "gemm_f64: lower_only needs M == N");
-  MGW_REQUIRE(h.dotpart == nullptr || (!h.lower_only && h.dotB), MGW_ERR_ARG,
-              "gemm_f64: dot epilogue needs dotB and a full tile grid");
+  MGW_REQUIRE(h.dotpart == nullptr || !h.lower_only, MGW_ERR_ARG, "gemm_f64: dot epilogue needs a full tile grid");
+  MGW_REQUIRE(!h.tri_a || (!h.transA && h.M == h.K), MGW_ERR_ARG,
+              "gemm_f64: a triangular A must be square and not transposed");
   MGW_REQUIRE(h.dotpart != nullptr || h.C != nullptr, MGW_ERR_ARG, "gemm_f64: no output");
   if (h.dotpart) return launch_cfg<128, 128, 4, 2, 16, true, MGW_KCLASS_GEMM_WB>(h, stream);
   // Skinny N: a 256 x 32 tile keeps the matrix cores busy when N is small.

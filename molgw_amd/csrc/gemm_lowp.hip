@@ -401,7 +401,8 @@ int launch_lowp(const GemmF64& h, hipStream_t stream) {
 
 int gemm_lowp(const GemmF64& h, int precision, hipStream_t stream) {
   if (h.M <= 0 || h.N <= 0 || h.batch <= 0) return MGW_OK;
-  MGW_REQUIRE(h.dotpart == nullptr && h.C != nullptr, MGW_ERR_ARG, "gemm_lowp: plain products only");
+  MGW_REQUIRE(h.dotpart == nullptr && h.C != nullptr && !h.tri_a && !h.no_mirror, MGW_ERR_ARG,
+              "gemm_lowp: plain products only");
   MGW_REQUIRE(!h.lower_only || h.M == h.N, MGW_ERR_ARG, "gemm_lowp: lower_only needs M == N");
   if (precision == MGW_BF16X3) {
     if (h.klass == MGW_KCLASS_GEMM_CHI0) return launch_lowp<128, 128, 4, 2, 4, MGW_KCLASS_GEMM_CHI0, true>(h, stream);

@@ -406,21 +406,23 @@ int launch_chi_finish(double* m, int n, int nomega, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ sigma imag
-// v[(p, m, iw)] = sum_rowblocks dotpart[iw][rb][col], col = p + np * m.
+// v[(p, m, iw)] = sum_rowblocks dotpart[iw][rb][col] - sub[col], col = p + np * m
+// (sub nullable).
 __global__ void dot_reduce_kernel(const double* __restrict__ part, int nrb, int64_t ncol, int nomega,
-                                  double* __restrict__ v) {
+                                  const double* __restrict__ sub, double* __restrict__ v) {
   const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int iw = blockIdx.y;
   if (col >= ncol) return;
   const double* q = part + (int64_t)nrb * ncol * iw;
   double s = 0.0;
   for (int rb = 0; rb < nrb; ++rb) s += q[(int64_t)rb * ncol + col];
-  v[col + ncol * iw] = s;
+  v[col + ncol * iw] = sub ? s - sub[col] : s;
 }
 
-int launch_dot_reduce(const double* part, int nrb, int64_t ncol, int nomega, double* v, hipStream_t s) {
+int launch_dot_reduce(const double* part, int nrb, int64_t ncol, int nomega, const double* sub, double* v,
+                      hipStream_t s) {
   hipLaunchKernelGGL(dot_reduce_kernel, dim3(nblocks(ncol, 256), nomega), dim3(256), 0, s, part, nrb,
-                     ncol, nomega, v);
+                     ncol, nomega, sub, v);
   MGW_LAUNCH_CHECK();
   return MGW_OK;
 }

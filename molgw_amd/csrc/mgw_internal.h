@@ -105,16 +105,19 @@ struct GemmF64 {
   double alpha = 1.0, beta = 0.0;
   int batch = 1;
   bool lower_only = false;
+  bool no_mirror = false;     // lower_only: do not mirror into the upper triangle
+  bool tri_a = false;         // A (not transposed, M == K) is lower triangular
   // Fused column dot: instead of storing C, accumulate
   // dotpart[(rowblock * N + n) + batch * strideDot] = sum_{m in rowblock} C(m,n) * D(m,n)
-  // with D(m, n) = dotB[m + n * lddot]. Used for diag(B^T chi B).
+  // with D(m, n) = dotB[m + n * lddot], or sum C(m,n)^2 when dotB is null.
   const double* dotB = nullptr; int64_t lddot = 0;
   double* dotpart = nullptr; int64_t strideDot = 0;
   // kernel class: the hot classes get their own kernel symbol so that
   // profiler per-kernel statistics separate them
   int klass = MGW_KCLASS_GEMM_OTHER;
   // arithmetic of the product: MGW_FP64 (default), MGW_FP32 or MGW_BF16X3
-  // (FP64 operands rounded on the way to LDS, FP64 accumulation)
+  // (FP64 operands rounded on the way to LDS, FP64 accumulation); the
+  // triangular and no-mirror options are FP64 only
   int precision = MGW_FP64;
 };
 int gemm_f64(const GemmF64& g, hipStream_t stream);
@@ -126,7 +129,20 @@ size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision);
 size_t chi0_split_workspace(int naux, int npole, int nomega, int precision);
 int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int nomega, int precision,
                     void* work, size_t work_bytes, double* chi, hipStream_t stream);
-int gemm_f64_rowblocks(int M);   // number of row blocks of the dot epilogue
+int gemm_f64_rowblocks(int M);
+// In-place inverse Cholesky factor of a batch of symmetric positive definite
+// matrices: A_f (n x n, lower triangle, stride n^2) becomes L_f^-1 (lower
+// triangle; the strict upper triangle is left undefined). logdet[f] receives
+// sum_i ln L_f(i, i); info[f] the 1-based column of the first non-positive
+// pivot (0 on success). work: n * 64 doubles per matrix.
+constexpr int CHOL_NB = 64;
+int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* info, double* work,
+                         hipStream_t stream);
+// tr[f] = sum of squares of the lower triangle of M_f (n x n, stride n^2).
+int launch_lower_frob2(const double* m, int n, int nbatch, double* partial, hipStream_t s);
+constexpr int FROB_BLOCKS = 64;
+// Zero the strict upper triangle of each n x n matrix of the batch.
+int launch_zero_upper(double* m, int n, int nbatch, hipStream_t s);   // number of row blocks of the dot epilogue
 int gemm_autotune_mode(int mode); // -1 query; returns the previous mode
 
 // ---------------------------------------------------------------- kernels
@@ -151,7 +167,8 @@ int launch_chi_weights(const int32_t* ia, int npole, const double* energy, const
 int launch_chi_trace(const double* sw, const double* n2, int npole, const double* mat, int naux,
                      int mode, int nomega, double* out, hipStream_t s);
 int launch_chi_finish(double* m, int n, int nomega, hipStream_t s);
-int launch_dot_reduce(const double* part, int nrb, int64_t ncol, int nomega, double* v, hipStream_t s);
+int launch_dot_reduce(const double* part, int nrb, int64_t ncol, int nomega, const double* sub, double* v,
+                      hipStream_t s);
 int launch_sigma_imag_sum(const double* v, int np, int nse, int nomega, const double* omega,
                           const double* weight, const double* ep, const double* wc, int ncalc,
                           double* sigma_calc, hipStream_t s);

@@ -417,7 +417,7 @@ def gw_selfenergy_imag(eri_pm: np.ndarray, chi: np.ndarray, omega_im: np.ndarray
     p_idx = np.arange(rng.ncore_G, rng.nvirtual_G - 1)
     sigma = np.zeros((len(omega_calc), nse), dtype=np.complex128)
     for mm in range(nse):
-        b = eri_pm[:, :, mm]
+        b = np.ascontiguousarray(eri_pm[:, :, mm])          # a strided slice makes BLAS crawl
         for iw in range(len(omega_im)):
             chib = chi[iw] @ b
             v = np.sum(b * chib, axis=0)                     # v_chi_v_p
