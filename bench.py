@@ -157,7 +157,7 @@ LOWP_CLASSES = ('gemm_ao2mo', 'gemm_chi0')
 PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0 / 3.0, 'bf16x6': 2500.0 / 6.0}
 
 
-def roofline_of(prof, precision='fp64'):
+def roofline_of(prof, precision='fp64', traffic_key=None):
     """Dominant hand-written kernel class by device time (the rocSOLVER eigensolver excluded).
 
     achieved = algorithmic flops per launch / average launch time. For the
@@ -180,21 +180,23 @@ def roofline_of(prof, precision='fp64'):
         r = {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_TFLOPS[prec], 'unit': 'TFLOP/s',
              'arith': prec}
     r['frac'] = r['achieved'] / r['peak']
-    r['traffic'] = _measured_traffic(name)
+    r['traffic'] = _measured_traffic(name, traffic_key)
     r['kernel'] = name
     r['avg_launch_ms'] = v['ms'] / v['launches']
     r['launches'] = v['launches']
     return name, r
 
 
-def _measured_traffic(kclass):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+def _measured_traffic(kclass, key=None):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    --pmc summary (profiles/traffic.json, keyed by workload/precision), if any."""
     path = os.path.join(ROOT, 'profiles', 'traffic.json')
     try:
         with open(path) as fh:
-            return json.load(fh).get(kclass, {}).get('bytes_per_launch')
+            table = json.load(fh)
     except (OSError, ValueError):
         return None
+    return table.get(key or '', {}).get(kclass, {}).get('bytes_per_launch')
 
 
 def cpu_baseline(system, budget_note=True):
@@ -392,7 +394,7 @@ def main():
                                                       args.precision)
     print(f'# gpu: {args.steps} solves in {elapsed:.3f}s; per-class device ms: '
           + ', '.join(f'{k}={v["ms"]:.1f}' for k, v in prof.items()), file=sys.stderr, flush=True)
-    kname, roof = roofline_of(prof, args.precision)
+    kname, roof = roofline_of(prof, args.precision, f'{args.workload}/{args.precision}')
     cpu = None
     if not args.no_cpu_baseline and technique == 'one_shot' and hasattr(system, 'eri_3center'):
         print('# cpu baseline (stage-sampled oracle) ...', file=sys.stderr, flush=True)

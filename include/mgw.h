@@ -99,6 +99,11 @@ int mgw_ctx_synchronize(mgw_ctx* ctx);
  * synchronises the whole GPU. No reference counterpart beyond
  * clean_deallocate (src/m_memory.f90:238-274). */
 int mgw_ctx_trim(mgw_ctx* ctx, int eager);
+/* With enable != 0, the batched Cholesky sequence of 1 - chi0 (about
+ * 6 nauxil / 64 kernel launches per solve) is captured into a HIP graph once
+ * per shape and buffer set and replayed on later solves. No reference
+ * counterpart (launch batching for the serving loop). */
+int mgw_ctx_graphs(mgw_ctx* ctx, int enable);
 /* GEMM tile autotuner (process-wide, no reference counterpart): mode 1 times
  * the candidate tile configurations on the first product of each new shape
  * class and keeps the fastest; 0 uses the table or the measured defaults;

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <tuple>
 #include <cstring>
 #include <numeric>
 
@@ -105,6 +106,16 @@ struct mgw_ctx {
   int sigma_ncalc = 0, sigma_calc_nse = 0;
   // frequencies of a reduced-precision chi0 rebuilt in FP64
   int64_t escalations = 0;
+  // HIP graphs of the batched Cholesky sequence, one per shape and buffer set
+  struct GraphKey {
+    int n, nbatch;
+    const void *a, *logdet, *info, *work;
+    bool operator<(const GraphKey& o) const {
+      return std::tie(n, nbatch, a, logdet, info, work) < std::tie(o.n, o.nbatch, o.a, o.logdet, o.info, o.work);
+    }
+  };
+  bool use_graphs = false;
+  std::map<GraphKey, hipGraphExec_t> graphs;
 };
 
 namespace {
@@ -490,6 +501,40 @@ int gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta,
   return MGW_OK;
 }
 
+// The batched Cholesky sequence (6 n / 64 launches, no host synchronisation)
+// is captured once per shape and buffer set into a HIP graph and replayed
+// when the context has graphs enabled; the autotuner's timing mode launches
+// eagerly because it synchronises.
+void clear_graphs(mgw_ctx* c) {
+  for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+}
+
+int run_chol(mgw_ctx* c, double* A, int n, int nbatch, double* logdet, int32_t* info, double* work) {
+  if (!c->use_graphs || gemm_autotune_mode(-1) == 1)
+    return chol_inverse_batched(A, n, nbatch, logdet, info, work, c->stream);
+  const mgw_ctx::GraphKey key{n, nbatch, A, logdet, info, work};
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    MGW_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = chol_inverse_batched(A, n, nbatch, logdet, info, work, c->stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc != MGW_OK) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    MGW_HIP(e);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    MGW_HIP(e2);
+    it = c->graphs.emplace(key, exec).first;
+  }
+  MGW_HIP(hipGraphLaunch(it->second, c->stream));
+  return MGW_OK;
+}
+
 // ------------------------------------------------------------------ A5
 int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int nomega,
                         const double* omega_im, const double* weight, double* erpa, double* egw,
@@ -559,7 +604,7 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
   {
     ProfScope chol(&c->prof, MGW_KCLASS_CHOLESKY, chol_flops * nomega, 16.0 * (double)naux * naux * nomega,
                    c->stream);
-    MGW_TRY(chol_inverse_batched(chi, naux, nomega, logdet, info_d, work, c->stream));
+    MGW_TRY(run_chol(c, chi, naux, nomega, logdet, info_d, work));
   }
   std::vector<int32_t> info(nomega);
   MGW_TRY(download(c, info.data(), info_d, nomega));
@@ -648,6 +693,7 @@ int gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int ncalc, const double*
   g.B = epm; g.ldb = naux; g.strideB = 0;
   g.dotpart = part; g.strideDot = (int64_t)nrb * ncol;
   g.batch = nomega;
+  g.batch_outer = true;   // one L^-1 stays in cache while B streams past it
   {
     ProfScope ps(&c->prof, MGW_KCLASS_GEMM_WB, (double)naux * (naux + 1.0) * ncol * nomega,
                  8.0 * (0.5 * (double)naux * naux * nomega + (double)naux * ncol), c->stream);
@@ -807,11 +853,20 @@ int mgw_ctx_destroy(mgw_ctx* c) {
   if (!c) return MGW_OK;
   MGW_HIP(hipSetDevice(c->device));
   MGW_HIP(hipStreamSynchronize(c->stream));
+  clear_graphs(c);
   for (auto& kv : c->ledger.bufs)
     if (kv.second.ptr) MGW_HIP(hipFree(kv.second.ptr));
   rocblas_destroy_handle(c->blas);
   MGW_HIP(hipStreamDestroy(c->stream));
   delete c;
+  return MGW_OK;
+}
+
+int mgw_ctx_graphs(mgw_ctx* c, int enable) {
+  MGW_CTX(c);
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  if (!enable) clear_graphs(c);
+  c->use_graphs = enable != 0;
   return MGW_OK;
 }
 
@@ -835,6 +890,7 @@ int mgw_ctx_trim(mgw_ctx* c, int eager) {
   for (auto& kv : c->ledger.bufs)
     if (!is_resident_input(kv.first)) names.push_back(kv.first);
   for (auto& n : names) MGW_TRY(release(c, n));
+  clear_graphs(c);
   c->has_eri_mo = false;
   c->has_wpol = false;
   c->has_chi = false;
@@ -1060,6 +1116,7 @@ int mgw_polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, i
     g.diag_add = mones;
     g.lower_only = true;
     g.batch = nomega;
+    g.batch_outer = true;
     MGW_TRY(gemm_f64(g, c->stream));
     MGW_TRY(download(c, chi_out, full, (size_t)n * n * nomega));
     MGW_TRY(release(c, "chi_full"));

@@ -22,6 +22,7 @@
 // The kernel feeds a = rows of the n block, b = rows of the m block, so lane
 // l, register r holds C(m = l & 15, n = 4 (l >> 4) + r) of the 16 x 16 tile.
 #include "mgw_internal.h"
+#include "tile_order.h"
 
 #include <algorithm>
 #include <cmath>
@@ -34,23 +35,8 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
-constexpr int NXCD = 8;
 constexpr int KT = 32;          // k per tile of the plane layout (one MFMA step)
 constexpr int PAD_ROWS = 128;   // P padding of the plane layout
-
-__device__ inline int64_t xcd_logical(int64_t w, int64_t G) {
-  const int64_t q = G / NXCD, r = G % NXCD;
-  const int64_t x = w % NXCD, i = w / NXCD;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
-}
-
-__device__ inline void tri_index(int t, int& bm, int& bn) {
-  int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((r + 1) * (r + 2) / 2 <= t) ++r;
-  while (r * (r + 1) / 2 > t) --r;
-  bm = r;
-  bn = t - r * (r + 1) / 2;
-}
 
 // ---------------------------------------------------------------- split
 // Block: 64 rows P x 32 transitions t (one k tile), all nf frequencies of
@@ -97,7 +83,7 @@ struct SyrkArgs {
   const __bf16* y;
   int64_t plane;       // elements between planes
   int64_t fstride;     // elements between frequencies
-  int ppad, nkt, naux, nf, tiles;
+  int ppad, nkt, naux, nf, tiles, tm;
   double* C;           // chi + f0 * naux^2
   int64_t strideC;
   int64_t nblocks;
@@ -130,7 +116,7 @@ chi_syrk_kernel(SyrkArgs g) {
   const int64_t L = xcd_logical(blockIdx.x, g.nblocks);
   const int f = (int)(L / g.tiles);
   int bm, bn;
-  tri_index((int)(L % g.tiles), bm, bn);
+  tri_grouped((int)(L % g.tiles), g.tm, bm, bn);
   const int m0 = bm * BM, n0 = bn * BM;
 
   const __bf16* __restrict__ Ya = g.y + (int64_t)f * g.fstride + (int64_t)m0 * KT;
@@ -179,21 +165,19 @@ chi_syrk_kernel(SyrkArgs g) {
       for (int j = 0; j < FN; ++j)
         b[q][j] = *reinterpret_cast<const bf8*>(&Bs[buf][q][wn * TN + j * 16 + fr][8 * fq]);
     }
+    // product-major: consecutive MFMAs write different accumulators, so
+    // none waits for the previous one's result (small products first)
+    // products (a plane, b plane): BF16x6 all six, BF16x3 the last three
+    constexpr int QA[6] = {NS == 3 ? 2 : 0, 0, 1, 1, 0, 0};
+    constexpr int QB[6] = {0, NS == 3 ? 2 : 0, 1, 0, 1, 0};
+    constexpr int P0 = NS == 3 ? 0 : 3;
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int pr = P0; pr < 6; ++pr)
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        f4 d = part[i][j];
-        if constexpr (NS == 3) {
-          d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0][j], a[2][i], d, 0, 0, 0);
-          d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[2][j], a[0][i], d, 0, 0, 0);
-          d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1][j], a[1][i], d, 0, 0, 0);
-        }
-        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0][j], a[1][i], d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1][j], a[0][i], d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0][j], a[0][i], d, 0, 0, 0);
-        part[i][j] = d;
-      }
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[QB[pr]][j], a[QA[pr]][i], part[i][j], 0, 0, 0);
     if ((s + 1) % FLUSH == 0 || s + 1 == nslab) {
 #pragma unroll
       for (int i = 0; i < FM; ++i)
@@ -272,15 +256,17 @@ chi_syrk_kernel(SyrkArgs g) {
 #undef MGW_SYRK_LOAD
 #undef MGW_SYRK_STORE
 
-// Pipeline of the SYRK: 2 (two register sets) unless MGW_CHI_PREFETCH selects
-// 1 (one register set) or 3 (two LDS buffers).
-int chi_prefetch_depth() {
-  static const int depth = [] {
+// Pipeline of the SYRK. Measured on MI355X (1200/3600/120, 32 frequencies):
+// two LDS buffers for two planes (BF16x3, 731 vs 850 ms with two register
+// sets), two register sets for three planes (BF16x6, 1392 vs 1419 ms).
+// MGW_CHI_PREFETCH = 1 (one register set), 2 or 3 overrides the choice.
+int chi_prefetch_depth(int ns) {
+  static const int forced = [] {
     const char* v = std::getenv("MGW_CHI_PREFETCH");
-    const int d = v ? std::atoi(v) : 2;
-    return (d >= 1 && d <= 3) ? d : 2;
+    const int d = v ? std::atoi(v) : 0;
+    return (d >= 1 && d <= 3) ? d : 0;
   }();
-  return depth;
+  return forced ? forced : (ns == 2 ? 3 : 2);
 }
 
 template <int NS>
@@ -303,11 +289,11 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
     MGW_HIP(hipGetLastError());
     SyrkArgs g;
     g.y = work; g.plane = plane; g.fstride = per_plane_f;
-    g.ppad = ppad; g.nkt = nkt; g.naux = naux; g.nf = nf; g.tiles = tiles;
+    g.ppad = ppad; g.nkt = nkt; g.naux = naux; g.nf = nf; g.tiles = tiles; g.tm = tm;
     g.C = chi + (int64_t)f0 * naux * naux; g.strideC = (int64_t)naux * naux;
     g.nblocks = (int64_t)tiles * nf;
     MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chi0 split: too many tiles");
-    const int depth = chi_prefetch_depth();
+    const int depth = chi_prefetch_depth(NS);
     if (depth == 1)
       hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 1>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
     else if (depth == 2)

@@ -100,6 +100,7 @@ __global__ void copy_block_kernel(const double* __restrict__ src, int64_t lds, i
 int gemm_chol(const GemmF64& g0, hipStream_t stream) {
   GemmF64 g = g0;
   g.klass = MGW_KCLASS_CHOLESKY;
+  g.batch_outer = true;   // the matrices of the batch share no operand
   return gemm_f64(g, stream);
 }
 

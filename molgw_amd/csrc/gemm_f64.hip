@@ -19,6 +19,7 @@
 // a row group hold 16 consecutive rows m of one column of the column-major C
 // and every store is a 128-byte contiguous segment.
 #include "mgw_internal.h"
+#include "tile_order.h"
 
 #include <algorithm>
 #include <cmath>
@@ -35,7 +36,6 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int PAD = 1;
-constexpr int NXCD = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int GROUP_M = 8;     // tile rows per group of the grouped ordering
 
 struct GemmDev {
@@ -51,28 +51,12 @@ struct GemmDev {
   int lower_only;
   int no_mirror;       // lower_only: leave the strict upper triangle untouched
   int tri_a;           // A (not transposed) is lower triangular: A(m, k) = 0 for k > m
+  int batch_outer;     // walk the tiles of one batch entry before the next
   int tiles_m, tiles_n, batch;
   int64_t nblocks;
   const double* dotB; int64_t lddot;
   double* dotpart; int64_t strideDot;
 };
-
-// Map a triangular tile index t (0-based) to (bm, bn) with bm >= bn.
-__device__ inline void tri_index(int t, int& bm, int& bn) {
-  int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((r + 1) * (r + 2) / 2 <= t) ++r;
-  while (r * (r + 1) / 2 > t) --r;
-  bm = r;
-  bn = t - r * (r + 1) / 2;
-}
-
-// Workgroup w is dispatched to XCD w % 8. Renumber so that consecutive
-// logical ids run on the same XCD (and share its L2): bijective on [0, G).
-__device__ inline int64_t xcd_logical(int64_t w, int64_t G) {
-  const int64_t q = G / NXCD, r = G % NXCD;
-  const int64_t x = w % NXCD, i = w / NXCD;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
-}
 
 template <int BM, int BN, int WM, int WN, int BK, bool TA, bool TB, bool DOT, int KLASS>
 __global__ void __launch_bounds__(64 * WM * WN)
@@ -98,14 +82,16 @@ gemm_f64_kernel(GemmDev g) {
   const int wm = wave % WM;
   const int wn = wave / WM;
 
-  // logical block: batch fastest (the frequencies of one tile read the same
-  // panels), then the tile in grouped order
+  // logical block: batch fastest when the batch entries share operands (the
+  // frequencies of one chi0 tile read the same panels), batch slowest when
+  // they do not; then the tile in grouped order
   const int64_t L = xcd_logical(blockIdx.x, g.nblocks);
-  const int batch = (int)(L % g.batch);
-  const int tile = (int)(L / g.batch);
+  const int64_t ntile = g.nblocks / g.batch;
+  const int batch = (int)(g.batch_outer ? L / ntile : L % g.batch);
+  const int tile = (int)(g.batch_outer ? L % ntile : L / g.batch);
   int bm, bn;
   if (g.lower_only && BM == BN) {
-    tri_index(tile, bm, bn);
+    tri_grouped(tile, g.tiles_m, bm, bn);
   } else {
     const int per_group = GROUP_M * g.tiles_n;
     const int group = tile / per_group;
@@ -290,6 +276,7 @@ int launch_cfg(const GemmF64& h, hipStream_t stream) {
   g.lower_only = h.lower_only ? 1 : 0;
   g.no_mirror = h.no_mirror ? 1 : 0;
   g.tri_a = h.tri_a ? 1 : 0;
+  g.batch_outer = h.batch_outer ? 1 : 0;
   g.tiles_m = (h.M + BM - 1) / BM;
   g.tiles_n = (h.N + BN - 1) / BN;
   g.batch = h.batch;

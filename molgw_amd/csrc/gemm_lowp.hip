@@ -24,6 +24,7 @@
 // As in gemm_f64 the kernels feed a = B^T, b = A^T, so d[i][j] = C(m = j, n = i)
 // and lane l, register r holds C(m = l & 15, n = 4 (l >> 4) + r).
 #include "mgw_internal.h"
+#include "tile_order.h"
 
 #include <cmath>
 
@@ -33,7 +34,6 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-constexpr int NXCD = 8;
 
 struct LowpDev {
   int M, N, K;
@@ -50,23 +50,9 @@ struct LowpDev {
   int64_t nblocks;
 };
 
-__device__ inline int64_t xcd_logical(int64_t w, int64_t G) {
-  const int64_t q = G / NXCD, r = G % NXCD;
-  const int64_t x = w % NXCD, i = w / NXCD;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
-}
-
-__device__ inline void tri_index(int t, int& bm, int& bn) {
-  int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((r + 1) * (r + 2) / 2 <= t) ++r;
-  while (r * (r + 1) / 2 > t) --r;
-  bm = r;
-  bn = t - r * (r + 1) / 2;
-}
-
 __device__ inline void tile_of(const LowpDev& g, int tile, int BMv, int BNv, int& bm, int& bn) {
   if (g.lower_only && BMv == BNv) {
-    tri_index(tile, bm, bn);
+    tri_grouped(tile, g.tiles_m, bm, bn);
   } else {
     constexpr int GROUP_M = 8;
     const int per_group = GROUP_M * g.tiles_n;
@@ -221,14 +207,22 @@ gemm_bf16x3_kernel(LowpDev g) {
       bh[j] = *reinterpret_cast<const bf8*>(&Bh[row][8 * fq]);
       bl[j] = *reinterpret_cast<const bf8*>(&Bl[row][8 * fq]);
     }
+    // product-major: consecutive MFMAs write different accumulators
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], part[i][j], 0, 0, 0);
+      for (int j = 0; j < FN; ++j)
         part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], part[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
         part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], part[i][j], 0, 0, 0);
-      }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], part[i][j], 0, 0, 0);
     if ((s + 1) % FLUSH == 0 || s + 1 == nslab) {
 #pragma unroll
       for (int i = 0; i < FM; ++i)

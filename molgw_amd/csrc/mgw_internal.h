@@ -107,6 +107,7 @@ struct GemmF64 {
   bool lower_only = false;
   bool no_mirror = false;     // lower_only: do not mirror into the upper triangle
   bool tri_a = false;         // A (not transposed, M == K) is lower triangular
+  bool batch_outer = false;   // tiles of one batch entry before the next (no shared operands)
   // Fused column dot: instead of storing C, accumulate
   // dotpart[(rowblock * N + n) + batch * strideDot] = sum_{m in rowblock} C(m,n) * D(m,n)
   // with D(m, n) = dotB[m + n * lddot], or sum C(m,n)^2 when dotB is null.

@@ -180,6 +180,10 @@ class Engine:
         check(self._lib.mgw_memory_report(self._h, C.byref(cur), C.byref(peak)))
         return cur.value, peak.value
 
+    def graphs(self, enable: bool = True) -> None:
+        """Replay the batched Cholesky sequence from a captured HIP graph."""
+        check(self._lib.mgw_ctx_graphs(self._h, 1 if enable else 0))
+
     def precision_escalations(self) -> int:
         """Frequencies whose reduced-precision 1 - chi0 was not positive
         definite and were rebuilt in FP64 (since the engine was created)."""

@@ -74,9 +74,11 @@ def device_memory_bytes(device: int = 0) -> int:
 
 
 class Server:
-    def __init__(self, nslots: int = 8, device: int = 0, hbm_budget: int | None = None):
+    def __init__(self, nslots: int = 8, device: int = 0, hbm_budget: int | None = None, graphs: bool = True):
         self.device = device
         self.slots = [Engine(device) for _ in range(nslots)]
+        for e in self.slots:
+            e.graphs(graphs)
         self.budget = hbm_budget if hbm_budget is not None else int(0.80 * device_memory_bytes(device))
         self._jobs = [queue.Queue() for _ in range(nslots)]
         self._done = queue.Queue()
