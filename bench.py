@@ -50,9 +50,11 @@ WORKLOADS = {
 
 
 def stream_bench(workload, args):
-    """Mixed-size request streams (configs 4 and 5): an FP64 pass that serves
-    one request at a time (the FP64 drop-in, also the QP reference), then the
-    serving pass (mixed precision, concurrent slots, continuous batching)."""
+    """Mixed-size request streams (the Poisson serving stream and the
+    64-alkane batch): an FP64 pass that serves one request at a time (the
+    FP64 drop-in, also the QP reference), then the serving pass (mixed
+    precision, concurrent slots, continuous batching). A step is one request;
+    the warm-up requests are served before the timed stream."""
     import torch
     from molgw_amd.serving import Request, Server, summarize, poisson_stream, alkane_batch
     from molgw_amd.workloads import synthetic_system_device, alkane_shape, water_cluster_shape
@@ -85,9 +87,12 @@ def stream_bench(workload, args):
     ref_sum = summarize(ref)
     print(f'# fp64 one at a time: {ref_sum}', file=sys.stderr, flush=True)
     with Server(nslots=args.slots, hbm_budget=budget) as srv:
-        srv.run(requests(args.precision)[:args.slots], realtime=False)  # warm-up
+        if args.warmup:
+            srv.run(requests(args.precision)[:args.warmup], realtime=False)  # untimed warm-up requests
         srv.profile(True)
+        torch.cuda.synchronize()
         res = srv.run(requests(args.precision), realtime=realtime)
+        torch.cuda.synchronize()
         prof = srv.profile_read()
         srv.profile(False)
         esc = srv.escalations()
@@ -331,25 +336,31 @@ def cpu_baseline_stream(spec, shapes, budget_s=15.0, window=8):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=5)
-    ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--workload', default='octane', choices=sorted(WORKLOADS) + ['serving', 'batch64'])
-    ap.add_argument('--requests', type=int, default=64, help='stream / batch length (serving, batch64)')
+    ap.add_argument('--steps', type=int, default=None,
+                    help='timed steps: solves (single-system workloads, default 5) or requests of the stream '
+                         '(serving, batch64; default 64)')
+    ap.add_argument('--warmup', type=int, default=None, help='untimed warm-up steps (default 1; streams: slots)')
+    ap.add_argument('--workload', default='serving', choices=sorted(WORKLOADS) + ['serving', 'batch64'])
+    ap.add_argument('--requests', type=int, default=None, help='stream / batch length (same as --steps)')
     ap.add_argument('--rate', type=float, default=20.0, help='Poisson arrival rate, requests/s (serving)')
     ap.add_argument('--slots', type=int, default=8, help='concurrent solve slots (serving, batch64)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--seed', type=int, default=0)
-    ap.add_argument('--precision', default='fp64', choices=['fp64', 'fp32', 'bf16x3', 'bf16x6'],
-                    help='arithmetic of the AO->MO transforms and the chi0/Casida build')
+    ap.add_argument('--precision', default=None, choices=['fp64', 'fp32', 'bf16x3', 'bf16x6'],
+                    help='arithmetic of the AO->MO transforms and the chi0/Casida build '
+                         '(default: bf16x6 for the streams, fp64 otherwise)')
     args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
-        print('bench.py: this engine runs one G0W0 solve per GPU on a single MI355X; multi-GPU runs are '
-              'not part of this build (each solve is independent: shard requests across GPUs).',
-              file=sys.stderr)
-        sys.exit(2)
-    if args.workload in ('serving', 'batch64'):
-        prec = args.precision if args.precision != 'fp64' else 'bf16x3'
-        args.precision = prec
+    stream = args.workload in ('serving', 'batch64')
+    if args.precision is None:
+        args.precision = 'bf16x6' if stream else 'fp64'
+    if args.requests is None:
+        args.requests = args.steps if args.steps is not None else 64
+    if args.steps is None:
+        args.steps = args.requests if stream else 5
+    if args.warmup is None:
+        args.warmup = args.slots if stream else 1
+    if stream:
+        prec = args.precision
         sm, ref, err, spec = stream_bench(args.workload, args)
         desc = ('Poisson stream of G0W0 requests: alkanes CnH2n+2 (n in 2..24) and water clusters (H2O)m '
                 f'(m in 1..32), cc-pVDZ/cc-pVDZ-RI shapes, {args.requests} requests at {args.rate}/s'
@@ -357,7 +368,7 @@ def main():
                 f'batch of {args.requests} alkanes CnH2n+2, n uniform in 1..16, cc-pVDZ/cc-pVDZ-RI shapes')
         line = {
             'metric': METRIC, 'value': sm['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
-            'steps': sm['solves'], 'warmup': args.slots, 'ms_per_step': 1e3 / sm['solves_per_s'],
+            'steps': sm['solves'], 'warmup': args.warmup, 'ms_per_step': 1e3 / sm['solves_per_s'],
             'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
             'dtype': f'{prec} products, f64 accumulation; f64 inversion and QP',
             'qp_error_vs_fp64_mev': err,
@@ -369,7 +380,8 @@ def main():
             'fp64_one_at_a_time': {'solves_per_s': ref['solves_per_s'], 'p50_ms': ref['p50_ms'],
                                    'p99_ms': ref['p99_ms']},
             'speedup_vs_fp64_one_at_a_time': sm['solves_per_s'] / ref['solves_per_s'],
-            'roofline': roofline_of(sm['profile'], prec)[1], 'cpu_baseline': sm['cpu_baseline'],
+            'roofline': roofline_of(sm['profile'], prec, f'{args.workload}/{prec}')[1],
+            'cpu_baseline': sm['cpu_baseline'],
             'stages_device_ms_summed_over_slots': {k: v['ms'] for k, v in sm['profile'].items()},
             'fp64_rebuilt_frequencies': sm['escalations'],
         }

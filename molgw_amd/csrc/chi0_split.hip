@@ -94,7 +94,7 @@ struct SyrkArgs {
 // memory while the MFMAs consume the LDS copy of the current one (two
 // barriers per slab). DEPTH 3: two LDS buffers; the next slab is written to
 // the idle buffer while the MFMAs read the other (one barrier per slab).
-template <int NS, int BM, int WM, int WN, int FLUSH, int DEPTH>
+template <int NS, int BM, int WM, int WN, int FLUSH, int DEPTH, int KS>
 __global__ void __launch_bounds__(64 * WM * WN)
 chi_syrk_kernel(SyrkArgs g) {
   constexpr int KP = KT + 8;              // padded LDS row: 80 bytes
@@ -106,10 +106,12 @@ chi_syrk_kernel(SyrkArgs g) {
   static_assert(CH % NT == 0, "chunks per thread");
   static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile");
   static_assert(DEPTH >= 1 && DEPTH <= 3, "prefetch depth");
+  static_assert(KS == 1 || KS == 2, "k tiles per slab");
   constexpr int NBUF = DEPTH == 3 ? 2 : 1;
 
-  __shared__ __attribute__((aligned(16))) __bf16 As[NBUF][NS][BM][KP];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[NBUF][NS][BM][KP];
+  // KS k tiles of 32 per slab (the plane layout pads the k tiles to a multiple of 2)
+  __shared__ __attribute__((aligned(16))) __bf16 As[NBUF][NS][KS][BM][KP];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[NBUF][NS][KS][BM][KP];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
@@ -124,21 +126,24 @@ chi_syrk_kernel(SyrkArgs g) {
   const int64_t slab = (int64_t)g.ppad * KT;
 
   // prefetch registers (native vectors, so they stay in VGPRs)
-  u4 ra0[NS][LPT], rb0[NS][LPT], ra1[NS][LPT], rb1[NS][LPT];
-#define MGW_SYRK_LOAD(RA, RB, S_)                                                             \
-  _Pragma("unroll") for (int q = 0; q < NS; ++q)                                               \
-    _Pragma("unroll") for (int i = 0; i < LPT; ++i) {                                          \
-      const int c = tid + i * NT;                                                              \
-      RA[q][i] = *reinterpret_cast<const u4*>(Ya + q * g.plane + (int64_t)(S_) * slab + 8 * c); \
-      RB[q][i] = *reinterpret_cast<const u4*>(Yb + q * g.plane + (int64_t)(S_) * slab + 8 * c); \
-    }
-#define MGW_SYRK_STORE(RA, RB, BUF)                                              \
-  _Pragma("unroll") for (int q = 0; q < NS; ++q)                                  \
-    _Pragma("unroll") for (int i = 0; i < LPT; ++i) {                             \
-      const int c = tid + i * NT;                                                 \
-      *reinterpret_cast<u4*>(&As[BUF][q][c >> 2][8 * (c & 3)]) = RA[q][i];        \
-      *reinterpret_cast<u4*>(&Bs[BUF][q][c >> 2][8 * (c & 3)]) = RB[q][i];        \
-    }
+  u4 ra0[NS][KS][LPT], rb0[NS][KS][LPT], ra1[NS][KS][LPT], rb1[NS][KS][LPT];
+#define MGW_SYRK_LOAD(RA, RB, S_)                                                                  \
+  _Pragma("unroll") for (int q = 0; q < NS; ++q)                                                    \
+    _Pragma("unroll") for (int kk = 0; kk < KS; ++kk)                                               \
+      _Pragma("unroll") for (int i = 0; i < LPT; ++i) {                                             \
+        const int c = tid + i * NT;                                                                 \
+        const int64_t off = q * g.plane + ((int64_t)(S_) * KS + kk) * slab + 8 * c;                 \
+        RA[q][kk][i] = *reinterpret_cast<const u4*>(Ya + off);                                      \
+        RB[q][kk][i] = *reinterpret_cast<const u4*>(Yb + off);                                      \
+      }
+#define MGW_SYRK_STORE(RA, RB, BUF)                                                  \
+  _Pragma("unroll") for (int q = 0; q < NS; ++q)                                      \
+    _Pragma("unroll") for (int kk = 0; kk < KS; ++kk)                                 \
+      _Pragma("unroll") for (int i = 0; i < LPT; ++i) {                               \
+        const int c = tid + i * NT;                                                   \
+        *reinterpret_cast<u4*>(&As[BUF][q][kk][c >> 2][8 * (c & 3)]) = RA[q][kk][i];  \
+        *reinterpret_cast<u4*>(&Bs[BUF][q][kk][c >> 2][8 * (c & 3)]) = RB[q][kk][i];  \
+      }
 
   double acc[FM][FN][4];
   f4 part[FM][FN];
@@ -152,32 +157,35 @@ chi_syrk_kernel(SyrkArgs g) {
     }
 
   const int fr = lane & 15, fq = lane >> 4;
-  const int nslab = g.nkt;
+  const int nslab = g.nkt / KS;
   // MFMAs of the slab in LDS, then the FP64 flush every FLUSH slabs
   auto compute = [&](int s, int buf) __attribute__((always_inline)) {
-    bf8 a[NS][FM], b[NS][FN];
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-        a[q][i] = *reinterpret_cast<const bf8*>(&As[buf][q][wm * TM + i * 16 + fr][8 * fq]);
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-        b[q][j] = *reinterpret_cast<const bf8*>(&Bs[buf][q][wn * TN + j * 16 + fr][8 * fq]);
-    }
-    // product-major: consecutive MFMAs write different accumulators, so
-    // none waits for the previous one's result (small products first)
     // products (a plane, b plane): BF16x6 all six, BF16x3 the last three
     constexpr int QA[6] = {NS == 3 ? 2 : 0, 0, 1, 1, 0, 0};
     constexpr int QB[6] = {0, NS == 3 ? 2 : 0, 1, 0, 1, 0};
     constexpr int P0 = NS == 3 ? 0 : 3;
 #pragma unroll
-    for (int pr = P0; pr < 6; ++pr)
+    for (int kk = 0; kk < KS; ++kk) {
+      bf8 a[NS][FM], b[NS][FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+      for (int q = 0; q < NS; ++q) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+          a[q][i] = *reinterpret_cast<const bf8*>(&As[buf][q][kk][wm * TM + i * 16 + fr][8 * fq]);
 #pragma unroll
         for (int j = 0; j < FN; ++j)
-          part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[QB[pr]][j], a[QA[pr]][i], part[i][j], 0, 0, 0);
+          b[q][j] = *reinterpret_cast<const bf8*>(&Bs[buf][q][kk][wn * TN + j * 16 + fr][8 * fq]);
+      }
+      // product-major: consecutive MFMAs write different accumulators, so
+      // none waits for the previous one's result (small products first)
+#pragma unroll
+      for (int pr = P0; pr < 6; ++pr)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[QB[pr]][j], a[QA[pr]][i], part[i][j], 0, 0, 0);
+    }
     if ((s + 1) % FLUSH == 0 || s + 1 == nslab) {
 #pragma unroll
       for (int i = 0; i < FM; ++i)
@@ -269,12 +277,25 @@ int chi_prefetch_depth(int ns) {
   return forced ? forced : (ns == 2 ? 3 : 2);
 }
 
+// k tiles of 32 per SYRK slab: MGW_CHI_KS = 1 or 2 (default 1; two tiles do
+// not fit two LDS buffers).
+int chi_k_tiles(int ns) {
+  static const int forced = [] {
+    const char* v = std::getenv("MGW_CHI_KS");
+    const int k = v ? std::atoi(v) : 0;
+    return (k == 1 || k == 2) ? k : 0;
+  }();
+  (void)ns;
+  const int k = forced ? forced : 1;
+  return (k == 2 && chi_prefetch_depth(ns) == 3) ? 1 : k;
+}
+
 template <int NS>
 int split_and_syrk(const double* x, int naux, int npole, const double* sw, int nomega, __bf16* work,
                    size_t work_bytes, double* chi, hipStream_t stream) {
   constexpr int BM = 128;
   const int ppad = (naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
-  const int nkt = (npole + KT - 1) / KT;
+  const int nkt = ((npole + KT - 1) / KT + 1) / 2 * 2;   // even: slabs of one or two k tiles
   const int64_t per_plane_f = (int64_t)nkt * ppad * KT;                 // elements
   const size_t per_f = (size_t)per_plane_f * NS * sizeof(__bf16);
   const int nf_max = (int)std::min<size_t>((size_t)nomega, work_bytes / per_f);
@@ -294,12 +315,16 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
     g.nblocks = (int64_t)tiles * nf;
     MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chi0 split: too many tiles");
     const int depth = chi_prefetch_depth(NS);
-    if (depth == 1)
-      hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 1>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
-    else if (depth == 2)
-      hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 2>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
-    else
-      hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 3>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
+    const int ks = chi_k_tiles(NS);
+    const dim3 grid((unsigned)g.nblocks), block(256);
+    if (ks == 2) {
+      if (depth == 1) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 2, 1, 2>), grid, block, 0, stream, g);
+      else hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 2, 2, 2>), grid, block, 0, stream, g);
+    } else {
+      if (depth == 1) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 1, 1>), grid, block, 0, stream, g);
+      else if (depth == 2) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 2, 1>), grid, block, 0, stream, g);
+      else hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 3, 1>), grid, block, 0, stream, g);
+    }
     MGW_HIP(hipGetLastError());
   }
   return MGW_OK;
@@ -310,7 +335,7 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
 size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision) {
   const int ns = precision == MGW_BF16X6 ? 3 : 2;
   const size_t ppad = (size_t)(naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
-  const size_t npad = (size_t)(npole + KT - 1) / KT * KT;
+  const size_t npad = (size_t)(npole + 2 * KT - 1) / (2 * KT) * (2 * KT);
   return ppad * npad * ns * sizeof(__bf16);
 }
 
