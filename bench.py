@@ -89,23 +89,31 @@ def stream_bench(workload, args):
     with Server(nslots=args.slots, hbm_budget=budget) as srv:
         if args.warmup:
             srv.run(requests(args.precision)[:args.warmup], realtime=False)  # untimed warm-up requests
+        # capacity: the whole stream available at t = 0, as in the FP64 pass
         srv.profile(True)
         torch.cuda.synchronize()
-        res = srv.run(requests(args.precision), realtime=realtime)
+        res = srv.run(requests(args.precision), realtime=False)
         torch.cuda.synchronize()
         prof = srv.profile_read()
         srv.profile(False)
+        # latency: the same requests arriving as the Poisson stream
+        live = srv.run(requests(args.precision), realtime=True) if realtime else res
+        torch.cuda.synchronize()
         esc = srv.escalations()
     sm = summarize(res)
+    lv = summarize(live)
     sm['profile'] = prof
     sm['escalations'] = esc
+    sm['live'] = lv
     print('# serving per-class device ms (summed over slots): '
           + ', '.join(f'{k}={v["ms"]:.1f}' for k, v in prof.items()), file=sys.stderr, flush=True)
     err = 0.0
-    for a, b in zip(res, ref):
-        sl = slice(a.nsemin - 1, a.nsemax)
-        err = max(err, float(np.max(np.abs(a.energy_qp[sl] - b.energy_qp[sl]))))
-    print(f'# serving: {sm}, max |dE_QP| = {err * 27211.38505:.4f} meV', file=sys.stderr, flush=True)
+    for run in (res, live):
+        for a, b in zip(run, ref):
+            sl = slice(a.nsemin - 1, a.nsemax)
+            err = max(err, float(np.max(np.abs(a.energy_qp[sl] - b.energy_qp[sl]))))
+    print(f'# serving capacity: {sm["solves_per_s"]:.2f} solves/s; live stream at {args.rate}/s: {lv}; '
+          f'max |dE_QP| = {err * 27211.38505:.4f} meV', file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         print('# cpu baseline (oracle on a sample of the stream) ...', file=sys.stderr, flush=True)
@@ -375,8 +383,11 @@ def main():
             'data': 'synthetic RI tensors at the molecules\' shapes (seeded, molecular magnitudes)',
             'config': {'workload': desc, 'technique': 'imaginary_pade', 'nomega_chi_imag': 32,
                        'nomega_sigma_calc': 16, 'states': 'HOMO-7..LUMO+8', 'slots': args.slots,
-                       'precision': prec},
-            'latency_ms': {'p50': sm['p50_ms'], 'p99': sm['p99_ms']},
+                       'precision': prec,
+                       'value': 'capacity: the whole stream available at t = 0 (as the FP64 pass)'},
+            'latency_ms': {'p50': sm['live']['p50_ms'], 'p99': sm['live']['p99_ms'],
+                           'stream': (f'Poisson arrivals at {args.rate}/s, {sm["live"]["solves_per_s"]:.2f} solves/s '
+                                      'delivered' if args.workload == 'serving' else 'batch at t = 0')},
             'fp64_one_at_a_time': {'solves_per_s': ref['solves_per_s'], 'p50_ms': ref['p50_ms'],
                                    'p99_ms': ref['p99_ms']},
             'speedup_vs_fp64_one_at_a_time': sm['solves_per_s'] / ref['solves_per_s'],

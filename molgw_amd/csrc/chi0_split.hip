@@ -87,6 +87,10 @@ struct SyrkArgs {
   double* C;           // chi + f0 * naux^2
   int64_t strideC;
   int64_t nblocks;
+  // K split: workgroup (f, kc, tile) multiplies slabs [kc * kc_slabs, ...) and
+  // writes its FP32 tile to part[((f * tiles + tile) * nkc + kc) * BM * BM]
+  int kc_slabs, nkc;
+  float* part;
 };
 
 // Lower-triangle tiles BM x BM, WM x WN waves, one 32-k slab per step.
@@ -94,8 +98,8 @@ struct SyrkArgs {
 // memory while the MFMAs consume the LDS copy of the current one (two
 // barriers per slab). DEPTH 3: two LDS buffers; the next slab is written to
 // the idle buffer while the MFMAs read the other (one barrier per slab).
-template <int NS, int BM, int WM, int WN, int FLUSH, int DEPTH, int KS>
-__global__ void __launch_bounds__(64 * WM * WN)
+template <int NS, int BM, int WM, int WN, int FLUSH, int DEPTH, int KS, bool KSPLIT>
+__global__ void __launch_bounds__(64 * WM * WN, KSPLIT ? 2 : 1)
 chi_syrk_kernel(SyrkArgs g) {
   constexpr int KP = KT + 8;              // padded LDS row: 80 bytes
   constexpr int NT = 64 * WM * WN;
@@ -116,9 +120,14 @@ chi_syrk_kernel(SyrkArgs g) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   const int64_t L = xcd_logical(blockIdx.x, g.nblocks);
-  const int f = (int)(L / g.tiles);
+  // K split: (frequency, chunk, tile) with the tile fastest, so concurrent
+  // workgroups read the same K range of their panels
+  const int64_t per_f = KSPLIT ? (int64_t)g.nkc * g.tiles : g.tiles;
+  const int f = (int)(L / per_f);
+  const int kc = KSPLIT ? (int)((L % per_f) / g.tiles) : 0;
+  const int tile = (int)(L % g.tiles);
   int bm, bn;
-  tri_grouped((int)(L % g.tiles), g.tm, bm, bn);
+  tri_grouped(tile, g.tm, bm, bn);
   const int m0 = bm * BM, n0 = bn * BM;
 
   const __bf16* __restrict__ Ya = g.y + (int64_t)f * g.fstride + (int64_t)m0 * KT;
@@ -158,6 +167,8 @@ chi_syrk_kernel(SyrkArgs g) {
 
   const int fr = lane & 15, fq = lane >> 4;
   const int nslab = g.nkt / KS;
+  const int s0 = KSPLIT ? kc * g.kc_slabs : 0;
+  const int s1 = KSPLIT ? min(nslab, s0 + g.kc_slabs) : nslab;
   // MFMAs of the slab in LDS, then the FP64 flush every FLUSH slabs
   auto compute = [&](int s, int buf) __attribute__((always_inline)) {
     // products (a plane, b plane): BF16x6 all six, BF16x3 the last three
@@ -186,7 +197,7 @@ chi_syrk_kernel(SyrkArgs g) {
           for (int j = 0; j < FN; ++j)
             part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[QB[pr]][j], a[QA[pr]][i], part[i][j], 0, 0, 0);
     }
-    if ((s + 1) % FLUSH == 0 || s + 1 == nslab) {
+    if (!KSPLIT && ((s + 1) % FLUSH == 0 || s + 1 == nslab)) {
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -199,46 +210,59 @@ chi_syrk_kernel(SyrkArgs g) {
   };
 
   if constexpr (DEPTH == 1) {
-    MGW_SYRK_LOAD(ra0, rb0, 0)
-    for (int s = 0; s < nslab; ++s) {
+    MGW_SYRK_LOAD(ra0, rb0, s0)
+    for (int s = s0; s < s1; ++s) {
       __syncthreads();                   // previous slab's fragments consumed
       MGW_SYRK_STORE(ra0, rb0, 0)
       __syncthreads();
-      if (s + 1 < nslab) { MGW_SYRK_LOAD(ra0, rb0, s + 1) }
+      if (s + 1 < s1) { MGW_SYRK_LOAD(ra0, rb0, s + 1) }
       compute(s, 0);
     }
   } else if constexpr (DEPTH == 2) {
-    MGW_SYRK_LOAD(ra0, rb0, 0)
-    if (nslab > 1) { MGW_SYRK_LOAD(ra1, rb1, 1) }
-    for (int s = 0; s < nslab; s += 2) {
+    MGW_SYRK_LOAD(ra0, rb0, s0)
+    if (s0 + 1 < s1) { MGW_SYRK_LOAD(ra1, rb1, s0 + 1) }
+    for (int s = s0; s < s1; s += 2) {
       __syncthreads();
       MGW_SYRK_STORE(ra0, rb0, 0)
       __syncthreads();
-      if (s + 2 < nslab) { MGW_SYRK_LOAD(ra0, rb0, s + 2) }
+      if (s + 2 < s1) { MGW_SYRK_LOAD(ra0, rb0, s + 2) }
       compute(s, 0);
-      if (s + 1 >= nslab) break;
+      if (s + 1 >= s1) break;
       __syncthreads();
       MGW_SYRK_STORE(ra1, rb1, 0)
       __syncthreads();
-      if (s + 3 < nslab) { MGW_SYRK_LOAD(ra1, rb1, s + 3) }
+      if (s + 3 < s1) { MGW_SYRK_LOAD(ra1, rb1, s + 3) }
       compute(s + 1, 0);
     }
   } else {
-    // buffer s & 1 holds slab s; slab s + 1 goes to the other buffer while
-    // slab s is multiplied, and the one barrier per slab orders both ways
-    MGW_SYRK_LOAD(ra0, rb0, 0)
+    // buffer (s - s0) & 1 holds slab s; slab s + 1 goes to the other buffer
+    // while slab s is multiplied, and the one barrier per slab orders both ways
+    MGW_SYRK_LOAD(ra0, rb0, s0)
     MGW_SYRK_STORE(ra0, rb0, 0)
-    if (nslab > 1) { MGW_SYRK_LOAD(ra0, rb0, 1) }
+    if (s0 + 1 < s1) { MGW_SYRK_LOAD(ra0, rb0, s0 + 1) }
     __syncthreads();
-    for (int s = 0; s < nslab; ++s) {
-      const int cur = s & 1;
+    for (int s = s0; s < s1; ++s) {
+      const int cur = (s - s0) & 1;
       compute(s, cur);
-      if (s + 1 < nslab) {
+      if (s + 1 < s1) {
         MGW_SYRK_STORE(ra0, rb0, cur ^ 1)
-        if (s + 2 < nslab) { MGW_SYRK_LOAD(ra0, rb0, s + 2) }
+        if (s + 2 < s1) { MGW_SYRK_LOAD(ra0, rb0, s + 2) }
       }
       __syncthreads();
     }
+  }
+
+  if constexpr (KSPLIT) {
+    // FP32 partial tile, column-major BM x BM; summed in FP64 by chi_ksplit_reduce_kernel
+    float* __restrict__ P = g.part + ((int64_t)(f * g.tiles + tile) * g.nkc + kc) * (BM * BM);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          P[(wm * TM + i * 16 + fr) + (wn * TN + j * 16 + 4 * fq + r) * BM] = part[i][j][r];
+    return;
   }
 
   // epilogue: lower triangle plus its mirror, 1 on the diagonal
@@ -263,6 +287,27 @@ chi_syrk_kernel(SyrkArgs g) {
 
 #undef MGW_SYRK_LOAD
 #undef MGW_SYRK_STORE
+
+// 1 - chi0 = I + sum over the K chunks of the FP32 partial tiles, summed in
+// FP64 in chunk order; lower triangle plus its mirror.
+template <int BM>
+__global__ void __launch_bounds__(256)
+chi_ksplit_reduce_kernel(const float* __restrict__ part, int nkc, int tiles, int tm, int naux,
+                         double* __restrict__ C, int64_t strideC) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  const int tile = blockIdx.y, f = blockIdx.z;
+  if (e >= BM * BM) return;
+  int bm, bn;
+  tri_grouped(tile, tm, bm, bn);
+  const int m = bm * BM + e % BM, n = bn * BM + e / BM;
+  if (m >= naux || n >= naux || m < n) return;
+  const float* __restrict__ p = part + (int64_t)(f * tiles + tile) * nkc * (BM * BM) + e;
+  double v = (m == n) ? 1.0 : 0.0;
+  for (int c = 0; c < nkc; ++c) v += (double)p[(int64_t)c * (BM * BM)];
+  double* __restrict__ out = C + (int64_t)f * strideC;
+  out[m + (int64_t)n * naux] = v;
+  if (m > n) out[n + (int64_t)m * naux] = v;
+}
 
 // Pipeline of the SYRK. Measured on MI355X (1200/3600/120, 32 frequencies):
 // two LDS buffers for two planes (BF16x3, 731 vs 850 ms with two register
@@ -290,40 +335,89 @@ int chi_k_tiles(int ns) {
   return (k == 2 && chi_prefetch_depth(ns) == 3) ? 1 : k;
 }
 
+// K split of the SYRK (FP32 partial tiles summed in FP64 by a second kernel):
+// frees the FP64 accumulators, so two workgroups share a CU, and multiplies
+// the workgroups of small problems. MGW_CHI_KSPLIT = 0 or 1 overrides the
+// default (on).
+bool chi_ksplit() {
+  static const int forced = [] {
+    const char* v = std::getenv("MGW_CHI_KSPLIT");
+    return v ? std::atoi(v) : -1;
+  }();
+  return forced != 0;
+}
+
+constexpr int KC_SLABS = 64;   // 2048 transitions per K chunk
+
+struct SplitLayout {
+  int ppad, nkt, tm, tiles, nkc;
+  int64_t per_plane_f;   // elements of one plane of one frequency
+  size_t planes_per_f;   // bytes of all planes of one frequency
+  size_t part_per_f;     // bytes of the FP32 partial tiles of one frequency
+};
+
+SplitLayout split_layout(int naux, int npole, int ns, bool ksplit) {
+  SplitLayout L;
+  L.ppad = (naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
+  L.nkt = ((npole + KT - 1) / KT + 1) / 2 * 2;   // even: slabs of one or two k tiles
+  L.tm = (naux + 127) / 128;
+  L.tiles = L.tm * (L.tm + 1) / 2;
+  L.nkc = (L.nkt + KC_SLABS - 1) / KC_SLABS;
+  L.per_plane_f = (int64_t)L.nkt * L.ppad * KT;
+  L.planes_per_f = (size_t)L.per_plane_f * ns * sizeof(__bf16);
+  L.part_per_f = ksplit ? (size_t)L.tiles * L.nkc * 128 * 128 * sizeof(float) : 0;
+  return L;
+}
+
 template <int NS>
-int split_and_syrk(const double* x, int naux, int npole, const double* sw, int nomega, __bf16* work,
+int split_and_syrk(const double* x, int naux, int npole, const double* sw, int nomega, void* work,
                    size_t work_bytes, double* chi, hipStream_t stream) {
   constexpr int BM = 128;
-  const int ppad = (naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
-  const int nkt = ((npole + KT - 1) / KT + 1) / 2 * 2;   // even: slabs of one or two k tiles
-  const int64_t per_plane_f = (int64_t)nkt * ppad * KT;                 // elements
-  const size_t per_f = (size_t)per_plane_f * NS * sizeof(__bf16);
-  const int nf_max = (int)std::min<size_t>((size_t)nomega, work_bytes / per_f);
+  const bool ksplit = chi_ksplit();
+  const SplitLayout L = split_layout(naux, npole, NS, ksplit);
+  const size_t per_f = L.planes_per_f + L.part_per_f;
+  MGW_REQUIRE(work_bytes > 256, MGW_ERR_ARG, "chi0 split: workspace too small");
+  const int nf_max = (int)std::min<size_t>((size_t)nomega, (work_bytes - 256) / per_f);
   MGW_REQUIRE(nf_max >= 1, MGW_ERR_ARG, "chi0 split: workspace smaller than one frequency");
-  const int tm = (naux + BM - 1) / BM;
-  const int tiles = tm * (tm + 1) / 2;
+  __bf16* planes = static_cast<__bf16*>(work);
+  float* part = reinterpret_cast<float*>(static_cast<char*>(work) + (L.planes_per_f * nf_max + 255) / 256 * 256);
   for (int f0 = 0; f0 < nomega; f0 += nf_max) {
     const int nf = std::min(nf_max, nomega - f0);
-    const int64_t plane = per_plane_f * nf;
-    hipLaunchKernelGGL((chi_split_kernel<NS>), dim3(ppad / 64, nkt), dim3(256), 0, stream, x, naux, npole, sw,
-                       f0, nf, ppad, nkt, work, plane);
+    const int64_t plane = L.per_plane_f * nf;
+    hipLaunchKernelGGL((chi_split_kernel<NS>), dim3(L.ppad / 64, L.nkt), dim3(256), 0, stream, x, naux, npole, sw,
+                       f0, nf, L.ppad, L.nkt, planes, plane);
     MGW_HIP(hipGetLastError());
     SyrkArgs g;
-    g.y = work; g.plane = plane; g.fstride = per_plane_f;
-    g.ppad = ppad; g.nkt = nkt; g.naux = naux; g.nf = nf; g.tiles = tiles; g.tm = tm;
+    g.y = planes; g.plane = plane; g.fstride = L.per_plane_f;
+    g.ppad = L.ppad; g.nkt = L.nkt; g.naux = naux; g.nf = nf; g.tiles = L.tiles; g.tm = L.tm;
     g.C = chi + (int64_t)f0 * naux * naux; g.strideC = (int64_t)naux * naux;
-    g.nblocks = (int64_t)tiles * nf;
-    MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chi0 split: too many tiles");
+    g.kc_slabs = KC_SLABS; g.nkc = L.nkc; g.part = part;
     const int depth = chi_prefetch_depth(NS);
+    const dim3 block(256);
+    if (ksplit) {
+      g.nblocks = (int64_t)L.tiles * L.nkc * nf;
+      MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chi0 split: too many tiles");
+      const dim3 grid((unsigned)g.nblocks);
+      // one LDS buffer and one register set: 162 (BF16x3) / 196 (BF16x6)
+      // VGPRs, two or three waves per SIMD; a second register set spills
+      hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
+      MGW_HIP(hipGetLastError());
+      hipLaunchKernelGGL((chi_ksplit_reduce_kernel<BM>), dim3(BM * BM / 256, L.tiles, nf), block, 0, stream, part,
+                         L.nkc, L.tiles, L.tm, naux, g.C, g.strideC);
+      MGW_HIP(hipGetLastError());
+      continue;
+    }
+    g.nblocks = (int64_t)L.tiles * nf;
+    MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chi0 split: too many tiles");
     const int ks = chi_k_tiles(NS);
-    const dim3 grid((unsigned)g.nblocks), block(256);
+    const dim3 grid((unsigned)g.nblocks);
     if (ks == 2) {
-      if (depth == 1) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 2, 1, 2>), grid, block, 0, stream, g);
-      else hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 2, 2, 2>), grid, block, 0, stream, g);
+      if (depth == 1) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 2, 1, 2, false>), grid, block, 0, stream, g);
+      else hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 2, 2, 2, false>), grid, block, 0, stream, g);
     } else {
-      if (depth == 1) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 1, 1>), grid, block, 0, stream, g);
-      else if (depth == 2) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 2, 1>), grid, block, 0, stream, g);
-      else hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 3, 1>), grid, block, 0, stream, g);
+      if (depth == 1) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 1, 1, false>), grid, block, 0, stream, g);
+      else if (depth == 2) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 2, 1, false>), grid, block, 0, stream, g);
+      else hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 3, 1, false>), grid, block, 0, stream, g);
     }
     MGW_HIP(hipGetLastError());
   }
@@ -333,19 +427,18 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
 }  // namespace
 
 size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision) {
-  const int ns = precision == MGW_BF16X6 ? 3 : 2;
-  const size_t ppad = (size_t)(naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
-  const size_t npad = (size_t)(npole + 2 * KT - 1) / (2 * KT) * (2 * KT);
-  return ppad * npad * ns * sizeof(__bf16);
+  const SplitLayout L = split_layout(naux, npole, precision == MGW_BF16X6 ? 3 : 2, chi_ksplit());
+  return L.planes_per_f + L.part_per_f;
 }
 
-// Workspace of the split planes: all frequencies when they fit in
-// max(4 x the FP64 operand, 2 GiB), else as many as fit (at least one).
+// Workspace of the split planes (and K-split partial tiles): all frequencies
+// when they fit in max(4 x the FP64 operand, 2 GiB), else as many as fit (at
+// least one).
 size_t chi0_split_workspace(int naux, int npole, int nomega, int precision) {
   const size_t per_f = chi0_split_bytes_per_frequency(naux, npole, precision);
   const size_t budget = std::max<size_t>((size_t)32 * naux * npole, (size_t)2 << 30);
   const size_t nf = std::max<size_t>(1, std::min<size_t>((size_t)nomega, budget / per_f));
-  return nf * per_f;
+  return nf * per_f + 256;
 }
 
 int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int nomega, int precision,
@@ -353,9 +446,9 @@ int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int 
   if (naux <= 0 || nomega <= 0) return MGW_OK;
   MGW_REQUIRE(npole > 0, MGW_ERR_ARG, "chi0 split: empty transition space");
   if (precision == MGW_BF16X3)
-    return split_and_syrk<2>(x, naux, npole, sw, nomega, static_cast<__bf16*>(work), work_bytes, chi, stream);
+    return split_and_syrk<2>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
   if (precision == MGW_BF16X6)
-    return split_and_syrk<3>(x, naux, npole, sw, nomega, static_cast<__bf16*>(work), work_bytes, chi, stream);
+    return split_and_syrk<3>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
   set_error("chi0 split: precision must be BF16X3 or BF16X6");
   return MGW_ERR_ARG;
 }

@@ -10,7 +10,7 @@
 // so the self-energy never needs the inverse itself.
 //
 // Blocked right-looking factorisation with 64-wide block columns: a
-// diagonal-block kernel factors L_kk in LDS and replaces it by its inverse
+// diagonal-block kernel factors L_kk in registers and replaces it by its inverse
 // T_k = L_kk^-1; the panel L21 = A21 T_k^T and the trailing update
 // A22 -= L21 L21^T are batched MFMA GEMMs. The inverse is then built from
 // the last block column backwards: (L^-1)21 = -(L^-1)22 L21 T_k, two GEMMs,
@@ -25,65 +25,76 @@ namespace {
 
 constexpr int NB = CHOL_NB;
 
-// One workgroup per matrix: factor the kb x kb diagonal block at (k, k),
-// add sum ln L_jj to logdet, write L_kk^-1 (zeros above the diagonal) in place.
+// Broadcast lane `lane` (a compile-time constant after unrolling) of a double.
+__device__ inline double bcast(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// One wave per matrix, lane i holding row i of the kb x kb diagonal block at
+// (k, k) in registers (rows beyond kb are identity rows): factor it, add
+// sum ln L_jj to logdet, and write L_kk^-1 (zeros above the diagonal) in
+// place. Fully unrolled: the column broadcasts are v_readlane, no barriers.
 // A non-positive pivot records its 1-based column in info (first failure
 // only) and is replaced by 1 so the batch stays finite.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
 chol_diag_kernel(double* __restrict__ A, int n, int64_t strideA, int k, int kb, double* __restrict__ logdet,
                  int32_t* __restrict__ info) {
-  __shared__ double L[NB][NB + 1];
-  __shared__ int failed;
-  const int f = blockIdx.x, tid = threadIdx.x;
+  const int f = blockIdx.x, i = threadIdx.x;
   double* __restrict__ a = A + (int64_t)f * strideA + k + (int64_t)k * n;
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int i = e % NB, j = e / NB;
-    L[i][j] = (i < kb && j < kb && i >= j) ? a[i + (int64_t)j * n] : 0.0;
-  }
-  if (tid == 0) failed = 0;
-  __syncthreads();
-  for (int j = 0; j < kb; ++j) {
-    if (tid == 0) {
-      double d = L[j][j];
-      if (!(d > 0.0)) {
-        if (!failed) failed = k + j + 1;
-        d = 1.0;
-      }
-      L[j][j] = sqrt(d);
+  double x[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+    x[j] = i < kb ? (j <= i ? a[i + (int64_t)j * n] : 0.0) : (i == j ? 1.0 : 0.0);
+  int failed = 0;
+  double ld = 0.0;
+  // right-looking Cholesky: x[j] of lane i becomes L(i, j)
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    double piv = bcast(x[j], j);
+    if (!(piv > 0.0)) {
+      if (!failed && j < kb) failed = k + j + 1;
+      piv = 1.0;
     }
-    __syncthreads();
-    const double djj = L[j][j];
-    for (int i = j + 1 + tid; i < kb; i += 256) L[i][j] /= djj;
-    __syncthreads();
-    const int m = kb - j - 1;
-    for (int e = tid; e < m * m; e += 256) {
-      const int i = j + 1 + e % m, l = j + 1 + e / m;
-      if (l <= i) L[i][l] -= L[i][j] * L[l][j];
+    const double d = sqrt(piv), rd = 1.0 / d;
+    if (j < kb) ld += log(d);
+    x[j] = (i == j) ? d : (i > j ? x[j] * rd : x[j]);
+#pragma unroll
+    for (int l = j + 1; l < NB; ++l) {
+      const double llj = bcast(x[j], l);
+      if (i >= l) x[l] -= x[j] * llj;
     }
-    __syncthreads();
   }
-  if (tid == 0) {
-    double s = 0.0;
-    for (int j = 0; j < kb; ++j) s += log(L[j][j]);
-    logdet[f] = (k == 0 ? 0.0 : logdet[f]) + s;
+  // inverse by rows: T(l, 0..l) is final once lane l has divided by L(l, l);
+  // every later row then subtracts L(i, l) T(l, j)
+  double t[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) t[j] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+  for (int l = 0; l < NB; ++l) {
+    const double rll = 1.0 / bcast(x[l], l);
+    if (i == l) {
+#pragma unroll
+      for (int j = 0; j <= l; ++j) t[j] *= rll;
+    }
+    const double lil = x[l];
+#pragma unroll
+    for (int j = 0; j <= l; ++j) {
+      const double tlj = bcast(t[j], l);
+      if (i > l) t[j] -= lil * tlj;
+    }
+  }
+  if (i == 0) {
+    logdet[f] = (k == 0 ? 0.0 : logdet[f]) + ld;
     if (k == 0) info[f] = 0;
     if (failed && info[f] == 0) info[f] = failed;
   }
-  // in-place inverse, last column first: T(i, j) = -sum_{l=j+1..i} T(i, l) L(l, j) / L(j, j)
-  for (int j = kb - 1; j >= 0; --j) {
-    const double ljj = L[j][j];
-    const int i = j + 1 + tid;
-    double x = 0.0;
-    if (i < kb)
-      for (int l = j + 1; l <= i; ++l) x += L[i][l] * L[l][j];
-    __syncthreads();
-    if (i < kb) L[i][j] = -x / ljj;
-    if (tid == 0) L[j][j] = 1.0 / ljj;
-    __syncthreads();
-  }
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int i = e % NB, j = e / NB;
-    if (i < kb && j < kb) a[i + (int64_t)j * n] = (i >= j) ? L[i][j] : 0.0;
+  if (i < kb) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+      if (j < kb) a[i + (int64_t)j * n] = (j <= i) ? t[j] : 0.0;
   }
 }
 
@@ -141,7 +152,7 @@ int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* 
   // factorisation
   for (int k = 0; k < n; k += NB) {
     const int kb = std::min(NB, n - k);
-    hipLaunchKernelGGL(chol_diag_kernel, dim3(nbatch), dim3(256), 0, stream, A, n, s2, k, kb, logdet, info);
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(nbatch), dim3(64), 0, stream, A, n, s2, k, kb, logdet, info);
     MGW_HIP(hipGetLastError());
     const int rest = n - k - kb;
     if (rest <= 0) continue;

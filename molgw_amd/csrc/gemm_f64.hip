@@ -428,8 +428,18 @@ int gemm_f64(const GemmF64& h, hipStream_t stream) {
   // Skinny N: a 256 x 32 tile keeps the matrix cores busy when N is small.
   if (h.N <= 32 && !h.lower_only) return launch_cfg<256, 32, 4, 1, 16, false, MGW_KCLASS_GEMM_AO2MO>(h, stream);
   // Defaults measured on MI355X: 256x128 with 16 waves for the large products
-  // (57-62 TF/s), 128x128 with 8 waves for the long-K symmetric updates.
+  // (57-62 TF/s), 128x128 with 8 waves for the long-K symmetric updates;
+  // 64x64 tiles when the default would leave most of the 256 CUs idle (the
+  // block columns of the batched Cholesky, small molecules).
   int cfg = h.lower_only ? 1 : 0;
+  {
+    auto wgs = [&](int bm, int bn) {
+      const int64_t tm = (h.M + bm - 1) / bm, tn = (h.N + bn - 1) / bn;
+      return (h.lower_only && bm == bn ? tm * (tm + 1) / 2 : tm * tn) * h.batch;
+    };
+    const int64_t have = h.lower_only ? wgs(128, 128) : wgs(256, 128);
+    if (have < 512 && wgs(64, 64) > have) cfg = 3;
+  }
   Tuner& t = tuner();
   const TuneKey key{h.klass, h.transA ? 1 : 0, h.transB ? 1 : 0, h.lower_only ? 1 : 0,
                     log2_bucket(h.M), log2_bucket(h.N), log2_bucket((int64_t)h.K * h.batch)};
