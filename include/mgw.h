@@ -139,6 +139,13 @@ int mgw_destroy_eri_3center(mgw_ctx* ctx);
 int mgw_set_mo(mgw_ctx* ctx, const double* c_matrix, int nbf, int nstate,
                const double* energy, const double* occupation, int where);
 
+/* Energies in G for the following self-energy calls (energy_g of
+ * selfenergy_evaluation, src/m_selfenergy_evaluation.f90:164-183: the QP
+ * energies of the previous step of a GnW0 / GnWn = evGW run), nstate values,
+ * host. The resident W and the energies in W (those of mgw_set_mo) are kept;
+ * NULL returns to the energies of mgw_set_mo, as does mgw_set_mo itself. */
+int mgw_set_energy_g(mgw_ctx* ctx, const double* energy_g);
+
 /* ------------------------------------------------ A2: AO -> MO transform */
 /* calculate_eri_3center_mo(c_matrix, mstate_min, mstate_max, nstate_min,
  * nstate_max) (src/m_eri_ao_mo.f90:416-582). Result eri_3center_mo(P, m, n)

@@ -44,6 +44,7 @@ SIGNATURES = {
     'mgw_ctx_trim': (C.c_int, [_vp, C.c_int]),
     'mgw_autotune': (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
     'mgw_ctx_graphs': (C.c_int, [_vp, C.c_int]),
+    'mgw_set_energy_g': (C.c_int, [_vp, C.POINTER(C.c_double)]),
     'mgw_memory_report': (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'mgw_precision_escalations': (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     'mgw_footprint_bytes': (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -86,10 +86,12 @@ struct mgw_ctx {
   // A1: resident RI tensor, dense B(P, k, l)
   bool has_ri = false;
   int nbf = 0, naux = 0;
-  // MO data of the single spin channel
+  // MO data of the single spin channel; energy_g (empty: energy) are the
+  // energies in G of the self-energy, energy those of W
   bool has_mo = false;
   int nstate = 0;
-  std::vector<double> energy, occupation;
+  std::vector<double> energy, occupation, energy_g;
+  const std::vector<double>& energy_in_g() const { return energy_g.empty() ? energy : energy_g; }
   // A2 module state eri_3center_mo(P, m, n)
   bool has_eri_mo = false;
   int mlo = 0, mhi = 0, nlo = 0, nhi = 0;
@@ -444,8 +446,9 @@ int gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta,
   MGW_TRY(ao2mo(c, r->nsemin, r->nsemax, glo, ghi, epi, precision));
   // small per-state vectors
   std::vector<double> e0p(se_energy0, se_energy0 + nse), ei(ng), fi(ng), ep(nse), occp(nse);
-  for (int k = 0; k < ng; ++k) { ei[k] = c->energy[glo - 1 + k]; fi[k] = c->occupation[glo - 1 + k]; }
-  for (int k = 0; k < nse; ++k) { ep[k] = c->energy[r->nsemin - 1 + k]; occp[k] = c->occupation[r->nsemin - 1 + k]; }
+  const std::vector<double>& eg = c->energy_in_g();
+  for (int k = 0; k < ng; ++k) { ei[k] = eg[glo - 1 + k]; fi[k] = c->occupation[glo - 1 + k]; }
+  for (int k = 0; k < nse; ++k) { ep[k] = eg[r->nsemin - 1 + k]; occp[k] = c->occupation[r->nsemin - 1 + k]; }
   double *om_d, *e0p_d, *ei_d, *fi_d, *ep_d, *occp_d;
   MGW_TRY(upload(c, "se_omega", omega, nomega, &om_d));
   MGW_TRY(upload(c, "se_e0p", e0p.data(), nse, &e0p_d));
@@ -701,7 +704,7 @@ int gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int ncalc, const double*
   }
   MGW_TRY(launch_dot_reduce(part, nrb, ncol, nomega, bn2, v, c->stream));
   std::vector<double> ep(np);
-  for (int k = 0; k < np; ++k) ep[k] = c->energy[glo - 1 + k];
+  for (int k = 0; k < np; ++k) ep[k] = c->energy_in_g()[glo - 1 + k];
   MGW_TRY(upload(c, "sgi_omega", c->chi_omega.data(), nomega, &om_d));
   MGW_TRY(upload(c, "sgi_weight", c->chi_weight.data(), nomega, &wt_d));
   MGW_TRY(upload(c, "sgi_ep", ep.data(), np, &ep_d));
@@ -859,6 +862,16 @@ int mgw_ctx_destroy(mgw_ctx* c) {
   rocblas_destroy_handle(c->blas);
   MGW_HIP(hipStreamDestroy(c->stream));
   delete c;
+  return MGW_OK;
+}
+
+int mgw_set_energy_g(mgw_ctx* c, const double* energy_g) {
+  MGW_CTX(c);
+  MGW_REQUIRE(c->has_mo, MGW_ERR_STATE, "MO data not set (mgw_set_mo)");
+  if (energy_g) c->energy_g.assign(energy_g, energy_g + c->nstate);
+  else c->energy_g.clear();
+  c->has_sigma = false;
+  c->has_sigma_calc = false;
   return MGW_OK;
 }
 
@@ -1033,6 +1046,7 @@ int mgw_set_mo(mgw_ctx* c, const double* c_matrix, int nbf, int nstate, const do
   MGW_TRY(upload(c, "occupation", c->occupation.data(), nstate, &od));
   MGW_HIP(hipStreamSynchronize(c->stream));
   c->nstate = nstate;
+  c->energy_g.clear();
   c->has_mo = true;
   c->has_eri_mo = false;
   c->has_wpol = false;
@@ -1189,7 +1203,7 @@ int mgw_g0w0(mgw_ctx* c, const mgw_g0w0_params* prm, const mgw_ranges* r, const 
   if (prm->technique == MGW_TECH_ONE_SHOT) {
     // se_init(one_shot): grid centred on each energy0 (src/m_selfenergy_tools.f90:682-690)
     for (int k = 0; k < nomega; ++k) omega[k] = prm->step_sigma * (k - nom);
-    for (int k = 0; k < nse; ++k) se0[k] = c->energy[r->nsemin - 1 + k];
+    for (int k = 0; k < nse; ++k) se0[k] = c->energy_in_g()[r->nsemin - 1 + k];
     MGW_TRY(polarizability_casida(c, r, prm->spin_fact, &en_rpa, nullptr, prec));
     MGW_TRY(gw_selfenergy(c, r, prm->spin_fact, prm->eta, nomega, omega.data(), se0.data(), &en_gw, prec));
   } else if (prm->technique == MGW_TECH_IMAGINARY_PADE) {
@@ -1197,7 +1211,7 @@ int mgw_g0w0(mgw_ctx* c, const mgw_g0w0_params* prm, const mgw_ranges* r, const 
                 "imaginary-axis technique needs nomega_chi_imag and nomega_sigma_calc");
     MGW_REQUIRE(r->nhomo_G >= 1 && r->nhomo_G < nstate, MGW_ERR_ARG, "nhomo_G out of range");
     // se_init(imaginary_axis_pade) (src/m_selfenergy_tools.f90:598-614)
-    const double efermi = 0.5 * (c->energy[r->nhomo_G - 1] + c->energy[r->nhomo_G]);
+    const double efermi = 0.5 * (c->energy_in_g()[r->nhomo_G - 1] + c->energy_in_g()[r->nhomo_G]);
     for (int k = 0; k < nomega; ++k) omega[k] = efermi + prm->step_sigma * (k - nom);
     std::fill(se0.begin(), se0.end(), 0.0);
     std::vector<double> wc(2 * prm->nomega_sigma_calc);

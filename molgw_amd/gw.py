@@ -180,6 +180,55 @@ class Engine:
         check(self._lib.mgw_memory_report(self._h, C.byref(cur), C.byref(peak)))
         return cur.value, peak.value
 
+    def set_energy_g(self, energy_g=None) -> None:
+        """Energies in G for the following self-energy calls (W is kept);
+        None returns to the energies of set_mo."""
+        if energy_g is None:
+            check(self._lib.mgw_set_energy_g(self._h, None))
+            return
+        e = _f64(energy_g).reshape(-1)
+        if len(e) != self.nstate:
+            raise ValueError('energy_g length must equal nstate')
+        check(self._lib.mgw_set_energy_g(self._h, _ptr(e)))
+
+    def evgw(self, nstep: int, approx: str = 'GnWn', rng: StateRanges | None = None, exchange_m_vxc=None,
+             eta: float = 0.001, spin_fact: float = 2.0, precision: str = 'fp64'):
+        """Eigenvalue self-consistent GW with analytic W (postscf GNW0 / GNWN =
+        EVGW; selfenergy_evaluation with EVSC, src/m_selfenergy_evaluation.f90:78-182,
+        589-606). Step n evaluates Sigma_c at the QP energies of step n-1 (step 1:
+        the set_mo energies) with G built from them; W is rebuilt from them for
+        GnWn and kept from step 1 for GnW0. E(p) = e(p) + Re Sigma_c(p) +
+        (Sigma_x - Vxc)(p) in [nsemin, nsemax] (find_qp_energy_linearization
+        without a grid, src/m_selfenergy_tools.f90:236-240), e(p) elsewhere.
+        Returns the QP energies of every step; the engine is left with the
+        set_mo energies."""
+        if approx not in ('GnW0', 'GnWn'):
+            raise ValueError('approx must be GnW0 or GnWn')
+        if rng is None:
+            rng = selfenergy_set_state_range(self.occupation)
+        e = self.energy.copy()
+        occ = self.occupation.copy()
+        exc = np.zeros(len(e)) if exchange_m_vxc is None else _f64(exchange_m_vxc).reshape(-1)
+        win = slice(rng.nsemin - 1, rng.nsemax)
+        eg = e.copy()
+        history = []
+        try:
+            for step in range(nstep):
+                if approx == 'GnWn':
+                    self.set_mo(self._c_matrix, eg, occ)          # energies in W (and in G)
+                    self.polarizability(rng, spin_fact=spin_fact, precision=precision)
+                elif step == 0:
+                    self.polarizability(rng, spin_fact=spin_fact, precision=precision)
+                self.set_energy_g(eg)
+                sig, _ = self.gw_selfenergy(rng, np.zeros(1), eg[win], eta=eta, spin_fact=spin_fact)
+                new = e.copy()
+                new[win] = e[win] + sig[0].real + exc[win]
+                history.append(new)
+                eg = new
+        finally:
+            self.set_mo(self._c_matrix, e, occ)
+        return history
+
     def graphs(self, enable: bool = True) -> None:
         """Replay the batched Cholesky sequence from a captured HIP graph."""
         check(self._lib.mgw_ctx_graphs(self._h, 1 if enable else 0))
@@ -223,6 +272,7 @@ class Engine:
                                    _lib.MGW_MEM_HOST))
         self.nstate = nstate
         self.energy, self.occupation = e.copy(), o.copy()
+        self._c_matrix = c
 
     # ------------------------------------------------------------ A2
     def calculate_eri_3center_mo(self, mstate_min, mstate_max, nstate_min, nstate_max, copy=True):

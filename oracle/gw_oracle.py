@@ -665,3 +665,57 @@ def g0w0_pade(eri_3center, index_basis, c_matrix, energy, occupation, nomega_chi
     res.extra = {'sigma_calc': sigma_calc, 'omega_calc': omega_calc, 'chi': chi,
                  'omega_im': omega_im, 'weight': weight}
     return res
+
+
+# ---------------------------------------------------------------------------
+# eigenvalue self-consistent GW (postscf GNW0 / GNWN = EVGW)
+# ---------------------------------------------------------------------------
+def evgw(eri_3center, index_basis, c_matrix, energy, occupation, nstep: int, approx: str = 'GnWn',
+         exchange_m_vxc=None, eta: float = 0.001, spin_fact: float = 2.0,
+         ranges: StateRanges | None = None):
+    """``selfenergy_evaluation`` with selfenergy_technique = EVSC
+    (src/m_selfenergy_evaluation.f90:78-182, 589-606; postscf keywords at
+    src/m_inputparam.f90:182-189), analytic W.
+
+    Step n (1-based): energy_g = the QP energies of step n-1 (``read_energy_qp``;
+    step 1: ``energy``, :164-170); energy_w = energy_g for GnWn, ``energy``
+    for GnW0 (:174-178), and GnW0 keeps the W of step 1 (:200-202).
+    se_init(EVSC) puts one point omega = 0 on top of se%energy0 = energy_g
+    (src/m_selfenergy_tools.f90:594-597, 683-684), so Sigma_c is evaluated at
+    energy_g(p); find_qp_energy_linearization without a grid (:236-240) gives
+    E(p) = energy(p) + Re Sigma_c(p) + (Sigma_x - Vxc)(p) in [nsemin, nsemax]
+    and energy(p) elsewhere (:216-217).
+
+    Returns (list of the QP energies of each step, list of the W poles used at
+    each step)."""
+    energy = np.asarray(energy, dtype=np.float64)
+    occupation = np.asarray(occupation, dtype=np.float64)
+    nstate = len(energy)
+    if exchange_m_vxc is None:
+        exchange_m_vxc = np.zeros(nstate)
+    if approx not in ('GnW0', 'GnWn'):
+        raise ValueError('approx must be GnW0 or GnWn')
+    rng = ranges or selfenergy_set_state_range(occupation[:, None])
+    ttable = transition_table(occupation[:, None], spin_fact, rng.ncore_W, rng.nvirtual_W)
+    wlo, whi = rng.ncore_W + 1, rng.nvirtual_W - 1
+    eri_w = calculate_eri_3center_mo(eri_3center, index_basis, c_matrix, wlo, whi, wlo, whi)
+    x_ia = casida_ia_matrix(eri_w, ttable, wlo)
+    del eri_w
+    eri_pi = calculate_eri_3center_mo(eri_3center, index_basis, c_matrix, rng.nsemin, rng.nsemax,
+                                      rng.ncore_G + 1, rng.nvirtual_G - 1)
+    win = slice(rng.nsemin - 1, rng.nsemax)
+    energy_g = energy.copy()
+    history, poles = [], []
+    pole = w_s = None
+    for step in range(nstep):
+        if approx == 'GnWn' or step == 0:
+            energy_w = energy_g if approx == 'GnWn' else energy
+            pole, w_s, _, _ = polarizability_casida(x_ia, energy_w, ttable, spin_fact)
+        sigma, _ = gw_selfenergy(eri_pi, w_s, pole, energy_g, occupation, rng, np.zeros(1),
+                                 energy_g[win], eta, spin_fact)
+        new = energy.copy()
+        new[win] = energy[win] + sigma[0].real + exchange_m_vxc[win]
+        history.append(new)
+        poles.append(pole)
+        energy_g = new
+    return history, poles

@@ -281,3 +281,25 @@ def test_c_pole_sum_equals_numpy_restatement():
     b, eb = O.gw_selfenergy_c(eri_w, ws, pole, s.energy, s.occupation, rng, omega, e0, 0.001)
     np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-13 * np.abs(a).max())
     assert abs(ea - eb) < 1e-12 * abs(ea)
+
+
+def test_evgw_first_step_is_the_g0w0_correction_and_w_policy():
+    """evGW (EVSC) step 1 evaluates Sigma_c at the input energies: it must
+    equal e + Re Sigma_c(omega = 0) of the pinned one-shot G0W0 on its grid
+    centre; GnW0 keeps the W of step 1, GnWn rebuilds it."""
+    from molgw_amd.workloads import synthetic_system
+    s = synthetic_system(30, 100, 5, seed=17, scale=0.1)
+    exc = np.linspace(-0.01, 0.01, s.nstate)
+    g = O.g0w0_casida(s.eri_3center, s.index_basis, s.c_matrix, s.energy, s.occupation, exchange_m_vxc=exc,
+                      nomega_sigma=3)
+    rng = O.selfenergy_set_state_range(s.occupation[:, None])
+    win = slice(rng.nsemin - 1, rng.nsemax)
+    hn, pn = O.evgw(s.eri_3center, s.index_basis, s.c_matrix, s.energy, s.occupation, 3, 'GnWn', exchange_m_vxc=exc)
+    h0, p0 = O.evgw(s.eri_3center, s.index_basis, s.c_matrix, s.energy, s.occupation, 3, 'GnW0', exchange_m_vxc=exc)
+    np.testing.assert_allclose(hn[0][win], s.energy[win] + g.sigma[1].real + exc[win], rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(hn[0], h0[0])
+    np.testing.assert_array_equal(p0[0], p0[2])                       # GnW0: W of step 1
+    assert np.abs(pn[1] - pn[0]).max() > 1e-6                         # GnWn: W from the new energies
+    outside = np.ones(s.nstate, bool)
+    outside[win] = False
+    np.testing.assert_array_equal(hn[2][outside], s.energy[outside])  # states outside the window keep e
