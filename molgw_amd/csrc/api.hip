@@ -601,7 +601,7 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
   double *logdet, *work, *frob;
   MGW_TRY(dbuf(c, "solver_info", std::max(64, nomega), &info_d));
   MGW_TRY(dbuf(c, "chol_logdet", nomega, &logdet));
-  MGW_TRY(dbuf(c, "chol_work", (size_t)naux * CHOL_NB * nomega, &work));
+  MGW_TRY(dbuf(c, "chol_work", chol_work_doubles(naux) * nomega, &work));
   MGW_TRY(dbuf(c, "chol_frob", (size_t)FROB_BLOCKS * nomega, &frob));
   const double chol_flops = 2.0 / 3.0 * (double)naux * naux * naux;
   {
@@ -970,12 +970,13 @@ int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int techn
               + na * npole;                       // w_s
   } else {
     stage_w = na * nb * no + na * npole + na * na * (int64_t)nomega_chi_imag + npole * nomega_chi_imag +
-              na * (int64_t)mgw::CHOL_NB * nomega_chi_imag;
+              (int64_t)mgw::chol_work_doubles((int)na) * nomega_chi_imag;
     if (precision == MGW_BF16X3 || precision == MGW_BF16X6)
       stage_w += (int64_t)(mgw::chi0_split_workspace((int)na, (int)npole, nomega_chi_imag, precision) / 8);
   }
   const int64_t stage_s = na * nb * ns + na * ns * ns + na * npole + (int64_t(1) << 27);
-  return 8 * (d + std::max(stage_w, stage_s));
+  // the temporaries of the W stage stay cached while the self-energy runs
+  return 8 * (d + stage_w + stage_s);
 }
 
 int mgw_set_eri_3center(mgw_ctx* c, const double* eri, int64_t npair, int nauxil,

@@ -9,12 +9,11 @@
 //   B^T chi B = ||L^-1 B||^2 - ||B||^2   (chi = (1 - chi0)^-1 - 1),
 // so the self-energy never needs the inverse itself.
 //
-// Blocked right-looking factorisation with 64-wide block columns: a
-// diagonal-block kernel factors L_kk in registers and replaces it by its inverse
-// T_k = L_kk^-1; the panel L21 = A21 T_k^T and the trailing update
-// A22 -= L21 L21^T are batched MFMA GEMMs. The inverse is then built from
-// the last block column backwards: (L^-1)21 = -(L^-1)22 L21 T_k, two GEMMs,
-// the second with a triangular A whose K range stops at the diagonal.
+// Recursive: the matrix is halved (at a multiple of 64) until 64 x 64
+// leaves, which a one-wave kernel factors and inverts in registers; the
+// off-diagonal work is a few large batched MFMA GEMMs per level (the trailing
+// update has K = m/2), so the matrix is swept O(log n) times instead of once
+// per 64-wide block column.
 #include "mgw_internal.h"
 
 #include <algorithm>
@@ -143,72 +142,106 @@ __global__ void zero_upper_kernel(double* __restrict__ m, int n) {
 
 }  // namespace
 
+size_t chol_work_doubles(int n) {
+  const size_t h = (size_t)(n + 1) / 2 + NB;
+  return h * h;
+}
+
+namespace {
+
+// Zero the strict upper triangle of the diagonal block [k0, k0 + m) of
+// every matrix (leading dimension n, stride n^2).
+__global__ void zero_upper_block_kernel(double* __restrict__ A, int n, int k0, int m) {
+  const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (i < m && j < m && i < j) A[(int64_t)blockIdx.z * n * n + (k0 + i) + (int64_t)(k0 + j) * n] = 0.0;
+}
+
+struct CholCtx {
+  double* A;
+  int n, nbatch;
+  double* logdet;
+  int32_t* info;
+  double* work;
+  int64_t sw;           // work doubles per matrix
+  hipStream_t stream;
+};
+
+// Recursive inverse Cholesky factor of the diagonal block [k0, k0 + m):
+//   [A11    ]   factor and invert A11 (recursion), L21 = A21 L11^-T,
+//   [A21 A22]   A22 -= L21 L21^T, factor and invert A22 (recursion),
+//               (L^-1)21 = -(L^-1)22 L21 (L^-1)11.
+// The products are large batched MFMA GEMMs; the 64 x 64 leaves are factored
+// and inverted in registers by chol_diag_kernel.
+int chol_rec(const CholCtx& c, int k0, int m) {
+  const int n = c.n;
+  const int64_t s2 = (int64_t)n * n;
+  if (m <= NB) {
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(c.nbatch), dim3(64), 0, c.stream, c.A, n, s2, k0, m, c.logdet,
+                       c.info);
+    MGW_HIP(hipGetLastError());
+    return MGW_OK;
+  }
+  int m1 = ((m + 1) / 2 + NB - 1) / NB * NB;   // a multiple of the leaf size
+  if (m1 >= m) m1 = m - NB;
+  const int m2 = m - m1;
+  double* a11 = c.A + k0 + (int64_t)k0 * n;
+  double* a21 = a11 + m1;
+  double* a22 = a11 + m1 + (int64_t)m1 * n;
+  MGW_TRY(chol_rec(c, k0, m1));
+  // (L^-1)11 with exact zeros above the diagonal: a plain operand below
+  hipLaunchKernelGGL(zero_upper_block_kernel, dim3((m1 + 15) / 16, (m1 + 15) / 16, c.nbatch), dim3(256), 0,
+                     c.stream, c.A, n, k0, m1);
+  MGW_HIP(hipGetLastError());
+  // L21 = A21 (L^-1)11^T   (work: m2 x m1, leading dimension m2)
+  GemmF64 p;
+  p.M = m2; p.N = m1; p.K = m1;
+  p.A = a21; p.lda = n; p.strideA = s2;
+  p.B = a11; p.ldb = n; p.strideB = s2; p.transB = true;
+  p.C = c.work; p.ldc = m2; p.strideC = c.sw;
+  p.batch = c.nbatch;
+  MGW_TRY(gemm_chol(p, c.stream));
+  // A22 -= L21 L21^T (lower triangle)
+  GemmF64 u;
+  u.M = m2; u.N = m2; u.K = m1;
+  u.A = c.work; u.lda = m2; u.strideA = c.sw;
+  u.B = c.work; u.ldb = m2; u.strideB = c.sw; u.transB = true;
+  u.C = a22; u.ldc = n; u.strideC = s2;
+  u.alpha = -1.0; u.beta = 1.0;
+  u.lower_only = true; u.no_mirror = true;
+  u.batch = c.nbatch;
+  MGW_TRY(gemm_chol(u, c.stream));
+  const int64_t cnt = (int64_t)m2 * m1;
+  hipLaunchKernelGGL(copy_block_kernel, dim3((unsigned)((cnt + 255) / 256), c.nbatch), dim3(256), 0, c.stream,
+                     c.work, (int64_t)m2, c.sw, a21, (int64_t)n, s2, m2, m1);
+  MGW_HIP(hipGetLastError());
+  MGW_TRY(chol_rec(c, k0 + m1, m2));
+  // (L^-1)21 = -(L^-1)22 (L21 (L^-1)11)
+  GemmF64 q;
+  q.M = m2; q.N = m1; q.K = m1;
+  q.A = a21; q.lda = n; q.strideA = s2;
+  q.B = a11; q.ldb = n; q.strideB = s2;
+  q.C = c.work; q.ldc = m2; q.strideC = c.sw;
+  q.batch = c.nbatch;
+  MGW_TRY(gemm_chol(q, c.stream));
+  GemmF64 t;
+  t.M = m2; t.N = m1; t.K = m2;
+  t.A = a22; t.lda = n; t.strideA = s2; t.tri_a = true;
+  t.B = c.work; t.ldb = m2; t.strideB = c.sw;
+  t.C = a21; t.ldc = n; t.strideC = s2;
+  t.alpha = -1.0;
+  t.batch = c.nbatch;
+  return gemm_chol(t, c.stream);
+}
+
+}  // namespace
+
 int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* info, double* work,
                          hipStream_t stream) {
   if (n <= 0 || nbatch <= 0) return MGW_OK;
   MGW_REQUIRE(nbatch <= 65535, MGW_ERR_ARG, "chol_inverse_batched: batch too large");
-  const int64_t s2 = (int64_t)n * n;
-  const int64_t sw = (int64_t)n * NB;    // W(f): rest x kb, leading dimension n
-  // factorisation
-  for (int k = 0; k < n; k += NB) {
-    const int kb = std::min(NB, n - k);
-    hipLaunchKernelGGL(chol_diag_kernel, dim3(nbatch), dim3(64), 0, stream, A, n, s2, k, kb, logdet, info);
-    MGW_HIP(hipGetLastError());
-    const int rest = n - k - kb;
-    if (rest <= 0) continue;
-    double* a21 = A + (k + kb) + (int64_t)k * n;
-    double* akk = A + k + (int64_t)k * n;
-    double* a22 = A + (k + kb) + (int64_t)(k + kb) * n;
-    // L21 = A21 T_k^T  (T_k = L_kk^-1 sits in the diagonal block)
-    GemmF64 p;
-    p.M = rest; p.N = kb; p.K = kb;
-    p.A = a21; p.lda = n; p.strideA = s2;
-    p.B = akk; p.ldb = n; p.strideB = s2; p.transB = true;
-    p.C = work; p.ldc = n; p.strideC = sw;
-    p.batch = nbatch;
-    MGW_TRY(gemm_chol(p, stream));
-    // A22 -= L21 L21^T (lower triangle only)
-    GemmF64 u;
-    u.M = rest; u.N = rest; u.K = kb;
-    u.A = work; u.lda = n; u.strideA = sw;
-    u.B = work; u.ldb = n; u.strideB = sw; u.transB = true;
-    u.C = a22; u.ldc = n; u.strideC = s2;
-    u.alpha = -1.0; u.beta = 1.0;
-    u.lower_only = true; u.no_mirror = true;
-    u.batch = nbatch;
-    MGW_TRY(gemm_chol(u, stream));
-    const int64_t cnt = (int64_t)rest * kb;
-    hipLaunchKernelGGL(copy_block_kernel, dim3((unsigned)((cnt + 255) / 256), nbatch), dim3(256), 0, stream,
-                       work, (int64_t)n, sw, a21, (int64_t)n, s2, rest, kb);
-    MGW_HIP(hipGetLastError());
-  }
-  // inverse, last block column first: (L^-1)21 = -(L^-1)22 (L21 T_k)
-  const int nblk = (n + NB - 1) / NB;
-  for (int b = nblk - 1; b >= 0; --b) {
-    const int k = b * NB;
-    const int kb = std::min(NB, n - k);
-    const int rest = n - k - kb;
-    if (rest <= 0) continue;
-    double* a21 = A + (k + kb) + (int64_t)k * n;
-    double* akk = A + k + (int64_t)k * n;
-    double* a22 = A + (k + kb) + (int64_t)(k + kb) * n;
-    GemmF64 p;
-    p.M = rest; p.N = kb; p.K = kb;
-    p.A = a21; p.lda = n; p.strideA = s2;
-    p.B = akk; p.ldb = n; p.strideB = s2;
-    p.C = work; p.ldc = n; p.strideC = sw;
-    p.batch = nbatch;
-    MGW_TRY(gemm_chol(p, stream));
-    GemmF64 t;
-    t.M = rest; t.N = kb; t.K = rest;
-    t.A = a22; t.lda = n; t.strideA = s2; t.tri_a = true;
-    t.B = work; t.ldb = n; t.strideB = sw;
-    t.C = a21; t.ldc = n; t.strideC = s2;
-    t.alpha = -1.0;
-    t.batch = nbatch;
-    MGW_TRY(gemm_chol(t, stream));
-  }
-  return MGW_OK;
+  const CholCtx c{A, n, nbatch, logdet, info, work, (int64_t)chol_work_doubles(n), stream};
+  return chol_rec(c, 0, n);
 }
 
 int launch_lower_frob2(const double* m, int n, int nbatch, double* partial, hipStream_t s) {

@@ -135,8 +135,9 @@ int gemm_f64_rowblocks(int M);
 // matrices: A_f (n x n, lower triangle, stride n^2) becomes L_f^-1 (lower
 // triangle; the strict upper triangle is left undefined). logdet[f] receives
 // sum_i ln L_f(i, i); info[f] the 1-based column of the first non-positive
-// pivot (0 on success). work: n * 64 doubles per matrix.
+// pivot (0 on success). work: chol_work_doubles(n) doubles per matrix.
 constexpr int CHOL_NB = 64;
+size_t chol_work_doubles(int n);
 int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* info, double* work,
                          hipStream_t stream);
 // tr[f] = sum of squares of the lower triangle of M_f (n x n, stride n^2).

@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .gw import Engine, selfenergy_set_state_range
+from .gw import PRECISION, Engine, selfenergy_set_state_range
 
 MGW_TECH = {'one_shot': _lib.MGW_TECH_ONE_SHOT, 'imaginary_pade': _lib.MGW_TECH_IMAGINARY_PADE}
 
@@ -65,7 +65,7 @@ def footprint_bytes(req: Request) -> int:
     s = req.system
     lib = _lib.load()
     return int(lib.mgw_footprint_bytes(s.nbf, s.naux, s.nstate, s.nocc, MGW_TECH[req.technique],
-                                       req.nomega_chi_imag, _lib.MGW_FP64))
+                                       req.nomega_chi_imag, PRECISION[req.precision]))
 
 
 def device_memory_bytes(device: int = 0) -> int:

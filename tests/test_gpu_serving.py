@@ -56,3 +56,20 @@ def test_request_larger_than_budget_is_refused():
     with Server(nslots=1, hbm_budget=1 << 20) as srv:
         with pytest.raises(ValueError):
             srv.run(reqs)
+
+
+@pytest.mark.parametrize('technique,precision', [('imaginary_pade', 'bf16x6'), ('imaginary_pade', 'fp64'),
+                                                 ('one_shot', 'fp64')])
+def test_footprint_bounds_the_measured_peak(technique, precision):
+    """Admission relies on mgw_footprint_bytes: it must bound the HBM the
+    solve actually holds (the context's ledger peak)."""
+    from molgw_amd.gw import Engine
+    from molgw_amd.serving import footprint_bytes
+    s = synthetic_system(*alkane_shape(4), seed=77, scale=0.3)
+    req = Request(0, s, technique=technique, precision=precision, nomega_chi_imag=16, nomega_sigma_calc=8)
+    with Server(nslots=1, hbm_budget=64 << 30) as srv:
+        srv.run([req], realtime=False)
+        peak = srv.slots[0].memory()[1]
+    fp = footprint_bytes(req)
+    print(f'{technique} {precision}: peak {peak / 2**20:.1f} MiB, footprint {fp / 2**20:.1f} MiB')
+    assert peak <= fp
