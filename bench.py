@@ -341,6 +341,93 @@ def cpu_baseline_stream(spec, shapes, budget_s=15.0, window=8):
             'seconds_for_stream': seconds}
 
 
+RI_BUILD = (492, 1188, 'n-octane-shaped RI tensor build, cc-pVTZ/cc-pVTZ-RI shape (nbf=492, npair=121278, '
+            'nauxil=1188): inverse square root of the 2-center metric (FP64 eigensolve, eigenvalues > 1e-6 kept) '
+            'and B = (mu nu|Q) V^-1/2 (FP64)')
+
+
+def ri_build_bench(args):
+    """Metric V^-1/2 and B build (src/m_eri_calculate.f90:1049-1262, :1877-1892)
+    at the octane shape, (mu nu|Q) resident in HBM. A step is one metric
+    eigendecomposition plus one B build over all pairs."""
+    import torch
+    from molgw_amd.gw import Engine
+    nbf, naux, desc = RI_BUILD
+    k, l = np.triu_indices(nbf)
+    ib = np.vstack([k + 1, l + 1]).astype(np.int32)
+    npair = ib.shape[1]
+    rng = np.random.default_rng(args.seed)
+    # metric with a Gaussian-RI-like spectrum (1e-8 .. 1e2): a few eigenvalues dropped
+    q, _ = np.linalg.qr(rng.standard_normal((naux, naux)))
+    lam = np.geomspace(1e-8, 1e2, naux)
+    v = np.asfortranarray((q * lam) @ q.T)
+    v = 0.5 * (v + v.T)
+    g = torch.Generator(device='cuda:0')
+    g.manual_seed(args.seed)
+    tmp = torch.randn((naux, npair), device='cuda:0', dtype=torch.float64, generator=g)  # column-major (npair, naux)
+    tmp *= 0.1
+    eng = Engine(0)
+    for _ in range(args.warmup):
+        eng.inverse_sqrt_eri_2center(v, copy=False)
+        eng.build_eri_3center_device(tmp.data_ptr(), 1, npair, npair, ib, nbf)
+    eng.synchronize()
+    eng.profile(True)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        kept, low, _ = eng.inverse_sqrt_eri_2center(v, copy=False)
+        eng.build_eri_3center_device(tmp.data_ptr(), 1, npair, npair, ib, nbf)
+        lat.append(time.perf_counter() - ts)
+    eng.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = eng.profile_read()
+    eng.profile(False)
+    eng.close()
+    print(f'# ri-build: {args.steps} builds in {elapsed:.3f}s, kept {kept} of {naux}; per-class device ms: '
+          + ', '.join(f'{c}={x["ms"]:.1f}' for c, x in prof.items()), file=sys.stderr, flush=True)
+    roof = None
+    if 'gemm_other' in prof:
+        x = prof['gemm_other']
+        t_avg = x['ms'] / x['launches'] * 1e-3
+        ach = x['flops'] / x['launches'] / t_avg / 1e12
+        roof = {'bound': 'mfma', 'achieved': ach, 'peak': MFMA_F64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                'frac': ach / MFMA_F64_PEAK_TFLOPS, 'traffic': None, 'kernel': 'gemm_other (B build)',
+                'avg_launch_ms': x['ms'] / x['launches'], 'launches': x['launches']}
+    cpu = None
+    if not args.no_cpu_baseline:
+        # oracle on the full metric and on a sample of the pairs, scaled by npair / sample
+        from oracle import gw_oracle as O
+        sample = 16384
+        tmp_h = tmp[:, :sample].T.cpu().numpy()
+        c0 = time.perf_counter()
+        sq, _ = O.inverse_sqrt_eri_2center(v)
+        c1 = time.perf_counter()
+        O.build_eri_3center(tmp_h, sq)
+        c2 = time.perf_counter()
+        per = (c1 - c0) + (c2 - c1) * npair / sample
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            cores = os.cpu_count()
+        cpu = {'value': 1.0 / per, 'unit': 'builds/s', 'cores': min(cores, int(os.environ.get('OMP_NUM_THREADS', cores))),
+               'kind': 'port', 'sample': f'oracle (numpy eigh + matmul, OpenBLAS): full metric eigensolve, B for '
+                                         f'{sample} of {npair} pairs scaled linearly',
+               'seconds_per_build': per}
+    line = {
+        'metric': 'RI tensor builds/s (metric inverse square root + B)', 'value': args.steps / elapsed,
+        'unit': 'builds/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': 1e3 * elapsed / args.steps, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic (seeded metric with a 1e-8..1e2 spectrum, '
+                                                       'seeded (mu nu|Q))',
+        'config': {'workload': desc, 'nbf': nbf, 'npair': npair, 'nauxil': naux, 'nauxil_kept': kept},
+        'latency_ms': {'p50': 1e3 * _percentile(lat, 50), 'p99': 1e3 * _percentile(lat, 99)},
+        'roofline': roof, 'cpu_baseline': cpu,
+        'stages_ms_per_build': {c: x['ms'] / args.steps for c, x in prof.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -348,7 +435,7 @@ def main():
                     help='timed steps: solves (single-system workloads, default 5) or requests of the stream '
                          '(serving, batch64; default 64)')
     ap.add_argument('--warmup', type=int, default=None, help='untimed warm-up steps (default 1; streams: slots)')
-    ap.add_argument('--workload', default='serving', choices=sorted(WORKLOADS) + ['serving', 'batch64'])
+    ap.add_argument('--workload', default='serving', choices=sorted(WORKLOADS) + ['serving', 'batch64', 'ri-build'])
     ap.add_argument('--requests', type=int, default=None, help='stream / batch length (same as --steps)')
     ap.add_argument('--rate', type=float, default=20.0, help='Poisson arrival rate, requests/s (serving)')
     ap.add_argument('--slots', type=int, default=8, help='concurrent solve slots (serving, batch64)')
@@ -358,6 +445,9 @@ def main():
                     help='arithmetic of the AO->MO transforms and the chi0/Casida build '
                          '(default: bf16x6 for the streams, fp64 otherwise)')
     args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit('bench.py measures one GPU: every box here has one, and the path has no multi-GPU code; '
+                 'run with --gpus 1')
     stream = args.workload in ('serving', 'batch64')
     if args.precision is None:
         args.precision = 'bf16x6' if stream else 'fp64'
@@ -367,6 +457,9 @@ def main():
         args.steps = args.requests if stream else 5
     if args.warmup is None:
         args.warmup = args.slots if stream else 1
+    if args.workload == 'ri-build':
+        ri_build_bench(args)
+        return
     if stream:
         prec = args.precision
         sm, ref, err, spec = stream_bench(args.workload, args)

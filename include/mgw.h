@@ -133,6 +133,34 @@ int mgw_set_eri_3center(mgw_ctx* ctx, const double* eri_3center, int64_t npair, 
                         const int32_t* index_basis, int nbf, int where);
 int mgw_destroy_eri_3center(mgw_ctx* ctx);
 
+/* Inverse square root of the 2-center Coulomb metric (P|1/r12|Q):
+ * calculate_inverse_sqrt_eri_2center_scalapack (src/m_eri_calculate.f90:1049-1262),
+ * full-range branch. eri_2center is nauxil_bas x nauxil_bas column-major
+ * (`where`: host or device), of which the lower triangle is read. The metric
+ * is diagonalised in FP64 on the device; eigenvalues <= 1e-6
+ * (TOO_LOW_EIGENVAL, src/m_eri.f90:27) are dropped and
+ * sqrtinv(:, j) = U(:, j + nneglect) / sqrt(lambda(j + nneglect)) stays
+ * resident for mgw_build_eri_3center. nauxil_kept receives the kept count
+ * (the reference's nauxil_global), lowest_eigval (nullable) the lowest
+ * eigenvalue, sqrtinv_out (nullable; host or device, nauxil_bas x nauxil_kept
+ * column-major, so nauxil_bas^2 doubles suffice) a copy of the matrix. */
+int mgw_inverse_sqrt_eri_2center(mgw_ctx* ctx, const double* eri_2center, int nauxil_bas, int where,
+                                 int* nauxil_kept, double* lowest_eigval, double* sqrtinv_out);
+/* Second part of calculate_eri_3center_scalapack (src/m_eri_calculate.f90:1877-1892):
+ * eri_3center(ipair, P) = sum_Q eri_3center_tmp(ipair, Q) sqrtinv(Q, P) for
+ * the batch of pairs ipair_first .. ipair_first + mpair - 1 (1-based) of
+ * npair, FP64. eri_3center_tmp is mpair x nauxil_bas column-major (`where`:
+ * host or device) and already carries the 1/2 of the diagonal pairs
+ * (src/m_eri_calculate.f90:1840-1841); index_basis is the full (2, npair)
+ * host list. Batches follow each other from ipair_first = 1; after the batch
+ * that ends at npair the tensor is resident exactly as after
+ * mgw_set_eri_3center with nauxil = nauxil_kept. eri_3center_out (nullable;
+ * host or device) is the caller's whole eri_3center(npair, nauxil_kept),
+ * of which this batch's rows are written. */
+int mgw_build_eri_3center(mgw_ctx* ctx, const double* eri_3center_tmp, int64_t ipair_first, int64_t mpair,
+                          int64_t npair, const int32_t* index_basis, int nbf, int where,
+                          double* eri_3center_out);
+
 /* MO coefficients, energies and occupations of the (single) spin channel:
  * the c_matrix(:,:,1), energy(:,1), occupation(:,1) arguments of the
  * reference routines. */
@@ -239,7 +267,7 @@ enum {
   MGW_KCLASS_GEMM_CASIDA = 1,  /* Casida SYRK and w_s = B (X+Y)            */
   MGW_KCLASS_GEMM_SIGMA = 2,   /* w_s^T (P|p i) of the pole self-energy    */
   MGW_KCLASS_SIGMA_POLES = 3,  /* pole sum (FP64 VALU)                     */
-  MGW_KCLASS_EIGEN = 4,        /* Casida eigensolver                       */
+  MGW_KCLASS_EIGEN = 4,        /* eigensolvers (Casida, 2-center metric)   */
   MGW_KCLASS_GEMM_CHI0 = 5,    /* 1 - chi0(iw), all frequencies            */
   MGW_KCLASS_CHOLESKY = 6,     /* (1 - chi0)^-1 by Cholesky                */
   MGW_KCLASS_GEMM_WB = 7,      /* chi(iw) (P|p m) with fused dot           */

@@ -50,6 +50,9 @@ SIGNATURES = {
     'mgw_footprint_bytes': (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     'mgw_set_eri_3center': (C.c_int, [_vp, _vp, C.c_int64, C.c_int, _ip, C.c_int, C.c_int]),
     'mgw_destroy_eri_3center': (C.c_int, [_vp]),
+    'mgw_inverse_sqrt_eri_2center': (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp, _vp]),
+    'mgw_build_eri_3center': (C.c_int, [_vp, _vp, C.c_int64, C.c_int64, C.c_int64, _ip, C.c_int, C.c_int,
+                                        _vp]),
     'mgw_set_mo': (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int]),
     'mgw_calculate_eri_3center_mo': (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     'mgw_destroy_eri_3center_mo': (C.c_int, [_vp]),

@@ -86,6 +86,12 @@ struct mgw_ctx {
   // A1: resident RI tensor, dense B(P, k, l)
   bool has_ri = false;
   int nbf = 0, naux = 0;
+  // A1 build: kept inverse square root of the 2-center metric ("ri_sqrtinv",
+  // naux_bas x naux_kept) and the next pair of a batched build
+  bool has_sqrtinv = false;
+  int naux_bas = 0, naux_kept = 0;
+  int64_t build_next = 0, build_npair = 0;
+  int build_nbf = 0;
   // MO data of the single spin channel; energy_g (empty: energy) are the
   // energies in G of the self-energy, energy those of W
   bool has_mo = false;
@@ -180,7 +186,8 @@ int scratch_done(mgw_ctx* c, const std::string& name) {
 
 // Buffers that hold the caller's resident inputs survive a trim.
 bool is_resident_input(const std::string& name) {
-  return name == "ri_dense" || name == "c_matrix" || name == "energy" || name == "occupation";
+  return name == "ri_dense" || name == "ri_sqrtinv" || name == "c_matrix" || name == "energy" ||
+         name == "occupation";
 }
 
 // End of a stage's use of a temporary. Temporaries stay cached for the next
@@ -509,7 +516,7 @@ int gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta,
 // when the context has graphs enabled; the autotuner's timing mode launches
 // eagerly because it synchronises.
 void clear_graphs(mgw_ctx* c) {
-  for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   c->graphs.clear();
 }
 
@@ -524,13 +531,13 @@ int run_chol(mgw_ctx* c, double* A, int n, int nbatch, double* logdet, int32_t* 
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc != MGW_OK) {
-      if (graph) hipGraphDestroy(graph);
+      if (graph) (void)hipGraphDestroy(graph);
       return rc;
     }
     MGW_HIP(e);
     hipGraphExec_t exec = nullptr;
     const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     MGW_HIP(e2);
     it = c->graphs.emplace(key, exec).first;
   }
@@ -1012,6 +1019,143 @@ int mgw_set_eri_3center(mgw_ctx* c, const double* eri, int64_t npair, int nauxil
   c->naux = nauxil;
   c->has_ri = true;
   c->has_eri_mo = false;
+  return MGW_OK;
+}
+
+// Threshold below which an eigenvalue of the 2-center metric is dropped
+// (TOO_LOW_EIGENVAL, src/m_eri.f90:27).
+constexpr double TOO_LOW_EIGENVAL = 1.0e-6;
+
+int mgw_inverse_sqrt_eri_2center(mgw_ctx* c, const double* eri_2center, int nauxil_bas, int where,
+                                 int* nauxil_kept, double* lowest_eigval, double* sqrtinv_out) {
+  MGW_CTX(c);
+  MGW_REQUIRE(eri_2center && nauxil_bas > 0 && nauxil_kept, MGW_ERR_ARG,
+              "mgw_inverse_sqrt_eri_2center: invalid arguments");
+  MGW_REQUIRE(where == MGW_MEM_HOST || where == MGW_MEM_DEVICE, MGW_ERR_ARG, "invalid memory kind");
+  const int n = nauxil_bas;
+  c->has_sqrtinv = false;
+  double *u = nullptr, *eig = nullptr, *ework = nullptr;
+  int32_t* info_d = nullptr;
+  MGW_TRY(dbuf(c, "metric_u", (size_t)n * n, &u));
+  MGW_TRY(dbuf(c, "metric_eig", n, &eig));
+  MGW_TRY(dbuf(c, "metric_ework", n, &ework));
+  MGW_TRY(dbuf(c, "solver_info", 64, &info_d));
+  MGW_HIP(hipMemcpyAsync(u, eri_2center, (size_t)n * n * sizeof(double),
+                         where == MGW_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, c->stream));
+  // the lower triangle is read, as by the reference's diagonalization
+  // (src/m_eri_calculate.f90:1106-1108)
+  {
+    const double n3 = (double)n * n * n;
+    ProfScope ps(&c->prof, MGW_KCLASS_EIGEN, (4.0 / 3.0 + 2.0) * n3, 16.0 * (double)n * n, c->stream);
+    MGW_BLAS(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_lower, n, u, n, eig, ework, info_d));
+  }
+  int32_t info = 0;
+  MGW_TRY(download(c, &info, info_d, 1));
+  MGW_REQUIRE(info == 0, MGW_ERR_SOLVER, "rocsolver_dsyevd (2-center metric) failed, info=" + std::to_string(info));
+  std::vector<double> ev(n);
+  MGW_TRY(download(c, ev.data(), eig, n));
+  // src/m_eri_calculate.f90:1117 and :1122
+  int kept = 0;
+  for (double v : ev) kept += v > TOO_LOW_EIGENVAL;
+  MGW_REQUIRE(kept > 0, MGW_ERR_SOLVER, "2-center metric has no eigenvalue above 1e-6");
+  const int neglect = n - kept;
+  double* s = nullptr;
+  MGW_TRY(dbuf(c, "ri_sqrtinv", (size_t)n * kept, &s));
+  MGW_TRY(launch_invsqrt_select(u, n, eig, neglect, kept, s, c->stream));
+  if (sqrtinv_out) {
+    MGW_HIP(hipMemcpyAsync(sqrtinv_out, s, (size_t)n * kept * sizeof(double), hipMemcpyDefault, c->stream));
+  }
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  MGW_TRY(release(c, "metric_u"));
+  MGW_TRY(release(c, "metric_eig"));
+  MGW_TRY(release(c, "metric_ework"));
+  c->naux_bas = n;
+  c->naux_kept = kept;
+  c->has_sqrtinv = true;
+  c->build_next = 0;
+  *nauxil_kept = kept;
+  if (lowest_eigval) *lowest_eigval = ev[0];
+  return MGW_OK;
+}
+
+int mgw_build_eri_3center(mgw_ctx* c, const double* eri_3center_tmp, int64_t ipair_first, int64_t mpair,
+                          int64_t npair, const int32_t* index_basis, int nbf, int where, double* eri_3center_out) {
+  MGW_CTX(c);
+  MGW_REQUIRE(c->has_sqrtinv, MGW_ERR_STATE, "no 2-center metric: call mgw_inverse_sqrt_eri_2center first");
+  MGW_REQUIRE(eri_3center_tmp && index_basis && npair > 0 && mpair > 0 && nbf > 0 && ipair_first >= 1 &&
+                  ipair_first - 1 + mpair <= npair,
+              MGW_ERR_ARG, "mgw_build_eri_3center: invalid arguments");
+  MGW_REQUIRE(where == MGW_MEM_HOST || where == MGW_MEM_DEVICE, MGW_ERR_ARG, "invalid memory kind");
+  const int nb = c->naux_bas, nk = c->naux_kept;
+  MGW_REQUIRE((int64_t)nk * nbf * (int64_t)nbf < (int64_t)1 << 40, MGW_ERR_ARG, "RI tensor too large");
+  if (ipair_first > 1) {
+    MGW_REQUIRE(ipair_first == c->build_next && npair == c->build_npair && nbf == c->build_nbf, MGW_ERR_STATE,
+                "mgw_build_eri_3center: batches must follow each other from ipair_first = 1 (expected " +
+                    std::to_string(c->build_next) + ")");
+  }
+  std::vector<int32_t> kl(2 * mpair);
+  for (int64_t p = 0; p < mpair; ++p) {
+    const int64_t q = ipair_first - 1 + p;
+    const int32_t k = index_basis[2 * q] - 1, l = index_basis[2 * q + 1] - 1;
+    MGW_REQUIRE(k >= 0 && k < nbf && l >= 0 && l < nbf, MGW_ERR_ARG,
+                "index_basis entry out of range at pair " + std::to_string(q + 1));
+    kl[2 * p] = k;
+    kl[2 * p + 1] = l;
+  }
+  double* dense = nullptr;
+  if (ipair_first == 1) {
+    c->has_ri = false;
+    c->has_eri_mo = false;
+    MGW_TRY(dbuf(c, "ri_dense", (size_t)nk * nbf * nbf, &dense));
+    MGW_TRY(launch_fill(dense, (int64_t)nk * nbf * nbf, 0.0, c->stream));
+    c->build_npair = npair;
+    c->build_nbf = nbf;
+  } else {
+    dense = find_buf(c, "ri_dense");
+    MGW_REQUIRE(dense, MGW_ERR_STATE, "mgw_build_eri_3center: the tensor of the first batch is gone");
+  }
+  const double* tmp = eri_3center_tmp;
+  if (where == MGW_MEM_HOST) {
+    double* d = nullptr;
+    MGW_TRY(upload(c, "ri_tmp", eri_3center_tmp, (size_t)mpair * nb, &d));
+    tmp = d;
+  }
+  double* packed = nullptr;
+  MGW_TRY(dbuf(c, "ri_packed", (size_t)mpair * nk, &packed));
+  // eri_3center(ipair, P) = sum_Q eri_3center_tmp(ipair, Q) sqrtinv(Q, P)
+  // (src/m_eri_calculate.f90:1889-1892); FP64
+  GemmF64 g;
+  g.M = (int)mpair; g.N = nk; g.K = nb;
+  g.A = tmp; g.lda = mpair;
+  g.B = find_buf(c, "ri_sqrtinv"); g.ldb = nb;
+  g.C = packed; g.ldc = mpair;
+  g.klass = MGW_KCLASS_GEMM_OTHER;
+  MGW_REQUIRE(mpair < ((int64_t)1 << 31), MGW_ERR_ARG, "mgw_build_eri_3center: batch too large");
+  {
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_OTHER, 2.0 * (double)mpair * nk * nb,
+                 8.0 * ((double)mpair * nb + (double)nb * nk + (double)mpair * nk), c->stream);
+    MGW_TRY(gemm_f64(g, c->stream));
+  }
+  int32_t* kl_d = nullptr;
+  MGW_TRY(upload(c, "ri_kl", kl.data(), kl.size(), &kl_d));
+  MGW_TRY(launch_unpack_ri(packed, mpair, nk, kl_d, nbf, dense, c->stream, false));
+  if (eri_3center_out) {
+    // rows ipair_first.. of the caller's eri_3center(npair, nauxil_kept)
+    MGW_HIP(hipMemcpy2DAsync(eri_3center_out + (ipair_first - 1), (size_t)npair * sizeof(double), packed,
+                             (size_t)mpair * sizeof(double), (size_t)mpair * sizeof(double), nk, hipMemcpyDefault,
+                             c->stream));
+  }
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  MGW_TRY(release(c, "ri_tmp"));
+  MGW_TRY(release(c, "ri_packed"));
+  MGW_TRY(release(c, "ri_kl"));
+  c->build_next = ipair_first + mpair;
+  if (c->build_next == npair + 1) {
+    c->nbf = nbf;
+    c->naux = nk;
+    c->has_ri = true;
+    c->build_next = 0;
+  }
   return MGW_OK;
 }
 

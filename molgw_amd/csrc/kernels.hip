@@ -75,10 +75,35 @@ unpack_ri_kernel(const double* __restrict__ packed, int64_t npair, int naux,
 }
 
 int launch_unpack_ri(const double* packed, int64_t npair, int naux, const int32_t* kl, int nbf,
-                     double* dense, hipStream_t s) {
-  MGW_TRY(launch_fill(dense, (int64_t)naux * nbf * nbf, 0.0, s));
+                     double* dense, hipStream_t s, bool zero_dense) {
+  if (zero_dense) MGW_TRY(launch_fill(dense, (int64_t)naux * nbf * nbf, 0.0, s));
   dim3 grid(nblocks(npair, 64), nblocks(naux, 64));
   hipLaunchKernelGGL(unpack_ri_kernel, grid, dim3(256), 0, s, packed, npair, naux, kl, nbf, dense);
+  MGW_LAUNCH_CHECK();
+  return MGW_OK;
+}
+
+// ------------------------------------------------------------------ metric
+// Kept columns of the eigenvectors of the 2-center metric, each divided by
+// the square root of its eigenvalue (src/m_eri_calculate.f90:1206-1211).
+// Eigenvalues ascend, so the dropped ones are the first `neglect`.
+__global__ void invsqrt_select_kernel(const double* __restrict__ u, int n, const double* __restrict__ eig,
+                                      int neglect, int kept, double* __restrict__ out) {
+  const int64_t total = (int64_t)n * kept;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int j = (int)(e / n);
+    const int64_t src = e + (int64_t)neglect * n;
+    out[e] = u[src] / sqrt(eig[j + neglect]);
+  }
+}
+
+int launch_invsqrt_select(const double* u, int n, const double* eig, int neglect, int kept, double* out,
+                          hipStream_t s) {
+  const int64_t total = (int64_t)n * kept;
+  if (total <= 0) return MGW_OK;
+  unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(invsqrt_select_kernel, dim3(g), dim3(256), 0, s, u, n, eig, neglect, kept, out);
   MGW_LAUNCH_CHECK();
   return MGW_OK;
 }

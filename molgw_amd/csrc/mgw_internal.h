@@ -148,8 +148,13 @@ int launch_zero_upper(double* m, int n, int nbatch, hipStream_t s);   // number 
 int gemm_autotune_mode(int mode); // -1 query; returns the previous mode
 
 // ---------------------------------------------------------------- kernels
+// zero_dense false: only the pairs listed are written (a later batch of a
+// tensor whose buffer was cleared before the first batch)
 int launch_unpack_ri(const double* packed, int64_t npair, int naux, const int32_t* kl,
-                     int nbf, double* dense, hipStream_t s);
+                     int nbf, double* dense, hipStream_t s, bool zero_dense = true);
+// out(:, j) = u(:, j + neglect) / sqrt(eig[j + neglect]), j < kept (u n x n)
+int launch_invsqrt_select(const double* u, int n, const double* eig, int neglect, int kept, double* out,
+                          hipStream_t s);
 int launch_fill(double* p, int64_t n, double v, hipStream_t s);
 int launch_gather_ia(const double* eri, int naux, int nm, int nn, const int32_t* ia_dev, int npole,
                      double* x, hipStream_t s);

@@ -49,6 +49,7 @@ module m_mgw
 
   public :: mgw_init, mgw_finalize, mgw_check
   public :: mgw_upload_eri_3center, mgw_upload_mo
+  public :: mgw_inverse_sqrt_eri_2center, mgw_build_eri_3center
   public :: mgw_eri_3center_mo, mgw_destroy_mo_integrals
   public :: mgw_polarizability_rpa, mgw_polarizability_imag
   public :: mgw_selfenergy_poles, mgw_selfenergy_imag, mgw_pade
@@ -86,6 +87,30 @@ module m_mgw
       integer(c_int), value :: nbf, where
       integer(c_int) :: ierr
     end function c_set_eri_3center
+
+    function c_inverse_sqrt_eri_2center(handle, eri_2center, nauxil_bas, where, nauxil_kept, lowest, sqrtinv) &
+        bind(C, name='mgw_inverse_sqrt_eri_2center') result(ierr)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      real(c_double), intent(in) :: eri_2center(*)
+      integer(c_int), value :: nauxil_bas, where
+      integer(c_int) :: nauxil_kept
+      real(c_double) :: lowest
+      real(c_double) :: sqrtinv(*)
+      integer(c_int) :: ierr
+    end function c_inverse_sqrt_eri_2center
+
+    function c_build_eri_3center(handle, tmp, ipair_first, mpair, npair, index_basis, nbf, where, eri) &
+        bind(C, name='mgw_build_eri_3center') result(ierr)
+      import :: c_int, c_int64_t, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: handle
+      real(c_double), intent(in) :: tmp(*)
+      integer(c_int64_t), value :: ipair_first, mpair, npair
+      integer(c_int32_t), intent(in) :: index_basis(*)
+      integer(c_int), value :: nbf, where
+      real(c_double) :: eri(*)
+      integer(c_int) :: ierr
+    end function c_build_eri_3center
 
     function c_set_mo(handle, c_matrix, nbf, nstate, energy, occupation, where) &
         bind(C, name='mgw_set_mo') result(ierr)
@@ -264,6 +289,45 @@ subroutine mgw_upload_eri_3center(eri_3center, index_basis, nbf)
                                    int(SIZE(eri_3center, DIM=2), c_int), ib, int(nbf, c_int), MGW_MEM_HOST), &
                  'mgw_upload_eri_3center')
 end subroutine mgw_upload_eri_3center
+
+!=========================================================================
+! calculate_inverse_sqrt_eri_2center_scalapack (src/m_eri_calculate.f90:1049-1262),
+! full-range branch: columns 1:nauxil_kept of eri_2center_sqrtinv are set.
+subroutine mgw_inverse_sqrt_eri_2center(eri_2center, nauxil_kept, eri_2center_sqrtinv)
+  real(dp), intent(in)    :: eri_2center(:, :)
+  integer, intent(out)    :: nauxil_kept
+  real(dp), intent(inout) :: eri_2center_sqrtinv(:, :)
+  !=====
+  integer(c_int) :: kept
+  real(c_double) :: lowest
+  !=====
+  if( SIZE(eri_2center_sqrtinv) < SIZE(eri_2center) ) then
+    write(*, '(a)') ' mgw_inverse_sqrt_eri_2center: eri_2center_sqrtinv is smaller than eri_2center'
+    error stop 'libmgw'
+  end if
+  call mgw_check(c_inverse_sqrt_eri_2center(ctx, eri_2center, int(SIZE(eri_2center, DIM=1), c_int), MGW_MEM_HOST, &
+                                            kept, lowest, eri_2center_sqrtinv), 'mgw_inverse_sqrt_eri_2center')
+  nauxil_kept = kept
+end subroutine mgw_inverse_sqrt_eri_2center
+
+!=========================================================================
+! Second part of calculate_eri_3center_scalapack (src/m_eri_calculate.f90:1877-1892)
+! for the pairs ipair_first:ipair_first+SIZE(eri_3center_tmp,1)-1; the RI
+! tensor is resident on the GPU once the last batch is in.
+subroutine mgw_build_eri_3center(eri_3center_tmp, ipair_first, index_basis, nbf, eri_3center)
+  real(dp), intent(in)    :: eri_3center_tmp(:, :)
+  integer, intent(in)     :: ipair_first, nbf
+  integer, intent(in)     :: index_basis(:, :)
+  real(dp), intent(inout) :: eri_3center(:, :)
+  !=====
+  integer(c_int32_t), allocatable :: ib(:, :)
+  !=====
+  ib = int(index_basis, c_int32_t)
+  call mgw_check(c_build_eri_3center(ctx, eri_3center_tmp, int(ipair_first, c_int64_t), &
+                                     int(SIZE(eri_3center_tmp, DIM=1), c_int64_t), &
+                                     int(SIZE(eri_3center, DIM=1), c_int64_t), ib, int(nbf, c_int), &
+                                     MGW_MEM_HOST, eri_3center), 'mgw_build_eri_3center')
+end subroutine mgw_build_eri_3center
 
 subroutine mgw_upload_mo(c_matrix, energy, occupation)
   real(dp), intent(in) :: c_matrix(:, :), energy(:), occupation(:)

@@ -134,6 +134,7 @@ class Engine:
         check(self._lib.mgw_ctx_create(int(device), C.byref(h)))
         self._h = h
         self.nbf = self.naux = self.nstate = 0
+        self._naux_kept = self._naux_bas = 0
         self.energy = self.occupation = None
 
     # ------------------------------------------------------------ lifetime
@@ -260,6 +261,71 @@ class Engine:
                                             ib.ctypes.data_as(C.POINTER(C.c_int32)), int(nbf),
                                             _lib.MGW_MEM_DEVICE))
         self.nbf, self.naux = int(nbf), int(naux)
+
+    def inverse_sqrt_eri_2center(self, eri_2center, copy: bool = True):
+        """calculate_inverse_sqrt_eri_2center_scalapack
+        (src/m_eri_calculate.f90:1049-1262): FP64 eigen-decomposition of the
+        2-center metric on the device, eigenvalues <= 1e-6 dropped. Returns
+        (nauxil_kept, lowest eigenvalue, sqrtinv(nauxil_bas, nauxil_kept) or
+        None); sqrtinv stays resident for build_eri_3center."""
+        v = _f64(eri_2center)
+        n = v.shape[0]
+        if v.shape != (n, n):
+            raise ValueError(f'eri_2center must be square, got {v.shape}')
+        out = np.zeros((n, n), order='F') if copy else None
+        kept = C.c_int(0)
+        low = C.c_double(0.0)
+        check(self._lib.mgw_inverse_sqrt_eri_2center(self._h, v.ctypes.data, n, _lib.MGW_MEM_HOST,
+                                                     C.byref(kept), C.byref(low),
+                                                     out.ctypes.data if copy else None))
+        k = kept.value
+        self._naux_kept, self._naux_bas = k, n
+        return k, low.value, (np.asfortranarray(out[:, :k]) if copy else None)
+
+    def build_eri_3center(self, eri_3center_tmp, index_basis, nbf: int, batch: int | None = None,
+                          copy: bool = True):
+        """B = eri_3center_tmp . sqrtinv (src/m_eri_calculate.f90:1877-1892) in
+        batches of `batch` pairs (all at once by default); eri_3center_tmp is
+        (npair, nauxil_bas) with the diagonal pairs already halved. The result
+        becomes the resident RI tensor; returns eri_3center(npair, nauxil_kept)
+        when copy is set."""
+        t = _f64(eri_3center_tmp)
+        ib = np.asfortranarray(np.asarray(index_basis, dtype=np.int32))
+        npair, nb = t.shape
+        if ib.shape != (2, npair):
+            raise ValueError(f'index_basis must be (2, {npair}), got {ib.shape}')
+        kept = getattr(self, '_naux_kept', 0)
+        if not kept:
+            raise RuntimeError('no 2-center metric: call inverse_sqrt_eri_2center first')
+        if nb != self._naux_bas:
+            raise ValueError(f'eri_3center_tmp has {nb} auxiliary columns, the metric {self._naux_bas}')
+        out = np.zeros((npair, kept), order='F') if copy else None
+        step = npair if batch is None else max(1, int(batch))
+        first = 0
+        while first < npair:
+            m = min(step, npair - first)
+            chunk = np.asfortranarray(t[first:first + m])
+            check(self._lib.mgw_build_eri_3center(self._h, chunk.ctypes.data, first + 1, m, npair,
+                                                  ib.ctypes.data_as(C.POINTER(C.c_int32)), int(nbf),
+                                                  _lib.MGW_MEM_HOST, out.ctypes.data if copy else None))
+            first += m
+        self.nbf, self.naux = int(nbf), kept
+        return out
+
+    def build_eri_3center_device(self, ptr: int, ipair_first: int, mpair: int, npair: int, index_basis,
+                                 nbf: int, out_ptr: int | None = None) -> None:
+        """build_eri_3center for one batch already in HBM: ``ptr`` is
+        eri_3center_tmp(mpair, nauxil_bas) column-major on the device, pairs
+        ipair_first.. (1-based); ``out_ptr`` (optional, device or host) is the
+        whole eri_3center(npair, nauxil_kept)."""
+        if not self._naux_kept:
+            raise RuntimeError('no 2-center metric: call inverse_sqrt_eri_2center first')
+        ib = np.asfortranarray(np.asarray(index_basis, dtype=np.int32))
+        check(self._lib.mgw_build_eri_3center(self._h, C.c_void_p(ptr), int(ipair_first), int(mpair), int(npair),
+                                              ib.ctypes.data_as(C.POINTER(C.c_int32)), int(nbf),
+                                              _lib.MGW_MEM_DEVICE, None if out_ptr is None else C.c_void_p(out_ptr)))
+        if ipair_first - 1 + mpair == npair:
+            self.nbf, self.naux = int(nbf), self._naux_kept
 
     def set_mo(self, c_matrix, energy, occupation) -> None:
         c = _f64(c_matrix)

@@ -13,7 +13,13 @@ Arrays keep the reference's column-major meaning: ``eri_mo[P, m, n]`` is
 ``eri_3center_mo(P, m, n, spin)``.
 
 Pinning (what checks each function against something other than itself)
-lives in ``tests/test_oracle_*.py``:
+lives in ``tests/test_oracle.py``:
+  * ``inverse_sqrt_eri_2center`` -- sqrtinv sqrtinv^T = V^-1 (``numpy.linalg.inv``)
+    for a full-rank metric; planted spectra fix the kept count and the
+    projected inverse; sqrtinv^T V sqrtinv = 1
+  * ``build_eri_3center`` -- from (mu nu|Q) = B V^1/2 (``scipy.linalg.sqrtm``)
+    the build returns B up to a rotation (B B^T), and the G0W0 QP energies
+    of the built tensor equal those of B
   * ``coeffs_gausslegint``  -- numpy ``leggauss``; integral of 2a/(w^2+a^2)
   * ``calculate_eri_3center_mo`` -- brute-force dense einsum; CC4S (11|11)
   * ``polarizability_casida`` -- lih_bulk08 RPA correlation golden; closed
@@ -173,6 +179,32 @@ def transition_table(occupation, spin_fact: float, ncore_W: int, nvirtual_W: int
 # ---------------------------------------------------------------------------
 # A1/A2: RI tensor and AO -> MO transform
 # ---------------------------------------------------------------------------
+TOO_LOW_EIGENVAL = 1.0e-6        # src/m_eri.f90:27
+
+
+def inverse_sqrt_eri_2center(eri_2center: np.ndarray):
+    """Kept inverse square root of the 2-center metric (P|1/r12|Q).
+
+    Restates the full-range branch of calculate_inverse_sqrt_eri_2center_scalapack
+    (src/m_eri_calculate.f90:1049-1262): diagonalise the lower triangle
+    (:1106-1108, eigenvalues ascending), keep the eigenvalues above
+    TOO_LOW_EIGENVAL (:1117, :1122), and scale the kept eigenvectors,
+    sqrtinv(:, j) = eigvec(:, j + nneglect) / sqrt(eigval(j + nneglect))
+    (:1206-1211). Returns (sqrtinv[nauxil_bas, nauxil_kept], eigval).
+    """
+    w, u = np.linalg.eigh(np.asarray(eri_2center, dtype=np.float64), UPLO='L')
+    kept = int(np.count_nonzero(w > TOO_LOW_EIGENVAL))
+    neglect = len(w) - kept
+    return u[:, neglect:] / np.sqrt(w[neglect:]), w
+
+
+def build_eri_3center(eri_3center_tmp: np.ndarray, sqrtinv: np.ndarray) -> np.ndarray:
+    """eri_3center(pair, P) = sum_Q eri_3center_tmp(pair, Q) sqrtinv(Q, P)
+    (src/m_eri_calculate.f90:1877-1892); the diagonal pairs of
+    eri_3center_tmp already carry the factor 1/2 (:1840-1841)."""
+    return np.asarray(eri_3center_tmp, dtype=np.float64) @ sqrtinv
+
+
 def pack_eri_3center(b_full: np.ndarray, index_basis: np.ndarray) -> np.ndarray:
     """Pack a dense symmetric B(P, mu, nu) into the reference layout.
 

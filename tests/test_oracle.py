@@ -303,3 +303,56 @@ def test_evgw_first_step_is_the_g0w0_correction_and_w_policy():
     outside = np.ones(s.nstate, bool)
     outside[win] = False
     np.testing.assert_array_equal(hn[2][outside], s.energy[outside])  # states outside the window keep e
+
+
+def _planted_metric(n, eigvals, seed):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (q * eigvals) @ q.T, q
+
+
+def test_metric_inverse_square_root_full_rank():
+    """No eigenvalue below the threshold: sqrtinv sqrtinv^T is the inverse
+    metric (LU inverse, a different algorithm) and whitens the metric."""
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((40, 40))
+    v = a @ a.T + 40 * np.eye(40)
+    s, w = O.inverse_sqrt_eri_2center(v)
+    assert s.shape == (40, 40) and np.all(np.diff(w) >= 0)
+    np.testing.assert_allclose(s @ s.T, np.linalg.inv(v), rtol=0, atol=1e-13)
+    np.testing.assert_allclose(s.T @ v @ s, np.eye(40), rtol=0, atol=1e-12)
+
+
+def test_metric_drops_eigenvalues_at_or_below_threshold():
+    """Planted spectrum with four eigenvalues at or below 1e-6 (one just above):
+    the kept count and the projected inverse are fixed by the plant."""
+    lam = np.concatenate([[1e-11, 3e-9, 4e-7, 9.9e-7, 2.0e-6], np.geomspace(1e-4, 30.0, 25)])
+    v, q = _planted_metric(len(lam), lam, 11)
+    s, w = O.inverse_sqrt_eri_2center(v)
+    keep = lam > 1e-6
+    assert s.shape == (30, int(keep.sum())) == (30, 26)
+    proj = (q[:, keep] / lam[keep]) @ q[:, keep].T
+    np.testing.assert_allclose(s @ s.T, proj, rtol=0, atol=1e-6 * np.abs(proj).max())
+    np.testing.assert_allclose(s.T @ v @ s, np.eye(26), rtol=0, atol=1e-8)
+    # the dropped directions are not seen: sqrtinv is orthogonal to them
+    assert np.abs(q[:, ~keep].T @ s).max() < 1e-6
+
+
+def test_b_build_recovers_b_up_to_rotation_and_the_qp_energies():
+    """(mu nu|Q) = B V^1/2 for a full-rank metric V: the build returns a B'
+    with B' B'^T = B B^T, and the one-shot G0W0 QP energies of B' equal those
+    of B (the path only sees B through B B^T)."""
+    import scipy.linalg as sla
+    from molgw_amd.workloads import synthetic_system
+    s = synthetic_system(24, 60, 4, seed=23, scale=0.1)
+    lam = np.geomspace(1e-3, 5.0, 60)
+    v, _ = _planted_metric(60, lam, 29)
+    vh = np.real(sla.sqrtm(v))
+    tmp = s.eri_3center @ vh
+    sq, _ = O.inverse_sqrt_eri_2center(v)
+    b = O.build_eri_3center(tmp, sq)
+    np.testing.assert_allclose(b @ b.T, s.eri_3center @ s.eri_3center.T, rtol=0,
+                               atol=1e-10 * np.abs(s.eri_3center).max() ** 2)
+    ref = O.g0w0_casida(s.eri_3center, s.index_basis, s.c_matrix, s.energy, s.occupation, nomega_sigma=5)
+    got = O.g0w0_casida(b, s.index_basis, s.c_matrix, s.energy, s.occupation, nomega_sigma=5)
+    np.testing.assert_allclose(got.energy_qp, ref.energy_qp, rtol=0, atol=1e-10)
