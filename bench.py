@@ -166,17 +166,22 @@ def gpu_bench(system, technique, steps, warmup, kw, precision='fp64'):
     return t1 - t0, lat, prof, out, mem, err_mev
 
 
-LOWP_CLASSES = ('gemm_ao2mo', 'gemm_chi0')
-PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0 / 3.0, 'bf16x6': 2500.0 / 6.0}
+PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0 / 3.0, 'bf16x6': 2500.0 / 6.0,
+               'fp16x3': 2500.0 / 3.0}
+# kernel classes that run below FP64 in each mode (the rest stay FP64)
+LOWP_CLASSES = {'fp32': ('gemm_ao2mo', 'gemm_chi0', 'gemm_casida'),
+                'bf16x3': ('gemm_ao2mo', 'gemm_chi0', 'gemm_casida', 'gemm_wb'),
+                'bf16x6': ('gemm_chi0', 'gemm_wb'),
+                'fp16x3': ('gemm_chi0', 'gemm_wb')}
 
 
 def roofline_of(prof, precision='fp64', traffic_key=None):
     """Dominant hand-written kernel class by device time (the rocSOLVER eigensolver excluded).
 
     achieved = algorithmic flops per launch / average launch time. For the
-    BF16x3 (BF16x6) products the ceiling is the published dense BF16 MFMA
-    peak (2.5 PFLOP/s) divided by the 3 (6) products each algorithmic product
-    costs; BF16x6 keeps the transforms in FP64."""
+    split products the ceiling is the published dense BF16/FP16 MFMA peak
+    (2.5 PFLOP/s) divided by the products each algorithmic product costs (3
+    for BF16x3 and FP16x3, 6 for BF16x6)."""
     own = {k: v for k, v in prof.items() if k != 'eigensolver'}
     if not own:
         return None, None
@@ -187,9 +192,7 @@ def roofline_of(prof, precision='fp64', traffic_key=None):
     if name == 'sigma_poles':
         r = {'bound': 'valu', 'achieved': achieved, 'peak': VALU_F64_PEAK_TFLOPS, 'unit': 'TFLOP/s'}
     else:
-        prec = precision if name in LOWP_CLASSES else 'fp64'
-        if prec == 'bf16x6' and name != 'gemm_chi0':
-            prec = 'fp64'
+        prec = precision if name in LOWP_CLASSES.get(precision, ()) else 'fp64'
         r = {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_TFLOPS[prec], 'unit': 'TFLOP/s',
              'arith': prec}
     r['frac'] = r['achieved'] / r['peak']
@@ -441,16 +444,16 @@ def main():
     ap.add_argument('--slots', type=int, default=8, help='concurrent solve slots (serving, batch64)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--seed', type=int, default=0)
-    ap.add_argument('--precision', default=None, choices=['fp64', 'fp32', 'bf16x3', 'bf16x6'],
-                    help='arithmetic of the AO->MO transforms and the chi0/Casida build '
-                         '(default: bf16x6 for the streams, fp64 otherwise)')
+    ap.add_argument('--precision', default=None, choices=['fp64', 'fp32', 'bf16x3', 'bf16x6', 'fp16x3'],
+                    help='arithmetic of the AO->MO transforms, the chi0/Casida build and the '
+                         'imaginary-axis self-energy product (default: fp16x3 for the streams, fp64 otherwise)')
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit('bench.py measures one GPU: every box here has one, and the path has no multi-GPU code; '
                  'run with --gpus 1')
     stream = args.workload in ('serving', 'batch64')
     if args.precision is None:
-        args.precision = 'bf16x6' if stream else 'fp64'
+        args.precision = 'fp16x3' if stream else 'fp64'
     if args.requests is None:
         args.requests = args.steps if args.steps is not None else 64
     if args.steps is None:
@@ -531,7 +534,8 @@ def main():
         'vs_baseline': None,
         'dtype': {'fp64': 'f64', 'fp32': 'f32 products, f64 accumulation',
                   'bf16x3': 'bf16x3 products, f64 accumulation',
-                  'bf16x6': 'bf16x6 products (chi0), f64 transforms, f64 accumulation'}[args.precision],
+                  'bf16x6': 'bf16x6 products (chi0, W.B), f64 transforms, f64 accumulation',
+                  'fp16x3': 'fp16x3 products (chi0, W.B), f64 transforms, f64 accumulation'}[args.precision],
         'qp_error_vs_fp64_mev': err_mev,
         'data': 'synthetic (seeded RI tensor, orthonormal C and energies at the molecule\'s shape; '
                 'no integral engine in the image)',

@@ -43,14 +43,17 @@ enum {
 
 enum { MGW_MEM_HOST = 0, MGW_MEM_DEVICE = 1 };
 
-/* Arithmetic of the transforms and of the chi0 / Casida-matrix build
- * (the metric, the (1 - chi0) factorisation, the eigensolve and the QP
- * solve are FP64 in every mode):
+/* Arithmetic of the transforms, the chi0 / Casida-matrix build and the
+ * imaginary-axis self-energy product (the metric, the (1 - chi0)
+ * factorisation, the eigensolve and the QP solve are FP64 in every mode):
  *   MGW_FP32   FP32 products on MFMA, FP64 accumulation
  *   MGW_BF16X3 two-way bf16 split, 3 products (~2^-17 relative)
- *   MGW_BF16X6 three-way bf16 split for chi0, 6 products (~2^-24, FP32
- *              level); the transforms stay FP64                           */
-enum { MGW_FP64 = 0, MGW_FP32 = 1, MGW_BF16X3 = 2, MGW_BF16X6 = 3 };
+ *   MGW_BF16X6 three-way bf16 split for chi0 and the self-energy product,
+ *              6 products (~2^-24, FP32 level); the transforms stay FP64
+ *   MGW_FP16X3 two-way fp16 split of power-of-two-scaled operands for chi0
+ *              and the self-energy product, 3 products (~2^-22); the
+ *              transforms stay FP64                                       */
+enum { MGW_FP64 = 0, MGW_FP32 = 1, MGW_BF16X3 = 2, MGW_BF16X6 = 3, MGW_FP16X3 = 4 };
 
 enum { MGW_TECH_ONE_SHOT = 0, MGW_TECH_IMAGINARY_PADE = 1 };
 

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libmgw.so')
 
 MGW_OK = 0
 MGW_MEM_HOST, MGW_MEM_DEVICE = 0, 1
-MGW_FP64, MGW_FP32, MGW_BF16X3, MGW_BF16X6 = 0, 1, 2, 3
+MGW_FP64, MGW_FP32, MGW_BF16X3, MGW_BF16X6, MGW_FP16X3 = 0, 1, 2, 3, 4
 MGW_TECH_ONE_SHOT, MGW_TECH_IMAGINARY_PADE = 0, 1
 
 

@@ -20,7 +20,7 @@ ROCM = os.environ.get('ROCM_PATH', '/opt/rocm')
 HIPCC = os.path.join(ROCM, 'bin', 'hipcc')
 ARCH = 'gfx950'
 
-SOURCES = ['gemm_f64.hip', 'gemm_lowp.hip', 'chi0_split.hip', 'chol.hip', 'kernels.hip', 'api.hip']
+SOURCES = ['gemm_f64.hip', 'gemm_lowp.hip', 'chi0_split.hip', 'wb_split.hip', 'chol.hip', 'kernels.hip', 'api.hip']
 
 CFLAGS = [
     f'--offload-arch={ARCH}', '-O3', '-std=c++17', '-fPIC', '-Wall', '-Wno-unused-result',

@@ -128,6 +128,9 @@ struct mgw_ctx {
 
 namespace {
 
+// modes whose chi0 and imaginary-axis self-energy product run on split operands
+bool split_precision(int p) { return p == MGW_BF16X3 || p == MGW_BF16X6 || p == MGW_FP16X3; }
+
 // ------------------------------------------------------------------ buffers
 int ensure(mgw_ctx* c, const std::string& name, size_t bytes, void** out) {
   DevBuf& b = c->ledger.bufs[name];
@@ -588,12 +591,12 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
     const int i = ia[2 * t], a = ia[2 * t + 1];
     if ((c->occupation[i] - c->occupation[a]) * (c->energy[a] - c->energy[i]) < 0.0) nonneg = false;
   }
-  if ((precision == MGW_BF16X3 || precision == MGW_BF16X6) && !nonneg)
+  if (split_precision(precision) && !nonneg)
     g.precision = MGW_FP32;   // per-element split kernels take signed weights at FP32
   {
     ProfScope ps(&c->prof, MGW_KCLASS_GEMM_CHI0, (double)naux * (naux + 1.0) * npole * nomega,
                  8.0 * ((double)naux * npole + 0.5 * (double)naux * naux * nomega), c->stream);
-    if ((precision == MGW_BF16X3 || precision == MGW_BF16X6) && nonneg) {
+    if (split_precision(precision) && nonneg) {
       const size_t bytes = chi0_split_workspace(naux, npole, nomega, precision);
       double* work = nullptr;
       MGW_TRY(dbuf(c, "chi_split", (bytes + 7) / 8, &work));
@@ -650,6 +653,9 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
     ++c->escalations;
   }
   MGW_TRY(launch_lower_frob2(chi, naux, nomega, frob, c->stream));
+  // D = L^-1 - 1 stays resident: chi = L^-T L^-1 - 1 = D^T D + D + D^T, and
+  // the self-energy product works on the screening part D alone
+  MGW_TRY(launch_diag_add(chi, naux, nomega, -1.0, c->stream));
   std::vector<double> h0(nomega), hl(nomega), fr((size_t)FROB_BLOCKS * nomega);
   MGW_TRY(download(c, h0.data(), tr0, nomega));
   MGW_TRY(download(c, hl.data(), logdet, nomega));
@@ -691,25 +697,33 @@ int gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int ncalc, const double*
   double *part, *v, *om_d, *wt_d, *ep_d, *wc_d, *sc;
   MGW_TRY(dbuf(c, "dot_part", (size_t)nrb * ncol * nomega, &part));
   MGW_TRY(dbuf(c, "dot_v", (size_t)ncol * nomega, &v));
-  // diag(B_m^T chi(iw) B_m) for every (p, m) and iw (src/m_gw_selfenergy_grid.f90:297-310),
-  // as ||L^-1 B_m||^2 - ||B_m||^2 with chi = (L L^T)^-1 - 1: a triangular
-  // product with the squared column norms fused into its epilogue
-  double* bn2 = nullptr;
-  MGW_TRY(dbuf(c, "eri_pm_n2", (size_t)ncol, &bn2));
-  MGW_TRY(launch_colnorm2(epm, naux, naux, (int)ncol, bn2, c->stream));
+  // diag(B_m^T chi(iw) B_m) for every (p, m) and iw (src/m_gw_selfenergy_grid.f90:297-310).
+  // With chi = L^-T L^-1 - 1 and D = L^-1 - 1 (lower triangular, resident):
+  // B^T chi B = ||D B||^2 + 2 B^T D B, a triangular product whose epilogue
+  // sums C (C + 2 B) over the rows; no cancellation against ||B||^2
   GemmF64 g;
   g.M = naux; g.N = (int)ncol; g.K = naux;
   g.A = find_buf(c, "wpol_linv"); g.lda = naux; g.strideA = (int64_t)naux * naux; g.tri_a = true;
   g.B = epm; g.ldb = naux; g.strideB = 0;
+  g.dotB = epm; g.lddot = naux; g.dot_cc = 1.0; g.dot_cd = 2.0;
   g.dotpart = part; g.strideDot = (int64_t)nrb * ncol;
   g.batch = nomega;
   g.batch_outer = true;   // one L^-1 stays in cache while B streams past it
+  const bool split = split_precision(precision);
   {
     ProfScope ps(&c->prof, MGW_KCLASS_GEMM_WB, (double)naux * (naux + 1.0) * ncol * nomega,
                  8.0 * (0.5 * (double)naux * naux * nomega + (double)naux * ncol), c->stream);
-    MGW_TRY(gemm_f64(g, c->stream));
+    if (split) {
+      const size_t bytes = wb_split_workspace(naux, ncol, nomega, precision);
+      double* work = nullptr;
+      MGW_TRY(dbuf(c, "wb_split", (bytes + 7) / 8, &work));
+      MGW_TRY(wb_split_dot(g.A, naux, nomega, epm, ncol, precision, work, bytes, part, c->stream));
+    } else {
+      MGW_TRY(gemm_f64(g, c->stream));
+    }
   }
-  MGW_TRY(launch_dot_reduce(part, nrb, ncol, nomega, bn2, v, c->stream));
+  if (split) MGW_TRY(scratch_done(c, "wb_split"));
+  MGW_TRY(launch_dot_reduce(part, nrb, ncol, nomega, nullptr, v, c->stream));
   std::vector<double> ep(np);
   for (int k = 0; k < np; ++k) ep[k] = c->energy_in_g()[glo - 1 + k];
   MGW_TRY(upload(c, "sgi_omega", c->chi_omega.data(), nomega, &om_d));
@@ -978,10 +992,12 @@ int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int techn
   } else {
     stage_w = na * nb * no + na * npole + na * na * (int64_t)nomega_chi_imag + npole * nomega_chi_imag +
               (int64_t)mgw::chol_work_doubles((int)na) * nomega_chi_imag;
-    if (precision == MGW_BF16X3 || precision == MGW_BF16X6)
+    if (split_precision(precision))
       stage_w += (int64_t)(mgw::chi0_split_workspace((int)na, (int)npole, nomega_chi_imag, precision) / 8);
   }
-  const int64_t stage_s = na * nb * ns + na * ns * ns + na * npole + (int64_t(1) << 27);
+  int64_t stage_s = na * nb * ns + na * ns * ns + na * npole + (int64_t(1) << 27);
+  if (technique != MGW_TECH_ONE_SHOT && split_precision(precision))
+    stage_s += (int64_t)((mgw::wb_split_workspace((int)na, ns * ns, nomega_chi_imag, precision) + 7) / 8);
   // the temporaries of the W stage stay cached while the self-energy runs
   return 8 * (d + stage_w + stage_s);
 }
@@ -1240,7 +1256,7 @@ int mgw_polarizability(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int pr
                        double* en_rpa, double* en_gw, double* pole_out, double* w_s_out) {
   MGW_CTX(c);
   MGW_TRY(check_ranges(c, r));
-  MGW_REQUIRE(precision >= MGW_FP64 && precision <= MGW_BF16X6, MGW_ERR_ARG,
+  MGW_REQUIRE(precision >= MGW_FP64 && precision <= MGW_FP16X3, MGW_ERR_ARG,
               "mgw_polarizability: unknown precision");
   MGW_TRY(polarizability_casida(c, r, spin_fact, en_rpa, en_gw, precision));
   if (pole_out) MGW_TRY(download(c, pole_out, find_buf(c, "wpol_pole"), c->npole));
@@ -1254,12 +1270,13 @@ int mgw_polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, i
                             double* erpa, double* egw, double* chi_out) {
   MGW_CTX(c);
   MGW_TRY(check_ranges(c, r));
-  MGW_REQUIRE(precision >= MGW_FP64 && precision <= MGW_BF16X6, MGW_ERR_ARG,
+  MGW_REQUIRE(precision >= MGW_FP64 && precision <= MGW_FP16X3, MGW_ERR_ARG,
               "mgw_polarizability_grid: unknown precision");
   MGW_REQUIRE(omega_im && weight_quad, MGW_ERR_ARG, "null frequency grid");
   MGW_TRY(polarizability_grid(c, r, spin_fact, nomega, omega_im, weight_quad, erpa, egw, precision));
   if (chi_out) {
-    // chi = (L L^T)^-1 - 1 = L^-T L^-1 - 1, formed only for callers that ask for it
+    // chi = (L L^T)^-1 - 1 = L^-T L^-1 - 1, formed only for callers that ask
+    // for it, from L^-1 = D + 1 (D is restored afterwards)
     const int n = c->naux;
     double* linv = find_buf(c, "wpol_linv");
     double *full, *mones;
@@ -1267,6 +1284,7 @@ int mgw_polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, i
     MGW_TRY(dbuf(c, "minus_ones", n, &mones));
     MGW_TRY(launch_fill(mones, n, -1.0, c->stream));
     MGW_TRY(launch_zero_upper(linv, n, nomega, c->stream));
+    MGW_TRY(launch_diag_add(linv, n, nomega, 1.0, c->stream));
     GemmF64 g;
     g.M = n; g.N = n; g.K = n;
     g.A = linv; g.lda = n; g.strideA = (int64_t)n * n; g.transA = true;
@@ -1277,6 +1295,7 @@ int mgw_polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, i
     g.batch = nomega;
     g.batch_outer = true;
     MGW_TRY(gemm_f64(g, c->stream));
+    MGW_TRY(launch_diag_add(linv, n, nomega, -1.0, c->stream));
     MGW_TRY(download(c, chi_out, full, (size_t)n * n * nomega));
     MGW_TRY(release(c, "chi_full"));
   }
@@ -1337,7 +1356,7 @@ int mgw_g0w0(mgw_ctx* c, const mgw_g0w0_params* prm, const mgw_ranges* r, const 
   MGW_REQUIRE(prm, MGW_ERR_ARG, "null parameters");
   MGW_TRY(check_ranges(c, r));
   const int prec = prm->precision;
-  MGW_REQUIRE(prec >= MGW_FP64 && prec <= MGW_BF16X6, MGW_ERR_ARG, "mgw_g0w0: unknown precision");
+  MGW_REQUIRE(prec >= MGW_FP64 && prec <= MGW_FP16X3, MGW_ERR_ARG, "mgw_g0w0: unknown precision");
   MGW_REQUIRE(prm->nomega_sigma >= 1, MGW_ERR_ARG, "nomega_sigma must be positive");
   const int nstate = c->nstate;
   const int nse = r->nsemax - r->nsemin + 1;

@@ -6,7 +6,11 @@
 // (src/m_gw_selfenergy_grid.f90:110-150 forms X diag(s) X^T with DSYRK in FP64).
 //
 // A pre-pass writes Y_f once per frequency, split into NS bf16 planes
-// y = y0 + y1 (+ y2), with y0 = bf16(y), y1 = bf16(y - y0), y2 = bf16(y - y0 - y1).
+// y = y0 + y1 (+ y2), with y0 = bf16(y), y1 = bf16(y - y0), y2 = bf16(y - y0 - y1),
+// or into two fp16 planes of 2^e_f Y_f (FP16X3): the power-of-two scale puts
+// the largest element near 2^14 (max|X| sqrt(max s_f) bounds it), so the
+// 11-bit hi and lo parts stay normal for every element that matters and the
+// result is unscaled exactly by 2^-2e_f.
 // Each plane is k-tiled, plane[f][kt][P][32] (P padded to a multiple of 128,
 // t padded to a multiple of 32, zeros in the padding), so the K slab of a
 // row block is one contiguous run and a lane's 8 consecutive k are one
@@ -14,6 +18,7 @@
 // v_mfma_f32_16x16x32_bf16:
 //   NS = 2 (BF16x3): y0 y0 + y0 y1 + y1 y0                  (~2^-17 relative per product)
 //   NS = 3 (BF16x6): + y1 y1 + y0 y2 + y2 y0                (~2^-24, FP32 level)
+//   NS = 2, fp16 (FP16X3): y0 y0 + y0 y1 + y1 y0 on v_mfma_f32_16x16x32_f16 (~2^-22)
 // FP32 partials are flushed into FP64 accumulators every FLUSH slabs of 32,
 // so sums over K ~ 10^5 transitions keep FP64 accumulation error.
 //
@@ -33,7 +38,18 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+// operand element and 8-vector of the planes: bf16, or fp16 when H
+template <bool H> struct PlaneT { typedef __bf16 E; typedef bf8 V; };
+template <> struct PlaneT<true> { typedef _Float16 E; typedef h8 V; };
+
+template <bool H>
+__device__ inline f4 mfma16x16x32(typename PlaneT<H>::V b, typename PlaneT<H>::V a, f4 c) {
+  if constexpr (H) return __builtin_amdgcn_mfma_f32_16x16x32_f16(b, a, c, 0, 0, 0);
+  else return __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a, c, 0, 0, 0);
+}
 
 constexpr int KT = 32;          // k per tile of the plane layout (one MFMA step)
 constexpr int PAD_ROWS = 128;   // P padding of the plane layout
@@ -42,10 +58,13 @@ constexpr int PAD_ROWS = 128;   // P padding of the plane layout
 // Block: 64 rows P x 32 transitions t (one k tile), all nf frequencies of
 // the chunk. X is read once through LDS (coalesced along P), each thread
 // then writes 8 consecutive t of one row per plane (one 16-byte store).
-template <int NS>
+template <int NS, bool H>
 __global__ void __launch_bounds__(256)
 chi_split_kernel(const double* __restrict__ x, int naux, int npole, const double* __restrict__ sw,
-                 int f0, int nf, int ppad, int nkt, __bf16* __restrict__ y, int64_t plane) {
+                 const double* __restrict__ scale, int f0, int nf, int ppad, int nkt,
+                 typename PlaneT<H>::E* __restrict__ y, int64_t plane) {
+  typedef typename PlaneT<H>::E E;
+  typedef typename PlaneT<H>::V V;
   __shared__ double tile[KT][65];
   const int tid = threadIdx.x;
   const int p0 = blockIdx.x * 64, kt = blockIdx.y, t0 = kt * KT;
@@ -60,27 +79,28 @@ chi_split_kernel(const double* __restrict__ x, int naux, int npole, const double
   const int64_t row = (int64_t)(p0 + p) * KT + 8 * c8;
   for (int f = 0; f < nf; ++f) {
     const double* __restrict__ s = sw + (int64_t)(f0 + f) * npole;
-    bf8 out[NS];
+    const double sc = scale ? scale[f0 + f] : 1.0;
+    V out[NS];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int t = t0 + 8 * c8 + e;
-      double v = (t < npole) ? tile[8 * c8 + e][p] * sqrt(s[t]) : 0.0;
+      double v = (t < npole) ? tile[8 * c8 + e][p] * sqrt(s[t]) * sc : 0.0;
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
-        const __bf16 h = (__bf16)(float)v;
+        const E h = (E)(float)v;
         out[q][e] = h;
         v -= (double)(float)h;
       }
     }
     const int64_t off = ((int64_t)f * nkt + kt) * (int64_t)ppad * KT + row;
 #pragma unroll
-    for (int q = 0; q < NS; ++q) *reinterpret_cast<bf8*>(y + q * plane + off) = out[q];
+    for (int q = 0; q < NS; ++q) *reinterpret_cast<V*>(y + q * plane + off) = out[q];
   }
 }
 
 // ---------------------------------------------------------------- SYRK
 struct SyrkArgs {
-  const __bf16* y;
+  const void* y;       // planes (bf16 or fp16)
   int64_t plane;       // elements between planes
   int64_t fstride;     // elements between frequencies
   int ppad, nkt, naux, nf, tiles, tm;
@@ -91,6 +111,7 @@ struct SyrkArgs {
   // writes its FP32 tile to part[((f * tiles + tile) * nkc + kc) * BM * BM]
   int kc_slabs, nkc;
   float* part;
+  const double* unscale;   // per frequency of the chunk (FP16X3), else null
 };
 
 // Lower-triangle tiles BM x BM, WM x WN waves, one 32-k slab per step.
@@ -98,9 +119,11 @@ struct SyrkArgs {
 // memory while the MFMAs consume the LDS copy of the current one (two
 // barriers per slab). DEPTH 3: two LDS buffers; the next slab is written to
 // the idle buffer while the MFMAs read the other (one barrier per slab).
-template <int NS, int BM, int WM, int WN, int FLUSH, int DEPTH, int KS, bool KSPLIT>
+template <int NS, bool H, int BM, int WM, int WN, int FLUSH, int DEPTH, int KS, bool KSPLIT>
 __global__ void __launch_bounds__(64 * WM * WN, KSPLIT ? 2 : 1)
 chi_syrk_kernel(SyrkArgs g) {
+  typedef typename PlaneT<H>::E E;
+  typedef typename PlaneT<H>::V V;
   constexpr int KP = KT + 8;              // padded LDS row: 80 bytes
   constexpr int NT = 64 * WM * WN;
   constexpr int TM = BM / WM, TN = BM / WN;
@@ -114,8 +137,8 @@ chi_syrk_kernel(SyrkArgs g) {
   constexpr int NBUF = DEPTH == 3 ? 2 : 1;
 
   // KS k tiles of 32 per slab (the plane layout pads the k tiles to a multiple of 2)
-  __shared__ __attribute__((aligned(16))) __bf16 As[NBUF][NS][KS][BM][KP];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[NBUF][NS][KS][BM][KP];
+  __shared__ __attribute__((aligned(16))) E As[NBUF][NS][KS][BM][KP];
+  __shared__ __attribute__((aligned(16))) E Bs[NBUF][NS][KS][BM][KP];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
@@ -130,8 +153,8 @@ chi_syrk_kernel(SyrkArgs g) {
   tri_grouped(tile, g.tm, bm, bn);
   const int m0 = bm * BM, n0 = bn * BM;
 
-  const __bf16* __restrict__ Ya = g.y + (int64_t)f * g.fstride + (int64_t)m0 * KT;
-  const __bf16* __restrict__ Yb = g.y + (int64_t)f * g.fstride + (int64_t)n0 * KT;
+  const E* __restrict__ Ya = static_cast<const E*>(g.y) + (int64_t)f * g.fstride + (int64_t)m0 * KT;
+  const E* __restrict__ Yb = static_cast<const E*>(g.y) + (int64_t)f * g.fstride + (int64_t)n0 * KT;
   const int64_t slab = (int64_t)g.ppad * KT;
 
   // prefetch registers (native vectors, so they stay in VGPRs)
@@ -177,15 +200,15 @@ chi_syrk_kernel(SyrkArgs g) {
     constexpr int P0 = NS == 3 ? 0 : 3;
 #pragma unroll
     for (int kk = 0; kk < KS; ++kk) {
-      bf8 a[NS][FM], b[NS][FN];
+      V a[NS][FM], b[NS][FN];
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
 #pragma unroll
         for (int i = 0; i < FM; ++i)
-          a[q][i] = *reinterpret_cast<const bf8*>(&As[buf][q][kk][wm * TM + i * 16 + fr][8 * fq]);
+          a[q][i] = *reinterpret_cast<const V*>(&As[buf][q][kk][wm * TM + i * 16 + fr][8 * fq]);
 #pragma unroll
         for (int j = 0; j < FN; ++j)
-          b[q][j] = *reinterpret_cast<const bf8*>(&Bs[buf][q][kk][wn * TN + j * 16 + fr][8 * fq]);
+          b[q][j] = *reinterpret_cast<const V*>(&Bs[buf][q][kk][wn * TN + j * 16 + fr][8 * fq]);
       }
       // product-major: consecutive MFMAs write different accumulators, so
       // none waits for the previous one's result (small products first)
@@ -195,7 +218,7 @@ chi_syrk_kernel(SyrkArgs g) {
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j)
-            part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[QB[pr]][j], a[QA[pr]][i], part[i][j], 0, 0, 0);
+            part[i][j] = mfma16x16x32<H>(b[QB[pr]][j], a[QA[pr]][i], part[i][j]);
     }
     if (!KSPLIT && ((s + 1) % FLUSH == 0 || s + 1 == nslab)) {
 #pragma unroll
@@ -268,6 +291,7 @@ chi_syrk_kernel(SyrkArgs g) {
   // epilogue: lower triangle plus its mirror, 1 on the diagonal
   double* __restrict__ C = g.C + (int64_t)f * g.strideC;
   const int n_ld = g.naux;
+  const double us = g.unscale ? g.unscale[f] : 1.0;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
     const int m = m0 + wm * TM + i * 16 + fr;
@@ -277,7 +301,7 @@ chi_syrk_kernel(SyrkArgs g) {
       for (int r = 0; r < 4; ++r) {
         const int n = n0 + wn * TN + j * 16 + 4 * fq + r;
         if (m >= g.naux || n >= g.naux || m < n) continue;
-        double v = acc[i][j][r];
+        double v = acc[i][j][r] * us;
         if (m == n) v += 1.0;
         C[m + (int64_t)n * n_ld] = v;
         if (m > n) C[n + (int64_t)m * n_ld] = v;
@@ -293,7 +317,7 @@ chi_syrk_kernel(SyrkArgs g) {
 template <int BM>
 __global__ void __launch_bounds__(256)
 chi_ksplit_reduce_kernel(const float* __restrict__ part, int nkc, int tiles, int tm, int naux,
-                         double* __restrict__ C, int64_t strideC) {
+                         const double* __restrict__ unscale, double* __restrict__ C, int64_t strideC) {
   const int e = blockIdx.x * 256 + threadIdx.x;
   const int tile = blockIdx.y, f = blockIdx.z;
   if (e >= BM * BM) return;
@@ -302,8 +326,10 @@ chi_ksplit_reduce_kernel(const float* __restrict__ part, int nkc, int tiles, int
   const int m = bm * BM + e % BM, n = bn * BM + e / BM;
   if (m >= naux || n >= naux || m < n) return;
   const float* __restrict__ p = part + (int64_t)(f * tiles + tile) * nkc * (BM * BM) + e;
-  double v = (m == n) ? 1.0 : 0.0;
+  double v = 0.0;
   for (int c = 0; c < nkc; ++c) v += (double)p[(int64_t)c * (BM * BM)];
+  if (unscale) v *= unscale[f];
+  if (m == n) v += 1.0;
   double* __restrict__ out = C + (int64_t)f * strideC;
   out[m + (int64_t)n * naux] = v;
   if (m > n) out[n + (int64_t)m * naux] = v;
@@ -369,41 +395,94 @@ SplitLayout split_layout(int naux, int npole, int ns, bool ksplit) {
   return L;
 }
 
-template <int NS>
+// ---------------------------------------------------------------- scales
+constexpr int NB_MAX = ABSMAX_BLOCKS;   // blocks of the |X| max reduction
+
+// Power-of-two scale of the fp16 planes of Y_f = X diag(sqrt(s_f)): the bound
+// max|X| sqrt(max_t s_f(t)) >= max|Y_f| is brought into [2^13, 2^14); the
+// product is unscaled by the exact inverse square.
+__global__ void __launch_bounds__(256)
+chi_scale_kernel(const double* __restrict__ xpart, int nb, const double* __restrict__ sw, int npole,
+                 double* __restrict__ scale, double* __restrict__ unscale) {
+  __shared__ double red[256];
+  const int f = blockIdx.x, tid = threadIdx.x;
+  const double* __restrict__ s = sw + (int64_t)f * npole;
+  double m = 0.0;
+  for (int t = tid; t < npole; t += 256) m = fmax(m, s[t]);
+  double xm = 0.0;
+  for (int b = tid; b < nb; b += 256) xm = fmax(xm, xpart[b]);
+  red[tid] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+    __syncthreads();
+  }
+  const double smax = red[0];
+  __syncthreads();
+  red[tid] = xm;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double bound = red[0] * sqrt(smax);
+    const int e = (bound > 0.0 && bound <= 1.0e300) ? ilogb(bound) : 0;   // bound in [2^e, 2^(e+1))
+    scale[f] = ldexp(1.0, 13 - e);
+    unscale[f] = ldexp(1.0, 2 * (e - 13));
+  }
+}
+
+// bytes in front of the planes: scale, unscale (per frequency), |X| partial maxima
+size_t scale_header_bytes(int nomega) { return ((size_t)(2 * nomega + NB_MAX) * sizeof(double) + 255) / 256 * 256; }
+
+template <int NS, bool H>
 int split_and_syrk(const double* x, int naux, int npole, const double* sw, int nomega, void* work,
                    size_t work_bytes, double* chi, hipStream_t stream) {
+  typedef typename PlaneT<H>::E E;
   constexpr int BM = 128;
   const bool ksplit = chi_ksplit();
   const SplitLayout L = split_layout(naux, npole, NS, ksplit);
   const size_t per_f = L.planes_per_f + L.part_per_f;
-  MGW_REQUIRE(work_bytes > 256, MGW_ERR_ARG, "chi0 split: workspace too small");
-  const int nf_max = (int)std::min<size_t>((size_t)nomega, (work_bytes - 256) / per_f);
+  const size_t head = scale_header_bytes(nomega);
+  MGW_REQUIRE(work_bytes > head + 256, MGW_ERR_ARG, "chi0 split: workspace too small");
+  const int nf_max = (int)std::min<size_t>((size_t)nomega, (work_bytes - head - 256) / per_f);
   MGW_REQUIRE(nf_max >= 1, MGW_ERR_ARG, "chi0 split: workspace smaller than one frequency");
-  __bf16* planes = static_cast<__bf16*>(work);
-  float* part = reinterpret_cast<float*>(static_cast<char*>(work) + (L.planes_per_f * nf_max + 255) / 256 * 256);
+  double* scl = static_cast<double*>(work);
+  double* uns = scl + nomega;
+  double* xpart = uns + nomega;
+  E* planes = reinterpret_cast<E*>(static_cast<char*>(work) + head);
+  float* part = reinterpret_cast<float*>(static_cast<char*>(work) + head +
+                                         (L.planes_per_f * nf_max + 255) / 256 * 256);
+  if (H) {
+    MGW_TRY(launch_absmax(x, (int64_t)naux * npole, xpart, stream));
+    hipLaunchKernelGGL(chi_scale_kernel, dim3(nomega), dim3(256), 0, stream, xpart, NB_MAX, sw, npole, scl, uns);
+    MGW_HIP(hipGetLastError());
+  }
   for (int f0 = 0; f0 < nomega; f0 += nf_max) {
     const int nf = std::min(nf_max, nomega - f0);
     const int64_t plane = L.per_plane_f * nf;
-    hipLaunchKernelGGL((chi_split_kernel<NS>), dim3(L.ppad / 64, L.nkt), dim3(256), 0, stream, x, naux, npole, sw,
-                       f0, nf, L.ppad, L.nkt, planes, plane);
+    hipLaunchKernelGGL((chi_split_kernel<NS, H>), dim3(L.ppad / 64, L.nkt), dim3(256), 0, stream, x, naux, npole,
+                       sw, H ? scl : nullptr, f0, nf, L.ppad, L.nkt, planes, plane);
     MGW_HIP(hipGetLastError());
     SyrkArgs g;
     g.y = planes; g.plane = plane; g.fstride = L.per_plane_f;
     g.ppad = L.ppad; g.nkt = L.nkt; g.naux = naux; g.nf = nf; g.tiles = L.tiles; g.tm = L.tm;
     g.C = chi + (int64_t)f0 * naux * naux; g.strideC = (int64_t)naux * naux;
     g.kc_slabs = KC_SLABS; g.nkc = L.nkc; g.part = part;
+    g.unscale = H ? uns + f0 : nullptr;
     const int depth = chi_prefetch_depth(NS);
     const dim3 block(256);
     if (ksplit) {
       g.nblocks = (int64_t)L.tiles * L.nkc * nf;
       MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chi0 split: too many tiles");
       const dim3 grid((unsigned)g.nblocks);
-      // one LDS buffer and one register set: 162 (BF16x3) / 196 (BF16x6)
+      // one LDS buffer and one register set: 162 (two planes) / 196 (three)
       // VGPRs, two or three waves per SIMD; a second register set spills
-      hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
+      hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
       MGW_HIP(hipGetLastError());
       hipLaunchKernelGGL((chi_ksplit_reduce_kernel<BM>), dim3(BM * BM / 256, L.tiles, nf), block, 0, stream, part,
-                         L.nkc, L.tiles, L.tm, naux, g.C, g.strideC);
+                         L.nkc, L.tiles, L.tm, naux, g.unscale, g.C, g.strideC);
       MGW_HIP(hipGetLastError());
       continue;
     }
@@ -412,12 +491,12 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
     const int ks = chi_k_tiles(NS);
     const dim3 grid((unsigned)g.nblocks);
     if (ks == 2) {
-      if (depth == 1) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 2, 1, 2, false>), grid, block, 0, stream, g);
-      else hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 2, 2, 2, false>), grid, block, 0, stream, g);
+      if (depth == 1) hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 2, 1, 2, false>), grid, block, 0, stream, g);
+      else hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 2, 2, 2, false>), grid, block, 0, stream, g);
     } else {
-      if (depth == 1) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 1, 1, false>), grid, block, 0, stream, g);
-      else if (depth == 2) hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 2, 1, false>), grid, block, 0, stream, g);
-      else hipLaunchKernelGGL((chi_syrk_kernel<NS, BM, 2, 2, 4, 3, 1, false>), grid, block, 0, stream, g);
+      if (depth == 1) hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 1, 1, false>), grid, block, 0, stream, g);
+      else if (depth == 2) hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 2, 1, false>), grid, block, 0, stream, g);
+      else hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 3, 1, false>), grid, block, 0, stream, g);
     }
     MGW_HIP(hipGetLastError());
   }
@@ -438,7 +517,7 @@ size_t chi0_split_workspace(int naux, int npole, int nomega, int precision) {
   const size_t per_f = chi0_split_bytes_per_frequency(naux, npole, precision);
   const size_t budget = std::max<size_t>((size_t)32 * naux * npole, (size_t)2 << 30);
   const size_t nf = std::max<size_t>(1, std::min<size_t>((size_t)nomega, budget / per_f));
-  return nf * per_f + 256;
+  return scale_header_bytes(nomega) + nf * per_f + 512;
 }
 
 int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int nomega, int precision,
@@ -446,10 +525,12 @@ int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int 
   if (naux <= 0 || nomega <= 0) return MGW_OK;
   MGW_REQUIRE(npole > 0, MGW_ERR_ARG, "chi0 split: empty transition space");
   if (precision == MGW_BF16X3)
-    return split_and_syrk<2>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
+    return split_and_syrk<2, false>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
   if (precision == MGW_BF16X6)
-    return split_and_syrk<3>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
-  set_error("chi0 split: precision must be BF16X3 or BF16X6");
+    return split_and_syrk<3, false>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
+  if (precision == MGW_FP16X3)
+    return split_and_syrk<2, true>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
+  set_error("chi0 split: precision must be BF16X3, BF16X6 or FP16X3");
   return MGW_ERR_ARG;
 }
 

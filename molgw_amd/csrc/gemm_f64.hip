@@ -55,6 +55,7 @@ struct GemmDev {
   int tiles_m, tiles_n, batch;
   int64_t nblocks;
   const double* dotB; int64_t lddot;
+  double dot_cc, dot_cd;
   double* dotpart; int64_t strideDot;
 };
 
@@ -199,8 +200,8 @@ gemm_f64_kernel(GemmDev g) {
   const double* __restrict__ dg = g.diag_add ? g.diag_add + (int64_t)batch * g.strideDiag : nullptr;
 
   if constexpr (DOT) {
-    // Column dots of this row block: sum_m C(m,n) * D(m,n), or sum_m C(m,n)^2
-    // without D.
+    // Column dots of this row block: sum_m C(m,n) (cc C(m,n) + cd D(m,n)),
+    // or sum_m C(m,n)^2 without D.
     __shared__ double red[WM][BN];
     const double* __restrict__ D = g.dotB ? g.dotB + (int64_t)batch * g.strideB : nullptr;  // batched as B
 #pragma unroll
@@ -215,7 +216,7 @@ gemm_f64_kernel(GemmDev g) {
           const int m = m0 + wm * TM + i * 16 + fr;
           if (m < g.M && n < g.N) {
             const double c = g.alpha * acc[i][j][r];
-            part += D ? c * D[m + (int64_t)n * g.lddot] : c * c;
+            part += D ? c * (g.dot_cc * c + g.dot_cd * D[m + (int64_t)n * g.lddot]) : c * c;
           }
         }
         part += __shfl_xor(part, 8, 16);
@@ -281,6 +282,7 @@ int launch_cfg(const GemmF64& h, hipStream_t stream) {
   g.tiles_n = (h.N + BN - 1) / BN;
   g.batch = h.batch;
   g.dotB = h.dotB; g.lddot = h.lddot; g.dotpart = h.dotpart; g.strideDot = h.strideDot;
+  g.dot_cc = h.dot_cc; g.dot_cd = h.dot_cd;
   // square-tile symmetric updates enumerate the lower triangle of tiles only;
   // other lower-only shapes launch the full grid and skip upper tiles
   const int64_t tiles = (h.lower_only && BM == BN) ? (int64_t)g.tiles_m * (g.tiles_m + 1) / 2
@@ -415,8 +417,9 @@ int gemm_autotune_mode(int mode) {
 
 int gemm_f64(const GemmF64& h, hipStream_t stream) {
   if (h.M <= 0 || h.N <= 0 || h.batch <= 0) return MGW_OK;
-  // MGW_BF16X6 splits only the chi0 operands (chi0_split.hip); every other
-  // product of that mode stays FP64
+  // MGW_BF16X6 and MGW_FP16X3 split only the chi0 and imaginary-axis
+  // self-energy operands (chi0_split.hip, wb_split.hip); every other product
+  // of those modes stays FP64
   if (h.precision == MGW_FP32 || h.precision == MGW_BF16X3) return gemm_lowp(h, h.precision, stream);
   MGW_REQUIRE(h.K >= 0, MGW_ERR_ARG, "gemm_f64: negative K");
   MGW_REQUIRE(!h.lower_only || h.M == h.N, MGW_ERR_ARG, "gemm_f64: lower_only needs M == N");

@@ -83,6 +83,47 @@ int launch_unpack_ri(const double* packed, int64_t npair, int naux, const int32_
   return MGW_OK;
 }
 
+// ------------------------------------------------------------------ max |x|
+__global__ void __launch_bounds__(256)
+absmax_kernel(const double* __restrict__ x, int64_t n, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + tid; i < n; i += (int64_t)gridDim.x * 256) m = fmax(m, fabs(x[i]));
+  red[tid] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) part[blockIdx.x] = red[0];
+}
+
+int launch_absmax(const double* x, int64_t n, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(absmax_kernel, dim3(ABSMAX_BLOCKS), dim3(256), 0, s, x, n, part);
+  MGW_LAUNCH_CHECK();
+  return MGW_OK;
+}
+
+// ------------------------------------------------------------------ diagonal
+__global__ void diag_add_kernel(double* __restrict__ m, int n, int nbatch, double v) {
+  const int64_t total = (int64_t)n * nbatch;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int64_t f = e / n, i = e - f * n;
+    m[f * n * n + i * (n + 1)] += v;
+  }
+}
+
+int launch_diag_add(double* m, int n, int nbatch, double v, hipStream_t s) {
+  const int64_t total = (int64_t)n * nbatch;
+  if (total <= 0) return MGW_OK;
+  unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(diag_add_kernel, dim3(g), dim3(256), 0, s, m, n, nbatch, v);
+  MGW_LAUNCH_CHECK();
+  return MGW_OK;
+}
+
 // ------------------------------------------------------------------ metric
 // Kept columns of the eigenvectors of the 2-center metric, each divided by
 // the square root of its eigenvalue (src/m_eri_calculate.f90:1206-1211).
@@ -406,26 +447,6 @@ int launch_chi_trace(const double* sw, const double* n2, int npole, const double
                      int mode, int nomega, double* out, hipStream_t s) {
   hipLaunchKernelGGL(chi_trace_kernel, dim3(nomega), dim3(256), 0, s, sw, n2, npole, mat, naux,
                      mode, out);
-  MGW_LAUNCH_CHECK();
-  return MGW_OK;
-}
-
-// chi = (1-chi0)^-1 - 1 from the lower triangle returned by the Cholesky
-// inverse: mirror to the upper triangle and subtract the identity.
-__global__ void chi_finish_kernel(double* __restrict__ m, int n) {
-  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
-  double* a = m + (int64_t)n * n * blockIdx.z;
-  if (i >= n || j >= n || i < j) return;   // (i, j) in the lower triangle
-  double v = a[i + (int64_t)n * j];
-  if (i == j) v -= 1.0;
-  a[i + (int64_t)n * j] = v;
-  a[j + (int64_t)n * i] = v;
-}
-
-int launch_chi_finish(double* m, int n, int nomega, hipStream_t s) {
-  dim3 grid(nblocks(n, 16), nblocks(n, 16), nomega);
-  hipLaunchKernelGGL(chi_finish_kernel, grid, dim3(256), 0, s, m, n);
   MGW_LAUNCH_CHECK();
   return MGW_OK;
 }

@@ -109,9 +109,12 @@ struct GemmF64 {
   bool tri_a = false;         // A (not transposed, M == K) is lower triangular
   bool batch_outer = false;   // tiles of one batch entry before the next (no shared operands)
   // Fused column dot: instead of storing C, accumulate
-  // dotpart[(rowblock * N + n) + batch * strideDot] = sum_{m in rowblock} C(m,n) * D(m,n)
-  // with D(m, n) = dotB[m + n * lddot], or sum C(m,n)^2 when dotB is null.
+  // dotpart[(rowblock * N + n) + batch * strideDot]
+  //   = sum_{m in rowblock} C(m,n) (dot_cc C(m,n) + dot_cd D(m,n))
+  // with D(m, n) = dotB[m + n * lddot] (batched with strideB), or
+  // sum C(m,n)^2 when dotB is null.
   const double* dotB = nullptr; int64_t lddot = 0;
+  double dot_cc = 0.0, dot_cd = 1.0;
   double* dotpart = nullptr; int64_t strideDot = 0;
   // kernel class: the hot classes get their own kernel symbol so that
   // profiler per-kernel statistics separate them
@@ -131,6 +134,13 @@ size_t chi0_split_workspace(int naux, int npole, int nomega, int precision);
 int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int nomega, int precision,
                     void* work, size_t work_bytes, double* chi, hipStream_t stream);
 int gemm_f64_rowblocks(int M);
+// v(n, f) = B_n^T chi(iw_f) B_n = sum_m C (C + 2 B), C = D_f B, from bf16-split
+// operands (MGW_BF16X3 / MGW_BF16X6) of D_f = L_f^-1 - 1 (lower triangle of
+// naux x naux, stride naux^2) and B (naux x ncol); writes the row-block
+// partials dotpart[(f * gemm_f64_rowblocks(naux) + rb) * ncol + n].
+size_t wb_split_workspace(int naux, int64_t ncol, int nomega, int precision);
+int wb_split_dot(const double* d, int naux, int nomega, const double* b, int64_t ncol, int precision, void* work,
+                 size_t work_bytes, double* dotpart, hipStream_t stream);
 // In-place inverse Cholesky factor of a batch of symmetric positive definite
 // matrices: A_f (n x n, lower triangle, stride n^2) becomes L_f^-1 (lower
 // triangle; the strict upper triangle is left undefined). logdet[f] receives
@@ -156,6 +166,11 @@ int launch_unpack_ri(const double* packed, int64_t npair, int naux, const int32_
 int launch_invsqrt_select(const double* u, int n, const double* eig, int neglect, int kept, double* out,
                           hipStream_t s);
 int launch_fill(double* p, int64_t n, double v, hipStream_t s);
+// part[b] = max |x_i| over the elements of block b, b < ABSMAX_BLOCKS
+constexpr int ABSMAX_BLOCKS = 256;
+int launch_absmax(const double* x, int64_t n, double* part, hipStream_t s);
+// m_f(i, i) += v for the nbatch n x n matrices m_f (stride n^2)
+int launch_diag_add(double* m, int n, int nbatch, double v, hipStream_t s);
 int launch_gather_ia(const double* eri, int naux, int nm, int nn, const int32_t* ia_dev, int npole,
                      double* x, hipStream_t s);
 int launch_colnorm2(const double* x, int64_t ld, int rows, int cols, double* out, hipStream_t s);
@@ -173,7 +188,6 @@ int launch_chi_weights(const int32_t* ia, int npole, const double* energy, const
                        const double* omega, int nomega, double* sw, hipStream_t s);
 int launch_chi_trace(const double* sw, const double* n2, int npole, const double* mat, int naux,
                      int mode, int nomega, double* out, hipStream_t s);
-int launch_chi_finish(double* m, int n, int nomega, hipStream_t s);
 int launch_dot_reduce(const double* part, int nrb, int64_t ncol, int nomega, const double* sub, double* v,
                       hipStream_t s);
 int launch_sigma_imag_sum(const double* v, int np, int nse, int nomega, const double* omega,

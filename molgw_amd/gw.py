@@ -46,7 +46,8 @@ from ._lib import G0W0Params, Ranges, check
 COMPLETELY_EMPTY = 1.0e-5   # src/m_definitions.f90:66
 HA_EV = 27.21138505         # src/m_definitions.f90:43
 
-PRECISION = {'fp64': _lib.MGW_FP64, 'fp32': _lib.MGW_FP32, 'bf16x3': _lib.MGW_BF16X3, 'bf16x6': _lib.MGW_BF16X6}
+PRECISION = {'fp64': _lib.MGW_FP64, 'fp32': _lib.MGW_FP32, 'bf16x3': _lib.MGW_BF16X3, 'bf16x6': _lib.MGW_BF16X6,
+             'fp16x3': _lib.MGW_FP16X3}
 
 
 def _f64(a) -> np.ndarray:

@@ -35,7 +35,7 @@ class Request:
     system: object                 # workloads.System (host) or workloads.DeviceSystem
     arrival: float = 0.0           # seconds after the start of the run
     technique: str = 'imaginary_pade'
-    precision: str = 'bf16x3'
+    precision: str = 'fp16x3'
     nomega_chi_imag: int = 32
     nomega_sigma_calc: int = 16
     window: int = 8                # states HOMO-window+1 .. LUMO+window-1 (nse = 2 window)

@@ -15,7 +15,6 @@ import pytest
 import scipy.linalg as sla
 
 from oracle import gw_oracle as O
-from molgw_amd._lib import MgwError
 from molgw_amd.workloads import synthetic_system
 
 pytestmark = pytest.mark.gpu
@@ -87,5 +86,15 @@ def test_b_build_batches_must_follow_each_other(engine):
     second = np.asfortranarray(tmp[30:40])
     # a batch that skips pairs 21..30 is refused before any device work
     assert lib.mgw_build_eri_3center(engine._h, second.ctypes.data, 31, 10, npair, ibp, s.nbf, 0, None) != 0
-    with pytest.raises(MgwError):
+    assert 'follow each other' in lib.mgw_last_error().decode()
+    # the host wrapper refuses a tensor whose auxiliary size is not the metric's
+    with pytest.raises(ValueError):
         engine.build_eri_3center(np.zeros((npair, 29)), s.index_basis, s.nbf)
+    # a build on a fresh context without a metric is refused by the library
+    from molgw_amd.gw import Engine
+    fresh = Engine(0)
+    try:
+        assert lib.mgw_build_eri_3center(fresh._h, first.ctypes.data, 1, 20, npair, ibp, s.nbf, 0, None) != 0
+        assert 'no 2-center metric' in lib.mgw_last_error().decode()
+    finally:
+        fresh.close()

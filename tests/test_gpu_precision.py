@@ -1,6 +1,6 @@
-"""Reduced-precision variants (FP32, BF16x3 split products with FP64
-accumulation) of the AO->MO transforms and the chi0 / Casida-matrix build,
-against the FP64 path on the GPU. Acceptance: every QP energy within 1 meV
+"""Reduced-precision variants (FP32, BF16x3 / BF16x6 split products with
+FP64 accumulation) of the AO->MO transforms, the chi0 / Casida-matrix build
+and the imaginary-axis self-energy product, against the FP64 path on the GPU. Acceptance: every QP energy within 1 meV
 of FP64 (the benchmark's accuracy condition); the measured errors are printed.
 """
 import numpy as np
@@ -29,8 +29,9 @@ def rel(a, b):
 
 
 # chi = (1 - chi0)^-1 - 1 amplifies the operand rounding of chi0 (FP32: 6e-8,
-# BF16x3 split: ~8e-6 per product, BF16x6 split: ~6e-8)
-@pytest.mark.parametrize('prec,tol_chi', [('fp32', 5e-5), ('bf16x3', 2e-4), ('bf16x6', 5e-5)])
+# BF16x3 split: ~8e-6 per product, BF16x6 split: ~6e-8, FP16x3 split of the
+# scaled operands: ~2e-7)
+@pytest.mark.parametrize('prec,tol_chi', [('fp32', 5e-5), ('bf16x3', 2e-4), ('bf16x6', 5e-5), ('fp16x3', 5e-5)])
 def test_chi_precision(engine, mol, prec, tol_chi):
     load(engine, mol)
     rng = selfenergy_set_state_range(mol.occupation)
@@ -42,10 +43,10 @@ def test_chi_precision(engine, mol, prec, tol_chi):
     assert err < tol_chi
 
 
-@pytest.mark.parametrize('prec', ['fp32', 'bf16x3', 'bf16x6'])
+@pytest.mark.parametrize('prec', ['fp32', 'bf16x3', 'bf16x6', 'fp16x3'])
 @pytest.mark.parametrize('technique', ['one_shot', 'imaginary_pade'])
 def test_qp_error_within_1_mev(engine, mol, prec, technique):
-    """FP32, BF16x6: every state within 1 meV. BF16x3 (~2^-17 relative per product):
+    """FP32, BF16x6, FP16x3: every state within 1 meV. BF16x3 (~2^-17 relative per product):
     the valence window HOMO-7..LUMO+8 that the serving path reports within
     1 meV at molecular magnitudes; deeper states are printed."""
     load(engine, mol)
@@ -57,7 +58,7 @@ def test_qp_error_within_1_mev(engine, mol, prec, technique):
     print(f'{technique} {prec}: max |dE_QP| all states {err.max() / MEV:.3e} meV, '
           f'window {err[win].max() / MEV:.4f} meV, HOMO {err[mol.nocc - 1] / MEV:.4f} meV, '
           f'LUMO {err[mol.nocc] / MEV:.4f} meV')
-    if prec in ('fp32', 'bf16x6'):
+    if prec in ('fp32', 'bf16x6', 'fp16x3'):
         assert err.max() < 1.0 * MEV
     else:
         assert err[win].max() < 1.0 * MEV
@@ -74,11 +75,13 @@ def test_benchmark_recipe_errors(engine):
     e32 = np.abs(engine.g0w0(precision='fp32').energy_qp - ref.energy_qp).max()
     e16 = np.abs(engine.g0w0(precision='bf16x3').energy_qp - ref.energy_qp).max()
     e6 = np.abs(engine.g0w0(precision='bf16x6').energy_qp - ref.energy_qp).max()
+    eh = np.abs(engine.g0w0(precision='fp16x3').energy_qp - ref.energy_qp).max()
     print(f'benchmark recipe: fp32 {e32 / MEV:.4f} meV, bf16x3 {e16 / MEV:.4f} meV, '
-          f'bf16x6 {e6 / MEV:.4f} meV (max |Sigma| {smax:.1f} Ha)')
+          f'bf16x6 {e6 / MEV:.4f} meV, fp16x3 {eh / MEV:.4f} meV (max |Sigma| {smax:.1f} Ha)')
     assert e32 < 1e-5 * smax
     assert e16 < 1e-4 * smax
     assert e6 < 1e-5 * smax
+    assert eh < 1e-5 * smax
 
 
 def test_split_chi0_matches_fp64_on_the_imaginary_grid(engine):
@@ -89,7 +92,7 @@ def test_split_chi0_matches_fp64_on_the_imaginary_grid(engine):
     rng = selfenergy_set_state_range(s.occupation)
     om, wt = imaginary_quad_grid(6)
     e64, g64, chi64 = engine.polarizability_grid(rng, om, wt, return_chi=True)
-    for prec, tol in (('bf16x3', 2e-4), ('bf16x6', 5e-5)):
+    for prec, tol in (('bf16x3', 2e-4), ('bf16x6', 5e-5), ('fp16x3', 5e-5)):
         e, g, chi = engine.polarizability_grid(rng, om, wt, return_chi=True, precision=prec)
         err = rel(chi, chi64)
         print(f'{prec}: chi rel err {err:.2e}, E_RPA err {abs(e - e64):.2e} Ha')
@@ -97,7 +100,7 @@ def test_split_chi0_matches_fp64_on_the_imaginary_grid(engine):
     assert engine.precision_escalations() == 0
 
 
-@pytest.mark.parametrize('prec', ['bf16x3', 'bf16x6'])
+@pytest.mark.parametrize('prec', ['bf16x3', 'bf16x6', 'fp16x3'])
 def test_large_strongly_coupled_grid_against_fp64(engine, prec):
     """A (H2O)19-shaped system (nbf 456, naux 1596, nocc 95) of the benchmark
     recipe: chi0 eigenvalues reach ~10^4, where a split-product 1 - chi0 can
@@ -113,3 +116,25 @@ def test_large_strongly_coupled_grid_against_fp64(engine, prec):
     esc = engine.precision_escalations() - before
     print(f'{prec}: E_RPA {e:.10f} vs {e64:.10f} (diff {abs(e - e64):.2e} Ha), {esc} frequencies rebuilt in FP64')
     assert abs(e - e64) < (1e-2 if prec == 'bf16x3' else 1e-4) * abs(e64)
+
+
+@pytest.mark.parametrize('prec', ['bf16x3', 'bf16x6', 'fp16x3'])
+def test_split_self_energy_product_against_fp64(engine, prec):
+    """The split-operand W.B product (D = L^-1 - 1 times (P|pm), C (C + 2B)
+    epilogue) at shapes that exercise its padding: naux 300 (three row
+    blocks, the last partial), 70 states x 16 = 1120 columns (a partial
+    column block). Sigma on the imaginary axis and the QP energies against
+    the FP64 solve; every state within 1 meV."""
+    s = synthetic_system(70, 300, 11, seed=34, scale=0.3)
+    load(engine, s)
+    rng = selfenergy_set_state_range(s.occupation)
+    rng.nsemin, rng.nsemax = rng.nhomo_G - 7, rng.nhomo_G + 8
+    kw = dict(rng=rng, technique='imaginary_pade', nomega_chi_imag=12, nomega_sigma_calc=8)
+    ref = engine.g0w0(**kw)
+    out = engine.g0w0(precision=prec, **kw)
+    win = slice(rng.nsemin - 1, rng.nsemax)
+    err = np.abs(out.energy_qp[win] - ref.energy_qp[win])
+    print(f'{prec}: max |dE_QP| {err.max() / MEV:.4f} meV, sigma rel {rel(out.sigma, ref.sigma):.2e}, '
+          f'E_GW diff {abs(out.en_gw - ref.en_gw):.2e} Ha')
+    assert err.max() < 1.0 * MEV
+    assert rel(out.sigma, ref.sigma) < (1e-3 if prec == 'bf16x3' else 1e-4)
