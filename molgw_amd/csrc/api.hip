@@ -583,6 +583,7 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
   g.diag_add = ones; g.strideDiag = 0;
   g.batch = nomega;
   g.lower_only = true;
+  g.upper_zero = true;   // the Cholesky wants the strict upper triangle zero
   g.klass = MGW_KCLASS_GEMM_CHI0;
   g.precision = precision;
   // the split-operand kernels factor s = sqrt(s) sqrt(s): every weight must be >= 0
@@ -1283,7 +1284,6 @@ int mgw_polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, i
     MGW_TRY(dbuf(c, "chi_full", (size_t)n * n * nomega, &full));
     MGW_TRY(dbuf(c, "minus_ones", n, &mones));
     MGW_TRY(launch_fill(mones, n, -1.0, c->stream));
-    MGW_TRY(launch_zero_upper(linv, n, nomega, c->stream));
     MGW_TRY(launch_diag_add(linv, n, nomega, 1.0, c->stream));
     GemmF64 g;
     g.M = n; g.N = n; g.K = n;

@@ -56,8 +56,11 @@ constexpr int PAD_ROWS = 128;   // P padding of the plane layout
 
 // ---------------------------------------------------------------- split
 // Block: 64 rows P x 32 transitions t (one k tile), all nf frequencies of
-// the chunk. X is read once through LDS (coalesced along P), each thread
-// then writes 8 consecutive t of one row per plane (one 16-byte store).
+// the chunk. X is read once through LDS (coalesced along P) into registers;
+// the weights sqrt(s_f(t)) (times the plane scale) of 32 frequencies at a
+// time are staged in LDS, so the frequency loop issues no global loads and
+// each thread writes 8 consecutive t of one row per plane per frequency (one
+// 16-byte store).
 template <int NS, bool H>
 __global__ void __launch_bounds__(256)
 chi_split_kernel(const double* __restrict__ x, int naux, int npole, const double* __restrict__ sw,
@@ -65,7 +68,9 @@ chi_split_kernel(const double* __restrict__ x, int naux, int npole, const double
                  typename PlaneT<H>::E* __restrict__ y, int64_t plane) {
   typedef typename PlaneT<H>::E E;
   typedef typename PlaneT<H>::V V;
+  constexpr int FB = 32;                    // frequencies per staged weight block
   __shared__ double tile[KT][65];
+  __shared__ double wt[FB][KT];
   const int tid = threadIdx.x;
   const int p0 = blockIdx.x * 64, kt = blockIdx.y, t0 = kt * KT;
 #pragma unroll
@@ -77,24 +82,38 @@ chi_split_kernel(const double* __restrict__ x, int naux, int npole, const double
   __syncthreads();
   const int p = tid >> 2, c8 = tid & 3;
   const int64_t row = (int64_t)(p0 + p) * KT + 8 * c8;
-  for (int f = 0; f < nf; ++f) {
-    const double* __restrict__ s = sw + (int64_t)(f0 + f) * npole;
-    const double sc = scale ? scale[f0 + f] : 1.0;
-    V out[NS];
+  double xv[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int t = t0 + 8 * c8 + e;
-      double v = (t < npole) ? tile[8 * c8 + e][p] * sqrt(s[t]) * sc : 0.0;
-#pragma unroll
-      for (int q = 0; q < NS; ++q) {
-        const E h = (E)(float)v;
-        out[q][e] = h;
-        v -= (double)(float)h;
+  for (int e = 0; e < 8; ++e) xv[e] = tile[8 * c8 + e][p];
+  for (int fb = 0; fb < nf; fb += FB) {
+    const int nb = min(FB, nf - fb);
+    __syncthreads();                        // previous block's weights consumed
+    for (int e = tid; e < FB * KT; e += 256) {
+      const int fl = e / KT, tl = e % KT, t = t0 + tl;
+      double w = 0.0;
+      if (fl < nb && t < npole) {
+        const int f = f0 + fb + fl;
+        w = sqrt(sw[(int64_t)f * npole + t]) * (scale ? scale[f] : 1.0);
       }
+      wt[fl][tl] = w;
     }
-    const int64_t off = ((int64_t)f * nkt + kt) * (int64_t)ppad * KT + row;
+    __syncthreads();
+    for (int fl = 0; fl < nb; ++fl) {
+      V out[NS];
 #pragma unroll
-    for (int q = 0; q < NS; ++q) *reinterpret_cast<V*>(y + q * plane + off) = out[q];
+      for (int e = 0; e < 8; ++e) {
+        double v = xv[e] * wt[fl][8 * c8 + e];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          const E h = (E)(float)v;
+          out[q][e] = h;
+          v -= (double)(float)h;
+        }
+      }
+      const int64_t off = ((int64_t)(fb + fl) * nkt + kt) * (int64_t)ppad * KT + row;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) *reinterpret_cast<V*>(y + q * plane + off) = out[q];
+    }
   }
 }
 
@@ -288,7 +307,8 @@ chi_syrk_kernel(SyrkArgs g) {
     return;
   }
 
-  // epilogue: lower triangle plus its mirror, 1 on the diagonal
+  // epilogue: lower triangle with 1 on the diagonal, zeros above it (the
+  // Cholesky that follows reads the lower triangle and wants the upper zero)
   double* __restrict__ C = g.C + (int64_t)f * g.strideC;
   const int n_ld = g.naux;
   const double us = g.unscale ? g.unscale[f] : 1.0;
@@ -304,7 +324,7 @@ chi_syrk_kernel(SyrkArgs g) {
         double v = acc[i][j][r] * us;
         if (m == n) v += 1.0;
         C[m + (int64_t)n * n_ld] = v;
-        if (m > n) C[n + (int64_t)m * n_ld] = v;
+        if (m > n) C[n + (int64_t)m * n_ld] = 0.0;
       }
   }
 }
@@ -313,7 +333,7 @@ chi_syrk_kernel(SyrkArgs g) {
 #undef MGW_SYRK_STORE
 
 // 1 - chi0 = I + sum over the K chunks of the FP32 partial tiles, summed in
-// FP64 in chunk order; lower triangle plus its mirror.
+// FP64 in chunk order; lower triangle, zeros above it.
 template <int BM>
 __global__ void __launch_bounds__(256)
 chi_ksplit_reduce_kernel(const float* __restrict__ part, int nkc, int tiles, int tm, int naux,
@@ -332,7 +352,7 @@ chi_ksplit_reduce_kernel(const float* __restrict__ part, int nkc, int tiles, int
   if (m == n) v += 1.0;
   double* __restrict__ out = C + (int64_t)f * strideC;
   out[m + (int64_t)n * naux] = v;
-  if (m > n) out[n + (int64_t)m * naux] = v;
+  if (m > n) out[n + (int64_t)m * naux] = 0.0;
 }
 
 // Pipeline of the SYRK. Measured on MI355X (1200/3600/120, 32 frequencies):
@@ -373,22 +393,43 @@ bool chi_ksplit() {
   return forced != 0;
 }
 
-constexpr int KC_SLABS = 64;   // 2048 transitions per K chunk
+// K chunk of the split SYRK: at least 64 slabs (2048 transitions), grown up
+// to 256 slabs while the grid keeps >= 4096 workgroups, so large problems
+// write fewer FP32 partial tiles; MGW_CHI_KC (slabs) overrides it.
+constexpr int KC_MIN = 64, KC_MAX = 256;
+constexpr int64_t KC_TARGET_WGS = 4096;
+
+int chi_kc_slabs(int tiles, int nomega, int nslab) {
+  static const int forced = [] {
+    const char* v = std::getenv("MGW_CHI_KC");
+    const int k = v ? std::atoi(v) : 0;
+    return k >= 1 ? k : 0;
+  }();
+  if (forced) return forced;
+  int kc = KC_MIN;
+  while (kc < KC_MAX) {
+    const int64_t nkc = (nslab + 2 * kc - 1) / (2 * kc);
+    if ((int64_t)tiles * nomega * nkc < KC_TARGET_WGS) break;
+    kc *= 2;
+  }
+  return kc;
+}
 
 struct SplitLayout {
-  int ppad, nkt, tm, tiles, nkc;
+  int ppad, nkt, tm, tiles, kc_slabs, nkc;
   int64_t per_plane_f;   // elements of one plane of one frequency
   size_t planes_per_f;   // bytes of all planes of one frequency
   size_t part_per_f;     // bytes of the FP32 partial tiles of one frequency
 };
 
-SplitLayout split_layout(int naux, int npole, int ns, bool ksplit) {
+SplitLayout split_layout(int naux, int npole, int ns, bool ksplit, int nomega) {
   SplitLayout L;
   L.ppad = (naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
   L.nkt = ((npole + KT - 1) / KT + 1) / 2 * 2;   // even: slabs of one or two k tiles
   L.tm = (naux + 127) / 128;
   L.tiles = L.tm * (L.tm + 1) / 2;
-  L.nkc = (L.nkt + KC_SLABS - 1) / KC_SLABS;
+  L.kc_slabs = chi_kc_slabs(L.tiles, nomega, L.nkt);
+  L.nkc = (L.nkt + L.kc_slabs - 1) / L.kc_slabs;
   L.per_plane_f = (int64_t)L.nkt * L.ppad * KT;
   L.planes_per_f = (size_t)L.per_plane_f * ns * sizeof(__bf16);
   L.part_per_f = ksplit ? (size_t)L.tiles * L.nkc * 128 * 128 * sizeof(float) : 0;
@@ -442,7 +483,7 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
   typedef typename PlaneT<H>::E E;
   constexpr int BM = 128;
   const bool ksplit = chi_ksplit();
-  const SplitLayout L = split_layout(naux, npole, NS, ksplit);
+  const SplitLayout L = split_layout(naux, npole, NS, ksplit, nomega);
   const size_t per_f = L.planes_per_f + L.part_per_f;
   const size_t head = scale_header_bytes(nomega);
   MGW_REQUIRE(work_bytes > head + 256, MGW_ERR_ARG, "chi0 split: workspace too small");
@@ -469,7 +510,7 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
     g.y = planes; g.plane = plane; g.fstride = L.per_plane_f;
     g.ppad = L.ppad; g.nkt = L.nkt; g.naux = naux; g.nf = nf; g.tiles = L.tiles; g.tm = L.tm;
     g.C = chi + (int64_t)f0 * naux * naux; g.strideC = (int64_t)naux * naux;
-    g.kc_slabs = KC_SLABS; g.nkc = L.nkc; g.part = part;
+    g.kc_slabs = L.kc_slabs; g.nkc = L.nkc; g.part = part;
     g.unscale = H ? uns + f0 : nullptr;
     const int depth = chi_prefetch_depth(NS);
     const dim3 block(256);
@@ -505,8 +546,8 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
 
 }  // namespace
 
-size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision) {
-  const SplitLayout L = split_layout(naux, npole, precision == MGW_BF16X6 ? 3 : 2, chi_ksplit());
+size_t chi0_split_bytes_per_frequency(int naux, int npole, int nomega, int precision) {
+  const SplitLayout L = split_layout(naux, npole, precision == MGW_BF16X6 ? 3 : 2, chi_ksplit(), nomega);
   return L.planes_per_f + L.part_per_f;
 }
 
@@ -514,7 +555,7 @@ size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision) {
 // when they fit in max(4 x the FP64 operand, 2 GiB), else as many as fit (at
 // least one).
 size_t chi0_split_workspace(int naux, int npole, int nomega, int precision) {
-  const size_t per_f = chi0_split_bytes_per_frequency(naux, npole, precision);
+  const size_t per_f = chi0_split_bytes_per_frequency(naux, npole, nomega, precision);
   const size_t budget = std::max<size_t>((size_t)32 * naux * npole, (size_t)2 << 30);
   const size_t nf = std::max<size_t>(1, std::min<size_t>((size_t)nomega, budget / per_f));
   return scale_header_bytes(nomega) + nf * per_f + 512;

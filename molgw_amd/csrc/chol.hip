@@ -6,14 +6,16 @@
 // Here 1 - chi0 = L L^T is factored for every frequency of the batch at
 // once, and L is inverted in place:
 //   ln det(1 - chi0) = 2 sum_i ln L_ii,   Tr (1 - chi0)^-1 = ||L^-1||_F^2,
-//   B^T chi B = ||L^-1 B||^2 - ||B||^2   (chi = (1 - chi0)^-1 - 1),
-// so the self-energy never needs the inverse itself.
+// and the self-energy works on D = L^-1 - 1 (chi = L^-T L^-1 - 1), so it
+// never needs the inverse itself.
 //
 // Recursive: the matrix is halved (at a multiple of 64) until 64 x 64
 // leaves, which a one-wave kernel factors and inverts in registers; the
-// off-diagonal work is a few large batched MFMA GEMMs per level (the trailing
+// off-diagonal work is four batched MFMA GEMMs per level (the trailing
 // update has K = m/2), so the matrix is swept O(log n) times instead of once
-// per 64-wide block column.
+// per 64-wide block column. The strict upper triangle of the input must be
+// zero (the 1 - chi0 builds write zeros there); the recursion keeps it zero,
+// so the inverted diagonal blocks are plain operands of the next products.
 #include "mgw_internal.h"
 
 #include <algorithm>
@@ -97,16 +99,6 @@ chol_diag_kernel(double* __restrict__ A, int n, int64_t strideA, int k, int kb, 
   }
 }
 
-// dst_f(rows x cols, ldd) = src_f(rows x cols, lds) for every matrix f.
-__global__ void copy_block_kernel(const double* __restrict__ src, int64_t lds, int64_t strideS,
-                                  double* __restrict__ dst, int64_t ldd, int64_t strideD, int rows, int cols) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int f = blockIdx.y;
-  if (e >= (int64_t)rows * cols) return;
-  const int i = (int)(e % rows), j = (int)(e / rows);
-  dst[(int64_t)f * strideD + i + j * ldd] = src[(int64_t)f * strideS + i + j * lds];
-}
-
 int gemm_chol(const GemmF64& g0, hipStream_t stream) {
   GemmF64 g = g0;
   g.klass = MGW_KCLASS_CHOLESKY;
@@ -134,28 +126,32 @@ lower_frob2_kernel(const double* __restrict__ m, int n, double* __restrict__ par
   if (threadIdx.x == 0) partial[(int64_t)f * FROB_BLOCKS + b] = red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ void zero_upper_kernel(double* __restrict__ m, int n) {
-  const int i = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (i < n && j < n && i < j) m[(int64_t)blockIdx.z * n * n + i + (int64_t)j * n] = 0.0;
+}  // namespace
+
+namespace {
+
+// Split of a diagonal block of size m > NB: m1 a multiple of the leaf size.
+inline int chol_split(int m) {
+  int m1 = ((m + 1) / 2 + NB - 1) / NB * NB;
+  if (m1 >= m) m1 = m - NB;
+  return m1;
+}
+
+// Work of the recursion on a block of size m: L21 (m2 x m1) stays alive while
+// A22 is processed in the work behind it; P = L21 (L^-1)11 then reuses that
+// space. A11 is processed before L21 exists.
+size_t chol_need(int m) {
+  if (m <= NB) return 0;
+  const int m1 = chol_split(m), m2 = m - m1;
+  const size_t l21 = (size_t)m2 * m1;
+  return std::max(chol_need(m1), l21 + std::max(chol_need(m2), l21));
 }
 
 }  // namespace
 
-size_t chol_work_doubles(int n) {
-  const size_t h = (size_t)(n + 1) / 2 + NB;
-  return h * h;
-}
+size_t chol_work_doubles(int n) { return std::max<size_t>(chol_need(n), 1); }
 
 namespace {
-
-// Zero the strict upper triangle of the diagonal block [k0, k0 + m) of
-// every matrix (leading dimension n, stride n^2).
-__global__ void zero_upper_block_kernel(double* __restrict__ A, int n, int k0, int m) {
-  const int i = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (i < m && j < m && i < j) A[(int64_t)blockIdx.z * n * n + (k0 + i) + (int64_t)(k0 + j) * n] = 0.0;
-}
 
 struct CholCtx {
   double* A;
@@ -171,9 +167,10 @@ struct CholCtx {
 //   [A11    ]   factor and invert A11 (recursion), L21 = A21 L11^-T,
 //   [A21 A22]   A22 -= L21 L21^T, factor and invert A22 (recursion),
 //               (L^-1)21 = -(L^-1)22 L21 (L^-1)11.
-// The products are large batched MFMA GEMMs; the 64 x 64 leaves are factored
-// and inverted in registers by chol_diag_kernel.
-int chol_rec(const CholCtx& c, int k0, int m) {
+// The products are batched MFMA GEMMs; the 64 x 64 leaves are factored and
+// inverted in registers by chol_diag_kernel. `w` is this block's work
+// (chol_need(m) doubles per matrix, matrix stride c.sw).
+int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   const int n = c.n;
   const int64_t s2 = (int64_t)n * n;
   if (m <= NB) {
@@ -182,52 +179,44 @@ int chol_rec(const CholCtx& c, int k0, int m) {
     MGW_HIP(hipGetLastError());
     return MGW_OK;
   }
-  int m1 = ((m + 1) / 2 + NB - 1) / NB * NB;   // a multiple of the leaf size
-  if (m1 >= m) m1 = m - NB;
-  const int m2 = m - m1;
+  const int m1 = chol_split(m), m2 = m - m1;
   double* a11 = c.A + k0 + (int64_t)k0 * n;
   double* a21 = a11 + m1;
   double* a22 = a11 + m1 + (int64_t)m1 * n;
-  MGW_TRY(chol_rec(c, k0, m1));
-  // (L^-1)11 with exact zeros above the diagonal: a plain operand below
-  hipLaunchKernelGGL(zero_upper_block_kernel, dim3((m1 + 15) / 16, (m1 + 15) / 16, c.nbatch), dim3(256), 0,
-                     c.stream, c.A, n, k0, m1);
-  MGW_HIP(hipGetLastError());
-  // L21 = A21 (L^-1)11^T   (work: m2 x m1, leading dimension m2)
+  double* l21 = w;                          // m2 x m1, leading dimension m2
+  double* rest = w + (int64_t)m2 * m1;      // A22's work, then P
+  MGW_TRY(chol_rec(c, k0, m1, w));
+  // L21 = A21 (L^-1)11^T   ((L^-1)11 has exact zeros above the diagonal)
   GemmF64 p;
   p.M = m2; p.N = m1; p.K = m1;
   p.A = a21; p.lda = n; p.strideA = s2;
   p.B = a11; p.ldb = n; p.strideB = s2; p.transB = true;
-  p.C = c.work; p.ldc = m2; p.strideC = c.sw;
+  p.C = l21; p.ldc = m2; p.strideC = c.sw;
   p.batch = c.nbatch;
   MGW_TRY(gemm_chol(p, c.stream));
-  // A22 -= L21 L21^T (lower triangle)
+  // A22 -= L21 L21^T (lower triangle; the upper stays zero)
   GemmF64 u;
   u.M = m2; u.N = m2; u.K = m1;
-  u.A = c.work; u.lda = m2; u.strideA = c.sw;
-  u.B = c.work; u.ldb = m2; u.strideB = c.sw; u.transB = true;
+  u.A = l21; u.lda = m2; u.strideA = c.sw;
+  u.B = l21; u.ldb = m2; u.strideB = c.sw; u.transB = true;
   u.C = a22; u.ldc = n; u.strideC = s2;
   u.alpha = -1.0; u.beta = 1.0;
   u.lower_only = true; u.no_mirror = true;
   u.batch = c.nbatch;
   MGW_TRY(gemm_chol(u, c.stream));
-  const int64_t cnt = (int64_t)m2 * m1;
-  hipLaunchKernelGGL(copy_block_kernel, dim3((unsigned)((cnt + 255) / 256), c.nbatch), dim3(256), 0, c.stream,
-                     c.work, (int64_t)m2, c.sw, a21, (int64_t)n, s2, m2, m1);
-  MGW_HIP(hipGetLastError());
-  MGW_TRY(chol_rec(c, k0 + m1, m2));
-  // (L^-1)21 = -(L^-1)22 (L21 (L^-1)11)
+  MGW_TRY(chol_rec(c, k0 + m1, m2, rest));
+  // (L^-1)21 = -(L^-1)22 P,  P = L21 (L^-1)11
   GemmF64 q;
   q.M = m2; q.N = m1; q.K = m1;
-  q.A = a21; q.lda = n; q.strideA = s2;
+  q.A = l21; q.lda = m2; q.strideA = c.sw;
   q.B = a11; q.ldb = n; q.strideB = s2;
-  q.C = c.work; q.ldc = m2; q.strideC = c.sw;
+  q.C = rest; q.ldc = m2; q.strideC = c.sw;
   q.batch = c.nbatch;
   MGW_TRY(gemm_chol(q, c.stream));
   GemmF64 t;
   t.M = m2; t.N = m1; t.K = m2;
   t.A = a22; t.lda = n; t.strideA = s2; t.tri_a = true;
-  t.B = c.work; t.ldb = m2; t.strideB = c.sw;
+  t.B = rest; t.ldb = m2; t.strideB = c.sw;
   t.C = a21; t.ldc = n; t.strideC = s2;
   t.alpha = -1.0;
   t.batch = c.nbatch;
@@ -241,7 +230,7 @@ int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* 
   if (n <= 0 || nbatch <= 0) return MGW_OK;
   MGW_REQUIRE(nbatch <= 65535, MGW_ERR_ARG, "chol_inverse_batched: batch too large");
   const CholCtx c{A, n, nbatch, logdet, info, work, (int64_t)chol_work_doubles(n), stream};
-  return chol_rec(c, 0, n);
+  return chol_rec(c, 0, n, work);
 }
 
 int launch_lower_frob2(const double* m, int n, int nbatch, double* partial, hipStream_t s) {
@@ -250,10 +239,5 @@ int launch_lower_frob2(const double* m, int n, int nbatch, double* partial, hipS
   return MGW_OK;
 }
 
-int launch_zero_upper(double* m, int n, int nbatch, hipStream_t s) {
-  hipLaunchKernelGGL(zero_upper_kernel, dim3((n + 15) / 16, (n + 15) / 16, nbatch), dim3(256), 0, s, m, n);
-  MGW_HIP(hipGetLastError());
-  return MGW_OK;
-}
 
 }  // namespace mgw

@@ -50,6 +50,7 @@ struct GemmDev {
   double alpha, beta;
   int lower_only;
   int no_mirror;       // lower_only: leave the strict upper triangle untouched
+  int upper_zero;      // lower_only: write zeros above the diagonal instead of the mirror
   int tri_a;           // A (not transposed) is lower triangular: A(m, k) = 0 for k > m
   int batch_outer;     // walk the tiles of one batch entry before the next
   int tiles_m, tiles_n, batch;
@@ -257,7 +258,7 @@ gemm_f64_kernel(GemmDev g) {
         const int64_t idx = m + (int64_t)n * g.ldc;
         if (g.beta != 0.0) v += g.beta * C[idx];
         C[idx] = v;
-        if (g.lower_only && m > n && !g.no_mirror) C[n + (int64_t)m * g.ldc] = v;
+        if (g.lower_only && m > n && !g.no_mirror) C[n + (int64_t)m * g.ldc] = g.upper_zero ? 0.0 : v;
       }
     }
   }
@@ -276,6 +277,7 @@ int launch_cfg(const GemmF64& h, hipStream_t stream) {
   g.alpha = h.alpha; g.beta = h.beta;
   g.lower_only = h.lower_only ? 1 : 0;
   g.no_mirror = h.no_mirror ? 1 : 0;
+  g.upper_zero = h.upper_zero ? 1 : 0;
   g.tri_a = h.tri_a ? 1 : 0;
   g.batch_outer = h.batch_outer ? 1 : 0;
   g.tiles_m = (h.M + BM - 1) / BM;

@@ -46,6 +46,7 @@ struct LowpDev {
   const double* diag_add; int64_t strideDiag;
   double alpha, beta;
   int lower_only;
+  int upper_zero;      // lower_only: zeros above the diagonal instead of the mirror
   int tiles_m, tiles_n, batch;
   int64_t nblocks;
 };
@@ -90,7 +91,7 @@ __device__ inline void store_tile(const LowpDev& g, double (&acc)[FM][FN][4], in
         const int64_t idx = m + (int64_t)n * g.ldc;
         if (g.beta != 0.0) v += g.beta * C[idx];
         C[idx] = v;
-        if (g.lower_only && m > n) C[n + (int64_t)m * g.ldc] = v;
+        if (g.lower_only && m > n) C[n + (int64_t)m * g.ldc] = g.upper_zero ? 0.0 : v;
       }
     }
   }
@@ -364,6 +365,7 @@ int launch_lowp(const GemmF64& h, hipStream_t stream) {
   g.diag_add = h.diag_add; g.strideDiag = h.strideDiag;
   g.alpha = h.alpha; g.beta = h.beta;
   g.lower_only = h.lower_only ? 1 : 0;
+  g.upper_zero = h.upper_zero ? 1 : 0;
   g.tiles_m = (h.M + BM - 1) / BM;
   g.tiles_n = (h.N + BN - 1) / BN;
   g.batch = h.batch;

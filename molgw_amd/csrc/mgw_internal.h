@@ -106,6 +106,7 @@ struct GemmF64 {
   int batch = 1;
   bool lower_only = false;
   bool no_mirror = false;     // lower_only: do not mirror into the upper triangle
+  bool upper_zero = false;    // lower_only: write zeros, not the mirror, above the diagonal
   bool tri_a = false;         // A (not transposed, M == K) is lower triangular
   bool batch_outer = false;   // tiles of one batch entry before the next (no shared operands)
   // Fused column dot: instead of storing C, accumulate
@@ -126,10 +127,10 @@ struct GemmF64 {
 };
 int gemm_f64(const GemmF64& g, hipStream_t stream);
 int gemm_lowp(const GemmF64& g, int precision, hipStream_t stream);
-// 1 - chi0(iw_f) = I + X diag(s_f) X^T (lower triangle and mirror) for all
+// 1 - chi0(iw_f) = I + X diag(s_f) X^T (lower triangle, zeros above it) for all
 // nomega frequencies from bf16-split operands (precision MGW_BF16X3 or
 // MGW_BF16X6); needs s_f >= 0 and a workspace of chi0_split_workspace bytes.
-size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision);
+size_t chi0_split_bytes_per_frequency(int naux, int npole, int nomega, int precision);
 size_t chi0_split_workspace(int naux, int npole, int nomega, int precision);
 int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int nomega, int precision,
                     void* work, size_t work_bytes, double* chi, hipStream_t stream);
@@ -142,8 +143,8 @@ size_t wb_split_workspace(int naux, int64_t ncol, int nomega, int precision);
 int wb_split_dot(const double* d, int naux, int nomega, const double* b, int64_t ncol, int precision, void* work,
                  size_t work_bytes, double* dotpart, hipStream_t stream);
 // In-place inverse Cholesky factor of a batch of symmetric positive definite
-// matrices: A_f (n x n, lower triangle, stride n^2) becomes L_f^-1 (lower
-// triangle; the strict upper triangle is left undefined). logdet[f] receives
+// matrices: A_f (n x n, lower triangle, stride n^2, strict upper triangle
+// zero on entry) becomes L_f^-1 (lower triangle; the upper stays zero). logdet[f] receives
 // sum_i ln L_f(i, i); info[f] the 1-based column of the first non-positive
 // pivot (0 on success). work: chol_work_doubles(n) doubles per matrix.
 constexpr int CHOL_NB = 64;
@@ -153,8 +154,6 @@ int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* 
 // tr[f] = sum of squares of the lower triangle of M_f (n x n, stride n^2).
 int launch_lower_frob2(const double* m, int n, int nbatch, double* partial, hipStream_t s);
 constexpr int FROB_BLOCKS = 64;
-// Zero the strict upper triangle of each n x n matrix of the batch.
-int launch_zero_upper(double* m, int n, int nbatch, hipStream_t s);   // number of row blocks of the dot epilogue
 int gemm_autotune_mode(int mode); // -1 query; returns the previous mode
 
 // ---------------------------------------------------------------- kernels
