@@ -168,11 +168,12 @@ def gpu_bench(system, technique, steps, warmup, kw, precision='fp64'):
 
 PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0 / 3.0, 'bf16x6': 2500.0 / 6.0,
                'fp16x3': 2500.0 / 3.0}
-# kernel classes that run below FP64 in each mode (the rest stay FP64)
-LOWP_CLASSES = {'fp32': ('gemm_ao2mo', 'gemm_chi0', 'gemm_casida'),
-                'bf16x3': ('gemm_ao2mo', 'gemm_chi0', 'gemm_casida', 'gemm_wb'),
-                'bf16x6': ('gemm_chi0', 'gemm_wb'),
-                'fp16x3': ('gemm_chi0', 'gemm_wb')}
+# arithmetic of the kernel classes that run below FP64 in each mode (the rest stay FP64)
+LOWP_CLASSES = {'fp32': {'gemm_ao2mo': 'fp32', 'gemm_chi0': 'fp32', 'gemm_casida': 'fp32'},
+                'bf16x3': {'gemm_ao2mo': 'bf16x3', 'gemm_chi0': 'bf16x3', 'gemm_casida': 'bf16x3',
+                           'gemm_wb': 'bf16x3'},
+                'bf16x6': {'gemm_chi0': 'bf16x6', 'gemm_wb': 'bf16x6'},
+                'fp16x3': {'gemm_chi0': 'fp16x3', 'gemm_wb': 'fp16x3'}}
 
 
 def roofline_of(prof, precision='fp64', traffic_key=None):
@@ -192,7 +193,7 @@ def roofline_of(prof, precision='fp64', traffic_key=None):
     if name == 'sigma_poles':
         r = {'bound': 'valu', 'achieved': achieved, 'peak': VALU_F64_PEAK_TFLOPS, 'unit': 'TFLOP/s'}
     else:
-        prec = precision if name in LOWP_CLASSES.get(precision, ()) else 'fp64'
+        prec = LOWP_CLASSES.get(precision, {}).get(name, 'fp64')
         r = {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_TFLOPS[prec], 'unit': 'TFLOP/s',
              'arith': prec}
     r['frac'] = r['achieved'] / r['peak']

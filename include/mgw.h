@@ -103,7 +103,7 @@ int mgw_ctx_synchronize(mgw_ctx* ctx);
  * clean_deallocate (src/m_memory.f90:238-274). */
 int mgw_ctx_trim(mgw_ctx* ctx, int eager);
 /* With enable != 0, the batched Cholesky sequence of 1 - chi0 (about
- * 6 nauxil / 64 kernel launches per solve) is captured into a HIP graph once
+ * 5 nauxil / 64 kernel launches per solve) is captured into a HIP graph once
  * per shape and buffer set and replayed on later solves. No reference
  * counterpart (launch batching for the serving loop). */
 int mgw_ctx_graphs(mgw_ctx* ctx, int enable);

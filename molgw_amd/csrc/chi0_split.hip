@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 
 namespace mgw {
 namespace {
@@ -52,6 +53,14 @@ __device__ inline f4 mfma16x16x32(typename PlaneT<H>::V b, typename PlaneT<H>::V
 }
 
 constexpr int KT = 32;          // k per tile of the plane layout (one MFMA step)
+
+// LDS rows of a slab are 64 bytes (32 k) without padding; 16-byte chunk j of
+// row r is stored at chunk j ^ ((r >> 1) & 3). Eight consecutive rows read at
+// one chunk index (the MFMA fragment loads), and the four chunks of two rows
+// written by eight consecutive lanes (the slab stores), then fall on eight
+// distinct 16-byte bank groups. Rows of a fragment start at multiples of 16,
+// so (fr >> 1) & 3 gives the swizzle of row base + fr.
+__device__ inline int lds_chunk(int r, int j) { return j ^ ((r >> 1) & 3); }
 constexpr int PAD_ROWS = 128;   // P padding of the plane layout
 
 // ---------------------------------------------------------------- split
@@ -69,8 +78,11 @@ chi_split_kernel(const double* __restrict__ x, int naux, int npole, const double
   typedef typename PlaneT<H>::E E;
   typedef typename PlaneT<H>::V V;
   constexpr int FB = 32;                    // frequencies per staged weight block
+  // two planes carry at most 22 bits: the product and the split run in FP32
+  // (24 bits); three planes need the FP64 product
+  typedef typename std::conditional<NS == 2, float, double>::type R;
   __shared__ double tile[KT][65];
-  __shared__ double wt[FB][KT];
+  __shared__ R wt[FB][KT];
   const int tid = threadIdx.x;
   const int p0 = blockIdx.x * 64, kt = blockIdx.y, t0 = kt * KT;
 #pragma unroll
@@ -82,9 +94,9 @@ chi_split_kernel(const double* __restrict__ x, int naux, int npole, const double
   __syncthreads();
   const int p = tid >> 2, c8 = tid & 3;
   const int64_t row = (int64_t)(p0 + p) * KT + 8 * c8;
-  double xv[8];
+  R xv[8];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) xv[e] = tile[8 * c8 + e][p];
+  for (int e = 0; e < 8; ++e) xv[e] = (R)tile[8 * c8 + e][p];
   for (int fb = 0; fb < nf; fb += FB) {
     const int nb = min(FB, nf - fb);
     __syncthreads();                        // previous block's weights consumed
@@ -95,19 +107,19 @@ chi_split_kernel(const double* __restrict__ x, int naux, int npole, const double
         const int f = f0 + fb + fl;
         w = sqrt(sw[(int64_t)f * npole + t]) * (scale ? scale[f] : 1.0);
       }
-      wt[fl][tl] = w;
+      wt[fl][tl] = (R)w;
     }
     __syncthreads();
     for (int fl = 0; fl < nb; ++fl) {
       V out[NS];
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        double v = xv[e] * wt[fl][8 * c8 + e];
+        R v = xv[e] * wt[fl][8 * c8 + e];
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
           const E h = (E)(float)v;
           out[q][e] = h;
-          v -= (double)(float)h;
+          v -= (R)(float)h;
         }
       }
       const int64_t off = ((int64_t)(fb + fl) * nkt + kt) * (int64_t)ppad * KT + row;
@@ -143,7 +155,7 @@ __global__ void __launch_bounds__(64 * WM * WN, KSPLIT ? 2 : 1)
 chi_syrk_kernel(SyrkArgs g) {
   typedef typename PlaneT<H>::E E;
   typedef typename PlaneT<H>::V V;
-  constexpr int KP = KT + 8;              // padded LDS row: 80 bytes
+  constexpr int KP = KT;                  // LDS row: 64 bytes, chunks swizzled (lds_chunk)
   constexpr int NT = 64 * WM * WN;
   constexpr int TM = BM / WM, TN = BM / WN;
   constexpr int FM = TM / 16, FN = TN / 16;
@@ -192,8 +204,8 @@ chi_syrk_kernel(SyrkArgs g) {
     _Pragma("unroll") for (int kk = 0; kk < KS; ++kk)                                 \
       _Pragma("unroll") for (int i = 0; i < LPT; ++i) {                               \
         const int c = tid + i * NT;                                                   \
-        *reinterpret_cast<u4*>(&As[BUF][q][kk][c >> 2][8 * (c & 3)]) = RA[q][kk][i];  \
-        *reinterpret_cast<u4*>(&Bs[BUF][q][kk][c >> 2][8 * (c & 3)]) = RB[q][kk][i];  \
+        *reinterpret_cast<u4*>(&As[BUF][q][kk][c >> 2][8 * lds_chunk(c >> 2, c & 3)]) = RA[q][kk][i];  \
+        *reinterpret_cast<u4*>(&Bs[BUF][q][kk][c >> 2][8 * lds_chunk(c >> 2, c & 3)]) = RB[q][kk][i];  \
       }
 
   double acc[FM][FN][4];
@@ -224,10 +236,10 @@ chi_syrk_kernel(SyrkArgs g) {
       for (int q = 0; q < NS; ++q) {
 #pragma unroll
         for (int i = 0; i < FM; ++i)
-          a[q][i] = *reinterpret_cast<const V*>(&As[buf][q][kk][wm * TM + i * 16 + fr][8 * fq]);
+          a[q][i] = *reinterpret_cast<const V*>(&As[buf][q][kk][wm * TM + i * 16 + fr][8 * lds_chunk(fr, fq)]);
 #pragma unroll
         for (int j = 0; j < FN; ++j)
-          b[q][j] = *reinterpret_cast<const V*>(&Bs[buf][q][kk][wn * TN + j * 16 + fr][8 * fq]);
+          b[q][j] = *reinterpret_cast<const V*>(&Bs[buf][q][kk][wn * TN + j * 16 + fr][8 * lds_chunk(fr, fq)]);
       }
       // product-major: consecutive MFMAs write different accumulators, so
       // none waits for the previous one's result (small products first)
@@ -393,26 +405,19 @@ bool chi_ksplit() {
   return forced != 0;
 }
 
-// K chunk of the split SYRK: at least 64 slabs (2048 transitions), grown up
-// to 256 slabs while the grid keeps >= 4096 workgroups, so large problems
-// write fewer FP32 partial tiles; MGW_CHI_KC (slabs) overrides it.
-constexpr int KC_MIN = 64, KC_MAX = 256;
-constexpr int64_t KC_TARGET_WGS = 4096;
+// K chunk of the split SYRK: 64 slabs (2048 transitions). Larger chunks
+// write fewer FP32 partial tiles but accumulate longer in FP32; measured at
+// 1200/3600/120 (FP16x3), 256-slab chunks were slower (fewer workgroups per
+// launch) and doubled the QP error. MGW_CHI_KC (slabs) overrides it.
+constexpr int KC_SLABS = 64;
 
-int chi_kc_slabs(int tiles, int nomega, int nslab) {
+int chi_kc_slabs() {
   static const int forced = [] {
     const char* v = std::getenv("MGW_CHI_KC");
     const int k = v ? std::atoi(v) : 0;
     return k >= 1 ? k : 0;
   }();
-  if (forced) return forced;
-  int kc = KC_MIN;
-  while (kc < KC_MAX) {
-    const int64_t nkc = (nslab + 2 * kc - 1) / (2 * kc);
-    if ((int64_t)tiles * nomega * nkc < KC_TARGET_WGS) break;
-    kc *= 2;
-  }
-  return kc;
+  return forced ? forced : KC_SLABS;
 }
 
 struct SplitLayout {
@@ -422,17 +427,18 @@ struct SplitLayout {
   size_t part_per_f;     // bytes of the FP32 partial tiles of one frequency
 };
 
-SplitLayout split_layout(int naux, int npole, int ns, bool ksplit, int nomega) {
+SplitLayout split_layout(int naux, int npole, int ns, bool ksplit) {
+  constexpr int bm = 128;
   SplitLayout L;
   L.ppad = (naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
   L.nkt = ((npole + KT - 1) / KT + 1) / 2 * 2;   // even: slabs of one or two k tiles
-  L.tm = (naux + 127) / 128;
+  L.tm = (naux + bm - 1) / bm;
   L.tiles = L.tm * (L.tm + 1) / 2;
-  L.kc_slabs = chi_kc_slabs(L.tiles, nomega, L.nkt);
+  L.kc_slabs = chi_kc_slabs();
   L.nkc = (L.nkt + L.kc_slabs - 1) / L.kc_slabs;
   L.per_plane_f = (int64_t)L.nkt * L.ppad * KT;
   L.planes_per_f = (size_t)L.per_plane_f * ns * sizeof(__bf16);
-  L.part_per_f = ksplit ? (size_t)L.tiles * L.nkc * 128 * 128 * sizeof(float) : 0;
+  L.part_per_f = ksplit ? (size_t)L.tiles * L.nkc * bm * bm * sizeof(float) : 0;
   return L;
 }
 
@@ -483,7 +489,7 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
   typedef typename PlaneT<H>::E E;
   constexpr int BM = 128;
   const bool ksplit = chi_ksplit();
-  const SplitLayout L = split_layout(naux, npole, NS, ksplit, nomega);
+  const SplitLayout L = split_layout(naux, npole, NS, ksplit);
   const size_t per_f = L.planes_per_f + L.part_per_f;
   const size_t head = scale_header_bytes(nomega);
   MGW_REQUIRE(work_bytes > head + 256, MGW_ERR_ARG, "chi0 split: workspace too small");
@@ -546,8 +552,8 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
 
 }  // namespace
 
-size_t chi0_split_bytes_per_frequency(int naux, int npole, int nomega, int precision) {
-  const SplitLayout L = split_layout(naux, npole, precision == MGW_BF16X6 ? 3 : 2, chi_ksplit(), nomega);
+size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision) {
+  const SplitLayout L = split_layout(naux, npole, precision == MGW_BF16X6 ? 3 : 2, chi_ksplit());
   return L.planes_per_f + L.part_per_f;
 }
 
@@ -555,7 +561,7 @@ size_t chi0_split_bytes_per_frequency(int naux, int npole, int nomega, int preci
 // when they fit in max(4 x the FP64 operand, 2 GiB), else as many as fit (at
 // least one).
 size_t chi0_split_workspace(int naux, int npole, int nomega, int precision) {
-  const size_t per_f = chi0_split_bytes_per_frequency(naux, npole, nomega, precision);
+  const size_t per_f = chi0_split_bytes_per_frequency(naux, npole, precision);
   const size_t budget = std::max<size_t>((size_t)32 * naux * npole, (size_t)2 << 30);
   const size_t nf = std::max<size_t>(1, std::min<size_t>((size_t)nomega, budget / per_f));
   return scale_header_bytes(nomega) + nf * per_f + 512;

@@ -130,7 +130,7 @@ int gemm_lowp(const GemmF64& g, int precision, hipStream_t stream);
 // 1 - chi0(iw_f) = I + X diag(s_f) X^T (lower triangle, zeros above it) for all
 // nomega frequencies from bf16-split operands (precision MGW_BF16X3 or
 // MGW_BF16X6); needs s_f >= 0 and a workspace of chi0_split_workspace bytes.
-size_t chi0_split_bytes_per_frequency(int naux, int npole, int nomega, int precision);
+size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision);
 size_t chi0_split_workspace(int naux, int npole, int nomega, int precision);
 int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int nomega, int precision,
                     void* work, size_t work_bytes, double* chi, hipStream_t stream);

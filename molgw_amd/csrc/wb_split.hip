@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 namespace mgw {
 namespace {
@@ -51,6 +52,14 @@ __device__ inline f4 mfma16x16x32(typename PlaneT<H>::V b, typename PlaneT<H>::V
 constexpr double D_SCALE = 8192.0;   // 2^13: |D_f| <= 1
 
 constexpr int KT = 32;    // k per tile of the plane layout (one MFMA step)
+
+// LDS rows of a slab are 64 bytes (32 k) without padding; 16-byte chunk j of
+// row r is stored at chunk j ^ ((r >> 1) & 3). Eight consecutive rows read at
+// one chunk index (the MFMA fragment loads), and the four chunks of two rows
+// written by eight consecutive lanes (the slab stores), then fall on eight
+// distinct 16-byte bank groups. Rows of a fragment start at multiples of 16,
+// so (fr >> 1) & 3 gives the swizzle of row base + fr.
+__device__ inline int lds_chunk(int r, int j) { return j ^ ((r >> 1) & 3); }
 constexpr int PAD = 128;  // row padding of the plane layout (the tile size)
 
 // ---------------------------------------------------------------- split
@@ -75,15 +84,17 @@ wb_split_d_kernel(const double* __restrict__ d, int n, int64_t strideD, int rpad
   }
   __syncthreads();
   const int rl = tid >> 2, c8 = tid & 3;
+  // two planes carry at most 22 bits: split in FP32; three planes in FP64
+  typedef typename std::conditional<NS == 2, float, double>::type R;
   V out[NS];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    double v = tile[8 * c8 + e][rl] * (H ? D_SCALE : 1.0);
+    R v = (R)(tile[8 * c8 + e][rl] * (H ? D_SCALE : 1.0));
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       const E h = (E)(float)v;
       out[q][e] = h;
-      v -= (double)(float)h;
+      v -= (R)(float)h;
     }
   }
   const int64_t off = (int64_t)f * fstride + ((int64_t)kt * rpad + r0 + rl) * KT + 8 * c8;
@@ -108,16 +119,17 @@ wb_split_b_kernel(const double* __restrict__ b, int krows, int64_t ncol, int64_t
   const int kt = (int)((e >> 2) % nkt);
   const int64_t n = (e >> 2) / nkt;
   const int k0 = kt * KT + 8 * c8;
+  typedef typename std::conditional<NS == 2, float, double>::type R;
   V out[NS];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int k = k0 + i;
-    double v = (n < ncol && k < krows) ? b[k + n * ldb] * sc : 0.0;
+    R v = (R)((n < ncol && k < krows) ? b[k + n * ldb] * sc : 0.0);
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       const E h = (E)(float)v;
       out[q][i] = h;
-      v -= (double)(float)h;
+      v -= (R)(float)h;
     }
   }
   const int64_t off = ((int64_t)kt * npad + n) * KT + 8 * c8;
@@ -148,7 +160,7 @@ __global__ void __launch_bounds__(64 * WM * WN)
 wb_dot_kernel(WbArgs g) {
   typedef typename PlaneT<H>::E E;
   typedef typename PlaneT<H>::V V;
-  constexpr int KP = KT + 8;              // padded LDS row: 80 bytes
+  constexpr int KP = KT;                  // LDS row: 64 bytes, chunks swizzled (lds_chunk)
   constexpr int NT = 64 * WM * WN;
   constexpr int TM = BM / WM, TN = BN / WN;
   constexpr int FM = TM / 16, FN = TN / 16;
@@ -198,11 +210,11 @@ wb_dot_kernel(WbArgs g) {
   _Pragma("unroll") for (int q = 0; q < NS; ++q) {                                  \
     _Pragma("unroll") for (int i = 0; i < LA; ++i) {                                \
       const int c = tid + i * NT;                                                   \
-      *reinterpret_cast<u4*>(&As[BUF][q][c >> 2][8 * (c & 3)]) = RA[q][i];          \
+      *reinterpret_cast<u4*>(&As[BUF][q][c >> 2][8 * lds_chunk(c >> 2, c & 3)]) = RA[q][i];  \
     }                                                                               \
     _Pragma("unroll") for (int i = 0; i < LB; ++i) {                                \
       const int c = tid + i * NT;                                                   \
-      *reinterpret_cast<u4*>(&Bs[BUF][q][c >> 2][8 * (c & 3)]) = RB[q][i];          \
+      *reinterpret_cast<u4*>(&Bs[BUF][q][c >> 2][8 * lds_chunk(c >> 2, c & 3)]) = RB[q][i];  \
     }                                                                               \
   }
 
@@ -229,10 +241,10 @@ wb_dot_kernel(WbArgs g) {
     for (int q = 0; q < NS; ++q) {
 #pragma unroll
       for (int i = 0; i < FM; ++i)
-        a[q][i] = *reinterpret_cast<const V*>(&As[buf][q][wm * TM + i * 16 + fr][8 * fq]);
+        a[q][i] = *reinterpret_cast<const V*>(&As[buf][q][wm * TM + i * 16 + fr][8 * lds_chunk(fr, fq)]);
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        b[q][j] = *reinterpret_cast<const V*>(&Bs[buf][q][wn * TN + j * 16 + fr][8 * fq]);
+        b[q][j] = *reinterpret_cast<const V*>(&Bs[buf][q][wn * TN + j * 16 + fr][8 * lds_chunk(fr, fq)]);
     }
     // product-major: consecutive MFMAs write different accumulators
 #pragma unroll
