@@ -169,6 +169,19 @@ int mgw_build_eri_3center(mgw_ctx* ctx, const double* eri_3center_tmp, int64_t i
  * reference routines. */
 int mgw_set_mo(mgw_ctx* ctx, const double* c_matrix, int nbf, int nstate,
                const double* energy, const double* occupation, int where);
+/* nspin (1 or 2) channels: c_matrix(nbf, nstate, nspin), energy(nstate, nspin),
+ * occupation(nstate, nspin), column-major like the reference's arrays. W sums
+ * the transitions of every channel (transition table spin outer,
+ * src/m_spectral_function.f90:239-266); the AO->MO, self-energy and QP calls
+ * work on the channel chosen by mgw_set_spin (channel 1 after this call),
+ * and mgw_g0w0 loops over the channels. With nspin = 2 pass spin_fact = 1
+ * (the reference's spin_fact = 3 - nspin, src/m_inputparam.f90:1059). */
+int mgw_set_mo_spin(mgw_ctx* ctx, const double* c_matrix, int nbf, int nstate, int nspin,
+                    const double* energy, const double* occupation, int where);
+/* Active channel (1-based) of the AO->MO, self-energy and QP calls: the
+ * reference's spin index of eri_3center_mo(:, :, :, ispin)
+ * (src/m_gw_selfenergy_grid.f90:279, src/m_gw_selfenergy_analytic.f90:80-102). */
+int mgw_set_spin(mgw_ctx* ctx, int ispin);
 
 /* Energies in G for the following self-energy calls (energy_g of
  * selfenergy_evaluation, src/m_selfenergy_evaluation.f90:164-183: the QP
@@ -254,7 +267,9 @@ int mgw_find_qp_energy(mgw_ctx* ctx, const mgw_ranges* ranges, int nomega, const
  * Sigma_x - Vxc (setup_exchange_m_vxc, src/m_selfenergy_tools.f90:720-802);
  * NULL means zero (converged HF). Outputs (all nullable): energy_qp, zz,
  * energy_qp_lin [nstate]; sigma_out (nomega_sigma x nse complex);
- * energies[2] = {E_RPA, E_GW}. */
+ * energies[2] = {E_RPA, E_GW}. With two spin channels (mgw_set_mo_spin)
+ * exchange_m_vxc and the state outputs are [nstate x nspin], sigma_out is
+ * [nomega_sigma x nse x nspin], and E_GW sums the channels. */
 int mgw_g0w0(mgw_ctx* ctx, const mgw_g0w0_params* params, const mgw_ranges* ranges,
              const double* exchange_m_vxc, double* energy_qp, double* zz,
              double* energy_qp_lin, double* sigma_out, double* energies);

@@ -54,6 +54,8 @@ SIGNATURES = {
     'mgw_build_eri_3center': (C.c_int, [_vp, _vp, C.c_int64, C.c_int64, C.c_int64, _ip, C.c_int, C.c_int,
                                         _vp]),
     'mgw_set_mo': (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int]),
+    'mgw_set_mo_spin': (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_int]),
+    'mgw_set_spin': (C.c_int, [_vp, C.c_int]),
     'mgw_calculate_eri_3center_mo': (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     'mgw_destroy_eri_3center_mo': (C.c_int, [_vp]),
     'mgw_npole': (C.c_int, [_vp, C.POINTER(Ranges), C.POINTER(C.c_int)]),

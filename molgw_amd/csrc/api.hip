@@ -92,12 +92,16 @@ struct mgw_ctx {
   int naux_bas = 0, naux_kept = 0;
   int64_t build_next = 0, build_npair = 0;
   int build_nbf = 0;
-  // MO data of the single spin channel; energy_g (empty: energy) are the
-  // energies in G of the self-energy, energy those of W
+  // MO data of nspin channels, each vector spin-major (nstate per channel);
+  // energy_g (empty: energy) are the energies in G of the self-energy, energy
+  // those of W. The self-energy, AO->MO and QP calls work on channel `spin`;
+  // W sums the transitions of all channels.
   bool has_mo = false;
-  int nstate = 0;
+  int nstate = 0, nspin = 1, spin = 0;
   std::vector<double> energy, occupation, energy_g;
   const std::vector<double>& energy_in_g() const { return energy_g.empty() ? energy : energy_g; }
+  // offset of the active channel in the spin-major vectors
+  int so() const { return spin * nstate; }
   // A2 module state eri_3center_mo(P, m, n)
   bool has_eri_mo = false;
   int mlo = 0, mhi = 0, nlo = 0, nhi = 0;
@@ -241,8 +245,9 @@ int check_ranges(mgw_ctx* c, const mgw_ranges* r) {
   return MGW_OK;
 }
 
-// Transition table (src/m_spectral_function.f90:224-266, skip_transition :359-378).
-// Entries are 0-based absolute state indices (i, a).
+// Transition table (src/m_spectral_function.f90:224-266, skip_transition :359-378),
+// spin outer. Entries are 0-based indices (i, a) into the spin-major state
+// vectors: state i of spin s is s * nstate + i.
 void transition_table(const mgw_ctx* c, const mgw_ranges* r, double spin_fact, std::vector<int32_t>& ia) {
   ia.clear();
   const double ce = 1.0e-5;
@@ -254,14 +259,17 @@ void transition_table(const mgw_ctx* c, const mgw_ranges* r, double spin_fact, s
     if (o1 > spin_fact - ce && o2 > spin_fact - ce) return true;
     return false;
   };
-  for (int i = 1; i <= ns; ++i)
-    for (int j = 1; j <= ns; ++j) {
-      const double oj = c->occupation[j - 1], oi = c->occupation[i - 1];
-      if (skip(j, i, oj, oi)) continue;
-      if (oj - oi > 0.0) continue;
-      ia.push_back(i - 1);
-      ia.push_back(j - 1);
-    }
+  for (int s = 0; s < c->nspin; ++s) {
+    const double* occ = c->occupation.data() + (size_t)s * ns;
+    for (int i = 1; i <= ns; ++i)
+      for (int j = 1; j <= ns; ++j) {
+        const double oj = occ[j - 1], oi = occ[i - 1];
+        if (skip(j, i, oj, oi)) continue;
+        if (oj - oi > 0.0) continue;
+        ia.push_back(s * ns + i - 1);
+        ia.push_back(s * ns + j - 1);
+      }
+  }
 }
 
 // Is the table the full product i in [i0, i1] (outer) x a in [a0, a1] (inner)?
@@ -283,7 +291,7 @@ bool rectangular(const std::vector<int32_t>& ia, int& i0, int& i1, int& a0, int&
 // is contracted first: H(P, k, f) = B(P k, l) C(l, f) is one GEMM over the
 // dense resident tensor, then a GEMM batched over f contracts k and writes
 // straight into the (P, m, n) layout of the reference.
-int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int precision = MGW_FP64) {
+int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int precision = MGW_FP64, int spin = -1) {
   MGW_REQUIRE(c->has_ri, MGW_ERR_STATE, "RI tensor not set (mgw_set_eri_3center)");
   MGW_REQUIRE(c->has_mo, MGW_ERR_STATE, "MO data not set (mgw_set_mo)");
   MGW_REQUIRE(mlo >= 1 && mhi <= c->nstate && mlo <= mhi && nlo >= 1 && nhi <= c->nstate && nlo <= nhi,
@@ -291,7 +299,8 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int preci
   const int nbf = c->nbf, naux = c->naux;
   const int nm = mhi - mlo + 1, nn = nhi - nlo + 1;
   const double* B = find_buf(c, "ri_dense");
-  const double* C = find_buf(c, "c_matrix");
+  // MO coefficients of the requested (default: active) spin channel
+  const double* C = find_buf(c, "c_matrix") + (int64_t)(spin < 0 ? c->spin : spin) * nbf * c->nstate;
   const bool first_n = nn <= nm;
   const int nf = first_n ? nn : nm;
   const int flo = first_n ? nlo : mlo;
@@ -331,7 +340,9 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int preci
   return MGW_OK;
 }
 
-// X(P, t) = (P| a_t i_t) for the W transition table, into buffer "x_ia".
+// X(P, t) = (P| a_t i_t) for the W transition table, into buffer "x_ia";
+// the transitions of each spin channel are contiguous (spin outer) and come
+// from that channel's AO->MO transform.
 int build_x_ia(mgw_ctx* c, const mgw_ranges* r, double spin_fact, std::vector<int32_t>& ia,
                double** x_out, int precision) {
   transition_table(c, r, spin_fact, ia);
@@ -339,22 +350,33 @@ int build_x_ia(mgw_ctx* c, const mgw_ranges* r, double spin_fact, std::vector<in
   MGW_REQUIRE(npole > 0, MGW_ERR_ARG, "empty transition space");
   double* x = nullptr;
   MGW_TRY(dbuf(c, "x_ia", (size_t)c->naux * npole, &x));
-  int i0, i1, a0, a1;
-  if (rectangular(ia, i0, i1, a0, a1)) {
-    // (P, a, i) with a inner is exactly X(P, t), t = (i outer, a inner)
-    MGW_TRY(ao2mo(c, a0 + 1, a1 + 1, i0 + 1, i1 + 1, x, precision));
-  } else {
-    const int wlo = r->ncore_W + 1, whi = r->nvirtual_W - 1;
-    const int nw = whi - wlo + 1;
-    double* e = nullptr;
-    MGW_TRY(dbuf(c, "ao2mo_w", (size_t)c->naux * nw * nw, &e));
-    MGW_TRY(ao2mo(c, wlo, whi, wlo, whi, e, precision));
-    std::vector<int32_t> rel(ia.size());
-    for (size_t k = 0; k < ia.size(); ++k) rel[k] = ia[k] - (wlo - 1);
-    int32_t* ia_d = nullptr;
-    MGW_TRY(upload(c, "ia_rel", rel.data(), rel.size(), &ia_d));
-    MGW_TRY(launch_gather_ia(e, c->naux, nw, nw, ia_d, npole, x, c->stream));
-    MGW_TRY(scratch_done(c, "ao2mo_w"));
+  const int ns = c->nstate;
+  int t0 = 0;
+  for (int s = 0; s < c->nspin && t0 < npole; ++s) {
+    int t1 = t0;
+    while (t1 < npole && ia[2 * t1] / ns == s) ++t1;
+    if (t1 == t0) continue;
+    // this channel's table with state indices relative to the channel
+    std::vector<int32_t> sub(ia.begin() + 2 * t0, ia.begin() + 2 * t1);
+    for (auto& v : sub) v -= s * ns;
+    double* xs = x + (int64_t)c->naux * t0;
+    int i0, i1, a0, a1;
+    if (rectangular(sub, i0, i1, a0, a1)) {
+      // (P, a, i) with a inner is exactly X(P, t), t = (i outer, a inner)
+      MGW_TRY(ao2mo(c, a0 + 1, a1 + 1, i0 + 1, i1 + 1, xs, precision, s));
+    } else {
+      const int wlo = r->ncore_W + 1, whi = r->nvirtual_W - 1;
+      const int nw = whi - wlo + 1;
+      double* e = nullptr;
+      MGW_TRY(dbuf(c, "ao2mo_w", (size_t)c->naux * nw * nw, &e));
+      MGW_TRY(ao2mo(c, wlo, whi, wlo, whi, e, precision, s));
+      for (auto& v : sub) v -= (wlo - 1);
+      int32_t* ia_d = nullptr;
+      MGW_TRY(upload(c, "ia_rel", sub.data(), sub.size(), &ia_d));
+      MGW_TRY(launch_gather_ia(e, c->naux, nw, nw, ia_d, t1 - t0, xs, c->stream));
+      MGW_TRY(scratch_done(c, "ao2mo_w"));
+    }
+    t0 = t1;
   }
   *x_out = x;
   return MGW_OK;
@@ -456,9 +478,11 @@ int gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta,
   MGW_TRY(ao2mo(c, r->nsemin, r->nsemax, glo, ghi, epi, precision));
   // small per-state vectors
   std::vector<double> e0p(se_energy0, se_energy0 + nse), ei(ng), fi(ng), ep(nse), occp(nse);
-  const std::vector<double>& eg = c->energy_in_g();
-  for (int k = 0; k < ng; ++k) { ei[k] = eg[glo - 1 + k]; fi[k] = c->occupation[glo - 1 + k]; }
-  for (int k = 0; k < nse; ++k) { ep[k] = eg[r->nsemin - 1 + k]; occp[k] = c->occupation[r->nsemin - 1 + k]; }
+  // energies and occupations of the active spin channel
+  const double* eg = c->energy_in_g().data() + c->so();
+  const double* oc = c->occupation.data() + c->so();
+  for (int k = 0; k < ng; ++k) { ei[k] = eg[glo - 1 + k]; fi[k] = oc[glo - 1 + k]; }
+  for (int k = 0; k < nse; ++k) { ep[k] = eg[r->nsemin - 1 + k]; occp[k] = oc[r->nsemin - 1 + k]; }
   double *om_d, *e0p_d, *ei_d, *fi_d, *ep_d, *occp_d;
   MGW_TRY(upload(c, "se_omega", omega, nomega, &om_d));
   MGW_TRY(upload(c, "se_e0p", e0p.data(), nse, &e0p_d));
@@ -726,7 +750,7 @@ int gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int ncalc, const double*
   if (split) MGW_TRY(scratch_done(c, "wb_split"));
   MGW_TRY(launch_dot_reduce(part, nrb, ncol, nomega, nullptr, v, c->stream));
   std::vector<double> ep(np);
-  for (int k = 0; k < np; ++k) ep[k] = c->energy_in_g()[glo - 1 + k];
+  for (int k = 0; k < np; ++k) ep[k] = c->energy_in_g()[c->so() + glo - 1 + k];
   MGW_TRY(upload(c, "sgi_omega", c->chi_omega.data(), nomega, &om_d));
   MGW_TRY(upload(c, "sgi_weight", c->chi_weight.data(), nomega, &wt_d));
   MGW_TRY(upload(c, "sgi_ep", ep.data(), np, &ep_d));
@@ -890,7 +914,7 @@ int mgw_ctx_destroy(mgw_ctx* c) {
 int mgw_set_energy_g(mgw_ctx* c, const double* energy_g) {
   MGW_CTX(c);
   MGW_REQUIRE(c->has_mo, MGW_ERR_STATE, "MO data not set (mgw_set_mo)");
-  if (energy_g) c->energy_g.assign(energy_g, energy_g + c->nstate);
+  if (energy_g) c->energy_g.assign(energy_g, energy_g + (size_t)c->nstate * c->nspin);
   else c->energy_g.clear();
   c->has_sigma = false;
   c->has_sigma_calc = false;
@@ -1183,36 +1207,58 @@ int mgw_destroy_eri_3center(mgw_ctx* c) {
   return MGW_OK;
 }
 
-int mgw_set_mo(mgw_ctx* c, const double* c_matrix, int nbf, int nstate, const double* energy,
-               const double* occupation, int where) {
+int mgw_set_mo_spin(mgw_ctx* c, const double* c_matrix, int nbf, int nstate, int nspin, const double* energy,
+                    const double* occupation, int where) {
   MGW_CTX(c);
-  MGW_REQUIRE(c_matrix && energy && occupation && nbf > 0 && nstate > 0, MGW_ERR_ARG,
-              "mgw_set_mo: invalid arguments");
+  MGW_REQUIRE(c_matrix && energy && occupation && nbf > 0 && nstate > 0 && (nspin == 1 || nspin == 2),
+              MGW_ERR_ARG, "mgw_set_mo_spin: invalid arguments");
+  MGW_REQUIRE(where == MGW_MEM_HOST || where == MGW_MEM_DEVICE, MGW_ERR_ARG, "invalid memory kind");
   MGW_REQUIRE(!c->has_ri || nbf == c->nbf, MGW_ERR_ARG, "mgw_set_mo: nbf differs from the RI tensor");
+  const size_t nc = (size_t)nbf * nstate * nspin, ne = (size_t)nstate * nspin;
   double* cd = nullptr;
-  MGW_TRY(dbuf(c, "c_matrix", (size_t)nbf * nstate, &cd));
-  c->energy.resize(nstate);
-  c->occupation.resize(nstate);
+  MGW_TRY(dbuf(c, "c_matrix", nc, &cd));
+  c->energy.resize(ne);
+  c->occupation.resize(ne);
   if (where == MGW_MEM_DEVICE) {
-    MGW_HIP(hipMemcpyAsync(cd, c_matrix, sizeof(double) * nbf * nstate, hipMemcpyDeviceToDevice, c->stream));
-    MGW_HIP(hipMemcpyAsync(c->energy.data(), energy, sizeof(double) * nstate, hipMemcpyDeviceToHost, c->stream));
-    MGW_HIP(hipMemcpyAsync(c->occupation.data(), occupation, sizeof(double) * nstate, hipMemcpyDeviceToHost, c->stream));
+    MGW_HIP(hipMemcpyAsync(cd, c_matrix, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+    MGW_HIP(hipMemcpyAsync(c->energy.data(), energy, sizeof(double) * ne, hipMemcpyDeviceToHost, c->stream));
+    MGW_HIP(hipMemcpyAsync(c->occupation.data(), occupation, sizeof(double) * ne, hipMemcpyDeviceToHost, c->stream));
     MGW_HIP(hipStreamSynchronize(c->stream));
   } else {
-    MGW_HIP(hipMemcpyAsync(cd, c_matrix, sizeof(double) * nbf * nstate, hipMemcpyHostToDevice, c->stream));
-    std::memcpy(c->energy.data(), energy, sizeof(double) * nstate);
-    std::memcpy(c->occupation.data(), occupation, sizeof(double) * nstate);
+    MGW_HIP(hipMemcpyAsync(cd, c_matrix, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream));
+    std::memcpy(c->energy.data(), energy, sizeof(double) * ne);
+    std::memcpy(c->occupation.data(), occupation, sizeof(double) * ne);
   }
   double *ed, *od;
-  MGW_TRY(upload(c, "energy", c->energy.data(), nstate, &ed));
-  MGW_TRY(upload(c, "occupation", c->occupation.data(), nstate, &od));
+  MGW_TRY(upload(c, "energy", c->energy.data(), ne, &ed));
+  MGW_TRY(upload(c, "occupation", c->occupation.data(), ne, &od));
   MGW_HIP(hipStreamSynchronize(c->stream));
   c->nstate = nstate;
+  c->nspin = nspin;
+  c->spin = 0;
   c->energy_g.clear();
   c->has_mo = true;
   c->has_eri_mo = false;
   c->has_wpol = false;
   c->has_chi = false;
+  return MGW_OK;
+}
+
+int mgw_set_mo(mgw_ctx* c, const double* c_matrix, int nbf, int nstate, const double* energy,
+               const double* occupation, int where) {
+  return mgw_set_mo_spin(c, c_matrix, nbf, nstate, 1, energy, occupation, where);
+}
+
+int mgw_set_spin(mgw_ctx* c, int ispin) {
+  MGW_CTX(c);
+  MGW_REQUIRE(c->has_mo, MGW_ERR_STATE, "MO data not set (mgw_set_mo)");
+  MGW_REQUIRE(ispin >= 1 && ispin <= c->nspin, MGW_ERR_ARG, "mgw_set_spin: spin channel out of range");
+  if (c->spin != ispin - 1) {
+    c->spin = ispin - 1;
+    c->has_eri_mo = false;
+    c->has_sigma = false;
+    c->has_sigma_calc = false;
+  }
   return MGW_OK;
 }
 
@@ -1358,24 +1404,55 @@ int mgw_g0w0(mgw_ctx* c, const mgw_g0w0_params* prm, const mgw_ranges* r, const 
   const int prec = prm->precision;
   MGW_REQUIRE(prec >= MGW_FP64 && prec <= MGW_FP16X3, MGW_ERR_ARG, "mgw_g0w0: unknown precision");
   MGW_REQUIRE(prm->nomega_sigma >= 1, MGW_ERR_ARG, "nomega_sigma must be positive");
-  const int nstate = c->nstate;
+  const int nstate = c->nstate, nspin = c->nspin;
   const int nse = r->nsemax - r->nsemin + 1;
   const int nom = prm->nomega_sigma / 2;
   const int nomega = 2 * nom + 1;
   std::vector<double> omega(nomega), se0(nse);
   double en_rpa = 0.0, en_gw = 0.0;
+  struct SpinGuard {     // the active channel is restored on every return
+    mgw_ctx* c;
+    int s;
+    ~SpinGuard() { c->spin = s; }
+  } guard{c, c->spin};
+  // W from the transitions of every channel, then Sigma and the QP equation
+  // channel by channel (the reference's ispin loops); outputs are spin-major
+  auto per_spin = [&](int s) {
+    const int64_t o = (int64_t)s * nstate;
+    return find_qp(c, r, nomega, omega.data(), nullptr, se0.data(), exc ? exc + o : nullptr,
+                   c->energy.data() + o, nstate, energy_qp ? energy_qp + o : nullptr, zz ? zz + o : nullptr,
+                   energy_qp_lin ? energy_qp_lin + o : nullptr);
+  };
+  auto sigma_copy = [&](int s) -> int {
+    if (sigma_out)
+      MGW_TRY(download(c, sigma_out + (int64_t)s * 2 * nomega * nse, find_buf(c, "sigma"), (size_t)2 * nomega * nse));
+    return MGW_OK;
+  };
   if (prm->technique == MGW_TECH_ONE_SHOT) {
     // se_init(one_shot): grid centred on each energy0 (src/m_selfenergy_tools.f90:682-690)
     for (int k = 0; k < nomega; ++k) omega[k] = prm->step_sigma * (k - nom);
-    for (int k = 0; k < nse; ++k) se0[k] = c->energy_in_g()[r->nsemin - 1 + k];
     MGW_TRY(polarizability_casida(c, r, prm->spin_fact, &en_rpa, nullptr, prec));
-    MGW_TRY(gw_selfenergy(c, r, prm->spin_fact, prm->eta, nomega, omega.data(), se0.data(), &en_gw, prec));
+    for (int s = 0; s < nspin; ++s) {
+      c->spin = s;
+      for (int k = 0; k < nse; ++k) se0[k] = c->energy_in_g()[(size_t)s * nstate + r->nsemin - 1 + k];
+      double eg = 0.0;
+      MGW_TRY(gw_selfenergy(c, r, prm->spin_fact, prm->eta, nomega, omega.data(), se0.data(), &eg, prec));
+      en_gw += eg;
+      MGW_TRY(per_spin(s));
+      MGW_TRY(sigma_copy(s));
+    }
   } else if (prm->technique == MGW_TECH_IMAGINARY_PADE) {
     MGW_REQUIRE(prm->nomega_chi_imag >= 1 && prm->nomega_sigma_calc >= 1, MGW_ERR_ARG,
                 "imaginary-axis technique needs nomega_chi_imag and nomega_sigma_calc");
     MGW_REQUIRE(r->nhomo_G >= 1 && r->nhomo_G < nstate, MGW_ERR_ARG, "nhomo_G out of range");
-    // se_init(imaginary_axis_pade) (src/m_selfenergy_tools.f90:598-614)
-    const double efermi = 0.5 * (c->energy_in_g()[r->nhomo_G - 1] + c->energy_in_g()[r->nhomo_G]);
+    // se_init(imaginary_axis_pade) (src/m_selfenergy_tools.f90:586-614):
+    // midway between the highest HOMO and the lowest LUMO over the channels
+    double ehomo = -1.0e300, elumo = 1.0e300;
+    for (int s = 0; s < nspin; ++s) {
+      ehomo = std::max(ehomo, c->energy_in_g()[(size_t)s * nstate + r->nhomo_G - 1]);
+      elumo = std::min(elumo, c->energy_in_g()[(size_t)s * nstate + r->nhomo_G]);
+    }
+    const double efermi = 0.5 * (ehomo + elumo);
     for (int k = 0; k < nomega; ++k) omega[k] = efermi + prm->step_sigma * (k - nom);
     std::fill(se0.begin(), se0.end(), 0.0);
     std::vector<double> wc(2 * prm->nomega_sigma_calc);
@@ -1387,15 +1464,17 @@ int mgw_g0w0(mgw_ctx* c, const mgw_g0w0_params* prm, const mgw_ranges* r, const 
     imaginary_grid(prm->nomega_chi_imag, om, wt);
     MGW_TRY(polarizability_grid(c, r, prm->spin_fact, prm->nomega_chi_imag, om.data(), wt.data(), &en_rpa, &en_gw,
                                 prec));
-    MGW_TRY(gw_selfenergy_imag(c, r, prm->nomega_sigma_calc, wc.data(), prec));
-    MGW_TRY(pade_fit(c, nse, prm->nomega_sigma_calc, wc.data(), nullptr, nomega, omega.data(), prm->eta));
+    for (int s = 0; s < nspin; ++s) {
+      c->spin = s;
+      MGW_TRY(gw_selfenergy_imag(c, r, prm->nomega_sigma_calc, wc.data(), prec));
+      MGW_TRY(pade_fit(c, nse, prm->nomega_sigma_calc, wc.data(), nullptr, nomega, omega.data(), prm->eta));
+      MGW_TRY(per_spin(s));
+      MGW_TRY(sigma_copy(s));
+    }
   } else {
     set_error("unknown technique");
     return MGW_ERR_ARG;
   }
-  MGW_TRY(find_qp(c, r, nomega, omega.data(), nullptr, se0.data(), exc, c->energy.data(), nstate, energy_qp,
-                  zz, energy_qp_lin));
-  if (sigma_out) MGW_TRY(download(c, sigma_out, find_buf(c, "sigma"), (size_t)2 * nomega * nse));
   if (energies) {
     energies[0] = en_rpa;
     energies[1] = en_gw;

@@ -48,7 +48,7 @@ module m_mgw
   end type mgw_g0w0_params
 
   public :: mgw_init, mgw_finalize, mgw_check
-  public :: mgw_upload_eri_3center, mgw_upload_mo
+  public :: mgw_upload_eri_3center, mgw_upload_mo, mgw_upload_mo_spin, mgw_select_spin
   public :: mgw_inverse_sqrt_eri_2center, mgw_build_eri_3center
   public :: mgw_eri_3center_mo, mgw_destroy_mo_integrals
   public :: mgw_polarizability_rpa, mgw_polarizability_imag
@@ -111,6 +111,22 @@ module m_mgw
       real(c_double) :: eri(*)
       integer(c_int) :: ierr
     end function c_build_eri_3center
+
+    function c_set_mo_spin(handle, c_matrix, nbf, nstate, nspin, energy, occupation, where) &
+        bind(C, name='mgw_set_mo_spin') result(ierr)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      real(c_double), intent(in) :: c_matrix(*), energy(*), occupation(*)
+      integer(c_int), value :: nbf, nstate, nspin, where
+      integer(c_int) :: ierr
+    end function c_set_mo_spin
+
+    function c_set_spin(handle, ispin) bind(C, name='mgw_set_spin') result(ierr)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: ispin
+      integer(c_int) :: ierr
+    end function c_set_spin
 
     function c_set_mo(handle, c_matrix, nbf, nstate, energy, occupation, where) &
         bind(C, name='mgw_set_mo') result(ierr)
@@ -328,6 +344,22 @@ subroutine mgw_build_eri_3center(eri_3center_tmp, ipair_first, index_basis, nbf,
                                      int(SIZE(eri_3center, DIM=1), c_int64_t), ib, int(nbf, c_int), &
                                      MGW_MEM_HOST, eri_3center), 'mgw_build_eri_3center')
 end subroutine mgw_build_eri_3center
+
+!=========================================================================
+! All spin channels at once: c_matrix(:,:,nspin), energy(:,nspin), occupation(:,nspin)
+! as held by the reference; W sums the channels, the self-energy calls work
+! on the channel chosen by mgw_select_spin.
+subroutine mgw_upload_mo_spin(c_matrix, energy, occupation)
+  real(dp), intent(in) :: c_matrix(:, :, :), energy(:, :), occupation(:, :)
+  call mgw_check(c_set_mo_spin(ctx, c_matrix, int(SIZE(c_matrix, DIM=1), c_int), int(SIZE(c_matrix, DIM=2), c_int), &
+                               int(SIZE(c_matrix, DIM=3), c_int), energy, occupation, MGW_MEM_HOST), &
+                 'mgw_upload_mo_spin')
+end subroutine mgw_upload_mo_spin
+
+subroutine mgw_select_spin(ispin)
+  integer, intent(in) :: ispin
+  call mgw_check(c_set_spin(ctx, int(ispin, c_int)), 'mgw_select_spin')
+end subroutine mgw_select_spin
 
 subroutine mgw_upload_mo(c_matrix, energy, occupation)
   real(dp), intent(in) :: c_matrix(:, :), energy(:), occupation(:)

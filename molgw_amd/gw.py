@@ -84,14 +84,17 @@ def selfenergy_set_state_range(occupation, ncoreg=0, nvirtualg=100000, ncorew=0,
                                selfenergy_state_range=100000) -> StateRanges:
     """State ranges as set by selfenergy_set_state_range
     (src/m_selfenergy_tools.f90:67-122) and sf_init (src/m_spectral_function.f90:183-184),
-    for one spin channel. ``selfenergy_state_range`` counts from the HOMO."""
-    occ = np.asarray(occupation, dtype=np.float64).reshape(-1)
-    nstate = len(occ)
+    for occupation (nstate,) or (nstate, nspin). ``selfenergy_state_range``
+    counts from the HOMO."""
+    occ = np.asarray(occupation, dtype=np.float64)
+    occ = occ.reshape(len(occ), -1)                      # (nstate, nspin)
+    nstate = occ.shape[0]
     ncore_G = ncoreg
     nvirtual_G = min(nvirtualg, nstate + 1)
-    filled = np.nonzero(~(occ < COMPLETELY_EMPTY))[0]
+    # HOMO: occupied in every channel; LUMO: empty in every channel
+    filled = np.nonzero(~np.any(occ < COMPLETELY_EMPTY, axis=1))[0]
     nhomo_G = max(1, int(filled.max()) + 1) if len(filled) else 1
-    empty = np.nonzero(~(occ > COMPLETELY_EMPTY))[0]
+    empty = np.nonzero(~np.any(occ > COMPLETELY_EMPTY, axis=1))[0]
     nlumo_G = int(empty.min()) + 1 if len(empty) else 100000
     nsemin = max(ncore_G + 1, selfenergy_state_min, 1, nhomo_G - selfenergy_state_range)
     nsemax = min(nvirtual_G - 1, selfenergy_state_max, nstate, nhomo_G + selfenergy_state_range)
@@ -136,6 +139,7 @@ class Engine:
         self._h = h
         self.nbf = self.naux = self.nstate = 0
         self._naux_kept = self._naux_bas = 0
+        self.nspin = 1
         self.energy = self.occupation = None
 
     # ------------------------------------------------------------ lifetime
@@ -188,9 +192,10 @@ class Engine:
         if energy_g is None:
             check(self._lib.mgw_set_energy_g(self._h, None))
             return
-        e = _f64(energy_g).reshape(-1)
-        if len(e) != self.nstate:
-            raise ValueError('energy_g length must equal nstate')
+        e = np.asfortranarray(np.asarray(energy_g, dtype=np.float64)).reshape(-1, order='F')
+        if len(e) != self.nstate * self.nspin:
+            raise ValueError('energy_g must hold nstate values per spin channel')
+        e = np.ascontiguousarray(e)
         check(self._lib.mgw_set_energy_g(self._h, _ptr(e)))
 
     def evgw(self, nstep: int, approx: str = 'GnWn', rng: StateRanges | None = None, exchange_m_vxc=None,
@@ -329,17 +334,34 @@ class Engine:
             self.nbf, self.naux = int(nbf), self._naux_kept
 
     def set_mo(self, c_matrix, energy, occupation) -> None:
-        c = _f64(c_matrix)
-        e = _f64(energy).reshape(-1)
-        o = _f64(occupation).reshape(-1)
-        nbf, nstate = c.shape
-        if len(e) != nstate or len(o) != nstate:
-            raise ValueError('energy/occupation length must equal nstate')
-        check(self._lib.mgw_set_mo(self._h, c.ctypes.data, nbf, nstate, e.ctypes.data, o.ctypes.data,
-                                   _lib.MGW_MEM_HOST))
-        self.nstate = nstate
-        self.energy, self.occupation = e.copy(), o.copy()
-        self._c_matrix = c
+        """MO coefficients c_matrix (nbf, nstate) or (nbf, nstate, nspin), energies
+        and occupations (nstate,) or (nstate, nspin). With two channels W sums
+        the transitions of both and the self-energy / QP calls work on the
+        channel chosen by set_spin (g0w0 loops over both); pass spin_fact = 1."""
+        c = np.asarray(c_matrix, dtype=np.float64)
+        e = np.asarray(energy, dtype=np.float64)
+        o = np.asarray(occupation, dtype=np.float64)
+        nspin = c.shape[2] if c.ndim == 3 else 1
+        nbf, nstate = c.shape[:2]
+        e = e.reshape(nstate, -1) if e.size == nstate * nspin else None
+        o = o.reshape(nstate, -1) if o.size == nstate * nspin else None
+        if e is None or o is None or e.shape[1] != nspin or o.shape[1] != nspin:
+            raise ValueError('energy/occupation must be (nstate,) or (nstate, nspin) matching c_matrix')
+        cf = np.asfortranarray(c.reshape(nbf, nstate, nspin))
+        ef, of = np.asfortranarray(e), np.asfortranarray(o)
+        check(self._lib.mgw_set_mo_spin(self._h, cf.ctypes.data, nbf, nstate, nspin, ef.ctypes.data,
+                                        of.ctypes.data, _lib.MGW_MEM_HOST))
+        self.nstate, self.nspin = nstate, nspin
+        if nspin == 1:
+            self.energy, self.occupation = ef[:, 0].copy(), of[:, 0].copy()
+            self._c_matrix = np.asfortranarray(cf[:, :, 0])
+        else:
+            self.energy, self.occupation = ef.copy(), of.copy()
+            self._c_matrix = cf
+
+    def set_spin(self, ispin: int) -> None:
+        """Channel (1-based) of the AO->MO, self-energy and QP calls."""
+        check(self._lib.mgw_set_spin(self._h, int(ispin)))
 
     # ------------------------------------------------------------ A2
     def calculate_eri_3center_mo(self, mstate_min, mstate_max, nstate_min, nstate_max, copy=True):
@@ -452,12 +474,14 @@ class Engine:
         prm = G0W0Params(tech, nomega_sigma, step_sigma, eta, nomega_chi_imag, nomega_sigma_calc,
                          step_sigma_calc, spin_fact, PRECISION[precision], 0)
         r = rng.to_c()
-        n = self.nstate
+        n, ns = self.nstate, getattr(self, 'nspin', 1)
         nom = nomega_sigma // 2
-        eqp, zz, elin = np.zeros(n), np.zeros(n), np.zeros(n)
-        sig = np.zeros((2 * nom + 1, rng.nse), dtype=np.complex128, order='F') if return_sigma else None
+        shape = (n,) if ns == 1 else (n, ns)
+        eqp, zz, elin = (np.zeros(shape, order='F') for _ in range(3))
+        sshape = (2 * nom + 1, rng.nse) if ns == 1 else (2 * nom + 1, rng.nse, ns)
+        sig = np.zeros(sshape, dtype=np.complex128, order='F') if return_sigma else None
         en = np.zeros(2)
-        exc = None if exchange_m_vxc is None else _f64(exchange_m_vxc)
+        exc = None if exchange_m_vxc is None else np.asfortranarray(np.asarray(exchange_m_vxc, dtype=np.float64))
         check(self._lib.mgw_g0w0(self._h, C.byref(prm), C.byref(r), _ptr(exc) if exc is not None else None,
                                  _ptr(eqp), _ptr(zz), _ptr(elin),
                                  sig.ctypes.data_as(C.POINTER(C.c_double)) if return_sigma else None,
@@ -465,6 +489,7 @@ class Engine:
         if technique == 'one_shot':
             omega = step_sigma * np.arange(-nom, nom + 1)
         else:
-            efermi = 0.5 * (self.energy[rng.nhomo_G - 1] + self.energy[rng.nhomo_G])
+            e2 = self.energy.reshape(n, -1)
+            efermi = 0.5 * (e2[rng.nhomo_G - 1].max() + e2[rng.nhomo_G].min())
             omega = efermi + step_sigma * np.arange(-nom, nom + 1)
         return G0W0Output(eqp, zz, elin, sig, omega, float(en[0]), float(en[1]))

@@ -293,12 +293,17 @@ def polarizability_casida(x_ia: np.ndarray, energy: np.ndarray, ttable: np.ndarr
       * E_RPA += 1/2 sum |Omega| (m_linear_response.f90:418)
       * w_s = sqrt(spin_fact) B (X+Y) and
         E_GM = 1/2 (sum w_s^2 - spin_fact sum B^2) (m_linear_response.f90:837-849)
-    ``energy`` is (nstate,) for the single spin channel.
+    ``energy`` is (nstate,) for one channel or (nstate, nspin) with the
+    transitions of ``ttable`` spin outer (x_ia columns in the same order).
     Returns (pole, w_s, en_rpa, en_gm).
     """
     i = ttable[:, 0] - 1
     a = ttable[:, 1] - 1
-    amb_diag = energy[a] - energy[i]
+    if energy.ndim == 2:                 # (nstate, nspin): each transition in its channel
+        sp = ttable[:, 2] - 1
+        amb_diag = energy[a, sp] - energy[i, sp]
+    else:
+        amb_diag = energy[a] - energy[i]
     apb = 2.0 * spin_fact * (x_ia.T @ x_ia)
     apb[np.diag_indices_from(apb)] += amb_diag
     en_rpa = -0.25 * np.trace(apb) - 0.25 * np.sum(amb_diag)
@@ -329,12 +334,20 @@ def polarizability_grid(eri_ai: np.ndarray, energy: np.ndarray, occupation: np.n
       chi = (1 - chi0)^-1 chi0 via LU inverse and DGEMM  (:175-187)
     Returns (chi[iw, P, Q], erpa, egw).
     """
-    naux = eri_ai.shape[0]
     i = ttable[:, 0] - 1
     a = ttable[:, 1] - 1
     x = eri_ai[:, a - (vlo - 1), i - (olo - 1)]
     docc = occupation[i] - occupation[a]
     de = energy[a] - energy[i]
+    return polarizability_grid_x(x, docc, de, omega_im, weight)
+
+
+def polarizability_grid_x(x: np.ndarray, docc: np.ndarray, de: np.ndarray, omega_im: np.ndarray,
+                          weight: np.ndarray):
+    """The frequency loop of ``polarizability_grid_scalapack``
+    (src/m_gw_selfenergy_grid.f90:118-191) on X(P, t) = (P|a_t i_t) with the
+    occupation and energy differences of each transition."""
+    naux = x.shape[0]
     chi = np.zeros((len(omega_im), naux, naux))
     erpa = 0.0
     egw = 0.0
@@ -751,3 +764,87 @@ def evgw(eri_3center, index_basis, c_matrix, energy, occupation, nstep: int, app
         poles.append(pole)
         energy_g = new
     return history, poles
+
+
+# ---------------------------------------------------------------------------
+# Spin-polarised one-shot G0W0 (nspin = 2)
+# ---------------------------------------------------------------------------
+def transition_columns(eri_3center, index_basis, c_matrix, ttable, wlo: int, whi: int) -> np.ndarray:
+    """X(P, t) = eri_3center_mo(P, a_t, i_t, s_t) for a spin-outer transition
+    table, each column from its own channel's AO->MO transform
+    (src/m_gw_selfenergy_grid.f90:126-137, src/m_linear_response.f90:837-842).
+    ``c_matrix`` is (nbf, nstate, nspin)."""
+    cols = []
+    for s in range(c_matrix.shape[2]):
+        sel = ttable[:, 2] == s + 1
+        if not sel.any():
+            continue
+        eri_s = calculate_eri_3center_mo(eri_3center, index_basis, c_matrix, wlo, whi, wlo, whi, spin=s)
+        cols.append(eri_s[:, ttable[sel, 1] - wlo, ttable[sel, 0] - wlo])
+    return np.concatenate(cols, axis=1)
+
+
+def g0w0_spin(eri_3center, index_basis, c_matrix, energy, occupation, technique: str = 'one_shot',
+              nomega_chi_imag: int = 0, exchange_m_vxc=None, nomega_sigma: int = 51, step_sigma: float = 0.01,
+              nomega_sigma_calc: int = 1, step_sigma_calc: float = 0.03, eta: float = 0.001,
+              spin_fact: float = 1.0, ranges: StateRanges | None = None):
+    """One-shot G0W0 with spin channels: ``c_matrix`` (nbf, nstate, nspin),
+    ``energy``/``occupation``/``exchange_m_vxc`` (nstate, nspin).
+
+    W from the transitions of every channel (spin-outer table,
+    src/m_spectral_function.f90:239-266; chi0 at
+    src/m_gw_selfenergy_grid.f90:118-148 or the Casida matrix of
+    src/m_linear_response.f90:33-498), then per channel the self-energy with
+    that channel's integrals and energies (the ispin loops of
+    src/m_gw_selfenergy_analytic.f90:80-196 and
+    src/m_gw_selfenergy_grid.f90:279-320) and its QP equation. For
+    'imaginary_pade' the real grid is centred midway between the highest HOMO
+    and the lowest LUMO over the channels (src/m_selfenergy_tools.f90:586-590).
+    Returns a dict with energy_qp, zz, energy_qp_lin (nstate, nspin), sigma
+    (nomega, nse, nspin), omega, en_rpa and en_gw (summed over the channels
+    for the pole self-energy)."""
+    c_matrix = np.asarray(c_matrix, dtype=np.float64)
+    energy = np.asarray(energy, dtype=np.float64)
+    occupation = np.asarray(occupation, dtype=np.float64)
+    nstate, nspin = energy.shape
+    exc = np.zeros((nstate, nspin)) if exchange_m_vxc is None else np.asarray(exchange_m_vxc, dtype=np.float64)
+    rng = ranges or selfenergy_set_state_range(occupation)
+    ttable = transition_table(occupation, spin_fact, rng.ncore_W, rng.nvirtual_W)
+    wlo, whi = rng.ncore_W + 1, rng.nvirtual_W - 1
+    x = transition_columns(eri_3center, index_basis, c_matrix, ttable, wlo, whi)
+    i, a, sp = ttable[:, 0] - 1, ttable[:, 1] - 1, ttable[:, 2] - 1
+    nom = nomega_sigma // 2
+    nse = rng.nsemax - rng.nsemin + 1
+    out = {k: np.zeros((nstate, nspin)) for k in ('energy_qp', 'zz', 'energy_qp_lin')}
+    sigma = np.zeros((2 * nom + 1, nse, nspin), dtype=np.complex128)
+    en_gw = 0.0
+    if technique == 'one_shot':
+        omega = step_sigma * np.arange(-nom, nom + 1, dtype=np.float64)
+        pole, w_s, en_rpa, _ = polarizability_casida(x, energy, ttable, spin_fact)
+        for s in range(nspin):
+            se_energy0 = energy[rng.nsemin - 1:rng.nsemax, s]
+            eri_pi = calculate_eri_3center_mo(eri_3center, index_basis, c_matrix, rng.nsemin, rng.nsemax,
+                                              rng.ncore_G + 1, rng.nvirtual_G - 1, spin=s)
+            sig, eg = gw_selfenergy(eri_pi, w_s, pole, energy[:, s], occupation[:, s], rng, omega, se_energy0,
+                                    eta, spin_fact)
+            en_gw += eg
+            sigma[:, :, s] = sig
+    else:
+        efermi = 0.5 * (energy[rng.nhomo_G - 1].max() + energy[rng.nhomo_G].min())
+        omega = efermi + step_sigma * np.arange(-nom, nom + 1, dtype=np.float64)
+        omega_calc = efermi + step_sigma_calc * np.arange(nomega_sigma_calc) * 1j
+        omega_im, weight = imaginary_quad_grid(nomega_chi_imag)
+        chi, en_rpa, en_gw = polarizability_grid_x(x, occupation[i, sp] - occupation[a, sp],
+                                                   energy[a, sp] - energy[i, sp], omega_im, weight)
+        for s in range(nspin):
+            eri_pm = calculate_eri_3center_mo(eri_3center, index_basis, c_matrix, rng.ncore_G + 1,
+                                              rng.nvirtual_G - 1, rng.nsemin, rng.nsemax, spin=s)
+            sc = gw_selfenergy_imag(eri_pm, chi, omega_im, weight, omega_calc, energy[:, s], rng)
+            sigma[:, :, s] = se_pade_fit(omega_calc, sc, omega, eta)
+    for s in range(nspin):
+        se_energy0 = (energy[rng.nsemin - 1:rng.nsemax, s] if technique == 'one_shot' else np.zeros(nse))
+        e_lin, _ = find_qp_energy_linearization(sigma[:, :, s], omega, se_energy0, exc[:, s], energy[:, s], rng)
+        e_qp, zz = find_qp_energy_graphical(sigma[:, :, s], omega, se_energy0, exc[:, s], energy[:, s], rng)
+        out['energy_qp'][:, s], out['zz'][:, s], out['energy_qp_lin'][:, s] = e_qp, zz, e_lin
+    out.update(sigma=sigma, omega=omega, en_rpa=en_rpa, en_gw=en_gw)
+    return out

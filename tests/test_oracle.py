@@ -356,3 +356,71 @@ def test_b_build_recovers_b_up_to_rotation_and_the_qp_energies():
     ref = O.g0w0_casida(s.eri_3center, s.index_basis, s.c_matrix, s.energy, s.occupation, nomega_sigma=5)
     got = O.g0w0_casida(b, s.index_basis, s.c_matrix, s.energy, s.occupation, nomega_sigma=5)
     np.testing.assert_allclose(got.energy_qp, ref.energy_qp, rtol=0, atol=1e-10)
+
+
+def _as_two_channels(s):
+    c2 = np.stack([s.c_matrix, s.c_matrix], axis=2)
+    e2 = np.stack([s.energy, s.energy], axis=1)
+    o2 = np.stack([s.occupation, s.occupation], axis=1) / 2.0
+    return c2, e2, o2
+
+
+@pytest.mark.parametrize('technique', ['one_shot', 'imaginary_pade'])
+def test_spin_closed_shell_as_two_channels_equals_one_channel(technique):
+    """A closed shell written as two identical channels (occupations 1,
+    spin_fact 1) must give the pinned single-channel G0W0 (occupations 2,
+    spin_fact 2): the same W (chi0 sums two equal halves), the same Sigma and
+    QP energies in both channels, E_RPA equal and E_GW equal."""
+    from molgw_amd.workloads import synthetic_system
+    s = synthetic_system(22, 60, 4, seed=41, scale=0.1)
+    c2, e2, o2 = _as_two_channels(s)
+    if technique == 'one_shot':
+        ref = O.g0w0_casida(s.eri_3center, s.index_basis, s.c_matrix, s.energy, s.occupation, nomega_sigma=7)
+        got = O.g0w0_spin(s.eri_3center, s.index_basis, c2, e2, o2, nomega_sigma=7)
+    else:
+        ref = O.g0w0_pade(s.eri_3center, s.index_basis, s.c_matrix, s.energy, s.occupation, 12,
+                          nomega_sigma=7, nomega_sigma_calc=6)
+        got = O.g0w0_spin(s.eri_3center, s.index_basis, c2, e2, o2, technique='imaginary_pade',
+                          nomega_chi_imag=12, nomega_sigma=7, nomega_sigma_calc=6)
+    for sp in range(2):
+        np.testing.assert_allclose(got['energy_qp'][:, sp], ref.energy_qp, rtol=0, atol=1e-11)
+        np.testing.assert_allclose(got['sigma'][:, :, sp], ref.sigma, rtol=0, atol=1e-11)
+    assert abs(got['en_rpa'] - ref.en_rpa) < 1e-11
+    assert abs(got['en_gw'] - ref.en_gw) < 1e-11
+
+
+def test_spin_empty_channel_reduces_to_one_channel():
+    """With the second channel empty, W comes from the first channel alone:
+    its QP energies equal a single-channel G0W0 of that channel with
+    spin_fact 1 (occupations 1 count as full)."""
+    from molgw_amd.workloads import synthetic_system
+    s = synthetic_system(20, 56, 4, seed=43, scale=0.1)
+    occ_up = s.occupation / 2.0
+    c2 = np.stack([s.c_matrix, s.c_matrix], axis=2)
+    # same energies in both channels, so the real grid's centre (midway
+    # between the highest HOMO and lowest LUMO over the channels) is the same
+    e2 = np.stack([s.energy, s.energy], axis=1)
+    o2 = np.stack([occ_up, np.zeros_like(occ_up)], axis=1)
+    rng = O.selfenergy_set_state_range(o2)
+    got = O.g0w0_spin(s.eri_3center, s.index_basis, c2, e2, o2, technique='imaginary_pade', nomega_chi_imag=10,
+                      nomega_sigma=7, nomega_sigma_calc=6, ranges=rng)
+    ref = O.g0w0_pade(s.eri_3center, s.index_basis, s.c_matrix, s.energy, occ_up, 10, nomega_sigma=7,
+                      nomega_sigma_calc=6, spin_fact=1.0, ranges=rng)
+    np.testing.assert_allclose(got['energy_qp'][:, 0], ref.energy_qp, rtol=0, atol=1e-11)
+    assert abs(got['en_rpa'] - ref.en_rpa) < 1e-11
+
+
+def test_spin_channel_swap_symmetry():
+    """Exchanging the two channels exchanges the per-channel results and
+    leaves W (E_RPA) unchanged."""
+    from molgw_amd.workloads import synthetic_system
+    s = synthetic_system(18, 50, 4, seed=47, scale=0.1)
+    rot = np.linalg.qr(np.random.default_rng(2).standard_normal((18, 18)) * 0.05 + np.eye(18))[0]
+    c2 = np.stack([s.c_matrix, s.c_matrix @ rot], axis=2)
+    e2 = np.stack([s.energy, s.energy + np.linspace(0.0, 0.02, 18)], axis=1)
+    o2 = np.stack([s.occupation / 2.0, s.occupation / 2.0], axis=1)
+    o2[3, 1] = 0.0                                     # one electron fewer in channel 2
+    a = O.g0w0_spin(s.eri_3center, s.index_basis, c2, e2, o2, nomega_sigma=5)
+    b = O.g0w0_spin(s.eri_3center, s.index_basis, c2[:, :, ::-1], e2[:, ::-1], o2[:, ::-1], nomega_sigma=5)
+    np.testing.assert_allclose(a['energy_qp'], b['energy_qp'][:, ::-1], rtol=0, atol=1e-11)
+    assert abs(a['en_rpa'] - b['en_rpa']) < 1e-11
