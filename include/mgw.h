@@ -215,6 +215,19 @@ int mgw_npole(mgw_ctx* ctx, const mgw_ranges* ranges, int* npole);
  * non-NULL), en_rpa and en_gw (Galitskii-Migdal). */
 int mgw_polarizability(mgw_ctx* ctx, const mgw_ranges* ranges, double spin_fact, int precision,
                        double* en_rpa, double* en_gw, double* pole_out, double* w_s_out);
+/* read_spectral_function (src/m_spectral_function.f90:508-620): W made
+ * resident from its poles pole[npole] and residues w_s(nauxil, npole)
+ * column-major (the arrays of a SCREENED_COULOMB file), in place of
+ * mgw_polarizability; nauxil must match the resident RI tensor. */
+int mgw_set_spectral_function(mgw_ctx* ctx, int npole, int nauxil, const double* pole, const double* w_s,
+                              int where);
+/* dump_gw_ingredients (src/m_gw_selfenergy_analytic.f90:801-881) for the
+ * active spin channel, states p, q in ncore_G+1 .. nvirtual_G-1 (ng of
+ * them): v_out(nauxil, ng, ng) = eri_3center_mo(P, p, q) (the v.bin
+ * records) and w_out(npole, ng, ng) = sum_P w_s(P, s) v(P, p, q) (the w.bin
+ * records). Both nullable; w_out needs the resident spectral function. */
+int mgw_gw_ingredients(mgw_ctx* ctx, const mgw_ranges* ranges, double* w_out, double* v_out);
+
 
 /* ----------------------------------------- A5: chi0 on the imaginary axis */
 /* polarizability_grid_scalapack (src/m_gw_selfenergy_grid.f90:33-209):

@@ -1027,6 +1027,21 @@ int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int techn
   return 8 * (d + stage_w + stage_s);
 }
 
+// Does the 0-based pair list (k, l) hold every unordered pair of nbf
+// functions exactly once?
+bool all_pairs(const std::vector<int32_t>& kl, int nbf) {
+  const int64_t npair = (int64_t)kl.size() / 2;
+  if (npair != (int64_t)nbf * (nbf + 1) / 2) return false;
+  std::vector<char> seen((size_t)nbf * nbf, 0);
+  for (int64_t p = 0; p < npair; ++p) {
+    const int a = std::min(kl[2 * p], kl[2 * p + 1]), b = std::max(kl[2 * p], kl[2 * p + 1]);
+    char& f = seen[(size_t)a * nbf + b];
+    if (f) return false;
+    f = 1;
+  }
+  return true;
+}
+
 int mgw_set_eri_3center(mgw_ctx* c, const double* eri, int64_t npair, int nauxil,
                         const int32_t* index_basis, int nbf, int where) {
   MGW_CTX(c);
@@ -1042,6 +1057,9 @@ int mgw_set_eri_3center(mgw_ctx* c, const double* eri, int64_t npair, int nauxil
     kl[2 * p] = k;
     kl[2 * p + 1] = l;
   }
+  // a list of every unordered pair writes every element of the dense tensor:
+  // the zero fill is needed only when pairs are screened out
+  const bool complete = all_pairs(kl, nbf);
   const double* packed = eri;
   if (where == MGW_MEM_HOST) {
     double* d = nullptr;
@@ -1052,7 +1070,7 @@ int mgw_set_eri_3center(mgw_ctx* c, const double* eri, int64_t npair, int nauxil
   MGW_TRY(upload(c, "ri_kl", kl.data(), kl.size(), &kl_d));
   double* dense = nullptr;
   MGW_TRY(dbuf(c, "ri_dense", (size_t)nauxil * nbf * nbf, &dense));
-  MGW_TRY(launch_unpack_ri(packed, npair, nauxil, kl_d, nbf, dense, c->stream));
+  MGW_TRY(launch_unpack_ri(packed, npair, nauxil, kl_d, nbf, dense, c->stream, !complete));
   MGW_HIP(hipStreamSynchronize(c->stream));
   MGW_TRY(release(c, "ri_packed"));
   MGW_TRY(release(c, "ri_kl"));
@@ -1309,6 +1327,57 @@ int mgw_polarizability(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int pr
   if (pole_out) MGW_TRY(download(c, pole_out, find_buf(c, "wpol_pole"), c->npole));
   if (w_s_out) MGW_TRY(download(c, w_s_out, find_buf(c, "wpol_ws"), (size_t)c->naux * c->npole));
   MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
+int mgw_set_spectral_function(mgw_ctx* c, int npole, int nauxil, const double* pole, const double* w_s,
+                              int where) {
+  MGW_CTX(c);
+  MGW_REQUIRE(pole && w_s && npole > 0 && nauxil > 0, MGW_ERR_ARG, "mgw_set_spectral_function: invalid arguments");
+  MGW_REQUIRE(where == MGW_MEM_HOST || where == MGW_MEM_DEVICE, MGW_ERR_ARG, "invalid memory kind");
+  MGW_REQUIRE(c->has_ri && nauxil == c->naux, MGW_ERR_ARG,
+              "mgw_set_spectral_function: nauxil differs from the resident RI tensor");
+  double *pd = nullptr, *wd = nullptr;
+  MGW_TRY(dbuf(c, "wpol_pole", npole, &pd));
+  MGW_TRY(dbuf(c, "wpol_ws", (size_t)nauxil * npole, &wd));
+  const hipMemcpyKind k = where == MGW_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  MGW_HIP(hipMemcpyAsync(pd, pole, sizeof(double) * npole, k, c->stream));
+  MGW_HIP(hipMemcpyAsync(wd, w_s, sizeof(double) * nauxil * (size_t)npole, k, c->stream));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  c->npole = npole;
+  c->has_wpol = true;
+  c->has_sigma = false;
+  return MGW_OK;
+}
+
+int mgw_gw_ingredients(mgw_ctx* c, const mgw_ranges* r, double* w_out, double* v_out) {
+  MGW_CTX(c);
+  MGW_TRY(check_ranges(c, r));
+  MGW_REQUIRE(!w_out || c->has_wpol, MGW_ERR_STATE, "no spectral function: call mgw_polarizability first");
+  const int glo = r->ncore_G + 1, ghi = r->nvirtual_G - 1;
+  const int ng = ghi - glo + 1, naux = c->naux;
+  MGW_REQUIRE(ng > 0, MGW_ERR_ARG, "mgw_gw_ingredients: empty state range");
+  double* v = nullptr;
+  MGW_TRY(dbuf(c, "ingr_v", (size_t)naux * ng * ng, &v));
+  MGW_TRY(ao2mo(c, glo, ghi, glo, ghi, v));
+  if (w_out) {
+    // w(s, p, q) = sum_P w_s(P, s) (P|p q)  (src/m_gw_selfenergy_analytic.f90:851-857)
+    const int npole = c->npole;
+    double* w = nullptr;
+    MGW_TRY(dbuf(c, "ingr_w", (size_t)npole * ng * ng, &w));
+    GemmF64 g;
+    g.M = npole; g.N = ng * ng; g.K = naux;
+    g.transA = true; g.A = find_buf(c, "wpol_ws"); g.lda = naux;
+    g.B = v; g.ldb = naux;
+    g.C = w; g.ldc = npole;
+    MGW_REQUIRE((int64_t)ng * ng < (1ll << 31), MGW_ERR_ARG, "mgw_gw_ingredients: state range too large");
+    MGW_TRY(gemm_f64(g, c->stream));
+    MGW_TRY(download(c, w_out, w, (size_t)npole * ng * ng));
+  }
+  if (v_out) MGW_TRY(download(c, v_out, v, (size_t)naux * ng * ng));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  MGW_TRY(release(c, "ingr_w"));
+  MGW_TRY(release(c, "ingr_v"));
   return MGW_OK;
 }
 

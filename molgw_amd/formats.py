@@ -5,6 +5,12 @@
 * molgw.yaml sections that ``molgw.py`` parses for GW results:
   ``<approx> energies`` (src/m_hamiltonian_tools.f90:532-560) and
   ``<approx> selfenergy`` (src/m_selfenergy_tools.f90:501-553).
+* SCREENED_COULOMB: the poles and residues of W (write_spectral_function /
+  read_spectral_function, src/m_spectral_function.f90:414-620), as a
+  sequential unformatted Fortran file (4-byte record markers, gfortran's
+  default).
+* dump_gw_ingredients (src/m_gw_selfenergy_analytic.f90:801-881):
+  energies.dat, omegas.dat, w.bin, v.dat, v.bin.
 
 Energies are in Hartree in the arrays and in eV in the yaml, as in the
 reference (Ha_eV of src/m_definitions.f90).
@@ -129,3 +135,94 @@ def append_gw_yaml(path: str, energy0, exchange_m_vxc, energy_qp, zz, nsemin: in
     with open(path, 'a') as fh:
         fh.write(yaml_qp_selfenergy(approx, energy0, exchange_m_vxc, energy_qp, zz, nsemin, nsemax))
         fh.write(yaml_energies(f'{approx} energies', energy_qp, nsemin, nsemax))
+
+
+# ---------------------------------------------------------------- SCREENED_COULOMB
+def _record(fh, payload: bytes) -> None:
+    n = np.array([len(payload)], dtype='<i4').tobytes()
+    fh.write(n + payload + n)
+
+
+def _records(path: str):
+    with open(path, 'rb') as fh:
+        data = fh.read()
+    pos = 0
+    while pos < len(data):
+        n = int(np.frombuffer(data, dtype='<i4', count=1, offset=pos)[0])
+        payload = data[pos + 4:pos + 4 + n]
+        tail = int(np.frombuffer(data, dtype='<i4', count=1, offset=pos + 4 + n)[0])
+        if tail != n:
+            raise ValueError(f'{path}: record markers differ ({n} vs {tail})')
+        yield payload
+        pos += n + 8
+
+
+def write_screened_coulomb(path: str, pole, w_s, postscf_name: str = 'GW') -> None:
+    """write_spectral_function (src/m_spectral_function.f90:414-445, the
+    serial branch): postscf_name (character(len=100)), nprodbasis_total,
+    npole_reso, pole(:), then w_s(:, s) for every pole, one record each."""
+    pole = np.asarray(pole, dtype='<f8').reshape(-1)
+    w_s = np.asarray(w_s, dtype='<f8')
+    if w_s.shape[1] != len(pole):
+        raise ValueError('w_s must be (nprodbasis, npole)')
+    with open(path, 'wb') as fh:
+        _record(fh, postscf_name.ljust(100)[:100].encode('ascii'))
+        _record(fh, np.array([w_s.shape[0]], dtype='<i4').tobytes())
+        _record(fh, np.array([len(pole)], dtype='<i4').tobytes())
+        _record(fh, pole.tobytes())
+        for s in range(len(pole)):
+            _record(fh, np.ascontiguousarray(w_s[:, s]).tobytes())
+
+
+def read_screened_coulomb(path: str):
+    """read_spectral_function (src/m_spectral_function.f90:508-560, the
+    serial branch). Returns (postscf_name, pole[npole], w_s[nprodbasis, npole])."""
+    recs = _records(path)
+    name = next(recs).decode('ascii').rstrip()
+    nprod = int(np.frombuffer(next(recs), dtype='<i4')[0])
+    npole = int(np.frombuffer(next(recs), dtype='<i4')[0])
+    pole = np.frombuffer(next(recs), dtype='<f8').copy()
+    if len(pole) != npole:
+        raise ValueError(f'{path}: {len(pole)} poles in a file that announces {npole}')
+    w_s = np.zeros((nprod, npole), order='F')
+    for s in range(npole):
+        col = np.frombuffer(next(recs), dtype='<f8')
+        if len(col) != nprod:
+            raise ValueError(f'{path}: residue {s + 1} has {len(col)} values, expected {nprod}')
+        w_s[:, s] = col
+    return name, pole, w_s
+
+
+# ---------------------------------------------------------------- GW ingredients
+def _ld(x) -> str:
+    """One list-directed value per line (the layout of list-directed output
+    is the compiler's; these values read back exactly)."""
+    if isinstance(x, (int, np.integer)):
+        return f'{int(x):12d}\n'
+    return f' {float(x): .17E}\n'
+
+
+def dump_gw_ingredients(directory: str, energy_g, pole, w_spin, v_spin) -> None:
+    """dump_gw_ingredients (src/m_gw_selfenergy_analytic.f90:801-881):
+    energies.dat (ng, then energy(p, 1) for the G states), omegas.dat (npole,
+    then the poles), w.bin and v.bin (stream files: for every spin channel and
+    every q, w(:, :, q) = (npole, ng) and v(:, :, q) = (nauxil, ng) in
+    column-major order) and v.dat (nauxil, ng, ng). ``w_spin``/``v_spin`` are
+    lists over the spin channels of the arrays of Engine.gw_ingredients;
+    ``energy_g`` the spin-1 energies of the G states."""
+    os.makedirs(directory, exist_ok=True)
+    e = np.asarray(energy_g, dtype=np.float64).reshape(-1)
+    pole = np.asarray(pole, dtype=np.float64).reshape(-1)
+    naux, ng = v_spin[0].shape[0], v_spin[0].shape[1]
+    with open(os.path.join(directory, 'energies.dat'), 'w') as fh:
+        fh.write(_ld(len(e)) + ''.join(_ld(x) for x in e))
+    with open(os.path.join(directory, 'omegas.dat'), 'w') as fh:
+        fh.write(_ld(len(pole)) + ''.join(_ld(x) for x in pole))
+    with open(os.path.join(directory, 'w.bin'), 'wb') as fh:
+        for w in w_spin:
+            fh.write(np.asarray(w, dtype='<f8').tobytes(order='F'))
+    with open(os.path.join(directory, 'v.dat'), 'w') as fh:
+        fh.write(_ld(naux) + _ld(ng) + _ld(ng))
+    with open(os.path.join(directory, 'v.bin'), 'wb') as fh:
+        for v in v_spin:
+            fh.write(np.asarray(v, dtype='<f8').tobytes(order='F'))

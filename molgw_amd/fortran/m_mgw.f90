@@ -48,7 +48,7 @@ module m_mgw
   end type mgw_g0w0_params
 
   public :: mgw_init, mgw_finalize, mgw_check
-  public :: mgw_upload_eri_3center, mgw_upload_mo, mgw_upload_mo_spin, mgw_select_spin
+  public :: mgw_upload_eri_3center, mgw_upload_mo, mgw_upload_mo_spin, mgw_select_spin, mgw_set_w
   public :: mgw_inverse_sqrt_eri_2center, mgw_build_eri_3center
   public :: mgw_eri_3center_mo, mgw_destroy_mo_integrals
   public :: mgw_polarizability_rpa, mgw_polarizability_imag
@@ -127,6 +127,15 @@ module m_mgw
       integer(c_int), value :: ispin
       integer(c_int) :: ierr
     end function c_set_spin
+
+    function c_set_spectral_function(handle, npole, nauxil, pole, w_s, where) &
+        bind(C, name='mgw_set_spectral_function') result(ierr)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      integer(c_int), value :: npole, nauxil, where
+      real(c_double), intent(in) :: pole(*), w_s(*)
+      integer(c_int) :: ierr
+    end function c_set_spectral_function
 
     function c_set_mo(handle, c_matrix, nbf, nstate, energy, occupation, where) &
         bind(C, name='mgw_set_mo') result(ierr)
@@ -360,6 +369,15 @@ subroutine mgw_select_spin(ispin)
   integer, intent(in) :: ispin
   call mgw_check(c_set_spin(ctx, int(ispin, c_int)), 'mgw_select_spin')
 end subroutine mgw_select_spin
+
+!=========================================================================
+! After read_spectral_function (src/m_spectral_function.f90:508-620): W from
+! the poles and residues read from SCREENED_COULOMB.
+subroutine mgw_set_w(pole, w_s)
+  real(dp), intent(in) :: pole(:), w_s(:, :)
+  call mgw_check(c_set_spectral_function(ctx, int(SIZE(pole), c_int), int(SIZE(w_s, DIM=1), c_int), pole, w_s, &
+                                         MGW_MEM_HOST), 'mgw_set_w')
+end subroutine mgw_set_w
 
 subroutine mgw_upload_mo(c_matrix, energy, occupation)
   real(dp), intent(in) :: c_matrix(:, :), energy(:), occupation(:)

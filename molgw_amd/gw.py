@@ -140,6 +140,7 @@ class Engine:
         self.nbf = self.naux = self.nstate = 0
         self._naux_kept = self._naux_bas = 0
         self.nspin = 1
+        self._npole_w = 0
         self.energy = self.occupation = None
 
     # ------------------------------------------------------------ lifetime
@@ -391,7 +392,30 @@ class Engine:
         erpa, egw = C.c_double(), C.c_double()
         check(self._lib.mgw_polarizability(self._h, C.byref(r), spin_fact, PRECISION[precision], C.byref(erpa),
                                            C.byref(egw), _ptr(pole), _ptr(ws) if return_ws else None))
+        self._npole_w = npole
         return erpa.value, egw.value, pole, ws
+
+    def set_spectral_function(self, pole, w_s) -> None:
+        """read_spectral_function: make W resident from its poles and residues
+        w_s(nauxil, npole) (e.g. from formats.read_screened_coulomb)."""
+        p = _f64(pole).reshape(-1)
+        w = _f64(w_s)
+        if w.shape != (self.naux, len(p)):
+            raise ValueError(f'w_s must be ({self.naux}, {len(p)}), got {w.shape}')
+        check(self._lib.mgw_set_spectral_function(self._h, len(p), self.naux, p.ctypes.data, w.ctypes.data,
+                                                  _lib.MGW_MEM_HOST))
+        self._npole_w = len(p)
+
+    def gw_ingredients(self, rng: StateRanges, with_w: bool = True):
+        """dump_gw_ingredients arrays of the active spin channel: (w(npole, ng, ng)
+        or None, v(nauxil, ng, ng)) for states ncore_G+1 .. nvirtual_G-1."""
+        ng = rng.nvirtual_G - 1 - rng.ncore_G
+        v = np.zeros((self.naux, ng, ng), order='F')
+        npole = self._npole_w or self.npole(rng)
+        w = np.zeros((npole, ng, ng), order='F') if with_w else None
+        r = rng.to_c()
+        check(self._lib.mgw_gw_ingredients(self._h, C.byref(r), _ptr(w) if with_w else None, _ptr(v)))
+        return w, v
 
     # ------------------------------------------------------------ A5
     def polarizability_grid(self, rng: StateRanges, omega_im, weight, spin_fact=2.0, return_chi=False,

@@ -61,3 +61,43 @@ def test_yaml_sections_parse_as_molgw_py_reads_them():
     # the line layout of the writer: 8 blanks, index left-adjusted in 6, ':', blank, ES18.8
     line = [ln for ln in text.splitlines() if ln.startswith('        3     :')][0]
     assert line == '        3     : ' + formats._es18_8(eqp[2] * HA_EV)
+
+
+def test_screened_coulomb_round_trip_and_record_layout(tmp_path):
+    from molgw_amd.formats import write_screened_coulomb, read_screened_coulomb
+    rng = np.random.default_rng(3)
+    pole = np.sort(rng.uniform(0.3, 3.0, 7))
+    w_s = np.asfortranarray(rng.standard_normal((11, 7)))
+    path = str(tmp_path / 'SCREENED_COULOMB')
+    write_screened_coulomb(path, pole, w_s, 'GW')
+    name, p2, w2 = read_screened_coulomb(path)
+    assert name == 'GW'
+    np.testing.assert_array_equal(p2, pole)
+    np.testing.assert_array_equal(w2, w_s)
+    # sequential unformatted layout: 4 + 7 records, each framed by its length
+    raw = open(path, 'rb').read()
+    assert len(raw) == (100 + 8) + 2 * (4 + 8) + (8 * 7 + 8) + 7 * (8 * 11 + 8)
+    assert np.frombuffer(raw[:4], '<i4')[0] == 100 and raw[4:6] == b'GW'
+    assert np.frombuffer(raw[108:112], '<i4')[0] == 4 and np.frombuffer(raw[112:116], '<i4')[0] == 11
+
+
+def test_dump_gw_ingredients_layout(tmp_path):
+    from molgw_amd.formats import dump_gw_ingredients
+    rng = np.random.default_rng(4)
+    ng, naux, npole = 5, 6, 4
+    w = [np.asfortranarray(rng.standard_normal((npole, ng, ng))) for _ in range(2)]
+    v = [np.asfortranarray(rng.standard_normal((naux, ng, ng))) for _ in range(2)]
+    e = np.linspace(-1, 1, ng)
+    pole = np.linspace(0.5, 2.0, npole)
+    d = str(tmp_path / 'ingr')
+    dump_gw_ingredients(d, e, pole, w, v)
+    vals = np.loadtxt(tmp_path / 'ingr' / 'energies.dat')
+    assert vals[0] == ng and np.array_equal(vals[1:], e)
+    vals = np.loadtxt(tmp_path / 'ingr' / 'omegas.dat')
+    assert vals[0] == npole and np.array_equal(vals[1:], pole)
+    assert list(np.loadtxt(tmp_path / 'ingr' / 'v.dat')) == [naux, ng, ng]
+    wb = np.fromfile(tmp_path / 'ingr' / 'w.bin', dtype='<f8')
+    # per spin, per q: wcoeff(npole, ng) column-major
+    np.testing.assert_array_equal(wb.reshape(2, ng, ng, npole)[1, 3, :, :].T, w[1][:, :, 3])
+    vb = np.fromfile(tmp_path / 'ingr' / 'v.bin', dtype='<f8')
+    np.testing.assert_array_equal(vb.reshape(2, ng, ng, naux)[0, 2, :, :].T, v[0][:, :, 2])
