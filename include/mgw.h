@@ -246,6 +246,16 @@ int mgw_polarizability_grid(mgw_ctx* ctx, const mgw_ranges* ranges, double spin_
 int mgw_gw_selfenergy(mgw_ctx* ctx, const mgw_ranges* ranges, double spin_fact, double eta,
                       int nomega, const double* omega, const double* se_energy0,
                       double* sigma_out, double* energy_gw);
+/* gw_selfenergy_qs, GW branch (src/m_gw_selfenergy_analytic.f90:653-797),
+ * with apply_qs_approximation (src/m_selfenergy_tools.f90:809-828), for the
+ * active spin channel on the resident Casida W: Sigma(p, q) = sum_{i,s}
+ * w(s,p,i) w(s,q,i) Re[f_i / (e_q - i eta - e_i + W_s) + (1 - f_i) /
+ * (e_q + i eta - e_i - W_s)] over p, q in nsemin..nsemax, symmetrised
+ * (Sigma + Sigma^T) / 2. selfenergy_mo (nstate x nstate, zero outside the
+ * window) and selfenergy_ao = S C Sigma C^T S (nbf x nbf, needs s_matrix,
+ * nbf x nbf) are nullable. */
+int mgw_gw_selfenergy_qs(mgw_ctx* ctx, const mgw_ranges* ranges, double spin_fact, double eta,
+                         const double* s_matrix, double* selfenergy_mo, double* selfenergy_ao);
 
 /* ------------------------------------------ A7: imaginary-axis self-energy */
 /* gw_selfenergy_imag_scalapack (src/m_gw_selfenergy_grid.f90:213-338) using

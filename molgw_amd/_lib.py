@@ -60,6 +60,7 @@ SIGNATURES = {
     'mgw_destroy_eri_3center_mo': (C.c_int, [_vp]),
     'mgw_npole': (C.c_int, [_vp, C.POINTER(Ranges), C.POINTER(C.c_int)]),
     'mgw_set_spectral_function': (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int]),
+    'mgw_gw_selfenergy_qs': (C.c_int, [_vp, C.POINTER(Ranges), C.c_double, C.c_double, _dp, _dp, _dp]),
     'mgw_gw_ingredients': (C.c_int, [_vp, C.POINTER(Ranges), _dp, _dp]),
     'mgw_polarizability': (C.c_int, [_vp, C.POINTER(Ranges), C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
     'mgw_polarizability_grid': (C.c_int, [_vp, C.POINTER(Ranges), C.c_double, C.c_int, C.c_int, _dp, _dp,

@@ -1418,6 +1418,102 @@ int mgw_polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, i
   return MGW_OK;
 }
 
+int mgw_gw_selfenergy_qs(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta, const double* s_matrix,
+                         double* selfenergy_mo, double* selfenergy_ao) {
+  MGW_CTX(c);
+  MGW_TRY(check_ranges(c, r));
+  MGW_REQUIRE(c->has_wpol, MGW_ERR_STATE, "no spectral function: call mgw_polarizability first");
+  MGW_REQUIRE(!selfenergy_ao || s_matrix, MGW_ERR_ARG, "mgw_gw_selfenergy_qs: the AO matrix needs s_matrix");
+  const int naux = c->naux, npole = c->npole, nbf = c->nbf, ns = c->nstate;
+  const int nse = r->nsemax - r->nsemin + 1;
+  const int glo = r->ncore_G + 1, ghi = r->nvirtual_G - 1;
+  const int ng = ghi - glo + 1;
+  double* epi = nullptr;
+  MGW_TRY(dbuf(c, "eri_pi", (size_t)naux * nse * ng, &epi));
+  MGW_TRY(ao2mo(c, r->nsemin, r->nsemax, glo, ghi, epi));
+  const double* eg = c->energy_in_g().data() + c->so();
+  const double* oc = c->occupation.data() + c->so();
+  std::vector<double> ei(ng), fi(ng), eq(nse);
+  for (int k = 0; k < ng; ++k) { ei[k] = eg[glo - 1 + k]; fi[k] = oc[glo - 1 + k]; }
+  for (int k = 0; k < nse; ++k) eq[k] = eg[r->nsemin - 1 + k];
+  double *ei_d, *fi_d, *eq_d, *sig;
+  MGW_TRY(upload(c, "qs_ei", ei.data(), ng, &ei_d));
+  MGW_TRY(upload(c, "qs_fi", fi.data(), ng, &fi_d));
+  MGW_TRY(upload(c, "qs_eq", eq.data(), nse, &eq_d));
+  MGW_TRY(dbuf(c, "qs_sigma", (size_t)nse * nse, &sig));
+  // chunk over i so that w(s, p, i) and its scaled copy stay within ~1 GiB each
+  const size_t per_i = (size_t)npole * nse * sizeof(double);
+  const int ic = (int)std::max<size_t>(1, std::min<size_t>((size_t)ng, (size_t(1) << 30) / per_i));
+  double *w, *b;
+  MGW_TRY(dbuf(c, "sigma_w", (size_t)npole * nse * ic, &w));
+  MGW_TRY(dbuf(c, "qs_b", (size_t)npole * nse * ic, &b));
+  const double* ws = find_buf(c, "wpol_ws");
+  const double* pole = find_buf(c, "wpol_pole");
+  for (int i0 = 0; i0 < ng; i0 += ic) {
+    const int cnt = std::min(ic, ng - i0);
+    // w(s, p, i) = sum_P w_s(P, s) (P | p i)   (src/m_gw_selfenergy_analytic.f90:711-715)
+    GemmF64 g;
+    g.M = npole; g.N = nse * cnt; g.K = naux;
+    g.transA = true; g.A = ws; g.lda = naux;
+    g.B = epi + (int64_t)naux * nse * i0; g.ldb = naux;
+    g.C = w; g.ldc = npole;
+    g.klass = MGW_KCLASS_GEMM_SIGMA;
+    MGW_TRY(gemm_f64(g, c->stream));
+    MGW_TRY(launch_qs_scale(w, b, npole, nse, cnt, pole, eq_d, ei_d + i0, fi_d + i0, spin_fact, eta, c->stream));
+    // Sigma(p, q) += sum_s w(s, p, i) b(s, q, i)   (:736-744)
+    for (int k = 0; k < cnt; ++k) {
+      GemmF64 h;
+      h.M = nse; h.N = nse; h.K = npole;
+      h.transA = true; h.A = w + (int64_t)npole * nse * k; h.lda = npole;
+      h.B = b + (int64_t)npole * nse * k; h.ldb = npole;
+      h.C = sig; h.ldc = nse;
+      h.beta = (i0 + k == 0) ? 0.0 : 1.0;
+      h.klass = MGW_KCLASS_GEMM_SIGMA;
+      MGW_TRY(gemm_f64(h, c->stream));
+    }
+  }
+  MGW_TRY(launch_symmetrize(sig, nse, c->stream));
+  if (selfenergy_mo) {
+    std::vector<double> h((size_t)nse * nse);
+    MGW_TRY(download(c, h.data(), sig, h.size()));
+    std::fill(selfenergy_mo, selfenergy_mo + (size_t)ns * ns, 0.0);
+    for (int q = 0; q < nse; ++q)
+      for (int p = 0; p < nse; ++p)
+        selfenergy_mo[(r->nsemin - 1 + p) + (size_t)ns * (r->nsemin - 1 + q)] = h[p + (size_t)nse * q];
+  }
+  if (selfenergy_ao) {
+    // h_mo_to_ao (src/m_selfenergy_tools.f90:826): S C Sigma C^T S over the window's MOs
+    double *sd, *t1, *t2;
+    MGW_TRY(upload(c, "qs_s", s_matrix, (size_t)nbf * nbf, &sd));
+    MGW_TRY(dbuf(c, "qs_t1", (size_t)nbf * nse, &t1));
+    MGW_TRY(dbuf(c, "qs_t2", (size_t)2 * nbf * nbf, &t2));
+    const double* cw = find_buf(c, "c_matrix") + (int64_t)c->spin * nbf * ns + (int64_t)(r->nsemin - 1) * nbf;
+    GemmF64 g1;   // T1 = C_w Sigma
+    g1.M = nbf; g1.N = nse; g1.K = nse;
+    g1.A = cw; g1.lda = nbf; g1.B = sig; g1.ldb = nse; g1.C = t1; g1.ldc = nbf;
+    MGW_TRY(gemm_f64(g1, c->stream));
+    GemmF64 g2;   // T2 = T1 C_w^T
+    g2.M = nbf; g2.N = nbf; g2.K = nse;
+    g2.A = t1; g2.lda = nbf; g2.B = cw; g2.ldb = nbf; g2.transB = true; g2.C = t2; g2.ldc = nbf;
+    MGW_TRY(gemm_f64(g2, c->stream));
+    double* t3 = t2 + (size_t)nbf * nbf;
+    GemmF64 g3;   // T3 = S T2
+    g3.M = nbf; g3.N = nbf; g3.K = nbf;
+    g3.A = sd; g3.lda = nbf; g3.B = t2; g3.ldb = nbf; g3.C = t3; g3.ldc = nbf;
+    MGW_TRY(gemm_f64(g3, c->stream));
+    GemmF64 g4;   // Sigma_AO = T3 S
+    g4.M = nbf; g4.N = nbf; g4.K = nbf;
+    g4.A = t3; g4.lda = nbf; g4.B = sd; g4.ldb = nbf; g4.C = t2; g4.ldc = nbf;
+    MGW_TRY(gemm_f64(g4, c->stream));
+    MGW_TRY(download(c, selfenergy_ao, t2, (size_t)nbf * nbf));
+  }
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  MGW_TRY(scratch_done(c, "sigma_w"));
+  MGW_TRY(scratch_done(c, "qs_b"));
+  MGW_TRY(scratch_done(c, "eri_pi"));
+  return MGW_OK;
+}
+
 int mgw_gw_selfenergy(mgw_ctx* c, const mgw_ranges* r, double spin_fact, double eta, int nomega,
                       const double* omega, const double* se_energy0, double* sigma_out,
                       double* energy_gw) {

@@ -83,6 +83,55 @@ int launch_unpack_ri(const double* packed, int64_t npair, int naux, const int32_
   return MGW_OK;
 }
 
+// ------------------------------------------------------------------ QS self-energy
+// b(s, q, i) = w(s, q, i) Re[f_i / (e_q - i eta - e_i + W_s) + (1 - f_i) / (e_q + i eta - e_i - W_s)]
+// with f_i = occ_i / spin_fact (src/m_gw_selfenergy_analytic.f90:738-744).
+__global__ void qs_scale_kernel(const double* __restrict__ w, double* __restrict__ b, int npole, int nse, int cnt,
+                                const double* __restrict__ pole, const double* __restrict__ eq,
+                                const double* __restrict__ ei, const double* __restrict__ fi, double spin_fact,
+                                double eta) {
+  const int64_t total = (int64_t)npole * nse * cnt;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int sp = (int)(e % npole);
+    const int64_t rest = e / npole;
+    const int q = (int)(rest % nse), i = (int)(rest / nse);
+    const double full = fi[i] / spin_fact, empty = (spin_fact - fi[i]) / spin_fact;
+    const double d = eq[q] - ei[i], om = pole[sp];
+    const double a = d + om, c = d - om;
+    b[e] = w[e] * (full * a / (a * a + eta * eta) + empty * c / (c * c + eta * eta));
+  }
+}
+
+int launch_qs_scale(const double* w, double* b, int npole, int nse, int cnt, const double* pole, const double* eq,
+                    const double* ei, const double* fi, double spin_fact, double eta, hipStream_t s) {
+  const int64_t total = (int64_t)npole * nse * cnt;
+  if (total <= 0) return MGW_OK;
+  unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(qs_scale_kernel, dim3(g), dim3(256), 0, s, w, b, npole, nse, cnt, pole, eq, ei, fi, spin_fact,
+                     eta);
+  MGW_LAUNCH_CHECK();
+  return MGW_OK;
+}
+
+// m = (m + m^T) / 2 for an n x n matrix (Kotani's hermitianization,
+// src/m_selfenergy_tools.f90:821-823)
+__global__ void symmetrize_kernel(double* __restrict__ m, int n) {
+  const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (i >= n || j >= n || i <= j) return;
+  const double v = 0.5 * (m[i + (int64_t)j * n] + m[j + (int64_t)i * n]);
+  m[i + (int64_t)j * n] = v;
+  m[j + (int64_t)i * n] = v;
+}
+
+int launch_symmetrize(double* m, int n, hipStream_t s) {
+  dim3 grid(nblocks(n, 16), nblocks(n, 16));
+  hipLaunchKernelGGL(symmetrize_kernel, grid, dim3(256), 0, s, m, n);
+  MGW_LAUNCH_CHECK();
+  return MGW_OK;
+}
+
 // ------------------------------------------------------------------ max |x|
 __global__ void __launch_bounds__(256)
 absmax_kernel(const double* __restrict__ x, int64_t n, double* __restrict__ part) {

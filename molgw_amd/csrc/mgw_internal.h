@@ -165,6 +165,11 @@ int launch_unpack_ri(const double* packed, int64_t npair, int naux, const int32_
 int launch_invsqrt_select(const double* u, int n, const double* eig, int neglect, int kept, double* out,
                           hipStream_t s);
 int launch_fill(double* p, int64_t n, double v, hipStream_t s);
+// b(s, q, i) = w(s, q, i) times the real part of the QS pole factor
+int launch_qs_scale(const double* w, double* b, int npole, int nse, int cnt, const double* pole, const double* eq,
+                    const double* ei, const double* fi, double spin_fact, double eta, hipStream_t s);
+// m = (m + m^T) / 2 (n x n)
+int launch_symmetrize(double* m, int n, hipStream_t s);
 // part[b] = max |x_i| over the elements of block b, b < ABSMAX_BLOCKS
 constexpr int ABSMAX_BLOCKS = 256;
 int launch_absmax(const double* x, int64_t n, double* part, hipStream_t s);

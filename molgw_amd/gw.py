@@ -444,6 +444,19 @@ class Engine:
                                           sig.ctypes.data_as(C.POINTER(C.c_double)), C.byref(egw)))
         return sig, egw.value
 
+    def gw_selfenergy_qs(self, rng: StateRanges, eta=0.001, spin_fact=2.0, s_matrix=None):
+        """QSGW self-energy matrix of the active channel on the resident Casida W:
+        (Sigma_MO[nstate, nstate] symmetrised, Sigma_AO[nbf, nbf] = S C Sigma C^T S
+        or None without s_matrix)."""
+        r = rng.to_c()
+        mo = np.zeros((self.nstate, self.nstate), order='F')
+        ao = np.zeros((self.nbf, self.nbf), order='F') if s_matrix is not None else None
+        sm = _f64(s_matrix) if s_matrix is not None else None
+        check(self._lib.mgw_gw_selfenergy_qs(self._h, C.byref(r), spin_fact, eta,
+                                             _ptr(sm) if sm is not None else None, _ptr(mo),
+                                             _ptr(ao) if ao is not None else None))
+        return mo, ao
+
     # ------------------------------------------------------------ A7
     def gw_selfenergy_imag(self, rng: StateRanges, omega_calc):
         r = rng.to_c()

@@ -413,6 +413,33 @@ def gw_selfenergy(eri_pi: np.ndarray, w_s: np.ndarray, pole: np.ndarray,
     return sigma, energy_gw
 
 
+def gw_selfenergy_qs(eri_pi: np.ndarray, w_s: np.ndarray, pole: np.ndarray, energy: np.ndarray,
+                     occupation: np.ndarray, rng: StateRanges, eta: float = 0.001, spin_fact: float = 2.0):
+    """``gw_selfenergy_qs`` GW branch (src/m_gw_selfenergy_analytic.f90:653-797)
+    and the hermitianization of apply_qs_approximation
+    (src/m_selfenergy_tools.f90:821-823), one spin channel.
+
+    ``eri_pi[P, p - nsemin, i - (ncore_G+1)]`` as for gw_selfenergy. For each
+    G state i (:699): w(s, p) = sum_P w_s(P, s) (P|p i) (:711-715), and
+    Sigma(p, q) += w(s, p) w(s, q) Re[f_i / (e_q - i eta - e_i + Omega_s)
+    + (1 - f_i) / (e_q + i eta - e_i - Omega_s)] (:736-744). Returns
+    (Sigma + Sigma^T) / 2 over the window (nse x nse).
+    """
+    nse = rng.nsemax - rng.nsemin + 1
+    eq = energy[rng.nsemin - 1:rng.nsemax]
+    sig = np.zeros((nse, nse))
+    for k, istate in enumerate(range(rng.ncore_G + 1, rng.nvirtual_G)):
+        w = w_s.T @ eri_pi[:, :, k]                        # (npole, nse)
+        f_full = occupation[istate - 1] / spin_fact
+        f_empty = (spin_fact - occupation[istate - 1]) / spin_fact
+        d = eq[None, :] - energy[istate - 1]
+        a = d + pole[:, None]
+        c = d - pole[:, None]
+        g = f_full * a / (a * a + eta * eta) + f_empty * c / (c * c + eta * eta)
+        sig += w.T @ (w * g)
+    return 0.5 * (sig + sig.T)
+
+
 def gw_selfenergy_c(eri_pi: np.ndarray, w_s: np.ndarray, pole: np.ndarray,
                     energy: np.ndarray, occupation: np.ndarray, rng: StateRanges,
                     omega: np.ndarray, energy0: np.ndarray, eta: float, spin_fact: float = 2.0):
