@@ -18,8 +18,9 @@
 // 32, zeros in the padding and, for D_f, above the diagonal). The product
 // kernel multiplies the planes on v_mfma_f32_16x16x32_bf16 / _f16 (NS = 2: three
 // products, NS = 3: six, as the chi0 SYRK), stops the K loop of a row block at
-// its last row (D_f is lower triangular), flushes FP32 partials into FP64
-// every FLUSH slabs, and reduces C (C + 2 B) over the rows of the block in
+// its last row (D_f is lower triangular), accumulates in FP32 MFMA registers
+// (NS = 2: over the whole K, at most naux terms; NS = 3: flushed into FP64
+// every FLUSH slabs), and reduces C (C + 2 B) over the rows of the block in
 // its epilogue (B in FP64): out[f][row block][n], summed over the row blocks
 // by dot_reduce_kernel.
 //
@@ -254,7 +255,7 @@ wb_dot_kernel(WbArgs g) {
 #pragma unroll
         for (int j = 0; j < FN; ++j)
           part[i][j] = mfma16x16x32<H>(b[QB[pr]][j], a[QA[pr]][i], part[i][j]);
-    if ((s + 1) % FLUSH == 0 || s + 1 == nslab) {
+    if constexpr (FLUSH > 0) if ((s + 1) % FLUSH == 0 || s + 1 == nslab) {
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -311,7 +312,8 @@ wb_dot_kernel(WbArgs g) {
       for (int i = 0; i < FM; ++i) {
         const int m = m0 + wm * TM + i * 16 + fr;
         if (m < g.naux && n < g.ncol) {
-          const double c = acc[i][j][r] * us;
+          // FLUSH 0: FP32 accumulation over the whole K (at most naux terms)
+          const double c = (FLUSH > 0 ? acc[i][j][r] : (double)part[i][j][r]) * us;
           sum += c * (c + 2.0 * g.bmat[m + n * g.ldb]);
         }
       }
@@ -423,8 +425,11 @@ int split_and_dot(const double* d, int naux, int nomega, const double* b, int64_
     g.tiles = (int64_t)L.tm * L.tn;
     g.nblocks = g.tiles * nf;
     MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "W.B split: too many tiles");
+    // two planes: FP32 accumulation over the whole K (the products of the
+    // 11-bit fp16 / 8-bit bf16 planes are exact in FP32; the FP64 accumulators
+    // would cost 128 VGPRs and half the occupancy)
     if (NS == 2)
-      hipLaunchKernelGGL((wb_dot_kernel<NS, H, BM, BN, 2, 2, 4, 3>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
+      hipLaunchKernelGGL((wb_dot_kernel<NS, H, BM, BN, 2, 2, 0, 3>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
     else
       hipLaunchKernelGGL((wb_dot_kernel<NS, H, BM, BN, 2, 2, 4, 2>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
     MGW_HIP(hipGetLastError());
