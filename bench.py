@@ -46,6 +46,10 @@ WORKLOADS = {
     'synthetic-imag': (1200, 3600, 120, 'imaginary_pade',
                        'synthetic imaginary-frequency kernel (nbf=1200, naux=3600, nocc=120), '
                        '32 GL frequencies, Pade continuation, 16 states HOMO-7..LUMO+8'),
+    'contour': (192, 672, 40, 'contour',
+                '(H2O)8-shaped FP64 G0W0 by contour deformation, cc-pVDZ/cc-pVDZ-RI shape (nbf=192, naux=672, '
+                'nocc=40), W on 16 imaginary and 6 real frequencies, 51-point real grid, 16 states '
+                'HOMO-7..LUMO+8'),
 }
 
 
@@ -294,6 +298,32 @@ def cpu_baseline(system, budget_note=True):
             'seconds_per_solve': total}
 
 
+def cpu_baseline_contour(system, kw, gpu_out):
+    """The oracle's whole contour-deformation solve (g0w0_contour: numpy/OpenBLAS
+    FP64, the reference's loops) of the same system and ranges, timed once;
+    its QP energies must match the GPU solve's."""
+    from oracle import gw_oracle as O
+    s, rng = system, kw['rng']
+    ranges = O.StateRanges(rng.ncore_G, rng.nvirtual_G, rng.nhomo_G, rng.nlumo_G, rng.nsemin, rng.nsemax,
+                           rng.ncore_W, rng.nvirtual_W)
+    t0 = time.perf_counter()
+    ref = O.g0w0_contour(s.eri_3center, s.index_basis, s.c_matrix, s.energy, s.occupation, kw['nomega_chi_imag'],
+                         kw['nomega_chi_real'], ranges=ranges)
+    dt = time.perf_counter() - t0
+    diff = float(np.max(np.abs(ref['energy_qp'][:, 0] - gpu_out.energy_qp)))
+    if diff > 1e-8:
+        raise RuntimeError(f'contour: GPU and oracle QP energies differ by {diff:.2e} Ha')
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    threads = int(os.environ.get('OMP_NUM_THREADS', cores))
+    return {'value': 1.0 / dt, 'unit': 'solves/s', 'cores': min(cores, threads), 'kind': 'port',
+            'sample': 'oracle g0w0_contour (numpy/OpenBLAS FP64), one whole solve of the same system; '
+                      f'its QP energies agree with the GPU solve to {diff:.1e} Ha',
+            'seconds_per_solve': dt}
+
+
 def _stream_flops(nbf, naux, nocc, nomega=32, nse=16):
     """Operation count of one imaginary-axis solve as the CPU oracle runs it:
     AO->MO (X and the Sigma block), the chi0 SYRK, the per-frequency
@@ -508,11 +538,12 @@ def main():
     print(f'# generated {args.workload}: nbf={nbf} naux={naux} nocc={nocc} in {time.perf_counter() - t0:.1f}s',
           file=sys.stderr, flush=True)
     kw = {}
-    if technique == 'imaginary_pade':
+    if technique in ('imaginary_pade', 'contour'):
         from molgw_amd.gw import selfenergy_set_state_range
         rng = selfenergy_set_state_range(system.occupation)
         rng.nsemin, rng.nsemax = rng.nhomo_G - 7, rng.nhomo_G + 8
-        kw = dict(rng=rng, nomega_chi_imag=32, nomega_sigma_calc=16)
+        kw = (dict(rng=rng, nomega_chi_imag=32, nomega_sigma_calc=16) if technique == 'imaginary_pade' else
+              dict(rng=rng, nomega_chi_imag=16, nomega_chi_real=6))
     elapsed, lat, prof, out, mem, err_mev = gpu_bench(system, technique, args.steps, args.warmup, kw,
                                                       args.precision)
     print(f'# gpu: {args.steps} solves in {elapsed:.3f}s; per-class device ms: '
@@ -524,6 +555,10 @@ def main():
         cpu = cpu_baseline(system)
         print(f'# cpu: {cpu["seconds_per_solve"]:.1f} s per solve, stages {cpu["stage_seconds"]}',
               file=sys.stderr, flush=True)
+    elif not args.no_cpu_baseline and technique == 'contour':
+        print('# cpu baseline (oracle, one whole solve) ...', file=sys.stderr, flush=True)
+        cpu = cpu_baseline_contour(system, kw, out)
+        print(f'# cpu: {cpu["seconds_per_solve"]:.1f} s per solve', file=sys.stderr, flush=True)
     value = args.steps / elapsed
     line = {
         'metric': METRIC,

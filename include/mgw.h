@@ -55,11 +55,13 @@ enum { MGW_MEM_HOST = 0, MGW_MEM_DEVICE = 1 };
  *              transforms stay FP64                                       */
 enum { MGW_FP64 = 0, MGW_FP32 = 1, MGW_BF16X3 = 2, MGW_BF16X6 = 3, MGW_FP16X3 = 4 };
 
-enum { MGW_TECH_ONE_SHOT = 0, MGW_TECH_IMAGINARY_PADE = 1 };
+enum { MGW_TECH_ONE_SHOT = 0, MGW_TECH_IMAGINARY_PADE = 1, MGW_TECH_CONTOUR = 2 };
 
 /* State ranges held as module variables in the reference:
- * ncore_G, nvirtual_G, nsemin, nsemax, nhomo_G (src/m_selfenergy_tools.f90:30-41)
- * ncore_W, nvirtual_W (src/m_spectral_function.f90:95-107). */
+ * ncore_G, nvirtual_G, nsemin, nsemax, nhomo_G, nlumo_G (src/m_selfenergy_tools.f90:30-41,
+ * :97-110) and ncore_W, nvirtual_W (src/m_spectral_function.f90:95-107).
+ * nlumo_G = 0 means nhomo_G + 1 (integer occupations); only the contour
+ * deformation self-energy reads it. */
 typedef struct mgw_ranges {
   int32_t ncore_G;
   int32_t nvirtual_G;
@@ -68,11 +70,11 @@ typedef struct mgw_ranges {
   int32_t nhomo_G;
   int32_t ncore_W;
   int32_t nvirtual_W;
-  int32_t reserved;
+  int32_t nlumo_G;
 } mgw_ranges;
 
 /* One-shot G0W0 parameters (input keywords, src/input_variables.yaml:
- * nomega_sigma :1164, step_sigma :1384, eta :1051, nomega_chi_imag :1148,
+ * nomega_sigma :1164, step_sigma :1384, eta :1051, nomega_chi_imag :1148, nomega_chi_real :1156,
  * nomega_sigma_calc :1173, step_sigma_calc :1392). */
 typedef struct mgw_g0w0_params {
   int32_t technique;          /* MGW_TECH_*                                   */
@@ -86,7 +88,8 @@ typedef struct mgw_g0w0_params {
   int32_t precision;          /* MGW_FP64 | MGW_FP32 | MGW_BF16X3 | MGW_BF16X6
                                  (transforms and chi0; metric/inversion/
                                  eigensolve and the QP solve always FP64)     */
-  int32_t reserved;
+  int32_t nomega_chi_real;    /* real-axis W grid, >= 2 (contour technique;
+                                 input keyword nomega_chi_real, default 2)    */
 } mgw_g0w0_params;
 
 /* ---------------------------------------------------------------- context */
@@ -122,7 +125,9 @@ int mgw_memory_report(mgw_ctx* ctx, int64_t* current_bytes, int64_t* peak_bytes)
  * reference has no reduced-precision build. */
 int mgw_precision_escalations(mgw_ctx* ctx, int64_t* count);
 /* Bytes of HBM one solve of this shape needs (resident RI tensor included);
- * seeded by the reference's evaluate_memory formulas (src/m_io.f90:3843-3936). */
+ * seeded by the reference's evaluate_memory formulas (src/m_io.f90:3843-3936).
+ * For MGW_TECH_CONTOUR pass nomega_chi_imag + nomega_chi_real as
+ * nomega_chi_imag. */
 int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int technique,
                             int nomega_chi_imag, int precision);
 
@@ -264,6 +269,24 @@ int mgw_gw_selfenergy_qs(mgw_ctx* ctx, const mgw_ranges* ranges, double spin_fac
 int mgw_gw_selfenergy_imag(mgw_ctx* ctx, const mgw_ranges* ranges, int nomega_calc,
                            const double* omega_calc, double* sigma_calc_out);
 
+/* ------------------------------- contour deformation self-energy (f)-3 */
+/* gw_selfenergy_contour (src/m_gw_selfenergy_grid.f90:342-551) for every spin
+ * channel, with W in the low-rank form of sf_vsqrt_chi_vsqrt_rpa(low_rank)
+ * (src/m_spectral_function.f90:768-949): chi0 of all channels' transitions
+ * (energies in G, as the reference passes energy_g) on the imaginary
+ * Gauss-Legendre grid (nomega_chi_imag points) and on the real grid of
+ * nomega_chi_real >= 2 points over [0, de_max] (:308-315), where de_max is the
+ * largest residue frequency of the reference's scan (:385-408); eigenpairs
+ * with |lambda / (1 - lambda)| <= 1e-3 dropped. omega[nomega] real (the grid
+ * se%omega around se_energy0), se_energy0 [nse x nspin]. sigma_out
+ * [nomega x nse x nspin complex, zero imaginary part] nullable; the active
+ * channel's sigma stays resident for mgw_find_qp_energy; de_max_out
+ * nullable. A residue frequency outside the real grid is an error, as in
+ * sf_interpolate_vsqrt_chi_vsqrt (:970-973). FP64. */
+int mgw_gw_selfenergy_contour(mgw_ctx* ctx, const mgw_ranges* ranges, double spin_fact, int nomega_chi_imag,
+                              int nomega_chi_real, double eta, int nomega, const double* omega,
+                              const double* se_energy0, double* sigma_out, double* de_max_out);
+
 /* ------------------------------------------------ A8: Pade continuation */
 /* se_pade_fit (src/m_selfenergy_tools.f90:965-999) with pade/calculate_pade_a
  * (src/m_numerical_tools.f90:86-145) on the resident sigma_calc (or on
@@ -292,7 +315,10 @@ int mgw_find_qp_energy(mgw_ctx* ctx, const mgw_ranges* ranges, int nomega, const
  * energy_qp_lin [nstate]; sigma_out (nomega_sigma x nse complex);
  * energies[2] = {E_RPA, E_GW}. With two spin channels (mgw_set_mo_spin)
  * exchange_m_vxc and the state outputs are [nstate x nspin], sigma_out is
- * [nomega_sigma x nse x nspin], and E_GW sums the channels. */
+ * [nomega_sigma x nse x nspin], and E_GW sums the channels.
+ * MGW_TECH_CONTOUR runs gw_selfenergy_contour with the one-shot grid
+ * (src/m_selfenergy_tools.f90:658-665, :679-680); its energies[] are 0, as
+ * the reference computes no RPA energy on that route. */
 int mgw_g0w0(mgw_ctx* ctx, const mgw_g0w0_params* params, const mgw_ranges* ranges,
              const double* exchange_m_vxc, double* energy_qp, double* zz,
              double* energy_qp_lin, double* sigma_out, double* energies);

@@ -13,21 +13,21 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libmgw.so')
 MGW_OK = 0
 MGW_MEM_HOST, MGW_MEM_DEVICE = 0, 1
 MGW_FP64, MGW_FP32, MGW_BF16X3, MGW_BF16X6, MGW_FP16X3 = 0, 1, 2, 3, 4
-MGW_TECH_ONE_SHOT, MGW_TECH_IMAGINARY_PADE = 0, 1
+MGW_TECH_ONE_SHOT, MGW_TECH_IMAGINARY_PADE, MGW_TECH_CONTOUR = 0, 1, 2
 
 
 class Ranges(C.Structure):
     """mgw_ranges: 1-based state ranges (module variables of the reference)."""
     _fields_ = [('ncore_G', C.c_int32), ('nvirtual_G', C.c_int32), ('nsemin', C.c_int32),
                 ('nsemax', C.c_int32), ('nhomo_G', C.c_int32), ('ncore_W', C.c_int32),
-                ('nvirtual_W', C.c_int32), ('reserved', C.c_int32)]
+                ('nvirtual_W', C.c_int32), ('nlumo_G', C.c_int32)]
 
 
 class G0W0Params(C.Structure):
     _fields_ = [('technique', C.c_int32), ('nomega_sigma', C.c_int32), ('step_sigma', C.c_double),
                 ('eta', C.c_double), ('nomega_chi_imag', C.c_int32), ('nomega_sigma_calc', C.c_int32),
                 ('step_sigma_calc', C.c_double), ('spin_fact', C.c_double), ('precision', C.c_int32),
-                ('reserved', C.c_int32)]
+                ('nomega_chi_real', C.c_int32)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -68,6 +68,8 @@ SIGNATURES = {
     'mgw_gw_selfenergy': (C.c_int, [_vp, C.POINTER(Ranges), C.c_double, C.c_double, C.c_int, _dp, _dp,
                                     _dp, _dp]),
     'mgw_gw_selfenergy_imag': (C.c_int, [_vp, C.POINTER(Ranges), C.c_int, _dp, _dp]),
+    'mgw_gw_selfenergy_contour': (C.c_int, [_vp, C.POINTER(Ranges), C.c_double, C.c_int, C.c_int, C.c_double,
+                                            C.c_int, _dp, _dp, _dp, _dp]),
     'mgw_pade_fit': (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.c_double, _dp]),
     'mgw_find_qp_energy': (C.c_int, [_vp, C.POINTER(Ranges), C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int,
                                      _dp, _dp, _dp]),

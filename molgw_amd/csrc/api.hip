@@ -111,6 +111,10 @@ struct mgw_ctx {
   bool has_chi = false;
   int nomega_chi = 0;
   std::vector<double> chi_omega, chi_weight;
+  // contour deformation: low-rank W ("cd_u" eigenvectors, "cd_w" weights) on
+  // cd_om_im (imaginary, weights cd_wt_im) then cd_om_re (real)
+  bool has_cd = false;
+  std::vector<double> cd_om_im, cd_wt_im, cd_om_re;
   // self-energy
   bool has_sigma = false;
   int sigma_nomega = 0, sigma_nse = 0;
@@ -862,6 +866,172 @@ void imaginary_grid(int n, std::vector<double>& om, std::vector<double>& wt) {
   }
 }
 
+// ------------------------------------------------------------------ contour deformation
+int nlumo_of(const mgw_ranges* r) { return r->nlumo_G > 0 ? r->nlumo_G : r->nhomo_G + 1; }
+
+// Largest residue frequency (src/m_gw_selfenergy_grid.f90:385-408): over the
+// channels, the states m and the grid, e_p - (E0_m + omega_k) for occupied p
+// in [ncore_G+1, nhomo_G] and E0_m + omega_k - e_p for empty p in
+// [nlumo_G, nvirtual_G-1], values >= -eta only; 0 if none.
+double contour_de_max(const mgw_ctx* c, const mgw_ranges* r, double eta, int nomega, const double* omega,
+                      const double* se_energy0) {
+  const int nse = r->nsemax - r->nsemin + 1, ns = c->nstate;
+  const std::vector<double>& e = c->energy_in_g();
+  double de_max = 0.0;
+  for (int s = 0; s < c->nspin; ++s)
+    for (int m = 0; m < nse; ++m)
+      for (int k = 0; k < nomega; ++k) {
+        const double w = se_energy0[m + (int64_t)nse * s] + omega[k];
+        for (int p = r->ncore_G + 1; p <= r->nhomo_G; ++p) {
+          const double de = e[(size_t)s * ns + p - 1] - w;
+          if (de >= -eta) de_max = std::max(de_max, de);
+        }
+        for (int p = nlumo_of(r); p <= r->nvirtual_G - 1; ++p) {
+          const double de = w - e[(size_t)s * ns + p - 1];
+          if (de >= -eta) de_max = std::max(de_max, de);
+        }
+      }
+  return de_max;
+}
+
+// W of sf_vsqrt_chi_vsqrt_rpa(low_rank) (src/m_spectral_function.f90:768-949)
+// on nimag imaginary and nreal real frequencies at once: chi0_f = -X diag(s_f) X^T
+// (one batched MFMA SYRK), the eigenpairs of every chi0_f (rocSOLVER dsyevd,
+// strided batched), and the weights lambda / (1 - lambda) where they exceed
+// 1e-3 in magnitude, 0 for the dropped eigenpairs (contour_eta_kernel).
+int contour_w(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int nimag, int nreal, double de_max) {
+  MGW_REQUIRE(nimag >= 1 && nreal >= 2 && nimag + nreal <= 65535, MGW_ERR_ARG,
+              "contour deformation needs nomega_chi_imag >= 1 and nomega_chi_real >= 2");
+  std::vector<int32_t> ia;
+  double* x = nullptr;
+  MGW_TRY(build_x_ia(c, r, spin_fact, ia, &x, MGW_FP64));
+  const int npole = (int)ia.size() / 2, naux = c->naux, nf = nimag + nreal;
+  int32_t* ia_d = nullptr;
+  MGW_TRY(upload(c, "ia", ia.data(), ia.size(), &ia_d));
+  std::vector<double> om_im, wt_im, om_re(nreal);
+  imaginary_grid(nimag, om_im, wt_im);
+  // REAL_LINEAR grid (src/m_spectral_function.f90:308-315)
+  for (int j = 0; j < nreal; ++j) om_re[j] = (double)j / (double)(nreal - 1) * de_max;
+  double *om_d, *eg_d, *sw, *u, *lam, *ework, *w;
+  std::vector<double> om_all(om_im);
+  om_all.insert(om_all.end(), om_re.begin(), om_re.end());
+  MGW_TRY(upload(c, "cd_omega", om_all.data(), nf, &om_d));
+  // the reference builds this W from the energies in G (energy_g)
+  MGW_TRY(upload(c, "cd_energy", c->energy_in_g().data(), c->energy_in_g().size(), &eg_d));
+  MGW_TRY(dbuf(c, "chi_sw", (size_t)npole * nf, &sw));
+  MGW_TRY(launch_chi_weights(ia_d, npole, eg_d, find_buf(c, "occupation"), om_d, nimag, sw, c->stream, 1.0));
+  MGW_TRY(launch_chi_weights(ia_d, npole, eg_d, find_buf(c, "occupation"), om_d + nimag, nreal,
+                             sw + (int64_t)npole * nimag, c->stream, -1.0));
+  MGW_TRY(dbuf(c, "cd_u", (size_t)naux * naux * nf, &u));
+  GemmF64 g;
+  g.M = naux; g.N = naux; g.K = npole;
+  g.A = x; g.lda = naux; g.strideA = 0;
+  g.B = x; g.ldb = naux; g.transB = true; g.strideB = 0;
+  g.C = u; g.ldc = naux; g.strideC = (int64_t)naux * naux;
+  g.kscale = sw; g.strideK = npole;
+  g.alpha = -1.0;                // chi0 itself
+  g.batch = nf;
+  g.lower_only = true; g.no_mirror = true;   // dsyevd reads the lower triangle
+  g.klass = MGW_KCLASS_GEMM_CHI0;
+  {
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_CHI0, (double)naux * (naux + 1.0) * npole * nf,
+                 8.0 * ((double)naux * npole + 0.5 * (double)naux * naux * nf), c->stream);
+    MGW_TRY(gemm_f64(g, c->stream));
+  }
+  MGW_TRY(scratch_done(c, "chi_sw"));
+  int32_t* info_d = nullptr;
+  MGW_TRY(dbuf(c, "solver_info", std::max(64, nf), &info_d));
+  MGW_TRY(dbuf(c, "cd_lambda", (size_t)naux * nf, &lam));
+  MGW_TRY(dbuf(c, "cd_ework", (size_t)naux * nf, &ework));
+  {
+    const double n3 = (double)naux * naux * naux;
+    ProfScope ps(&c->prof, MGW_KCLASS_EIGEN, (4.0 / 3.0 + 2.0) * n3 * nf, 16.0 * (double)naux * naux * nf,
+                 c->stream);
+    MGW_BLAS(rocsolver_dsyevd_strided_batched(c->blas, rocblas_evect_original, rocblas_fill_lower, naux, u, naux,
+                                              (rocblas_stride)naux * naux, lam, naux, ework, naux, info_d, nf));
+  }
+  std::vector<int32_t> info(nf);
+  MGW_TRY(download(c, info.data(), info_d, nf));
+  for (int f = 0; f < nf; ++f)
+    MGW_REQUIRE(info[f] == 0, MGW_ERR_SOLVER,
+                "rocsolver_dsyevd (contour W) failed at frequency " + std::to_string(f) + ", info=" +
+                    std::to_string(info[f]));
+  MGW_TRY(dbuf(c, "cd_w", (size_t)naux * nf, &w));
+  MGW_TRY(launch_contour_eta(lam, (int64_t)naux * nf, 1.0e-3, w, c->stream));
+  MGW_TRY(scratch_done(c, "cd_ework"));
+  c->has_cd = true;
+  c->cd_om_im = om_im;
+  c->cd_wt_im = wt_im;
+  c->cd_om_re = om_re;
+  return MGW_OK;
+}
+
+// The self-energy of gw_selfenergy_contour (src/m_gw_selfenergy_grid.f90:429-530)
+// for the active channel: v(p, m, f) = B_pm^T W_f B_pm on every frequency of
+// both grids in one batched MFMA product with a row-weighted column-dot
+// epilogue (C = U_f^T B, v = sum_e w_e C_e^2), then sigma_contour_kernel.
+int contour_sigma(mgw_ctx* c, const mgw_ranges* r, double eta, int nomega, const double* omega,
+                  const double* se_energy0) {
+  MGW_REQUIRE(c->has_cd, MGW_ERR_STATE, "no contour W");
+  const int naux = c->naux, nimag = (int)c->cd_om_im.size(), nreal = (int)c->cd_om_re.size();
+  const int nf = nimag + nreal;
+  const int nse = r->nsemax - r->nsemin + 1;
+  const int glo = r->ncore_G + 1, ghi = r->nvirtual_G - 1;
+  const int np = ghi - glo + 1;
+  const int64_t ncol = (int64_t)np * nse;
+  MGW_REQUIRE(ncol < (1ll << 31), MGW_ERR_ARG, "sigma_contour: too many columns");
+  double* epm = nullptr;
+  MGW_TRY(dbuf(c, "eri_pm", (size_t)naux * ncol, &epm));
+  MGW_TRY(ao2mo(c, glo, ghi, r->nsemin, r->nsemax, epm, MGW_FP64));
+  const int nrb = gemm_f64_rowblocks(naux);
+  double *part, *v;
+  MGW_TRY(dbuf(c, "dot_part", (size_t)nrb * ncol * nf, &part));
+  MGW_TRY(dbuf(c, "dot_v", (size_t)ncol * nf, &v));
+  GemmF64 g;
+  g.M = naux; g.N = (int)ncol; g.K = naux;
+  g.transA = true; g.A = find_buf(c, "cd_u"); g.lda = naux; g.strideA = (int64_t)naux * naux;
+  g.B = epm; g.ldb = naux; g.strideB = 0;
+  g.dot_w = find_buf(c, "cd_w"); g.strideDotW = naux;
+  g.dotpart = part; g.strideDot = (int64_t)nrb * ncol;
+  g.batch = nf;
+  g.batch_outer = true;
+  {
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_WB, 2.0 * naux * (double)naux * ncol * nf,
+                 8.0 * ((double)naux * naux * nf + (double)naux * ncol), c->stream);
+    MGW_TRY(gemm_f64(g, c->stream));
+  }
+  MGW_TRY(launch_dot_reduce(part, nrb, ncol, nf, nullptr, v, c->stream));
+  std::vector<double> ep(np);
+  std::vector<int32_t> ps(np);
+  for (int k = 0; k < np; ++k) {
+    ep[k] = c->energy_in_g()[c->so() + glo - 1 + k];
+    ps[k] = glo + k;
+  }
+  double *omi_d, *wti_d, *omr_d, *ep_d, *e0_d, *om_d, *sigma;
+  int32_t *ps_d, *err_d;
+  MGW_TRY(upload(c, "cd_om_im", c->cd_om_im.data(), nimag, &omi_d));
+  MGW_TRY(upload(c, "cd_wt_im", c->cd_wt_im.data(), nimag, &wti_d));
+  MGW_TRY(upload(c, "cd_om_re", c->cd_om_re.data(), nreal, &omr_d));
+  MGW_TRY(upload(c, "cd_ep", ep.data(), np, &ep_d));
+  MGW_TRY(upload(c, "cd_pstate", ps.data(), np, &ps_d));
+  MGW_TRY(upload(c, "se_e0p", se_energy0, nse, &e0_d));
+  MGW_TRY(upload(c, "se_omega", omega, nomega, &om_d));
+  MGW_TRY(dbuf(c, "sigma", (size_t)2 * nomega * nse, &sigma));
+  const int32_t zero = 0;
+  MGW_TRY(upload(c, "cd_err", &zero, 1, &err_d));
+  MGW_TRY(launch_sigma_contour(v, np, nse, nimag, nreal, omi_d, wti_d, omr_d, ep_d, ps_d, r->nhomo_G, nlumo_of(r),
+                               e0_d, om_d, nomega, eta, sigma, err_d, c->stream));
+  int32_t err = 0;
+  MGW_TRY(download(c, &err, err_d, 1));
+  MGW_REQUIRE(err == 0, MGW_ERR_ARG, "sigma_contour: a residue frequency lies outside the real W grid");
+  c->has_sigma = true;
+  c->sigma_nomega = nomega;
+  c->sigma_nse = nse;
+  MGW_TRY(scratch_done(c, "dot_part"));
+  MGW_TRY(scratch_done(c, "eri_pm"));
+  return MGW_OK;
+}
+
 }  // namespace
 
 // ======================================================================
@@ -1014,6 +1184,10 @@ int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int techn
     stage_w = na * nb * no + na * npole           // half transform + (P|ia)
               + 2 * npole * npole                 // Casida matrix + eigen workspace
               + na * npole;                       // w_s
+  } else if (technique == MGW_TECH_CONTOUR) {
+    // eigenvectors of chi0 on both grids, eigenvalues, weights
+    stage_w = na * nb * no + na * npole + na * na * (int64_t)nomega_chi_imag + npole * nomega_chi_imag +
+              3 * na * (int64_t)nomega_chi_imag;
   } else {
     stage_w = na * nb * no + na * npole + na * na * (int64_t)nomega_chi_imag + npole * nomega_chi_imag +
               (int64_t)mgw::chol_work_doubles((int)na) * nomega_chi_imag;
@@ -1021,6 +1195,8 @@ int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int techn
       stage_w += (int64_t)(mgw::chi0_split_workspace((int)na, (int)npole, nomega_chi_imag, precision) / 8);
   }
   int64_t stage_s = na * nb * ns + na * ns * ns + na * npole + (int64_t(1) << 27);
+  if (technique == MGW_TECH_CONTOUR)   // column-dot partials and v on every frequency
+    stage_s += (int64_t)(mgw::gemm_f64_rowblocks((int)na) + 1) * ns * ns * nomega_chi_imag;
   if (technique != MGW_TECH_ONE_SHOT && split_precision(precision))
     stage_s += (int64_t)((mgw::wb_split_workspace((int)na, ns * ns, nomega_chi_imag, precision) + 7) / 8);
   // the temporaries of the W stage stay cached while the self-energy runs
@@ -1539,6 +1715,31 @@ int mgw_gw_selfenergy_imag(mgw_ctx* c, const mgw_ranges* r, int nomega_calc, con
   return MGW_OK;
 }
 
+int mgw_gw_selfenergy_contour(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int nomega_chi_imag,
+                              int nomega_chi_real, double eta, int nomega, const double* omega,
+                              const double* se_energy0, double* sigma_out, double* de_max_out) {
+  MGW_CTX(c);
+  MGW_TRY(check_ranges(c, r));
+  MGW_REQUIRE(omega && se_energy0 && nomega >= 1, MGW_ERR_ARG, "invalid frequency grid");
+  const int nse = r->nsemax - r->nsemin + 1;
+  const double de_max = contour_de_max(c, r, eta, nomega, omega, se_energy0);
+  MGW_TRY(contour_w(c, r, spin_fact, nomega_chi_imag, nomega_chi_real, de_max));
+  // every channel, the active one last so that its sigma stays resident
+  const int active = c->spin;
+  for (int k = 0; k < c->nspin; ++k) {
+    const int s = k == c->nspin - 1 ? active : (k < active ? k : k + 1);
+    c->spin = s;
+    const int rc = contour_sigma(c, r, eta, nomega, omega, se_energy0 + (int64_t)nse * s);
+    if (rc == MGW_OK && sigma_out)
+      MGW_TRY(download(c, sigma_out + (int64_t)2 * nomega * nse * s, find_buf(c, "sigma"), (size_t)2 * nomega * nse));
+    c->spin = active;
+    MGW_TRY(rc);
+  }
+  if (de_max_out) *de_max_out = de_max;
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  return MGW_OK;
+}
+
 int mgw_pade_fit(mgw_ctx* c, int nse, int nomega_calc, const double* omega_calc, const double* sigma_calc,
                  int nomega, const double* omega, double eta, double* sigma_out) {
   MGW_CTX(c);
@@ -1633,6 +1834,23 @@ int mgw_g0w0(mgw_ctx* c, const mgw_g0w0_params* prm, const mgw_ranges* r, const 
       c->spin = s;
       MGW_TRY(gw_selfenergy_imag(c, r, prm->nomega_sigma_calc, wc.data(), prec));
       MGW_TRY(pade_fit(c, nse, prm->nomega_sigma_calc, wc.data(), nullptr, nomega, omega.data(), prm->eta));
+      MGW_TRY(per_spin(s));
+      MGW_TRY(sigma_copy(s));
+    }
+  } else if (prm->technique == MGW_TECH_CONTOUR) {
+    MGW_REQUIRE(prec == MGW_FP64, MGW_ERR_ARG, "the contour deformation technique runs in FP64");
+    // se_init(contour_deformation): the one-shot grid around energy0
+    // (src/m_selfenergy_tools.f90:658-665, :679-680)
+    for (int k = 0; k < nomega; ++k) omega[k] = prm->step_sigma * (k - nom);
+    std::vector<double> se0_all((size_t)nse * nspin);
+    for (int s = 0; s < nspin; ++s)
+      for (int k = 0; k < nse; ++k) se0_all[k + (size_t)nse * s] = c->energy_in_g()[(size_t)s * nstate + r->nsemin - 1 + k];
+    const double de_max = contour_de_max(c, r, prm->eta, nomega, omega.data(), se0_all.data());
+    MGW_TRY(contour_w(c, r, prm->spin_fact, prm->nomega_chi_imag, prm->nomega_chi_real, de_max));
+    for (int s = 0; s < nspin; ++s) {
+      c->spin = s;
+      std::copy(se0_all.begin() + (size_t)nse * s, se0_all.begin() + (size_t)nse * (s + 1), se0.begin());
+      MGW_TRY(contour_sigma(c, r, prm->eta, nomega, omega.data(), se0.data()));
       MGW_TRY(per_spin(s));
       MGW_TRY(sigma_copy(s));
     }

@@ -58,6 +58,7 @@ struct GemmDev {
   const double* dotB; int64_t lddot;
   double dot_cc, dot_cd;
   double* dotpart; int64_t strideDot;
+  const double* dot_w; int64_t strideDotW;
 };
 
 template <int BM, int BN, int WM, int WN, int BK, bool TA, bool TB, bool DOT, int KLASS>
@@ -202,9 +203,10 @@ gemm_f64_kernel(GemmDev g) {
 
   if constexpr (DOT) {
     // Column dots of this row block: sum_m C(m,n) (cc C(m,n) + cd D(m,n)),
-    // or sum_m C(m,n)^2 without D.
+    // or sum_m w(m) C(m,n)^2 without D (w = 1 without row weights).
     __shared__ double red[WM][BN];
     const double* __restrict__ D = g.dotB ? g.dotB + (int64_t)batch * g.strideB : nullptr;  // batched as B
+    const double* __restrict__ W = g.dot_w ? g.dot_w + (int64_t)batch * g.strideDotW : nullptr;
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
 #pragma unroll
@@ -217,7 +219,7 @@ gemm_f64_kernel(GemmDev g) {
           const int m = m0 + wm * TM + i * 16 + fr;
           if (m < g.M && n < g.N) {
             const double c = g.alpha * acc[i][j][r];
-            part += D ? c * (g.dot_cc * c + g.dot_cd * D[m + (int64_t)n * g.lddot]) : c * c;
+            part += D ? c * (g.dot_cc * c + g.dot_cd * D[m + (int64_t)n * g.lddot]) : (W ? W[m] : 1.0) * c * c;
           }
         }
         part += __shfl_xor(part, 8, 16);
@@ -285,6 +287,7 @@ int launch_cfg(const GemmF64& h, hipStream_t stream) {
   g.batch = h.batch;
   g.dotB = h.dotB; g.lddot = h.lddot; g.dotpart = h.dotpart; g.strideDot = h.strideDot;
   g.dot_cc = h.dot_cc; g.dot_cd = h.dot_cd;
+  g.dot_w = h.dot_w; g.strideDotW = h.strideDotW;
   // square-tile symmetric updates enumerate the lower triangle of tiles only;
   // other lower-only shapes launch the full grid and skip upper tiles
   const int64_t tiles = (h.lower_only && BM == BN) ? (int64_t)g.tiles_m * (g.tiles_m + 1) / 2
@@ -422,6 +425,7 @@ int gemm_f64(const GemmF64& h, hipStream_t stream) {
   // MGW_BF16X6 and MGW_FP16X3 split only the chi0 and imaginary-axis
   // self-energy operands (chi0_split.hip, wb_split.hip); every other product
   // of those modes stays FP64
+  MGW_REQUIRE(h.dot_w == nullptr || h.precision == MGW_FP64, MGW_ERR_ARG, "gemm_f64: row-weighted dots are FP64 only");
   if (h.precision == MGW_FP32 || h.precision == MGW_BF16X3) return gemm_lowp(h, h.precision, stream);
   MGW_REQUIRE(h.K >= 0, MGW_ERR_ARG, "gemm_f64: negative K");
   MGW_REQUIRE(!h.lower_only || h.M == h.N, MGW_ERR_ARG, "gemm_f64: lower_only needs M == N");

@@ -449,7 +449,7 @@ int sigma_part_doubles(int npole, int nse) {
 __global__ void chi_weights_kernel(const int32_t* __restrict__ ia, int npole,
                                    const double* __restrict__ energy,
                                    const double* __restrict__ occ, const double* __restrict__ omega,
-                                   int nomega, double* __restrict__ sw) {
+                                   int nomega, double wsign, double* __restrict__ sw) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int iw = blockIdx.y;
   if (t >= npole) return;
@@ -457,13 +457,110 @@ __global__ void chi_weights_kernel(const int32_t* __restrict__ ia, int npole,
   const double docc = occ[i] - occ[a];
   const double de = energy[a] - energy[i];
   const double w = omega[iw];
-  sw[t + (int64_t)npole * iw] = 2.0 * docc * de / (w * w + de * de);
+  sw[t + (int64_t)npole * iw] = 2.0 * docc * de / (wsign * w * w + de * de);
 }
 
 int launch_chi_weights(const int32_t* ia, int npole, const double* energy, const double* occ,
-                       const double* omega, int nomega, double* sw, hipStream_t s) {
+                       const double* omega, int nomega, double* sw, hipStream_t s, double wsign) {
   hipLaunchKernelGGL(chi_weights_kernel, dim3(nblocks(npole, 256), nomega), dim3(256), 0, s, ia,
-                     npole, energy, occ, omega, nomega, sw);
+                     npole, energy, occ, omega, nomega, wsign, sw);
+  MGW_LAUNCH_CHECK();
+  return MGW_OK;
+}
+
+// ------------------------------------------------------------------ contour deformation
+// Low-rank W from the eigenvalues lambda of chi0 (src/m_spectral_function.f90:892-913):
+// w = lambda / (1 - lambda) where |w| > tol, else 0 (the dropped eigenpairs).
+__global__ void contour_eta_kernel(const double* __restrict__ lam, int64_t n, double tol,
+                                   double* __restrict__ w) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double l = lam[k];
+  const double e = l / (1.0 - l);
+  w[k] = fabs(e) > tol ? e : 0.0;
+}
+
+int launch_contour_eta(const double* lam, int64_t n, double tol, double* w, hipStream_t s) {
+  hipLaunchKernelGGL(contour_eta_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, lam, n, tol, w);
+  MGW_LAUNCH_CHECK();
+  return MGW_OK;
+}
+
+// sigma(k, m) of gw_selfenergy_contour (src/m_gw_selfenergy_grid.f90:429-530)
+// for x = E0_m + omega_k - e_p and v = B_pm^T W B_pm:
+//   imaginary axis, |x| > eta: -sum_f wt_f / (2 pi) [1/(x + i w_f) + 1/(x - i w_f)] v_f
+//   occupied p (p < nlumo), de = e_p - (E0_m + omega_k) >= -eta:  -v(|de|) c
+//   empty p (p > nhomo),    de = E0_m + omega_k - e_p >= -eta:    +v(|de|) c
+// with c = 1/2 when |de| < eta, else 1, and v(|de|) at the real-grid point
+// nearest to |de| (the first on a tie, MINLOC). v holds the imaginary
+// frequencies first, then the real ones: v[(p + np m) + ncol f]. A |de|
+// outside the real grid sets *err. One workgroup per (k, m).
+__global__ void sigma_contour_kernel(const double* __restrict__ v, int np, int nse, int nimag, int nreal,
+                                     const double* __restrict__ om_im, const double* __restrict__ wt_im,
+                                     const double* __restrict__ om_re, const double* __restrict__ ep,
+                                     const int32_t* __restrict__ pstate, int nhomo, int nlumo,
+                                     const double* __restrict__ e0, const double* __restrict__ omega,
+                                     int nomega, double eta, double* __restrict__ sigma, int32_t* __restrict__ err) {
+  __shared__ double red[4];
+  const int k = blockIdx.x, m = blockIdx.y;
+  const int64_t ncol = (int64_t)np * nse;
+  const double w = e0[m] + omega[k];
+  double lo = om_re[0], hi = om_re[0];
+  for (int j = 1; j < nreal; ++j) { lo = fmin(lo, om_re[j]); hi = fmax(hi, om_re[j]); }
+  double acc = 0.0;
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    const int64_t col = p + (int64_t)np * m;
+    const double x = w - ep[p];
+    if (fabs(x) > eta) {
+      double s = 0.0;
+      for (int f = 0; f < nimag; ++f) {
+        const double wf = om_im[f];
+        s += wt_im[f] * (2.0 * x / (x * x + wf * wf)) * v[col + ncol * f];
+      }
+      acc -= s / (2.0 * 3.14159265358979323);
+    }
+    const int ps = pstate[p];
+    for (int branch = 0; branch < 2; ++branch) {
+      double de;
+      if (branch == 0) {
+        if (ps >= nlumo) continue;
+        de = ep[p] - w;
+      } else {
+        if (ps <= nhomo) continue;
+        de = w - ep[p];
+      }
+      if (de < -eta) continue;
+      const double a = fabs(de);
+      if (a > hi || a < lo) {
+        err[0] = 1;
+        continue;
+      }
+      int jbest = 0;
+      double dbest = fabs(om_re[0] - a);
+      for (int j = 1; j < nreal; ++j) {
+        const double d = fabs(om_re[j] - a);
+        if (d < dbest) { dbest = d; jbest = j; }
+      }
+      const double r = v[col + ncol * (nimag + jbest)] * (a < eta ? 0.5 : 1.0);
+      acc += branch == 0 ? -r : r;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double* o = sigma + 2 * ((int64_t)m * nomega + k);
+    o[0] = red[0] + red[1] + red[2] + red[3];
+    o[1] = 0.0;
+  }
+}
+
+int launch_sigma_contour(const double* v, int np, int nse, int nimag, int nreal, const double* om_im,
+                         const double* wt_im, const double* om_re, const double* ep, const int32_t* pstate,
+                         int nhomo, int nlumo, const double* e0, const double* omega, int nomega, double eta,
+                         double* sigma, int32_t* err, hipStream_t s) {
+  hipLaunchKernelGGL(sigma_contour_kernel, dim3(nomega, nse), dim3(256), 0, s, v, np, nse, nimag, nreal, om_im,
+                     wt_im, om_re, ep, pstate, nhomo, nlumo, e0, omega, nomega, eta, sigma, err);
   MGW_LAUNCH_CHECK();
   return MGW_OK;
 }

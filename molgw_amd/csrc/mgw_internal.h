@@ -113,10 +113,12 @@ struct GemmF64 {
   // dotpart[(rowblock * N + n) + batch * strideDot]
   //   = sum_{m in rowblock} C(m,n) (dot_cc C(m,n) + dot_cd D(m,n))
   // with D(m, n) = dotB[m + n * lddot] (batched with strideB), or
-  // sum C(m,n)^2 when dotB is null.
+  // sum w(m) C(m,n)^2 when dotB is null, w = dot_w[m + batch * strideDotW]
+  // (FP64 only) or 1.
   const double* dotB = nullptr; int64_t lddot = 0;
   double dot_cc = 0.0, dot_cd = 1.0;
   double* dotpart = nullptr; int64_t strideDot = 0;
+  const double* dot_w = nullptr; int64_t strideDotW = 0;
   // kernel class: the hot classes get their own kernel symbol so that
   // profiler per-kernel statistics separate them
   int klass = MGW_KCLASS_GEMM_OTHER;
@@ -188,8 +190,16 @@ int launch_sigma_poles(const double* w, int npole, int nse, int ni, const double
                        const double* fi, const double* ep, const double* occp, double spin_fact,
                        double eta, double* part, int accumulate, double* sigma, double* egw_p,
                        hipStream_t s);
+// sw[t + npole f] = 2 docc de / (wsign omega_f^2 + de^2): wsign = 1 for
+// imaginary frequencies i omega_f, -1 for real ones (then 1 - chi0 = 1 + X diag(sw) X^T
+// on both axes)
 int launch_chi_weights(const int32_t* ia, int npole, const double* energy, const double* occ,
-                       const double* omega, int nomega, double* sw, hipStream_t s);
+                       const double* omega, int nomega, double* sw, hipStream_t s, double wsign = 1.0);
+int launch_contour_eta(const double* lam, int64_t n, double tol, double* w, hipStream_t s);
+int launch_sigma_contour(const double* v, int np, int nse, int nimag, int nreal, const double* om_im,
+                         const double* wt_im, const double* om_re, const double* ep, const int32_t* pstate,
+                         int nhomo, int nlumo, const double* e0, const double* omega, int nomega, double eta,
+                         double* sigma, int32_t* err, hipStream_t s);
 int launch_chi_trace(const double* sw, const double* n2, int npole, const double* mat, int naux,
                      int mode, int nomega, double* out, hipStream_t s);
 int launch_dot_reduce(const double* part, int nrb, int64_t ncol, int nomega, const double* sub, double* v,

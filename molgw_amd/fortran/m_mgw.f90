@@ -21,7 +21,7 @@ module m_mgw
 
   integer(c_int), parameter, public :: MGW_MEM_HOST = 0, MGW_MEM_DEVICE = 1
   integer(c_int), parameter, public :: MGW_FP64 = 0, MGW_FP32 = 1, MGW_BF16X3 = 2
-  integer(c_int), parameter, public :: MGW_TECH_ONE_SHOT = 0, MGW_TECH_IMAGINARY_PADE = 1
+  integer(c_int), parameter, public :: MGW_TECH_ONE_SHOT = 0, MGW_TECH_IMAGINARY_PADE = 1, MGW_TECH_CONTOUR = 2
 
   type, bind(C), public :: mgw_ranges
     integer(c_int32_t) :: ncore_G = 0
@@ -31,7 +31,7 @@ module m_mgw
     integer(c_int32_t) :: nhomo_G = 0
     integer(c_int32_t) :: ncore_W = 0
     integer(c_int32_t) :: nvirtual_W = 0
-    integer(c_int32_t) :: reserved = 0
+    integer(c_int32_t) :: nlumo_G = 0        ! 0: nhomo_G + 1
   end type mgw_ranges
 
   type, bind(C), public :: mgw_g0w0_params
@@ -44,7 +44,7 @@ module m_mgw
     real(c_double)     :: step_sigma_calc = 0.03_dp
     real(c_double)     :: spin_fact = 2.0_dp
     integer(c_int32_t) :: precision = 0
-    integer(c_int32_t) :: reserved = 0
+    integer(c_int32_t) :: nomega_chi_real = 2
   end type mgw_g0w0_params
 
   public :: mgw_init, mgw_finalize, mgw_check
@@ -52,7 +52,7 @@ module m_mgw
   public :: mgw_inverse_sqrt_eri_2center, mgw_build_eri_3center
   public :: mgw_eri_3center_mo, mgw_destroy_mo_integrals
   public :: mgw_polarizability_rpa, mgw_polarizability_imag
-  public :: mgw_selfenergy_poles, mgw_selfenergy_imag, mgw_pade
+  public :: mgw_selfenergy_poles, mgw_selfenergy_imag, mgw_selfenergy_contour, mgw_pade
   public :: mgw_qp_energy, mgw_one_shot
 
   type(c_ptr), save :: ctx = c_null_ptr
@@ -217,6 +217,19 @@ module m_mgw
       complex(c_double_complex) :: sigma_calc(*)
       integer(c_int) :: ierr
     end function c_gw_selfenergy_imag
+
+    function c_gw_selfenergy_contour(handle, ranges, spin_fact, nimag, nreal, eta, nomega, omega, se_energy0, &
+                                     sigma, de_max) bind(C, name='mgw_gw_selfenergy_contour') result(ierr)
+      import :: c_int, c_ptr, c_double, mgw_ranges, c_double_complex
+      type(c_ptr), value :: handle
+      type(mgw_ranges), intent(in) :: ranges
+      real(c_double), value :: spin_fact, eta
+      integer(c_int), value :: nimag, nreal, nomega
+      real(c_double), intent(in) :: omega(*), se_energy0(*)
+      complex(c_double_complex) :: sigma(*)
+      real(c_double) :: de_max
+      integer(c_int) :: ierr
+    end function c_gw_selfenergy_contour
 
     function c_pade_fit(handle, nse, ncalc, omega_calc, sigma_calc, nomega, omega, eta, sigma) &
         bind(C, name='mgw_pade_fit') result(ierr)
@@ -452,6 +465,20 @@ subroutine mgw_selfenergy_imag(ranges, omega_calc, sigma_calc)
   call mgw_check(c_gw_selfenergy_imag(ctx, ranges, int(SIZE(omega_calc), c_int), omega_calc, sigma_calc), &
                  'mgw_selfenergy_imag')
 end subroutine mgw_selfenergy_imag
+
+! gw_selfenergy_contour (src/m_gw_selfenergy_grid.f90:342-551): se%sigma of
+! every spin channel, sigma(-nomega:nomega, nsemin:nsemax, nspin) as in se_init
+subroutine mgw_selfenergy_contour(ranges, spin_fact, nomega_chi_imag, nomega_chi_real, eta, omega, se_energy0, &
+                                  sigma)
+  type(mgw_ranges), intent(in) :: ranges
+  real(dp), intent(in)     :: spin_fact, eta, omega(:), se_energy0(:, :)
+  integer, intent(in)      :: nomega_chi_imag, nomega_chi_real
+  complex(dp), intent(out) :: sigma(:, :, :)
+  real(c_double) :: de_max
+  call mgw_check(c_gw_selfenergy_contour(ctx, ranges, spin_fact, int(nomega_chi_imag, c_int), &
+                                         int(nomega_chi_real, c_int), eta, int(SIZE(omega), c_int), omega, &
+                                         se_energy0, sigma, de_max), 'mgw_selfenergy_contour')
+end subroutine mgw_selfenergy_contour
 
 ! se_pade_fit (src/m_selfenergy_tools.f90:965-999)
 subroutine mgw_pade(omega_calc, sigma_calc, omega, eta, sigma)

@@ -72,7 +72,7 @@ class StateRanges:
 
     def to_c(self) -> Ranges:
         return Ranges(self.ncore_G, self.nvirtual_G, self.nsemin, self.nsemax, self.nhomo_G,
-                      self.ncore_W, self.nvirtual_W, 0)
+                      self.ncore_W, self.nvirtual_W, self.nlumo_G if self.nlumo_G <= self.nvirtual_G else 0)
 
     @property
     def nse(self) -> int:
@@ -467,6 +467,26 @@ class Engine:
                                                out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    # ------------------------------------------------------------ contour deformation
+    def gw_selfenergy_contour(self, rng: StateRanges, omega, se_energy0, nomega_chi_imag: int,
+                              nomega_chi_real: int = 2, eta=0.001, spin_fact=None):
+        """gw_selfenergy_contour for every channel: returns (sigma[nomega, nse]
+        (or [nomega, nse, nspin]) complex with zero imaginary part, de_max).
+        ``se_energy0`` is (nse,) or (nse, nspin)."""
+        r = rng.to_c()
+        ns = getattr(self, 'nspin', 1)
+        if spin_fact is None:
+            spin_fact = 2.0 if ns == 1 else 1.0
+        om = _f64(omega)
+        e0 = np.asfortranarray(np.asarray(se_energy0, dtype=np.float64).reshape(rng.nse, ns, order='F'))
+        shape = (len(om), rng.nse) if ns == 1 else (len(om), rng.nse, ns)
+        sig = np.zeros(shape, dtype=np.complex128, order='F')
+        de_max = C.c_double()
+        check(self._lib.mgw_gw_selfenergy_contour(self._h, C.byref(r), spin_fact, nomega_chi_imag,
+                                                  nomega_chi_real, eta, len(om), _ptr(om), _ptr(e0),
+                                                  sig.ctypes.data_as(C.POINTER(C.c_double)), C.byref(de_max)))
+        return sig, de_max.value
+
     # ------------------------------------------------------------ A8
     def pade_fit(self, omega_calc, sigma_calc, omega, eta=0.001):
         wc = np.asfortranarray(np.asarray(omega_calc, dtype=np.complex128))
@@ -499,17 +519,19 @@ class Engine:
     def g0w0(self, rng: StateRanges | None = None, technique='one_shot', nomega_sigma=51,
              step_sigma=0.01, eta=0.001, nomega_chi_imag=0, nomega_sigma_calc=1,
              step_sigma_calc=0.03, exchange_m_vxc=None, spin_fact=2.0, return_sigma=True,
-             precision='fp64') -> G0W0Output:
+             precision='fp64', nomega_chi_real=2) -> G0W0Output:
         """One-shot G0W0 (postscf='GW' for technique='one_shot', 'G0W0_PADE' for
-        technique='imaginary_pade') on the resident tensors. ``precision``
+        technique='imaginary_pade', 'G0W0_CONTOUR' for technique='contour', FP64
+        only, with nomega_chi_imag and nomega_chi_real) on the resident tensors. ``precision``
         ('fp64', 'fp32', 'bf16x3', 'bf16x6') applies to the AO->MO transforms and the
         chi0 / Casida-matrix build; the eigensolve, the (1 - chi0) inversion,
         the self-energy products and the QP solve are FP64."""
         if rng is None:
             rng = selfenergy_set_state_range(self.occupation)
-        tech = {'one_shot': _lib.MGW_TECH_ONE_SHOT, 'imaginary_pade': _lib.MGW_TECH_IMAGINARY_PADE}[technique]
+        tech = {'one_shot': _lib.MGW_TECH_ONE_SHOT, 'imaginary_pade': _lib.MGW_TECH_IMAGINARY_PADE,
+                'contour': _lib.MGW_TECH_CONTOUR}[technique]
         prm = G0W0Params(tech, nomega_sigma, step_sigma, eta, nomega_chi_imag, nomega_sigma_calc,
-                         step_sigma_calc, spin_fact, PRECISION[precision], 0)
+                         step_sigma_calc, spin_fact, PRECISION[precision], nomega_chi_real)
         r = rng.to_c()
         n, ns = self.nstate, getattr(self, 'nspin', 1)
         nom = nomega_sigma // 2
@@ -523,7 +545,7 @@ class Engine:
                                  _ptr(eqp), _ptr(zz), _ptr(elin),
                                  sig.ctypes.data_as(C.POINTER(C.c_double)) if return_sigma else None,
                                  _ptr(en)))
-        if technique == 'one_shot':
+        if technique in ('one_shot', 'contour'):
             omega = step_sigma * np.arange(-nom, nom + 1)
         else:
             e2 = self.energy.reshape(n, -1)

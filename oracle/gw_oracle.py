@@ -32,6 +32,14 @@ lives in ``tests/test_oracle.py``:
   * ``pade`` -- exact on rational functions
   * ``find_fixed_point`` / ``find_qp_energy_*`` -- hand-built piecewise-linear
     cases with known roots, Z factors and fallbacks
+  * ``vsqrt_chi_vsqrt_low_rank`` -- U diag(lambda/(1-lambda)) U^T equals the
+    pinned polarizability_grid chi (imaginary axis) and a dense solve of
+    (1 - chi0) chi = chi0 (real axis); the kept count equals the number of
+    eigenvalues of that chi above the threshold
+  * ``gw_selfenergy_contour`` -- with W at exactly |de| and nothing dropped,
+    equals Re Sigma_c of the pinned pole self-energy (eta -> 0); on a grid
+    holding every |de| the nearest-point rule is exact; out-of-grid error;
+    closed shell as two channels equals one channel
 """
 from __future__ import annotations
 
@@ -874,4 +882,194 @@ def g0w0_spin(eri_3center, index_basis, c_matrix, energy, occupation, technique:
         e_qp, zz = find_qp_energy_graphical(sigma[:, :, s], omega, se_energy0, exc[:, s], energy[:, s], rng)
         out['energy_qp'][:, s], out['zz'][:, s], out['energy_qp_lin'][:, s] = e_qp, zz, e_lin
     out.update(sigma=sigma, omega=omega, en_rpa=en_rpa, en_gw=en_gw)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Contour deformation (postscf='G0W0_CONTOUR'): low-rank W on the imaginary
+# and the real axis, imaginary-axis integral plus residues
+# ---------------------------------------------------------------------------
+TOL_LOW_EIGVAL_CHI = 1.0e-3      # src/m_spectral_function.f90:775
+
+
+def vsqrt_chi_vsqrt_low_rank(x: np.ndarray, docc: np.ndarray, de: np.ndarray, omega2: np.ndarray,
+                             tol: float = TOL_LOW_EIGVAL_CHI):
+    """``sf_vsqrt_chi_vsqrt_rpa`` with low_rank (src/m_spectral_function.f90:768-949)
+    on X(P, t) = (P|a_t i_t).
+
+    Per frequency, with ``omega2[k]`` = omega^2 (-w^2 on the imaginary axis,
+    w^2 on the real axis):
+      chi0 = X diag(2 docc de / (omega^2 - de^2)) X^T          (:857-877)
+      lambda, U = eig(chi0); eta = lambda / (1 - lambda)        (:892-895)
+      keep the eigenpairs with |eta| > tol                      (:896-913)
+    Returns a list of (eigvec[naux, neig], eta[neig])."""
+    out = []
+    for w2 in np.asarray(omega2, dtype=np.float64):
+        factor = 2.0 * docc * de / (w2 - de ** 2)
+        chi0 = (x * factor[None, :]) @ x.T
+        lam, u = np.linalg.eigh(chi0)
+        eta = lam / (1.0 - lam)
+        keep = np.abs(eta) > tol
+        out.append((u[:, keep], eta[keep]))
+    return out
+
+
+def contour_de_max(energy: np.ndarray, se_energy0: np.ndarray, omega: np.ndarray, rng: StateRanges,
+                   eta: float) -> float:
+    """Largest real frequency the residues need (src/m_gw_selfenergy_grid.f90:385-408):
+    over the channels, the states m of [nsemin, nsemax] and the grid omega_k,
+    e_p - (E0_m + omega_k) for occupied p in [ncore_G+1, nhomo_G] and
+    E0_m + omega_k - e_p for empty p in [nlumo_G, nvirtual_G-1], counting only
+    values >= -eta; 0 if there are none. ``energy`` (nstate, nspin),
+    ``se_energy0`` (nse, nspin)."""
+    de_max = 0.0
+    nse = rng.nsemax - rng.nsemin + 1
+    for s in range(energy.shape[1]):
+        for m in range(nse):
+            w = se_energy0[m, s] + omega
+            for p in range(rng.ncore_G + 1, rng.nhomo_G + 1):
+                de = energy[p - 1, s] - w
+                de = de[de >= -eta]
+                if de.size:
+                    de_max = max(de_max, float(de.max()))
+            for p in range(rng.nlumo_G, rng.nvirtual_G):
+                de = w - energy[p - 1, s]
+                de = de[de >= -eta]
+                if de.size:
+                    de_max = max(de_max, float(de.max()))
+    return de_max
+
+
+def real_linear_grid(nomega: int, omega_max: float) -> np.ndarray:
+    """REAL_LINEAR grid of sf_init (src/m_spectral_function.f90:308-315):
+    omega_j = (j - 1) / (n - 1) omega_max, j = 1..n."""
+    if nomega < 2:
+        raise ValueError('nomega_chi_real must be at least 2')
+    return np.arange(nomega, dtype=np.float64) / (nomega - 1) * omega_max
+
+
+def _v_chi_v(w, b):
+    u, eta = w
+    if eta.size == 0:
+        return np.zeros(b.shape[1])
+    t = u.T @ b
+    return np.sum(t ** 2 * eta[:, None], axis=0)
+
+
+def gw_selfenergy_contour(eri_pm: np.ndarray, w_imag, omega_im: np.ndarray, weight: np.ndarray, w_real,
+                          omega_re: np.ndarray, energy: np.ndarray, rng: StateRanges, omega: np.ndarray,
+                          se_energy0: np.ndarray, eta: float, w_real_at=None) -> np.ndarray:
+    """The self-energy loop of ``gw_selfenergy_contour``
+    (src/m_gw_selfenergy_grid.f90:429-530) for one channel.
+
+    ``eri_pm[P, p - (ncore_G+1), m - nsemin]``; ``w_imag`` / ``w_real`` the
+    low-rank W of ``vsqrt_chi_vsqrt_low_rank`` on the imaginary grid
+    (``omega_im``, ``weight``) and on the real grid ``omega_re``; ``energy``
+    this channel's energies (nstate,); ``se_energy0`` (nse,). For
+    x = E0_m + omega_k - e_p and v = B_pm^T W B_pm:
+      imaginary axis (:436-469), where |x| > eta:
+        sigma -= weight / (2 pi) [1/(x + i w) + 1/(x - i w)] v(iw)
+      occupied p (p < nlumo_G), de = e_p - (E0_m + omega_k) >= -eta (:475-500):
+        sigma -= v(|de|) (x 1/2 when |de| < eta)
+      empty p (p > nhomo_G), de = E0_m + omega_k - e_p >= -eta (:505-530):
+        sigma += v(|de|) (x 1/2 when |de| < eta)
+    v(|de|) takes the real-grid point nearest to |de|, the first one on a tie
+    (sf_interpolate_vsqrt_chi_vsqrt, src/m_spectral_function.f90:953-981);
+    a |de| outside the grid is an error, as there. ``w_real_at(w)``, when
+    given, returns the low-rank W at exactly w instead (tests only).
+    Returns sigma[iomega, m] complex (zero imaginary part)."""
+    nse = rng.nsemax - rng.nsemin + 1
+    p_idx = np.arange(rng.ncore_G, rng.nvirtual_G - 1)
+    sigma = np.zeros((len(omega), nse), dtype=np.complex128)
+    lo, hi = float(np.min(omega_re)), float(np.max(omega_re))
+
+    def residue(b_col, w):
+        if w_real_at is not None:
+            return float(_v_chi_v(w_real_at(w), b_col[:, None])[0])
+        if w > hi or w < lo:
+            raise ValueError(f'requested frequency {w} out of range')
+        j = int(np.argmin(np.abs(omega_re - w)))
+        return float(_v_chi_v(w_real[j], b_col[:, None])[0])
+
+    for mm in range(nse):
+        b = np.ascontiguousarray(eri_pm[:, :, mm])
+        x = se_energy0[mm] + omega[:, None] - energy[p_idx][None, :]          # (nomega, np)
+        for iw in range(len(omega_im)):
+            if w_imag[iw][1].size == 0:
+                continue
+            v = _v_chi_v(w_imag[iw], b)
+            g = 1.0 / (x + 1j * omega_im[iw]) + 1.0 / (x - 1j * omega_im[iw])
+            g[np.abs(x) <= eta] = 0.0
+            sigma[:, mm] -= weight[iw] * (g @ v) / (2.0 * PI)
+        for pl, p in enumerate(p_idx):
+            if p + 1 < rng.nlumo_G:
+                for k in range(len(omega)):
+                    de = energy[p] - (se_energy0[mm] + omega[k])
+                    if de < -eta:
+                        continue
+                    factor = 0.5 if abs(de) < eta else 1.0
+                    sigma[k, mm] -= residue(b[:, pl], abs(de)) * factor
+            if p + 1 > rng.nhomo_G:
+                for k in range(len(omega)):
+                    de = se_energy0[mm] + omega[k] - energy[p]
+                    if de < -eta:
+                        continue
+                    factor = 0.5 if abs(de) < eta else 1.0
+                    sigma[k, mm] += residue(b[:, pl], abs(de)) * factor
+    return sigma
+
+
+def g0w0_contour(eri_3center, index_basis, c_matrix, energy, occupation, nomega_chi_imag: int,
+                 nomega_chi_real: int, exchange_m_vxc=None, nomega_sigma: int = 51, step_sigma: float = 0.01,
+                 eta: float = 0.001, spin_fact: float | None = None, ranges: StateRanges | None = None,
+                 tol: float = TOL_LOW_EIGVAL_CHI):
+    """One-shot G0W0 by contour deformation (postscf='G0W0_CONTOUR'):
+    src/m_selfenergy_evaluation.f90:218-240 with se_init(contour_deformation)
+    (src/m_selfenergy_tools.f90:658-665, :679-680: omega_k = step k around
+    E0 = the input energies) and gw_selfenergy_contour
+    (src/m_gw_selfenergy_grid.f90:342-551): W from the transitions of every
+    channel on both grids, the self-energy and the graphical QP solve per
+    channel. ``energy``/``occupation`` (nstate,) or (nstate, nspin),
+    ``c_matrix`` (nbf, nstate) or (nbf, nstate, nspin). Returns a dict with
+    energy_qp, zz, energy_qp_lin (nstate, nspin), sigma (nomega, nse, nspin),
+    omega, omega_re (the real grid) and de_max."""
+    energy = np.asarray(energy, dtype=np.float64)
+    occupation = np.asarray(occupation, dtype=np.float64)
+    c_matrix = np.asarray(c_matrix, dtype=np.float64)
+    if energy.ndim == 1:
+        energy, occupation, c_matrix = energy[:, None], occupation[:, None], c_matrix[:, :, None]
+    nstate, nspin = energy.shape
+    if spin_fact is None:
+        spin_fact = 2.0 if nspin == 1 else 1.0
+    exc = np.zeros((nstate, nspin)) if exchange_m_vxc is None else np.asarray(exchange_m_vxc, dtype=np.float64)
+    if exc.ndim == 1:
+        exc = exc[:, None]
+    rng = ranges or selfenergy_set_state_range(occupation)
+    nom = nomega_sigma // 2
+    omega = step_sigma * np.arange(-nom, nom + 1, dtype=np.float64)
+    nse = rng.nsemax - rng.nsemin + 1
+    se_energy0 = energy[rng.nsemin - 1:rng.nsemax, :]
+    ttable = transition_table(occupation, spin_fact, rng.ncore_W, rng.nvirtual_W)
+    wlo, whi = rng.ncore_W + 1, rng.nvirtual_W - 1
+    x = transition_columns(eri_3center, index_basis, c_matrix, ttable, wlo, whi)
+    i, a, sp = ttable[:, 0] - 1, ttable[:, 1] - 1, ttable[:, 2] - 1
+    docc = occupation[i, sp] - occupation[a, sp]
+    de = energy[a, sp] - energy[i, sp]
+    omega_im, weight = imaginary_quad_grid(nomega_chi_imag)
+    w_imag = vsqrt_chi_vsqrt_low_rank(x, docc, de, -omega_im ** 2, tol)
+    de_max = contour_de_max(energy, se_energy0, omega, rng, eta)
+    omega_re = real_linear_grid(nomega_chi_real, de_max)
+    w_real = vsqrt_chi_vsqrt_low_rank(x, docc, de, omega_re ** 2, tol)
+    out = {k: np.zeros((nstate, nspin)) for k in ('energy_qp', 'zz', 'energy_qp_lin')}
+    sigma = np.zeros((len(omega), nse, nspin), dtype=np.complex128)
+    for s in range(nspin):
+        eri_pm = calculate_eri_3center_mo(eri_3center, index_basis, c_matrix, rng.ncore_G + 1,
+                                          rng.nvirtual_G - 1, rng.nsemin, rng.nsemax, spin=s)
+        sigma[:, :, s] = gw_selfenergy_contour(eri_pm, w_imag, omega_im, weight, w_real, omega_re, energy[:, s],
+                                               rng, omega, se_energy0[:, s], eta)
+        e_lin, _ = find_qp_energy_linearization(sigma[:, :, s], omega, se_energy0[:, s], exc[:, s],
+                                                energy[:, s], rng)
+        e_qp, zz = find_qp_energy_graphical(sigma[:, :, s], omega, se_energy0[:, s], exc[:, s], energy[:, s], rng)
+        out['energy_qp'][:, s], out['zz'][:, s], out['energy_qp_lin'][:, s] = e_qp, zz, e_lin
+    out.update(sigma=sigma, omega=omega, omega_re=omega_re, de_max=de_max)
     return out
