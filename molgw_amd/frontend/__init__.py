@@ -1,0 +1,7 @@
+"""Native integral and RI-HF front end: basis sets in the reference's file
+format, Gaussian integrals, and the closed-shell RI-HF that produces the
+inputs of the G0W0 path."""
+from .basis import BasisSet, read_basis_file
+from .scf import Molecule, RIHFResult, run_rihf
+
+__all__ = ['BasisSet', 'read_basis_file', 'Molecule', 'RIHFResult', 'run_rihf']

@@ -1,0 +1,155 @@
+"""RI Hartree-Fock front end: the molecule, its integrals, the SCF, and the
+inputs of the G0W0 path (packed eri_3center_tmp, the 2-center metric, C,
+energies, occupations).
+
+Follows the reference's RI-HF flow (src/m_scf_loop.f90; Hartree and exchange
+through the RI tensor, src/m_hamiltonian_twobodies.f90): the orbital basis
+is filtered by the overlap eigenvalues (setup_x_matrix,
+src/m_hamiltonian_tools.f90:1171-1219, min_overlap 1e-5), the metric by
+eigenvalues above 1e-6 (src/m_eri_calculate.f90:1049-1262), and the pair
+tensor carries the 1/2 on diagonal pairs (src/m_eri_calculate.f90:1840-1841).
+The SCF itself is a textbook closed-shell Roothaan-Hall iteration with DIIS
+from the core-Hamiltonian guess; at convergence its energy and orbitals are
+those of any converged RI-HF run.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import integrals as I
+from .basis import ANGSTROM_BOHR, BasisSet, atomic_number
+from ..workloads import upper_pairs
+
+MIN_OVERLAP = 1.0e-5        # src/input_variables.yaml:1804-1811
+TOO_LOW_EIGENVAL = 1.0e-6   # src/m_eri_calculate.f90 (metric filter)
+
+
+@dataclass
+class Molecule:
+    atoms: list          # atomic numbers
+    coords: np.ndarray   # bohr, (natom, 3)
+
+    @classmethod
+    def from_lines(cls, lines, unit: str = 'angstrom'):
+        """Atoms as in a MOLGW input: ``symbol-or-Z x y z`` per line."""
+        atoms, xyz = [], []
+        for ln in lines:
+            f = ln.split()
+            if not f:
+                continue
+            atoms.append(atomic_number(f[0]))
+            xyz.append([float(v) for v in f[1:4]])
+        xyz = np.array(xyz, dtype=np.float64)
+        if unit.lower().startswith('ang'):
+            xyz = xyz * ANGSTROM_BOHR
+        return cls(atoms, xyz)
+
+    @property
+    def nelectron(self) -> int:
+        return int(sum(self.atoms))
+
+
+@dataclass
+class RIHFResult:
+    energy: float
+    components: dict
+    c_matrix: np.ndarray        # (nbf, nstate)
+    energy_mo: np.ndarray       # (nstate,)
+    occupation: np.ndarray      # (nstate,)
+    eri_3center_tmp: np.ndarray  # packed (npair, naux_bas), diagonal pairs x1/2, before V^-1/2
+    eri_2center: np.ndarray     # (naux_bas, naux_bas)
+    index_basis: np.ndarray     # (2, npair), 1-based
+    nbf: int
+    iterations: int
+
+
+def _pack(eri3: np.ndarray, index_basis: np.ndarray) -> np.ndarray:
+    k, l = index_basis[0] - 1, index_basis[1] - 1
+    out = eri3[k, l, :].copy()
+    out[k == l] *= 0.5
+    return out
+
+
+def run_rihf(mol: Molecule, basis: str, auxil_basis: str, basis_dir: str, gaussian_type: str = 'PURE',
+             tol: float = 1e-10, max_iter: int = 100, diis: int = 8) -> RIHFResult:
+    """Closed-shell RI-HF. Returns the converged energy, its parts (kinetic,
+    nucleus, Hartree, exchange, nucleus-nucleus) and the G0W0 inputs."""
+    if mol.nelectron % 2:
+        raise ValueError('closed shell only')
+    bs = BasisSet(mol.atoms, mol.coords, basis, basis_dir, gaussian_type)
+    aux = BasisSet(mol.atoms, mol.coords, auxil_basis, basis_dir, gaussian_type)
+    charges = [float(z) for z in mol.atoms]
+    s, t, v = I.one_electron(bs, charges, mol.coords)
+    m2 = I.eri_2center(aux)
+    e3 = I.eri_3center(bs, aux)
+    enuc = I.nuclear_repulsion(charges, mol.coords)
+    # metric V^-1/2 over the kept eigenvalues, and B(mu nu, P)
+    w, u = np.linalg.eigh(m2)
+    keep = w > TOO_LOW_EIGENVAL
+    sqrtinv = u[:, keep] / np.sqrt(w[keep])
+    b = e3 @ sqrtinv                                            # (n, n, naux_kept)
+    # orthogonalisation X = U s^-1/2 over the kept overlap eigenvalues
+    sw, su = np.linalg.eigh(s)
+    xk = sw > MIN_OVERLAP
+    x = su[:, xk] / np.sqrt(sw[xk])
+    nstate = x.shape[1]
+    nocc = mol.nelectron // 2
+    h = t + v
+
+    def fock(d):
+        j = np.einsum('mnP,P->mn', b, np.einsum('lsP,ls->P', b, d), optimize=True)
+        bd = np.einsum('mlP,ls->msP', b, d, optimize=True)
+        k = np.einsum('msP,snP->mn', bd, b, optimize=True)
+        return h + j - 0.5 * k, j, k
+
+    def solve(f):
+        e, cp = np.linalg.eigh(x.T @ f @ x)
+        return e, x @ cp
+
+    eps, c = solve(h)
+    d = 2.0 * c[:, :nocc] @ c[:, :nocc].T
+    e_old = 0.0
+    fs, errs = [], []
+    it = 0
+    for it in range(1, max_iter + 1):
+        f, j, k = fock(d)
+        energy = 0.5 * np.sum(d * (h + f)) + enuc
+        err = x.T @ (f @ d @ s - s @ d @ f) @ x
+        fs.append(f)
+        errs.append(err)
+        if len(fs) > diis:
+            fs.pop(0)
+            errs.pop(0)
+        if len(fs) > 1:
+            nd = len(fs)
+            bm = -np.ones((nd + 1, nd + 1))
+            bm[nd, nd] = 0.0
+            for a_ in range(nd):
+                for b_ in range(nd):
+                    bm[a_, b_] = np.sum(errs[a_] * errs[b_])
+            rhs = np.zeros(nd + 1)
+            rhs[nd] = -1.0
+            coef = np.linalg.solve(bm, rhs)[:nd]
+            f_use = sum(cf * ff for cf, ff in zip(coef, fs))
+        else:
+            f_use = f
+        eps, c = solve(f_use)
+        d = 2.0 * c[:, :nocc] @ c[:, :nocc].T
+        if abs(energy - e_old) < tol and np.abs(err).max() < 1e-7:
+            break
+        e_old = energy
+    # final orbitals from the converged Fock matrix itself
+    f, j, k = fock(d)
+    eps, c = solve(f)
+    d = 2.0 * c[:, :nocc] @ c[:, :nocc].T
+    f, j, k = fock(d)
+    energy = 0.5 * np.sum(d * (h + f)) + enuc
+    comps = {'kinetic': float(np.sum(d * t)), 'nucleus': float(np.sum(d * v)),
+             'hartree': float(0.5 * np.sum(d * j)), 'exchange': float(-0.25 * np.sum(d * k)),
+             'nucleus_nucleus': enuc}
+    occ = np.zeros(nstate)
+    occ[:nocc] = 2.0
+    ib = upper_pairs(bs.nbf)
+    return RIHFResult(float(energy), comps, c, eps, occ, _pack(e3, ib), m2, ib, bs.nbf, it)

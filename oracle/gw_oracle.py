@@ -22,6 +22,9 @@ lives in ``tests/test_oracle.py``:
     of the built tensor equal those of B
   * ``coeffs_gausslegint``  -- numpy ``leggauss``; integral of 2a/(w^2+a^2)
   * ``calculate_eri_3center_mo`` -- brute-force dense einsum; CC4S (11|11)
+  * ``g0w0_casida`` end to end -- the reference's lih_gw.in golden (G0W0@HF
+    HOMO-LUMO gap of LiH, cc-pVQZ, 8.053158 eV) on the integrals and RI-HF
+    orbitals of molgw_amd.frontend (tests/test_frontend.py)
   * ``polarizability_casida`` -- lih_bulk08 RPA correlation golden; closed
     form for one transition; Tr[Sigma_c G] = E_GM identity
   * ``polarizability_grid`` -- chi(iw) from Casida poles (analytic identity);
