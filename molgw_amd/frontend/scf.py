@@ -51,6 +51,26 @@ class Molecule:
         return int(sum(self.atoms))
 
 
+def frozen_core_states(atoms, frozencore: str = 'MOLGW') -> int:
+    """ncore_G = ncore_W of the frozencore keyword (src/m_selfenergy_tools.f90:82-87,
+    src/m_spectral_function.f90:186-191): the sum over the atoms of
+    element_core ('MOLGW'/'YES': none up to Be, 1 up to Mg, 5 up to Zn, 9 up
+    to Cd, else 18) or element_core_gaussian ('GAUSSIAN': none up to He, 1 up
+    to Ne, then as MOLGW) of src/m_elements.f90:40-100 (all-electron atoms);
+    0 for 'NO'."""
+    rule = frozencore.strip().upper()
+    if rule in ('NO', ''):
+        return 0
+    if rule not in ('MOLGW', 'YES', 'GAUSSIAN'):
+        raise ValueError(f'unknown frozencore {frozencore!r}')
+    first = 2 if rule == 'GAUSSIAN' else 4
+    n = 0
+    for z in atoms:
+        n += 0 if z <= first else 1 if z <= (10 if rule == 'GAUSSIAN' else 12) else 5 if z <= 30 else \
+            9 if z <= 48 else 18
+    return n
+
+
 @dataclass
 class RIHFResult:
     energy: float

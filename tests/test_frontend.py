@@ -9,6 +9,9 @@ Goldens (Ha unless noted, tolerance as in the testsuite):
   lih_bse-i.in    LiH (3 bohr), HF cc-pVQZ (Cartesian) / cc-pVQZ-RI: SCF -7.9872035011
   lih_gw.in       the same LiH, one-shot G0W0@HF: HOMO-LUMO gap 8.053158 eV (1e-4),
                   checked through the oracle's G0W0 on the front end's integrals
+  lih_gw_frozencore_molgw.in / _gaussian.in  the same with frozencore 'molgw'
+                  (no Li core frozen) / 'gaussian' (Li 1s frozen in G and W),
+                  selfenergy_state_range 1: GW HOMO -8.270983 / -8.265769 eV
 """
 import math
 import os
@@ -17,7 +20,7 @@ import numpy as np
 import pytest
 from scipy.integrate import quad
 
-from molgw_amd.frontend import Molecule, run_rihf
+from molgw_amd.frontend import Molecule, frozen_core_states, run_rihf
 from molgw_amd.frontend import integrals as I
 from molgw_amd.frontend.basis import BasisSet, Shell
 
@@ -111,3 +114,22 @@ def test_g0w0_gap_of_lih_through_the_oracle(lih_qz):
     res = O.g0w0_casida(b, lih_qz.index_basis, lih_qz.c_matrix, lih_qz.energy_mo, lih_qz.occupation)
     gap = (res.energy_qp[2] - res.energy_qp[1]) * HA_EV
     assert abs(gap - 8.053158) < 1e-4
+
+
+@pytest.mark.parametrize('rule,homo', [('molgw', -8.270983), ('gaussian', -8.265769)])
+def test_g0w0_frozen_core_goldens_of_lih_through_the_oracle(lih_qz, rule, homo):
+    from oracle import gw_oracle as O
+    nc = frozen_core_states([3, 1], rule)
+    assert nc == (0 if rule == 'molgw' else 1)
+    sq, _ = O.inverse_sqrt_eri_2center(lih_qz.eri_2center)
+    b = lih_qz.eri_3center_tmp @ sq
+    rng = O.selfenergy_set_state_range(lih_qz.occupation[:, None], ncoreg=nc, ncorew=nc, selfenergy_state_range=1)
+    res = O.g0w0_casida(b, lih_qz.index_basis, lih_qz.c_matrix, lih_qz.energy_mo, lih_qz.occupation, ranges=rng)
+    assert abs(res.energy_qp[1] * HA_EV - homo) < 1e-4
+
+
+def test_frozen_core_rules():
+    assert frozen_core_states([8, 1, 1], 'molgw') == 1
+    assert frozen_core_states([8, 1, 1], 'gaussian') == 1
+    assert frozen_core_states([3, 1], 'molgw') == 0 and frozen_core_states([3, 1], 'gaussian') == 1
+    assert frozen_core_states([14, 6], 'molgw') == 6 and frozen_core_states([14, 6], 'no') == 0

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import gw_oracle as O
-from molgw_amd.frontend import Molecule, run_rihf
+from molgw_amd.frontend import Molecule, frozen_core_states, run_rihf
 from molgw_amd.gw import selfenergy_set_state_range
 
 pytestmark = pytest.mark.gpu
@@ -68,3 +68,16 @@ def test_lih_imaginary_and_contour_match_oracle(engine, lih):
           f'{(cont.energy_qp[2] - cont.energy_qp[1]) * HA_EV:.6f} eV; vs oracle {dp:.2e} / {dc:.2e} Ha')
     assert dp < 1e-8
     assert dc < 1e-8
+
+
+@pytest.mark.parametrize('rule,homo', [('molgw', -8.270983), ('gaussian', -8.265769)])
+def test_lih_frozen_core_goldens(engine, lih, rule, homo):
+    """lih_gw_frozencore_molgw.in / lih_gw_frozencore_gaussian.in: the GW HOMO
+    with the frozen-core ranges of each rule, selfenergy_state_range 1."""
+    r, b = lih
+    _load(engine, r)
+    nc = frozen_core_states([3, 1], rule)
+    rng = selfenergy_set_state_range(r.occupation, ncoreg=nc, ncorew=nc, selfenergy_state_range=1)
+    out = engine.g0w0(rng, technique='one_shot')
+    print(f'  frozencore {rule}: GW HOMO {out.energy_qp[1] * HA_EV:.6f} eV (golden {homo})')
+    assert abs(out.energy_qp[1] * HA_EV - homo) < 1e-4
