@@ -112,7 +112,21 @@ def stream_bench(workload, args):
         live = srv.run(requests(args.precision), realtime=True) if realtime else res
         torch.cuda.synchronize()
         esc = srv.escalations()
+    # the same requests one at a time, after the timed passes: each kernel
+    # class alone on the device, so its HIP-event time is its kernels' time
+    # (as a kernel trace of the pass shows them)
+    with Server(nslots=1, hbm_budget=budget) as srv:
+        srv.profile(True)
+        torch.cuda.synchronize()
+        i0 = time.clock_gettime_ns(time.CLOCK_MONOTONIC)
+        srv.run(requests(args.precision), realtime=False)
+        torch.cuda.synchronize()
+        i1 = time.clock_gettime_ns(time.CLOCK_MONOTONIC)
+        print(f'# isolated window ns: monotonic {i0} {i1}', file=sys.stderr, flush=True)
+        prof_iso = srv.profile_read()
+        srv.profile(False)
     sm = summarize(res)
+    sm['profile_isolated'] = prof_iso
     lv = summarize(live)
     sm['profile'] = prof
     sm['escalations'] = esc
@@ -548,6 +562,9 @@ def main():
                                    'p99_ms': ref['p99_ms']},
             'speedup_vs_fp64_one_at_a_time': sm['solves_per_s'] / ref['solves_per_s'],
             'roofline': roofline_of(sm['profile'], prec, f'{args.workload}/{prec}', by='flops')[1],
+            'roofline_isolated': dict(roofline_of(sm['profile_isolated'], prec, f'{args.workload}/{prec}',
+                                                  by='flops')[1],
+                                      **{'pass': 'the same requests one at a time after the timed passes'}),
             'cpu_baseline': sm['cpu_baseline'],
             'stages_device_ms_summed_over_slots': {k: v['ms'] for k, v in sm['profile'].items()},
             'fp64_rebuilt_frequencies': sm['escalations'],
