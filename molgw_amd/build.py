@@ -58,8 +58,22 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print('built', LIB)
+    build_ints(verbose)
     build_fortran(verbose)
     return LIB
+
+
+INTS_SRC = os.path.join(HERE, 'frontend', 'ints.cpp')
+INTS_LIB = os.path.join(HERE, 'frontend', 'libmgw_ints.so')
+
+
+def build_ints(verbose: bool = False) -> str:
+    """The front end's native integral engine (host C++, OpenMP)."""
+    _run(['g++', '-O3', '-fopenmp', '-std=c++17', '-shared', '-fPIC', '-Wall', INTS_SRC, '-o', INTS_LIB + '.tmp'])
+    os.replace(INTS_LIB + '.tmp', INTS_LIB)
+    if verbose:
+        print('built', INTS_LIB)
+    return INTS_LIB
 
 
 FLANG = os.path.join(ROCM, 'lib', 'llvm', 'bin', 'amdflang')

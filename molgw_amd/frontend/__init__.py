@@ -2,6 +2,6 @@
 format, Gaussian integrals, and the closed-shell RI-HF that produces the
 inputs of the G0W0 path."""
 from .basis import BasisSet, read_basis_file
-from .scf import Molecule, RIHFResult, frozen_core_states, run_rihf
+from .scf import Molecule, RIHFResult, alkane, frozen_core_states, run_rihf
 
-__all__ = ['BasisSet', 'read_basis_file', 'Molecule', 'RIHFResult', 'frozen_core_states', 'run_rihf']
+__all__ = ['BasisSet', 'read_basis_file', 'Molecule', 'RIHFResult', 'alkane', 'frozen_core_states', 'run_rihf']

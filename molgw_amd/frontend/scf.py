@@ -14,6 +14,7 @@ those of any converged RI-HF run.
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -93,17 +94,25 @@ def _pack(eri3: np.ndarray, index_basis: np.ndarray) -> np.ndarray:
 
 
 def run_rihf(mol: Molecule, basis: str, auxil_basis: str, basis_dir: str, gaussian_type: str = 'PURE',
-             tol: float = 1e-10, max_iter: int = 100, diis: int = 8) -> RIHFResult:
+             tol: float = 1e-10, max_iter: int = 100, diis: int = 8, engine: str = 'native') -> RIHFResult:
     """Closed-shell RI-HF. Returns the converged energy, its parts (kinetic,
-    nucleus, Hartree, exchange, nucleus-nucleus) and the G0W0 inputs."""
+    nucleus, Hartree, exchange, nucleus-nucleus) and the G0W0 inputs.
+    ``engine``: 'native' (ints.cpp, OpenMP; the default) or 'numpy' (the
+    integrals module, the same algorithm, which pins the native one)."""
     if mol.nelectron % 2:
         raise ValueError('closed shell only')
     bs = BasisSet(mol.atoms, mol.coords, basis, basis_dir, gaussian_type)
     aux = BasisSet(mol.atoms, mol.coords, auxil_basis, basis_dir, gaussian_type)
     charges = [float(z) for z in mol.atoms]
-    s, t, v = I.one_electron(bs, charges, mol.coords)
-    m2 = I.eri_2center(aux)
-    e3 = I.eri_3center(bs, aux)
+    if engine == 'native':
+        from . import native as ints
+    elif engine == 'numpy':
+        ints = I
+    else:
+        raise ValueError(f'unknown integral engine {engine!r}')
+    s, t, v = ints.one_electron(bs, charges, mol.coords)
+    m2 = ints.eri_2center(aux)
+    e3 = ints.eri_3center(bs, aux)
     enuc = I.nuclear_repulsion(charges, mol.coords)
     # metric V^-1/2 over the kept eigenvalues, and B(mu nu, P)
     w, u = np.linalg.eigh(m2)
@@ -173,3 +182,34 @@ def run_rihf(mol: Molecule, basis: str, auxil_basis: str, basis_dir: str, gaussi
     occ[:nocc] = 2.0
     ib = upper_pairs(bs.nbf)
     return RIHFResult(float(energy), comps, c, eps, occ, _pack(e3, ib), m2, ib, bs.nbf, it)
+
+
+def alkane(n: int, cc: float = 1.54, ch: float = 1.09) -> Molecule:
+    """All-trans n-alkane CnH2n+2 from bond rules: C-C cc, C-H ch (Angstrom),
+    tetrahedral angles, the carbon zigzag in the xy plane."""
+    th = math.acos(-1.0 / 3.0)                    # tetrahedral angle
+    dx, dy = cc * math.sin(th / 2), cc * math.cos(th / 2)
+    cs = [np.array([k * dx, dy * (k % 2), 0.0]) for k in range(n)]
+    atoms, xyz = [6] * n, list(cs)
+    ez = np.array([0.0, 0.0, 1.0])
+    for k, c in enumerate(cs):
+        nb = [cs[j] for j in (k - 1, k + 1) if 0 <= j < n]
+        us = [(b - c) / np.linalg.norm(b - c) for b in nb]
+        if len(us) == 2:
+            bis = -(us[0] + us[1])
+            bis /= np.linalg.norm(bis)
+            dirs = [bis * math.cos(th / 2) + s * ez * math.sin(th / 2) for s in (1, -1)]
+        else:
+            u = us[0] if us else np.array([1.0, 0.0, 0.0])
+            e1 = np.cross(ez, u)
+            e1 /= np.linalg.norm(e1)
+            if len(cs) > 1 and np.dot(e1, np.array([0.0, 1.0, 0.0])) * (1 if k % 2 else -1) > 0:
+                e1 = -e1
+            dirs = [-u / 3.0 + math.sqrt(8.0) / 3.0 * (math.cos(p) * e1 + math.sin(p) * ez)
+                    for p in (0.0, 2 * math.pi / 3, 4 * math.pi / 3)]
+            if not us:                               # methane: a fourth hydrogen
+                dirs.append(u)
+        for d in dirs:
+            atoms.append(1)
+            xyz.append(c + ch * d)
+    return Molecule(atoms, np.array(xyz) * ANGSTROM_BOHR)

@@ -133,3 +133,29 @@ def test_frozen_core_rules():
     assert frozen_core_states([8, 1, 1], 'gaussian') == 1
     assert frozen_core_states([3, 1], 'molgw') == 0 and frozen_core_states([3, 1], 'gaussian') == 1
     assert frozen_core_states([14, 6], 'molgw') == 6 and frozen_core_states([14, 6], 'no') == 0
+
+
+def test_native_engine_equals_the_numpy_engine():
+    """The native integral engine (ints.cpp) against the numpy one that the
+    goldens above pin: Cartesian and pure, up to h functions (LiH cc-pVQZ-RI)."""
+    from molgw_amd.frontend import native as N
+    cases = [(['8 .0 .0 .493003', '6 .0 .0 -.657337'], 'angstrom', 'cc-pVDZ_gaussian', 'cc-pVDZ-RI', 'CART'),
+             (['O 0.0 0.0 0.1', 'H 0.0 0.76 -0.5', 'H 0.0 -0.76 -0.5'], 'angstrom', 'cc-pVDZ', 'cc-pVDZ-RI', 'PURE'),
+             (['Li 0 0 1.5', 'H 0 0 -1.5'], 'bohr', 'cc-pVQZ', 'cc-pVQZ-RI', 'CART')]
+    for lines, unit, bas, auxb, gt in cases:
+        m = Molecule.from_lines(lines, unit=unit)
+        bs = BasisSet(m.atoms, m.coords, bas, BASIS, gt)
+        ax = BasisSet(m.atoms, m.coords, auxb, BASIS, gt)
+        q = [float(z) for z in m.atoms]
+        for x, y in zip(N.one_electron(bs, q, m.coords), I.one_electron(bs, q, m.coords)):
+            assert np.abs(x - y).max() < 1e-13 * np.abs(y).max()
+        e3 = I.eri_3center(bs, ax)
+        assert np.abs(N.eri_3center(bs, ax) - e3).max() < 1e-13 * np.abs(e3).max()
+        m2 = I.eri_2center(ax)
+        assert np.abs(N.eri_2center(ax) - m2).max() < 1e-13 * np.abs(m2).max()
+
+
+def test_rihf_engines_agree():
+    r1 = run_rihf(Molecule.from_lines(['Ne 0 0 0']), 'cc-pVDZ', 'cc-pVDZ-RI', BASIS, engine='numpy')
+    r2 = run_rihf(Molecule.from_lines(['Ne 0 0 0']), 'cc-pVDZ', 'cc-pVDZ-RI', BASIS, engine='native')
+    assert abs(r1.energy - r2.energy) < 1e-11
