@@ -50,6 +50,10 @@ WORKLOADS = {
                'LiH (3 bohr) FP64 G0W0@HF with real integrals from the native front end: cc-pVQZ / cc-pVQZ-RI, '
                'Cartesian (nbf=105, naux=244 kept of 251), analytic Casida W, 51-point real grid, all states; '
                'the reference golden (lih_gw.in) gap is 8.053158 eV'),
+    'octane-real': (492, 1188, 33, 'one_shot',
+                    'all-trans n-octane FP64 G0W0@HF with real integrals and RI-HF orbitals from the native front end '
+                    '(geometry from bond rules: C-C 1.54 A, C-H 1.09 A, tetrahedral), cc-pVTZ / cc-pVTZ-RI '
+                    '(nbf=492, naux=1188, nocc=33, npole=15147), analytic Casida W, 51-point real grid, all 492 states'),
     'contour': (192, 672, 40, 'contour',
                 '(H2O)8-shaped FP64 G0W0 by contour deformation, cc-pVDZ/cc-pVDZ-RI shape (nbf=192, naux=672, '
                 'nocc=40), W on 16 imaginary and 6 real frequencies, 51-point real grid, 16 states '
@@ -320,22 +324,36 @@ def cpu_baseline(system, budget_note=True):
             'seconds_per_solve': total}
 
 
-def real_system_lih():
-    """LiH at 3 bohr, cc-pVQZ / cc-pVQZ-RI Cartesian (the reference's lih_gw.in):
-    integrals and RI-HF orbitals from molgw_amd.frontend, the RI tensor built
-    on the device (metric inverse square root + B). Not timed."""
+def _real_system(hf):
+    """The RI tensor of a front-end RI-HF result built on the device (metric
+    inverse square root + B), as a system for gpu_bench."""
     from types import SimpleNamespace
-    from molgw_amd.frontend import Molecule, run_rihf
     from molgw_amd.gw import Engine
-    basis_dir = os.path.join(ROOT, 'tests', 'golden', 'basis')
-    m = Molecule.from_lines(['Li 0.000000 0.0 1.50', 'H 0.000000 0.0 -1.50'], unit='bohr')
-    r = run_rihf(m, 'cc-pVQZ', 'cc-pVQZ-RI', basis_dir, gaussian_type='CART')
     eng = Engine(0)
-    kept, _, _ = eng.inverse_sqrt_eri_2center(r.eri_2center)
-    b = eng.build_eri_3center(r.eri_3center_tmp, r.index_basis, r.nbf)
+    kept, _, _ = eng.inverse_sqrt_eri_2center(hf.eri_2center)
+    b = eng.build_eri_3center(hf.eri_3center_tmp, hf.index_basis, hf.nbf)
     eng.close()
-    return SimpleNamespace(eri_3center=b, index_basis=r.index_basis, nbf=r.nbf, naux=kept, nstate=len(r.energy_mo),
-                           c_matrix=r.c_matrix, energy=r.energy_mo, occupation=r.occupation, scf_energy=r.energy)
+    return SimpleNamespace(eri_3center=b, index_basis=hf.index_basis, nbf=hf.nbf, naux=kept, nstate=len(hf.energy_mo),
+                           c_matrix=hf.c_matrix, energy=hf.energy_mo, occupation=hf.occupation, scf_energy=hf.energy)
+
+
+def real_system_alkane(n):
+    """All-trans CnH2n+2 at cc-pVTZ / cc-pVTZ-RI from the native front end. Not timed."""
+    from molgw_amd.frontend import alkane, run_rihf
+    t0 = time.perf_counter()
+    hf = run_rihf(alkane(n), 'cc-pVTZ', 'cc-pVTZ-RI', os.path.join(ROOT, 'tests', 'golden', 'basis'))
+    print(f'# front end: RI-HF {hf.energy:.8f} Ha in {time.perf_counter() - t0:.0f}s ({hf.iterations} iterations)',
+          file=sys.stderr, flush=True)
+    return _real_system(hf)
+
+
+def real_system_lih():
+    """LiH at 3 bohr, cc-pVQZ / cc-pVQZ-RI Cartesian (the reference's lih_gw.in)
+    from the native front end. Not timed."""
+    from molgw_amd.frontend import Molecule, run_rihf
+    m = Molecule.from_lines(['Li 0.000000 0.0 1.50', 'H 0.000000 0.0 -1.50'], unit='bohr')
+    return _real_system(run_rihf(m, 'cc-pVQZ', 'cc-pVQZ-RI', os.path.join(ROOT, 'tests', 'golden', 'basis'),
+                                 gaussian_type='CART'))
 
 
 def cpu_baseline_contour(system, kw, gpu_out):
@@ -577,6 +595,9 @@ def main():
     if args.workload == 'lih-qz':
         system = real_system_lih()
         nbf, naux = system.nbf, system.naux
+    elif args.workload == 'octane-real':
+        system = real_system_alkane(8)
+        nbf, naux = system.nbf, system.naux
     elif naux * nbf * nbf > 1e9:
         system = synthetic_system_device(nbf, naux, nocc, seed=args.seed, name=args.workload)
     else:
@@ -634,6 +655,9 @@ def main():
         'result_check': {'homo_qp_ha': float(out.energy_qp[nocc - 1]), 'lumo_qp_ha': float(out.energy_qp[nocc]),
                          'en_rpa': out.en_rpa, 'en_gw': out.en_gw},
     }
+    if args.workload == 'octane-real':
+        line['data'] = 'real integrals and RI-HF orbitals from the native front end (no synthetic data)'
+        line['result_check']['scf_energy_ha'] = system.scf_energy
     if args.workload == 'lih-qz':
         line['data'] = 'real integrals and RI-HF orbitals from the native front end (no synthetic data)'
         gap = float(out.energy_qp[nocc] - out.energy_qp[nocc - 1]) * 27.21138505
