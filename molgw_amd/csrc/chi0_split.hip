@@ -526,6 +526,8 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
       const dim3 grid((unsigned)g.nblocks);
       // one LDS buffer and one register set: 162 (two planes) / 196 (three)
       // VGPRs, two or three waves per SIMD; a second register set spills
+      // (two LDS buffers with one barrier per slab, or two k tiles per slab:
+      // 64 KB of LDS and two waves per SIMD, measured slower)
       hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
       MGW_HIP(hipGetLastError());
       hipLaunchKernelGGL((chi_ksplit_reduce_kernel<BM>), dim3(BM * BM / 256, L.tiles, nf), block, 0, stream, part,

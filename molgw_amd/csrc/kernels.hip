@@ -731,6 +731,58 @@ __global__ void pade_kernel(const double* __restrict__ wc, const double* __restr
   }
 }
 
+// The same continued fraction with one wave per state (2 ncalc - 1 <= 64):
+// lane j keeps point j and its table entry in registers, the row's pivot
+// g(ip-1) comes by a lane broadcast, and the lanes then evaluate the
+// fraction at one real frequency each. Every element sees the operations of
+// pade_kernel in the same order.
+__device__ inline cplx shfl_c(cplx v, int src) { return {__shfl(v.r, src, 64), __shfl(v.i, src, 64)}; }
+
+__global__ void __launch_bounds__(64)
+pade_wave_kernel(const double* __restrict__ wc, const double* __restrict__ sc, int ncalc, int nse,
+                 const double* __restrict__ omega, int nomega, double eta, double* __restrict__ sigma) {
+  __shared__ cplx zs[64], aas[64];
+  const int p = blockIdx.x, lane = threadIdx.x;
+  const int np = 2 * ncalc - 1;
+  const cplx* sp = reinterpret_cast<const cplx*>(sc) + (int64_t)p * ncalc;
+  const cplx* w = reinterpret_cast<const cplx*>(wc);
+  cplx zj = {0.0, 0.0}, gj = {0.0, 0.0};
+  if (lane < np) {
+    const int k = lane - (ncalc - 1);
+    if (k >= 0) {
+      zj = w[k];
+      gj = sp[k];
+    } else {
+      zj = {w[-k].r, -w[-k].i};
+      gj = {sp[-k].r, -sp[-k].i};
+    }
+  }
+  zs[lane] = zj;
+  if (lane == 0) aas[0] = gj;
+  __syncthreads();
+  for (int ip = 1; ip < np; ++ip) {
+    const cplx gd = shfl_c(gj, ip - 1);
+    if (lane >= ip && lane < np) gj = cdiv(csub(gd, gj), cmul(csub(zj, zs[ip - 1]), gj));
+    if (lane == ip) aas[ip] = gj;
+  }
+  __syncthreads();
+  for (int k = lane; k < nomega; k += 64) {
+    const double sgn = (omega[k] - w[0].r) >= 0.0 ? -1.0 : 1.0;
+    const cplx zo = {omega[k], eta * sgn};
+    cplx a_prev = {0.0, 0.0}, a_cur = aas[0], b_prev = {1.0, 0.0}, b_cur = {1.0, 0.0};
+    for (int ip = 1; ip < np; ++ip) {
+      const cplx f = cmul(csub(zo, zs[ip - 1]), aas[ip]);
+      const cplx a_next = cadd(a_cur, cmul(f, a_prev));
+      const cplx b_next = cadd(b_cur, cmul(f, b_prev));
+      a_prev = a_cur; a_cur = a_next;
+      b_prev = b_cur; b_cur = b_next;
+    }
+    const cplx r = cdiv(a_cur, b_cur);
+    sigma[2 * ((int64_t)p * nomega + k)] = r.r;
+    sigma[2 * ((int64_t)p * nomega + k) + 1] = r.i;
+  }
+}
+
 int pade_work_doubles(int ncalc, int nse) {
   const int np = 2 * ncalc - 1;
   return nse * (2 * np) * 2 + nse * np * 2;
@@ -738,8 +790,11 @@ int pade_work_doubles(int ncalc, int nse) {
 
 int launch_pade(const double* wc, const double* sc, int ncalc, int nse, const double* omega,
                 int nomega, double eta, double* work, double* sigma, hipStream_t s) {
-  hipLaunchKernelGGL(pade_kernel, dim3(nblocks(nse, 64)), dim3(64), 0, s, wc, sc, ncalc, nse, omega,
-                     nomega, eta, work, sigma);
+  if (2 * ncalc - 1 <= 64)
+    hipLaunchKernelGGL(pade_wave_kernel, dim3(nse), dim3(64), 0, s, wc, sc, ncalc, nse, omega, nomega, eta, sigma);
+  else
+    hipLaunchKernelGGL(pade_kernel, dim3(nblocks(nse, 64)), dim3(64), 0, s, wc, sc, ncalc, nse, omega,
+                       nomega, eta, work, sigma);
   MGW_LAUNCH_CHECK();
   return MGW_OK;
 }
