@@ -120,6 +120,7 @@ def stream_bench(workload, args):
     # class alone on the device, so its HIP-event time is its kernels' time
     # (as a kernel trace of the pass shows them)
     with Server(nslots=1, hbm_budget=budget) as srv:
+        srv.run(requests(args.precision), realtime=False)   # warm: buffers of every shape allocated
         srv.profile(True)
         torch.cuda.synchronize()
         i0 = time.clock_gettime_ns(time.CLOCK_MONOTONIC)

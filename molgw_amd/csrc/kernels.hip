@@ -137,9 +137,18 @@ __global__ void __launch_bounds__(256)
 absmax_kernel(const double* __restrict__ x, int64_t n, double* __restrict__ part) {
   __shared__ double red[256];
   const int tid = threadIdx.x;
-  double m = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + tid; i < n; i += (int64_t)gridDim.x * 256) m = fmax(m, fabs(x[i]));
-  red[tid] = m;
+  // four independent loads in flight per thread (max is order-independent)
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  int64_t i = (int64_t)blockIdx.x * 256 + tid;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    m0 = fmax(m0, fabs(x[i]));
+    m1 = fmax(m1, fabs(x[i + stride]));
+    m2 = fmax(m2, fabs(x[i + 2 * stride]));
+    m3 = fmax(m3, fabs(x[i + 3 * stride]));
+  }
+  for (; i < n; i += stride) m0 = fmax(m0, fabs(x[i]));
+  red[tid] = fmax(fmax(m0, m1), fmax(m2, m3));
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);

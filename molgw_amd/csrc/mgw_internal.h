@@ -173,7 +173,7 @@ int launch_qs_scale(const double* w, double* b, int npole, int nse, int cnt, con
 // m = (m + m^T) / 2 (n x n)
 int launch_symmetrize(double* m, int n, hipStream_t s);
 // part[b] = max |x_i| over the elements of block b, b < ABSMAX_BLOCKS
-constexpr int ABSMAX_BLOCKS = 256;
+constexpr int ABSMAX_BLOCKS = 1024;
 int launch_absmax(const double* x, int64_t n, double* part, hipStream_t s);
 // m_f(i, i) += v for the nbatch n x n matrices m_f (stride n^2)
 int launch_diag_add(double* m, int n, int nbatch, double v, hipStream_t s);
