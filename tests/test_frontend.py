@@ -7,6 +7,7 @@ Goldens (Ha unless noted, tolerance as in the testsuite):
                   HOMO -22.639065 eV, LUMO 46.110304 eV (1e-4)
   co_gw_dm.in     CO, HF cc-pVDZ_gaussian (Cartesian) / cc-pVDZ-RI: SCF -112.7480846685
   lih_bse-i.in    LiH (3 bohr), HF cc-pVQZ (Cartesian) / cc-pVQZ-RI: SCF -7.9872035011
+  ch4_gwtdhf.in   CH4, HF aug-cc-pVTZ (pure, diffuse) / aug-cc-pVTZ-RI: SCF -40.2135693443
   lih_gw.in       the same LiH, one-shot G0W0@HF: HOMO-LUMO gap 8.053158 eV (1e-4),
                   checked through the oracle's G0W0 on the front end's integrals
   lih_gw_frozencore_molgw.in / _gaussian.in  the same with frozencore 'molgw'
@@ -95,6 +96,14 @@ def test_rihf_co_cartesian_golden():
     r = run_rihf(m, 'cc-pVDZ_gaussian', 'cc-pVDZ-RI', BASIS, gaussian_type='CART')
     assert r.nbf == 30
     assert abs(r.energy - (-112.7480846685)) < 1e-7
+
+
+def test_rihf_methane_augmented_triple_zeta_golden():
+    m = Molecule.from_lines(['6 .000000 .000000 .000000', '1 .629118 .629118 .629118', '1 -.629118 -.629118 .629118',
+                             '1 .629118 -.629118 -.629118', '1 -.629118 .629118 -.629118'])
+    r = run_rihf(m, 'aug-cc-pVTZ', 'aug-cc-pVTZ-RI', BASIS)
+    assert r.nbf == 138
+    assert abs(r.energy - (-40.2135693443)) < 1e-7
 
 
 def test_rihf_lih_quadruple_zeta_golden(lih_qz):
