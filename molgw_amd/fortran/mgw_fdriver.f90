@@ -9,7 +9,9 @@
 !                  real64 eri_3center(npair, nauxil); int32 index_basis(2, npair);
 !                  real64 c_matrix(nbf, nstate), energy(nstate), occupation(nstate)
 ! output (stream): real64 energy_qp(nstate), zz(nstate), energy_qp_lin(nstate),
-!                  en_rpa, en_gw, energy_qp_one_shot(nstate)
+!                  en_rpa, en_gw, energy_qp_one_shot(nstate),
+!                  energy_qp_contour(nstate) (gw_selfenergy_contour, 12 imaginary
+!                  and 6 real W frequencies, then the graphical QP solve)
 !=========================================================================
 program mgw_fdriver
   use m_mgw
@@ -21,7 +23,8 @@ program mgw_fdriver
   integer, allocatable :: index_basis(:, :)
   real(dp), allocatable :: pole(:), omega(:), se_energy0(:), exc(:)
   real(dp), allocatable :: energy_qp(:), zz(:), energy_qp_lin(:), eqp2(:), zz2(:), elin2(:)
-  complex(dp), allocatable :: sigma(:, :), sigma2(:, :)
+  real(dp), allocatable :: eqp3(:), zz3(:), elin3(:), se_energy0_spin(:, :)
+  complex(dp), allocatable :: sigma(:, :), sigma2(:, :), sigma3(:, :, :)
   real(dp) :: en_rpa, en_gw, energies(2)
   type(mgw_ranges) :: ranges
   type(mgw_g0w0_params) :: params
@@ -74,10 +77,16 @@ program mgw_fdriver
   ! the same solve in one call
   allocate(eqp2(nstate), zz2(nstate), elin2(nstate), sigma2(nomega_sigma, nstate))
   call mgw_one_shot(params, ranges, exc, eqp2, zz2, elin2, sigma2, energies)
+
+  ! contour deformation on the same grid, then the QP equation
+  allocate(eqp3(nstate), zz3(nstate), elin3(nstate), sigma3(nomega_sigma, nstate, 1), se_energy0_spin(nstate, 1))
+  se_energy0_spin(:, 1) = energy(:)
+  call mgw_selfenergy_contour(ranges, 2.0_dp, 12, 6, eta, omega, se_energy0_spin, sigma3)
+  call mgw_qp_energy(ranges, omega, sigma3(:, :, 1), se_energy0, exc, energy, eqp3, zz3, elin3)
   call mgw_finalize()
 
   open(newunit=iu, file=TRIM(fout), access='stream', form='unformatted', status='replace', action='write')
-  write(iu) energy_qp, zz, energy_qp_lin, en_rpa, en_gw, eqp2
+  write(iu) energy_qp, zz, energy_qp_lin, en_rpa, en_gw, eqp2, eqp3
   close(iu)
   write(*, '(a,i0,a,f16.10)') ' mgw_fdriver: nstate=', nstate, ' E_RPA=', en_rpa
 end program mgw_fdriver

@@ -38,9 +38,13 @@ def test_fortran_driver_matches_oracle(tmp_path):
     eqp, zz, elin = raw[:n], raw[n:2 * n], raw[2 * n:3 * n]
     en_rpa, en_gw = raw[3 * n], raw[3 * n + 1]
     eqp_one = raw[3 * n + 2:4 * n + 2]
+    eqp_contour = raw[4 * n + 2:5 * n + 2]
     ref = O.g0w0_casida(s.eri_3center, s.index_basis, s.c_matrix, s.energy, s.occupation)
     assert abs(en_rpa - ref.en_rpa) < 1e-10
     assert abs(en_gw - ref.en_gw) < 1e-10
     assert np.max(np.abs(eqp - ref.energy_qp)) < 1e-8
     assert np.max(np.abs(elin - ref.energy_qp_lin)) < 1e-8
     np.testing.assert_array_equal(eqp, eqp_one)
+    # contour deformation through its iso_c_binding wrapper
+    refc = O.g0w0_contour(s.eri_3center, s.index_basis, s.c_matrix, s.energy, s.occupation, 12, 6)
+    assert np.max(np.abs(eqp_contour - refc['energy_qp'][:, 0])) < 1e-8
