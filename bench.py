@@ -113,9 +113,12 @@ def stream_bench(workload, args):
         prof = srv.profile_read()
         srv.profile(False)
         # latency: the same requests arriving as the Poisson stream
+        trims_cap = srv.trims
         live = srv.run(requests(args.precision), realtime=True) if realtime else res
         torch.cuda.synchronize()
         esc = srv.escalations()
+        print(f'# admission trims: capacity pass {trims_cap}, live pass {srv.trims - trims_cap}', file=sys.stderr,
+              flush=True)
     # the same requests one at a time, after the timed passes: each kernel
     # class alone on the device, so its HIP-event time is its kernels' time
     # (as a kernel trace of the pass shows them)

@@ -80,6 +80,7 @@ class Server:
         for e in self.slots:
             e.graphs(graphs)
         self.budget = hbm_budget if hbm_budget is not None else int(0.80 * device_memory_bytes(device))
+        self.trims = 0       # cached-buffer releases forced by admission
         self._jobs = [queue.Queue() for _ in range(nslots)]
         self._done = queue.Queue()
         self._threads = [threading.Thread(target=self._worker, args=(k,), daemon=True) for k in range(nslots)]
@@ -127,10 +128,21 @@ class Server:
     def _used(self) -> int:
         return sum(self._slot_bytes(k) for k in range(len(self.slots)))
 
-    def _trim_idle(self) -> None:
-        for k, eng in enumerate(self.slots):
-            if not self._busy[k]:
-                eng.trim()
+    def _trim_until(self, k: int, req) -> None:
+        """Free the cached buffers of idle slots, largest first, until ``req``
+        fits on slot ``k``. Freeing device memory waits for the whole device,
+        so every in-flight solve stalls on it: free no more than needed."""
+        def fits():
+            others = sum(self._slot_bytes(j) for j in range(len(self.slots)) if j != k)
+            return others + max(req.footprint, self.slots[k].memory()[0]) <= self.budget
+        idle = sorted((j for j in range(len(self.slots)) if not self._busy[j] and j != k),
+                      key=lambda j: -self.slots[j].memory()[0])
+        for j in [k] + idle:
+            if fits():
+                return
+            if self.slots[j].memory()[0] > 0:
+                self.slots[j].trim()
+                self.trims += 1
 
     # ------------------------------------------------------------ run
     def run(self, requests: list[Request], realtime: bool = True) -> list[Result]:
@@ -158,7 +170,7 @@ class Server:
                 others = sum(self._slot_bytes(j) for j in range(len(self.slots)) if j != k)
                 need = max(req.footprint, self.slots[k].memory()[0])
                 if others + need > self.budget:
-                    self._trim_idle()
+                    self._trim_until(k, req)
                     others = sum(self._slot_bytes(j) for j in range(len(self.slots)) if j != k)
                     need = max(req.footprint, self.slots[k].memory()[0])
                 if others + need > self.budget:
