@@ -144,17 +144,19 @@ int ensure(mgw_ctx* c, const std::string& name, size_t bytes, void** out) {
   DevBuf& b = c->ledger.bufs[name];
   if (bytes == 0) bytes = 8;
   if (b.bytes < bytes) {
+    // stream-ordered: the old buffer returns to the device's pool once this
+    // stream's earlier work is done, without waiting for the other streams
+    // of the device (hipFree would wait for all of them)
     if (b.ptr) {
-      MGW_HIP(hipStreamSynchronize(c->stream));
-      MGW_HIP(hipFree(b.ptr));
+      MGW_HIP(hipFreeAsync(b.ptr, c->stream));
       c->ledger.current -= (int64_t)b.bytes;
       b.ptr = nullptr;
       b.bytes = 0;
     }
-    hipError_t e = hipMalloc(&b.ptr, bytes);
+    hipError_t e = hipMallocAsync(&b.ptr, bytes, c->stream);
     if (e != hipSuccess) {
       b.ptr = nullptr;
-      set_error("hipMalloc(" + name + ", " + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+      set_error("hipMallocAsync(" + name + ", " + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
       (void)hipGetLastError();
       return MGW_ERR_NOMEM;
     }
@@ -183,8 +185,7 @@ int release(mgw_ctx* c, const std::string& name) {
   auto it = c->ledger.bufs.find(name);
   if (it == c->ledger.bufs.end()) return MGW_OK;
   if (it->second.ptr) {
-    MGW_HIP(hipStreamSynchronize(c->stream));
-    MGW_HIP(hipFree(it->second.ptr));
+    MGW_HIP(hipFreeAsync(it->second.ptr, c->stream));
     c->ledger.current -= (int64_t)it->second.bytes;
   }
   c->ledger.bufs.erase(it);
@@ -202,9 +203,9 @@ bool is_resident_input(const std::string& name) {
 }
 
 // End of a stage's use of a temporary. Temporaries stay cached for the next
-// solve (hipFree synchronises the whole device, which would serialise
-// contexts that share the GPU); mgw_ctx_trim returns them. When the context
-// is set to trim eagerly, they are freed here.
+// solve; mgw_ctx_trim returns them to the device's pool (stream-ordered frees,
+// so other contexts sharing the GPU keep running). When the context is set to
+// trim eagerly, they are freed here.
 int scratch_done(mgw_ctx* c, const std::string& name);
 
 template <class T>
@@ -1049,6 +1050,16 @@ int mgw_ctx_create(int device, mgw_ctx** out) {
   MGW_HIP(hipGetDeviceCount(&ndev));
   MGW_REQUIRE(device >= 0 && device < ndev, MGW_ERR_ARG, "invalid device " + std::to_string(device));
   MGW_HIP(hipSetDevice(device));
+  // freed buffers stay in the device's pool for the next allocation of any
+  // context of this process instead of going back to the driver at each
+  // synchronisation
+  {
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+  }
   mgw_ctx* c = new mgw_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1074,7 +1085,8 @@ int mgw_ctx_destroy(mgw_ctx* c) {
   MGW_HIP(hipStreamSynchronize(c->stream));
   clear_graphs(c);
   for (auto& kv : c->ledger.bufs)
-    if (kv.second.ptr) MGW_HIP(hipFree(kv.second.ptr));
+    if (kv.second.ptr) MGW_HIP(hipFreeAsync(kv.second.ptr, c->stream));
+  MGW_HIP(hipStreamSynchronize(c->stream));
   rocblas_destroy_handle(c->blas);
   MGW_HIP(hipStreamDestroy(c->stream));
   delete c;
