@@ -16,10 +16,11 @@
  *    code and mgw_last_error() describes it (the Fortran shim calls die()).
  *  - Output pointers marked "nullable" may be NULL: the result then stays
  *    resident in the context for the next call.
- *  - One context per process and GPU; a context is not re-entrant. Calls on
- *    one context are ordered on the context's HIP stream.
- *  - Restricted closed-shell only (nspin = 1, spin_fact = 2): the reference
- *    calls are made with spin index 1.
+ *  - A context is not re-entrant; calls on one context are ordered on the
+ *    context's HIP stream. Several contexts (one per concurrent solve) may
+ *    share a GPU and run concurrently from different host threads.
+ *  - Restricted (nspin = 1, spin_fact = 2, mgw_set_mo) or spin-polarised
+ *    (nspin = 2, spin_fact = 1, mgw_set_mo_spin) MO data.
  */
 #ifndef MGW_H
 #define MGW_H
@@ -86,8 +87,10 @@ typedef struct mgw_g0w0_params {
   double  step_sigma_calc;    /* Ha (Pade technique only)                     */
   double  spin_fact;          /* 2 for restricted closed shell                */
   int32_t precision;          /* MGW_FP64 | MGW_FP32 | MGW_BF16X3 | MGW_BF16X6
-                                 (transforms and chi0; metric/inversion/
-                                 eigensolve and the QP solve always FP64)     */
+                                 | MGW_FP16X3 (transforms, chi0 and the
+                                 imaginary-axis self-energy product; metric,
+                                 inversion, eigensolve and the QP solve are
+                                 always FP64)                                 */
   int32_t nomega_chi_real;    /* real-axis W grid, >= 2 (contour technique;
                                  input keyword nomega_chi_real, default 2)    */
 } mgw_g0w0_params;
@@ -119,6 +122,13 @@ int mgw_ctx_graphs(mgw_ctx* ctx, int enable);
 int mgw_autotune(int mode, int* previous);
 /* HBM ledger: replaces the clean_allocate bookkeeping (src/m_memory.f90:19-80). */
 int mgw_memory_report(mgw_ctx* ctx, int64_t* current_bytes, int64_t* peak_bytes);
+/* Bytes the device's default memory pool (from which every context
+ * allocates, stream-ordered) holds now and at most since the process began
+ * or the last reset (nullable outputs); reset_high != 0 restarts the
+ * high-water mark after reading it. mgw_ctx_trim hands the pool's unused
+ * memory back to the device. No reference counterpart (instrumentation of
+ * the ledger). */
+int mgw_device_pool_report(int device, int reset_high, int64_t* reserved_bytes, int64_t* reserved_high_bytes);
 /* Frequencies whose reduced-precision 1 - chi0 lost positive definiteness
  * in the Cholesky factorisation and were rebuilt in FP64 on the device,
  * counted since the context was created. No reference counterpart: the

@@ -46,6 +46,7 @@ SIGNATURES = {
     'mgw_ctx_graphs': (C.c_int, [_vp, C.c_int]),
     'mgw_set_energy_g': (C.c_int, [_vp, C.POINTER(C.c_double)]),
     'mgw_memory_report': (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    'mgw_device_pool_report': (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'mgw_precision_escalations': (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     'mgw_footprint_bytes': (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     'mgw_set_eri_3center': (C.c_int, [_vp, _vp, C.c_int64, C.c_int, _ip, C.c_int, C.c_int]),

@@ -58,11 +58,20 @@ using namespace mgw;
 static_assert(sizeof(mgw_ranges) == 32, "mgw_ranges layout is part of the ABI");
 static_assert(sizeof(mgw_g0w0_params) == 56, "mgw_g0w0_params layout is part of the ABI");
 
+// A library call: its status, then any launch error its kernels left, so
+// that the error is reported at this call and not at the next launch.
 #define MGW_BLAS(expr)                                                                  \
   do {                                                                                  \
     rocblas_status st_ = (expr);                                                        \
     if (st_ != rocblas_status_success) {                                                \
       set_error(std::string(#expr) + ": " + rocblas_status_to_string(st_));             \
+      (void)hipGetLastError();                                                          \
+      return MGW_ERR_SOLVER;                                                            \
+    }                                                                                   \
+    hipError_t he_ = hipGetLastError();                                                 \
+    if (he_ != hipSuccess) {                                                            \
+      set_error(std::string(#expr) + ": launch error " + hipGetErrorString(he_) + " (" +  \
+                __FILE__ + ":" + std::to_string(__LINE__) + ")");                      \
       return MGW_ERR_SOLVER;                                                            \
     }                                                                                   \
   } while (0)
@@ -98,6 +107,7 @@ struct mgw_ctx {
   // W sums the transitions of all channels.
   bool has_mo = false;
   int nstate = 0, nspin = 1, spin = 0;
+  int mo_nbf = 0;   // rows of c_matrix (must equal nbf of the resident RI tensor)
   std::vector<double> energy, occupation, energy_g;
   const std::vector<double>& energy_in_g() const { return energy_g.empty() ? energy : energy_g; }
   // offset of the active channel in the spin-major vectors
@@ -138,6 +148,27 @@ namespace {
 
 // modes whose chi0 and imaginary-axis self-energy product run on split operands
 bool split_precision(int p) { return p == MGW_BF16X3 || p == MGW_BF16X6 || p == MGW_FP16X3; }
+
+// Everything computed from the RI tensor and the MO data: new inputs make it stale.
+void invalidate_derived(mgw_ctx* c) {
+  c->has_eri_mo = false;
+  c->has_wpol = false;
+  c->has_chi = false;
+  c->has_cd = false;
+  c->has_sigma = false;
+  c->has_sigma_calc = false;
+}
+
+// The MO coefficients must have as many rows as the resident RI tensor has
+// basis functions (a new RI tensor of another molecule needs new MO data).
+int check_mo_matches_ri(const mgw_ctx* c) {
+  MGW_REQUIRE(c->has_mo, MGW_ERR_STATE, "MO data not set (mgw_set_mo)");
+  MGW_REQUIRE(c->has_ri, MGW_ERR_STATE, "RI tensor not set (mgw_set_eri_3center)");
+  MGW_REQUIRE(c->mo_nbf == c->nbf, MGW_ERR_STATE,
+              "MO data (nbf " + std::to_string(c->mo_nbf) + ") do not match the resident RI tensor (nbf " +
+                  std::to_string(c->nbf) + "): call mgw_set_mo again");
+  return MGW_OK;
+}
 
 // ------------------------------------------------------------------ buffers
 int ensure(mgw_ctx* c, const std::string& name, size_t bytes, void** out) {
@@ -241,6 +272,7 @@ int dsum(mgw_ctx* c, const double* x, int64_t n, int square, double* result) {
 int check_ranges(mgw_ctx* c, const mgw_ranges* r) {
   MGW_REQUIRE(r, MGW_ERR_ARG, "null ranges");
   MGW_REQUIRE(c->has_mo, MGW_ERR_STATE, "MO data not set (mgw_set_mo)");
+  if (c->has_ri) MGW_TRY(check_mo_matches_ri(c));
   const int ns = c->nstate;
   MGW_REQUIRE(r->ncore_G >= 0 && r->nvirtual_G <= ns + 1 && r->ncore_G + 1 <= r->nvirtual_G - 1,
               MGW_ERR_ARG, "invalid ncore_G / nvirtual_G");
@@ -297,8 +329,7 @@ bool rectangular(const std::vector<int32_t>& ia, int& i0, int& i1, int& a0, int&
 // dense resident tensor, then a GEMM batched over f contracts k and writes
 // straight into the (P, m, n) layout of the reference.
 int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int precision = MGW_FP64, int spin = -1) {
-  MGW_REQUIRE(c->has_ri, MGW_ERR_STATE, "RI tensor not set (mgw_set_eri_3center)");
-  MGW_REQUIRE(c->has_mo, MGW_ERR_STATE, "MO data not set (mgw_set_mo)");
+  MGW_TRY(check_mo_matches_ri(c));
   MGW_REQUIRE(mlo >= 1 && mhi <= c->nstate && mlo <= mhi && nlo >= 1 && nhi <= c->nstate && nlo <= nhi,
               MGW_ERR_ARG, "ao2mo: invalid state range");
   const int nbf = c->nbf, naux = c->naux;
@@ -1132,12 +1163,16 @@ int mgw_ctx_trim(mgw_ctx* c, int eager) {
     if (!is_resident_input(kv.first)) names.push_back(kv.first);
   for (auto& n : names) MGW_TRY(release(c, n));
   clear_graphs(c);
-  c->has_eri_mo = false;
-  c->has_wpol = false;
-  c->has_chi = false;
-  c->has_sigma = false;
-  c->has_sigma_calc = false;
+  invalidate_derived(c);
   c->trim_eagerly = eager != 0;
+  // the device pool keeps freed memory (release threshold at the maximum, so
+  // that later allocations do not go back to the driver): hand what no
+  // context holds back to the device, so that the pool's reserved size stays
+  // within what the contexts' ledgers account for
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  hipMemPool_t pool;
+  MGW_HIP(hipDeviceGetDefaultMemPool(&pool, c->device));
+  MGW_HIP(hipMemPoolTrimTo(pool, 0));
   return MGW_OK;
 }
 
@@ -1175,6 +1210,25 @@ int mgw_profile_read(mgw_ctx* c, int kernel_class, int64_t* launches, double* to
 int mgw_precision_escalations(mgw_ctx* c, int64_t* count) {
   MGW_REQUIRE(c && count, MGW_ERR_ARG, "mgw_precision_escalations: null argument");
   *count = c->escalations;
+  return MGW_OK;
+}
+
+int mgw_device_pool_report(int device, int reset_high, int64_t* reserved_bytes, int64_t* reserved_high_bytes) {
+  int ndev = 0;
+  MGW_HIP(hipGetDeviceCount(&ndev));
+  MGW_REQUIRE(device >= 0 && device < ndev, MGW_ERR_ARG, "invalid device " + std::to_string(device));
+  MGW_HIP(hipSetDevice(device));
+  hipMemPool_t pool;
+  MGW_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+  uint64_t cur = 0, high = 0;
+  MGW_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &cur));
+  MGW_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemHigh, &high));
+  if (reset_high) {
+    uint64_t zero = 0;   // the high-water mark restarts from the current size
+    MGW_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReservedMemHigh, &zero));
+  }
+  if (reserved_bytes) *reserved_bytes = (int64_t)cur;
+  if (reserved_high_bytes) *reserved_high_bytes = (int64_t)high;
   return MGW_OK;
 }
 
@@ -1265,7 +1319,7 @@ int mgw_set_eri_3center(mgw_ctx* c, const double* eri, int64_t npair, int nauxil
   c->nbf = nbf;
   c->naux = nauxil;
   c->has_ri = true;
-  c->has_eri_mo = false;
+  invalidate_derived(c);
   return MGW_OK;
 }
 
@@ -1402,6 +1456,7 @@ int mgw_build_eri_3center(mgw_ctx* c, const double* eri_3center_tmp, int64_t ipa
     c->naux = nk;
     c->has_ri = true;
     c->build_next = 0;
+    invalidate_derived(c);
   }
   return MGW_OK;
 }
@@ -1410,6 +1465,7 @@ int mgw_destroy_eri_3center(mgw_ctx* c) {
   MGW_CTX(c);
   MGW_TRY(release(c, "ri_dense"));
   c->has_ri = false;
+  invalidate_derived(c);
   return MGW_OK;
 }
 
@@ -1444,9 +1500,8 @@ int mgw_set_mo_spin(mgw_ctx* c, const double* c_matrix, int nbf, int nstate, int
   c->spin = 0;
   c->energy_g.clear();
   c->has_mo = true;
-  c->has_eri_mo = false;
-  c->has_wpol = false;
-  c->has_chi = false;
+  c->mo_nbf = nbf;
+  invalidate_derived(c);
   return MGW_OK;
 }
 

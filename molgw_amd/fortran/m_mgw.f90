@@ -54,6 +54,7 @@ module m_mgw
   public :: mgw_polarizability_rpa, mgw_polarizability_imag
   public :: mgw_selfenergy_poles, mgw_selfenergy_imag, mgw_selfenergy_contour, mgw_pade
   public :: mgw_qp_energy, mgw_one_shot
+  public :: mgw_selfenergy_qs, mgw_gw_ingredients
 
   type(c_ptr), save :: ctx = c_null_ptr
 
@@ -268,6 +269,25 @@ module m_mgw
       real(c_double) :: energies(2)
       integer(c_int) :: ierr
     end function c_g0w0
+
+    function c_gw_selfenergy_qs(handle, ranges, spin_fact, eta, s_matrix, selfenergy_mo, selfenergy_ao) &
+        bind(C, name='mgw_gw_selfenergy_qs') result(ierr)
+      import :: c_int, c_ptr, c_double, mgw_ranges
+      type(c_ptr), value :: handle
+      type(mgw_ranges), intent(in) :: ranges
+      real(c_double), value :: spin_fact, eta
+      real(c_double), intent(in) :: s_matrix(*)
+      real(c_double) :: selfenergy_mo(*), selfenergy_ao(*)
+      integer(c_int) :: ierr
+    end function c_gw_selfenergy_qs
+
+    function c_gw_ingredients(handle, ranges, w, v) bind(C, name='mgw_gw_ingredients') result(ierr)
+      import :: c_int, c_ptr, c_double, mgw_ranges
+      type(c_ptr), value :: handle
+      type(mgw_ranges), intent(in) :: ranges
+      real(c_double) :: w(*), v(*)
+      integer(c_int) :: ierr
+    end function c_gw_ingredients
 
     function c_strlen(s) bind(C, name='strlen') result(n)
       import :: c_ptr, c_size_t
@@ -512,5 +532,27 @@ subroutine mgw_one_shot(params, ranges, exchange_m_vxc, energy_qp, zz, energy_qp
   call mgw_check(c_g0w0(ctx, params, ranges, exchange_m_vxc, energy_qp, zz, energy_qp_lin, sigma, energies), &
                  'mgw_one_shot')
 end subroutine mgw_one_shot
+
+!=========================================================================
+! gw_selfenergy_qs, GW branch (src/m_gw_selfenergy_analytic.f90:653-797) with
+! the symmetrisation of apply_qs_approximation (src/m_selfenergy_tools.f90:809-828)
+! on the resident Casida W, active spin channel: selfenergy_mo(nstate, nstate)
+! (zero outside nsemin..nsemax) and selfenergy_ao = S C Sigma C^T S (nbf, nbf)
+subroutine mgw_selfenergy_qs(ranges, spin_fact, eta, s_matrix, selfenergy_mo, selfenergy_ao)
+  type(mgw_ranges), intent(in) :: ranges
+  real(dp), intent(in)  :: spin_fact, eta, s_matrix(:, :)
+  real(dp), intent(out) :: selfenergy_mo(:, :), selfenergy_ao(:, :)
+  call mgw_check(c_gw_selfenergy_qs(ctx, ranges, spin_fact, eta, s_matrix, selfenergy_mo, selfenergy_ao), &
+                 'mgw_selfenergy_qs')
+end subroutine mgw_selfenergy_qs
+
+! dump_gw_ingredients (src/m_gw_selfenergy_analytic.f90:801-881): the arrays of
+! the w.bin and v.bin records for G states ncore_G+1 .. nvirtual_G-1 (ng):
+! w(npole, ng, ng) on the resident W and v(nauxil, ng, ng)
+subroutine mgw_gw_ingredients(ranges, w, v)
+  type(mgw_ranges), intent(in) :: ranges
+  real(dp), intent(out) :: w(:, :, :), v(:, :, :)
+  call mgw_check(c_gw_ingredients(ctx, ranges, w, v), 'mgw_gw_ingredients')
+end subroutine mgw_gw_ingredients
 
 end module m_mgw

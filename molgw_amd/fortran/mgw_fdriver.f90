@@ -11,7 +11,11 @@
 ! output (stream): real64 energy_qp(nstate), zz(nstate), energy_qp_lin(nstate),
 !                  en_rpa, en_gw, energy_qp_one_shot(nstate),
 !                  energy_qp_contour(nstate) (gw_selfenergy_contour, 12 imaginary
-!                  and 6 real W frequencies, then the graphical QP solve)
+!                  and 6 real W frequencies, then the graphical QP solve),
+!                  selfenergy_mo(nstate, nstate), selfenergy_ao(nbf, nbf) (the
+!                  QSGW matrix on the Casida W, S = identity),
+!                  w(npole, nstate, nstate), v(nauxil, nstate, nstate)
+!                  (dump_gw_ingredients arrays, all states), w_s(nauxil, npole)
 !=========================================================================
 program mgw_fdriver
   use m_mgw
@@ -26,6 +30,8 @@ program mgw_fdriver
   real(dp), allocatable :: eqp3(:), zz3(:), elin3(:), se_energy0_spin(:, :)
   complex(dp), allocatable :: sigma(:, :), sigma2(:, :), sigma3(:, :, :)
   real(dp) :: en_rpa, en_gw, energies(2)
+  real(dp), allocatable :: s_matrix(:, :), sigma_qs_mo(:, :), sigma_qs_ao(:, :), w_ing(:, :, :), v_ing(:, :, :)
+  real(dp), allocatable :: w_s(:, :)
   type(mgw_ranges) :: ranges
   type(mgw_g0w0_params) :: params
   character(len=512) :: fin, fout
@@ -63,7 +69,7 @@ program mgw_fdriver
   call mgw_upload_mo(c_matrix, energy, occupation)
 
   ! W by Casida, then Sigma_c on se%omega = step_sigma * k centred on energy0
-  call mgw_polarizability_rpa(ranges, en_rpa, en_gw, pole)
+  call mgw_polarizability_rpa(ranges, en_rpa, en_gw, pole, nauxil, w_s)
   allocate(omega(nomega_sigma), se_energy0(nstate), exc(nstate), sigma(nomega_sigma, nstate))
   do k = 1, nomega_sigma
     omega(k) = step_sigma * (k - 1 - nomega_sigma / 2)
@@ -73,6 +79,16 @@ program mgw_fdriver
   call mgw_selfenergy_poles(ranges, eta, omega, se_energy0, sigma, en_gw)
   allocate(energy_qp(nstate), zz(nstate), energy_qp_lin(nstate))
   call mgw_qp_energy(ranges, omega, sigma, se_energy0, exc, energy, energy_qp, zz, energy_qp_lin)
+
+  ! the QSGW self-energy matrix and the GW ingredients on the same resident W
+  allocate(s_matrix(nbf, nbf), sigma_qs_mo(nstate, nstate), sigma_qs_ao(nbf, nbf))
+  s_matrix(:, :) = 0.0_dp
+  do k = 1, nbf
+    s_matrix(k, k) = 1.0_dp
+  end do
+  call mgw_selfenergy_qs(ranges, 2.0_dp, eta, s_matrix, sigma_qs_mo, sigma_qs_ao)
+  allocate(w_ing(SIZE(pole), nstate, nstate), v_ing(nauxil, nstate, nstate))
+  call mgw_gw_ingredients(ranges, w_ing, v_ing)
 
   ! the same solve in one call
   allocate(eqp2(nstate), zz2(nstate), elin2(nstate), sigma2(nomega_sigma, nstate))
@@ -87,6 +103,7 @@ program mgw_fdriver
 
   open(newunit=iu, file=TRIM(fout), access='stream', form='unformatted', status='replace', action='write')
   write(iu) energy_qp, zz, energy_qp_lin, en_rpa, en_gw, eqp2, eqp3
+  write(iu) sigma_qs_mo, sigma_qs_ao, w_ing, v_ing, w_s
   close(iu)
   write(*, '(a,i0,a,f16.10)') ' mgw_fdriver: nstate=', nstate, ' E_RPA=', en_rpa
 end program mgw_fdriver

@@ -137,6 +137,7 @@ class Engine:
         h = C.c_void_p()
         check(self._lib.mgw_ctx_create(int(device), C.byref(h)))
         self._h = h
+        self.device = int(device)
         self.nbf = self.naux = self.nstate = 0
         self._naux_kept = self._naux_bas = 0
         self.nspin = 1
@@ -186,6 +187,14 @@ class Engine:
         cur, peak = C.c_int64(), C.c_int64()
         check(self._lib.mgw_memory_report(self._h, C.byref(cur), C.byref(peak)))
         return cur.value, peak.value
+
+    def pool_memory(self, reset_high: bool = False):
+        """(reserved, reserved high-water) bytes of the device's memory pool,
+        which every engine of the process allocates from; reset_high restarts
+        the high-water mark from the current size after reading it."""
+        cur, high = C.c_int64(), C.c_int64()
+        check(self._lib.mgw_device_pool_report(self.device, 1 if reset_high else 0, C.byref(cur), C.byref(high)))
+        return cur.value, high.value
 
     def set_energy_g(self, energy_g=None) -> None:
         """Energies in G for the following self-energy calls (W is kept);

@@ -48,3 +48,25 @@ def test_fortran_driver_matches_oracle(tmp_path):
     # contour deformation through its iso_c_binding wrapper
     refc = O.g0w0_contour(s.eri_3center, s.index_basis, s.c_matrix, s.energy, s.occupation, 12, 6)
     assert np.max(np.abs(eqp_contour - refc['energy_qp'][:, 0])) < 1e-8
+    # QSGW matrix and GW ingredients through their iso_c_binding wrappers
+    off = 5 * n + 2
+    qs_mo = raw[off:off + n * n].reshape((n, n), order='F')
+    off += n * n
+    qs_ao = raw[off:off + s.nbf ** 2].reshape((s.nbf, s.nbf), order='F')
+    off += s.nbf ** 2
+    npole = ref.pole.size
+    w = raw[off:off + npole * n * n].reshape((npole, n, n), order='F')
+    off += npole * n * n
+    v = raw[off:off + s.naux * n * n].reshape((s.naux, n, n), order='F')
+    off += s.naux * n * n
+    ws = raw[off:off + s.naux * npole].reshape((s.naux, npole), order='F')
+    assert off + s.naux * npole == raw.size
+    rng = O.selfenergy_set_state_range(s.occupation[:, None])
+    eri = O.calculate_eri_3center_mo(s.eri_3center, s.index_basis, s.c_matrix, 1, n, 1, n)
+    ref_qs = O.gw_selfenergy_qs(eri, ws, ref.pole, s.energy, s.occupation, rng)
+    scale = np.max(np.abs(ref_qs))
+    assert np.max(np.abs(qs_mo - ref_qs)) < 1e-10 * scale
+    assert np.max(np.abs(qs_ao - s.c_matrix @ ref_qs @ s.c_matrix.T)) < 1e-10 * scale
+    assert np.max(np.abs(v - eri)) < 1e-12 * np.max(np.abs(eri))
+    ref_w = np.einsum('Ps,Ppq->spq', ws, eri)      # w_s of the same run (eigenvector signs)
+    assert np.max(np.abs(w - ref_w)) < 1e-11 * np.max(np.abs(ref_w))

@@ -291,3 +291,25 @@ def test_errors_are_reported(engine):
         engine.g0w0(bad)
     with pytest.raises(MgwError):
         engine.set_eri_3center(s.eri_3center, s.index_basis * 0, s.nbf)   # index 0 is out of range
+
+
+def test_new_ri_tensor_needs_matching_mo_data(engine):
+    """A new RI tensor of a larger molecule leaves the old MO data stale:
+    the transforms refuse to run (MGW_ERR_STATE) until set_mo is called again,
+    and the resident self-energy of the old molecule is no longer accepted."""
+    from molgw_amd._lib import MgwError
+    a = synthetic_system(12, 30, 3, seed=1)
+    b = synthetic_system(20, 44, 4, seed=2)
+    load(engine, a)
+    engine.g0w0()
+    engine.set_eri_3center(b.eri_3center, b.index_basis, b.nbf)
+    with pytest.raises(MgwError) as exc:
+        engine.g0w0()
+    assert exc.value.code == 4     # MGW_ERR_STATE
+    with pytest.raises(MgwError) as exc:
+        engine.calculate_eri_3center_mo(1, 3, 1, 3)
+    assert exc.value.code == 4
+    engine.set_mo(b.c_matrix, b.energy, b.occupation)
+    out = engine.g0w0()
+    ref = O.g0w0_casida(b.eri_3center, b.index_basis, b.c_matrix, b.energy, b.occupation)
+    assert np.max(np.abs(out.energy_qp - ref.energy_qp)) < 1e-8

@@ -73,3 +73,28 @@ def test_footprint_bounds_the_measured_peak(technique, precision):
     fp = footprint_bytes(req)
     print(f'{technique} {precision}: peak {peak / 2**20:.1f} MiB, footprint {fp / 2**20:.1f} MiB')
     assert peak <= fp
+
+
+def test_device_pool_stays_within_the_budget():
+    """Admission counts the contexts' ledgers; the device pool they allocate
+    from must not hold more than that (trimmed idle slots hand their memory
+    back to the device), on a mixed-size stream whose budget forces trims."""
+    from molgw_amd.serving import footprint_bytes
+    shapes = [alkane_shape(6), water_cluster_shape(2), alkane_shape(8), water_cluster_shape(7),
+              alkane_shape(3), water_cluster_shape(9), alkane_shape(7)]
+    reqs = [Request(k, synthetic_system(*sh, seed=300 + k, scale=0.3), technique='imaginary_pade',
+                    precision='fp16x3', nomega_chi_imag=16, nomega_sigma_calc=8)
+            for k, sh in enumerate(shapes)]
+    fps = sorted(footprint_bytes(r) for r in reqs)
+    budget = int(1.05 * (fps[-1] + fps[-2]))
+    with Server(nslots=3, hbm_budget=budget) as srv:
+        for e in srv.slots:
+            e.trim()
+        srv.slots[0].pool_memory(reset_high=True)
+        res = srv.run(reqs, realtime=False)
+        _, high = srv.slots[0].pool_memory()
+        trims = srv.trims
+    assert len(res) == len(reqs)
+    assert trims >= 1
+    print(f'budget {budget / 2**20:.0f} MiB, pool high-water {high / 2**20:.0f} MiB, {trims} trims')
+    assert high <= budget + (64 << 20)

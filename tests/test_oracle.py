@@ -8,6 +8,7 @@ Golden values from the reference's test database (tests/inputs/testsuite):
 The fixture files are the reference's own tests/data_for_tests/lih_bulk08_*.
 """
 import hashlib
+import os
 import math
 
 import numpy as np
@@ -35,9 +36,20 @@ def lih_packed(lih):
 
 
 def test_fixture_checksums():
-    h = hashlib.sha256(open(LIH_ROOT + '_CoulombVertex.elements', 'rb').read()).hexdigest()
-    assert len(h) == 64
-    assert open(LIH_ROOT + '_CoulombVertex.elements', 'rb').read(16) != b'\0' * 16
+    """Every committed fixture (the reference's CC4S files and basis files)
+    is byte-identical to the copy whose SHA-256 was recorded when it was
+    taken from the reference's tests/data_for_tests and basis directories."""
+    root = os.path.join(os.path.dirname(__file__), 'golden')
+    listed = {}
+    for line in open(os.path.join(root, 'SHA256SUMS')):
+        digest, name = line.split()
+        listed[name] = digest
+    on_disk = sorted(os.path.relpath(os.path.join(d, f), root) for d, _, fs in os.walk(root) for f in fs
+                     if f != 'SHA256SUMS' and not f.endswith('.py') and '__pycache__' not in d)
+    assert sorted(listed) == on_disk
+    for name, digest in listed.items():
+        with open(os.path.join(root, name), 'rb') as fh:
+            assert hashlib.sha256(fh.read()).hexdigest() == digest, name
 
 
 # ---------------------------------------------------------------- grids
