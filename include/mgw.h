@@ -129,6 +129,12 @@ int mgw_memory_report(mgw_ctx* ctx, int64_t* current_bytes, int64_t* peak_bytes)
  * memory back to the device. No reference counterpart (instrumentation of
  * the ledger). */
 int mgw_device_pool_report(int device, int reset_high, int64_t* reserved_bytes, int64_t* reserved_high_bytes);
+/* Release threshold of the device's default memory pool: at each
+ * synchronisation the pool hands memory that no allocation uses back to the
+ * device while it holds more than `bytes` (a serving loop passes its HBM
+ * budget). Contexts start with no limit (the pool keeps freed memory for
+ * the next allocation). No reference counterpart. */
+int mgw_device_pool_limit(int device, int64_t bytes);
 /* Frequencies whose reduced-precision 1 - chi0 lost positive definiteness
  * in the Cholesky factorisation and were rebuilt in FP64 on the device,
  * counted since the context was created. No reference counterpart: the
@@ -140,6 +146,62 @@ int mgw_precision_escalations(mgw_ctx* ctx, int64_t* count);
  * nomega_chi_imag. */
 int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int technique,
                             int nomega_chi_imag, int precision);
+
+/* ---------------------------------- integral front end (upstream of A1) */
+/* A set of contracted Gaussian shells: the reference's basis%shell(:)
+ * (am, ng, alpha, coeff, x0, istart; src/m_basis_set.f90:90-364) with the
+ * map from Cartesian components to the final (pure or Cartesian)
+ * functions (src/m_cart_to_pure.f90). Host arrays over nshell shells:
+ *   am[s]                angular momentum
+ *   x0[3 s + d]          centre (bohr)
+ *   prim_offset[s]       primitives of shell s: prim_offset[s] .. prim_offset[s+1]-1
+ *                        (nshell + 1 entries) into
+ *   alpha[], coeff[]     exponents and weights of the unnormalised Cartesian
+ *                        primitives x^i y^j z^k exp(-alpha r^2) (normalisation
+ *                        of the contraction included)
+ *   transform_offset[s]  start in transform[] of shell s's (nfunc x ncart)
+ *                        row-major map; Cartesian components ordered
+ *                        (l - i, i - k, k) for i = 0..l, k = 0..i
+ *   function_offset[s]   first final function (0-based; nshell + 1 entries)
+ */
+typedef struct mgw_shells {
+  int32_t nshell;
+  const int32_t* am;
+  const double* x0;
+  const int32_t* prim_offset;
+  const double* alpha;
+  const double* coeff;
+  const int32_t* transform_offset;
+  const double* transform;
+  const int32_t* function_offset;
+} mgw_shells;
+
+/* 2-centre Coulomb integrals (P|1/r12|Q) over the final auxiliary functions:
+ * the integral part of calculate_integrals_eri_2center_scalapack
+ * (src/m_eri_calculate.f90:703-945), which the reference takes from libcint
+ * (cint2c2e, src/m_libcint_tools.f90). eri_2center (naux x naux,
+ * column-major; `where` host or device) receives both triangles; it is the
+ * input of mgw_inverse_sqrt_eri_2center. Computed on the GPU (orbital
+ * shells l <= 3, auxiliary shells l <= 4). */
+int mgw_eri_2center_ao(mgw_ctx* ctx, const mgw_shells* auxil, int where, double* eri_2center);
+/* 3-centre Coulomb integrals (mu nu|P) of the pairs index_basis(2, npair)
+ * (1-based, host): the integral part of calculate_integrals_eri_3center_scalapack
+ * (src/m_eri_calculate.f90:1266-1580), i.e. eri_3center_tmp(npair, naux_bas)
+ * with the diagonal pairs x1/2 (:1840-1841), column-major. eri_3center_tmp
+ * (`where` host or device) is nullable: the result then stays resident in
+ * the context and mgw_build_eri_3center called with eri_3center_tmp = NULL
+ * turns it into the RI tensor without a copy. Computed on the GPU. */
+int mgw_eri_3center_ao(mgw_ctx* ctx, const mgw_shells* basis, const mgw_shells* auxil, int64_t npair,
+                       const int32_t* index_basis, int where, double* eri_3center_tmp);
+/* Hartree and exchange matrices through the resident RI tensor, one spin
+ * channel: setup_hartree_ri (src/m_hamiltonian_twobodies.f90:280-381) and
+ * setup_exchange_ri (:707-786). x (nbf x nx, host, column-major) holds the
+ * occupied orbitals scaled by sqrt(occupation / spin_fact) (the reference's
+ * c_t); the density is P = spin_fact x x^T. hartree_ao(mu, nu) =
+ * sum_P B(P, mu nu) sum_kl B(P, k l) P(k, l) and exchange_ao = -sum_P
+ * (B_P x)(B_P x)^T (nbf x nbf, host, nullable). FP64 MFMA products. */
+int mgw_hartree_exchange_ri(mgw_ctx* ctx, const double* x, int nbf, int nx, double spin_fact,
+                            double* hartree_ao, double* exchange_ao);
 
 /* ------------------------------------------------- A1: resident RI tensor */
 /* Replaces the module state eri_3center(npair, nauxil) and index_basis(2, npair)
@@ -170,7 +232,8 @@ int mgw_inverse_sqrt_eri_2center(mgw_ctx* ctx, const double* eri_2center, int na
  * npair, FP64. eri_3center_tmp is mpair x nauxil_bas column-major (`where`:
  * host or device) and already carries the 1/2 of the diagonal pairs
  * (src/m_eri_calculate.f90:1840-1841); index_basis is the full (2, npair)
- * host list. Batches follow each other from ipair_first = 1; after the batch
+ * host list. eri_3center_tmp = NULL takes the rows from the tensor that
+ * mgw_eri_3center_ao left resident (same npair and naux_bas). Batches follow each other from ipair_first = 1; after the batch
  * that ends at npair the tensor is resident exactly as after
  * mgw_set_eri_3center with nauxil = nauxil_kept. eri_3center_out (nullable;
  * host or device) is the caller's whole eri_3center(npair, nauxil_kept),
@@ -204,6 +267,17 @@ int mgw_set_spin(mgw_ctx* ctx, int ispin);
  * host. The resident W and the energies in W (those of mgw_set_mo) are kept;
  * NULL returns to the energies of mgw_set_mo, as does mgw_set_mo itself. */
 int mgw_set_energy_g(mgw_ctx* ctx, const double* energy_g);
+
+/* ------------------------------------------------ A10: Sigma_x - Vxc */
+/* setup_exchange_m_vxc (src/m_selfenergy_tools.f90:720-802), the branch
+ * without a DFT core-valence split (:786-797): exchange_m_vxc(p, q, s) =
+ * (C^T F C)(p, q) - delta_pq energy(p, s) over the resident MO data of every
+ * spin channel, with hamiltonian_fock (nbf x nbf x nspin, host,
+ * column-major) the Fock matrix of the starting point (for HF: h + J +
+ * exchange at the final density). exchange_m_vxc (nstate x nstate x nspin)
+ * and diag (nstate x nspin: its diagonal, the exchange_m_vxc argument of
+ * mgw_g0w0 and mgw_find_qp_energy) are host arrays, both nullable. FP64. */
+int mgw_exchange_m_vxc(mgw_ctx* ctx, const double* hamiltonian_fock, double* exchange_m_vxc, double* diag);
 
 /* ------------------------------------------------ A2: AO -> MO transform */
 /* calculate_eri_3center_mo(c_matrix, mstate_min, mstate_max, nstate_min,

@@ -30,6 +30,14 @@ class G0W0Params(C.Structure):
                 ('nomega_chi_real', C.c_int32)]
 
 
+class Shells(C.Structure):
+    """mgw_shells: a set of contracted Gaussian shells (host arrays)."""
+    _fields_ = [('nshell', C.c_int32), ('am', C.POINTER(C.c_int32)), ('x0', C.POINTER(C.c_double)),
+                ('prim_offset', C.POINTER(C.c_int32)), ('alpha', C.POINTER(C.c_double)),
+                ('coeff', C.POINTER(C.c_double)), ('transform_offset', C.POINTER(C.c_int32)),
+                ('transform', C.POINTER(C.c_double)), ('function_offset', C.POINTER(C.c_int32))]
+
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _vp = C.c_void_p
@@ -47,8 +55,13 @@ SIGNATURES = {
     'mgw_set_energy_g': (C.c_int, [_vp, C.POINTER(C.c_double)]),
     'mgw_memory_report': (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'mgw_device_pool_report': (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    'mgw_device_pool_limit': (C.c_int, [C.c_int, C.c_int64]),
     'mgw_precision_escalations': (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     'mgw_footprint_bytes': (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    'mgw_eri_2center_ao': (C.c_int, [_vp, C.POINTER(Shells), C.c_int, _vp]),
+    'mgw_eri_3center_ao': (C.c_int, [_vp, C.POINTER(Shells), C.POINTER(Shells), C.c_int64, _ip, C.c_int, _vp]),
+    'mgw_hartree_exchange_ri': (C.c_int, [_vp, _dp, C.c_int, C.c_int, C.c_double, _dp, _dp]),
+    'mgw_exchange_m_vxc': (C.c_int, [_vp, _dp, _dp, _dp]),
     'mgw_set_eri_3center': (C.c_int, [_vp, _vp, C.c_int64, C.c_int, _ip, C.c_int, C.c_int]),
     'mgw_destroy_eri_3center': (C.c_int, [_vp]),
     'mgw_inverse_sqrt_eri_2center': (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp, _vp]),

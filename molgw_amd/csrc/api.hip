@@ -4,6 +4,9 @@
 // src/m_selfenergy_evaluation.f90 (MOLGW 3.4).
 #include "mgw_internal.h"
 
+#include <mutex>
+#include <set>
+
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -99,6 +102,11 @@ struct mgw_ctx {
   // naux_bas x naux_kept) and the next pair of a batched build
   bool has_sqrtinv = false;
   int naux_bas = 0, naux_kept = 0;
+  // integral front end: eri_3center_tmp left resident by mgw_eri_3center_ao
+  // ("ri_tmp3", npair x naux_bas) for mgw_build_eri_3center
+  bool has_ri_tmp3 = false;
+  int64_t ri_tmp3_npair = 0;
+  int ri_tmp3_naux = 0;
   int64_t build_next = 0, build_npair = 0;
   int build_nbf = 0;
   // MO data of nspin channels, each vector spin-major (nstate per channel);
@@ -1085,10 +1093,15 @@ int mgw_ctx_create(int device, mgw_ctx** out) {
   // context of this process instead of going back to the driver at each
   // synchronisation
   {
+    // once per device and process, so that a limit set by mgw_device_pool_limit stays
+    static std::mutex m;
+    static std::set<int> done;
+    std::lock_guard<std::mutex> lock(m);
     hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+    if (!done.count(device) && hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
       uint64_t keep = UINT64_MAX;
       (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+      done.insert(device);
     }
   }
   mgw_ctx* c = new mgw_ctx();
@@ -1229,6 +1242,19 @@ int mgw_device_pool_report(int device, int reset_high, int64_t* reserved_bytes, 
   }
   if (reserved_bytes) *reserved_bytes = (int64_t)cur;
   if (reserved_high_bytes) *reserved_high_bytes = (int64_t)high;
+  return MGW_OK;
+}
+
+int mgw_device_pool_limit(int device, int64_t bytes) {
+  MGW_REQUIRE(bytes >= 0, MGW_ERR_ARG, "mgw_device_pool_limit: negative limit");
+  int ndev = 0;
+  MGW_HIP(hipGetDeviceCount(&ndev));
+  MGW_REQUIRE(device >= 0 && device < ndev, MGW_ERR_ARG, "invalid device " + std::to_string(device));
+  MGW_HIP(hipSetDevice(device));
+  hipMemPool_t pool;
+  MGW_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+  uint64_t v = (uint64_t)bytes;
+  MGW_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &v));
   return MGW_OK;
 }
 
@@ -1383,9 +1409,14 @@ int mgw_build_eri_3center(mgw_ctx* c, const double* eri_3center_tmp, int64_t ipa
                           int64_t npair, const int32_t* index_basis, int nbf, int where, double* eri_3center_out) {
   MGW_CTX(c);
   MGW_REQUIRE(c->has_sqrtinv, MGW_ERR_STATE, "no 2-center metric: call mgw_inverse_sqrt_eri_2center first");
-  MGW_REQUIRE(eri_3center_tmp && index_basis && npair > 0 && mpair > 0 && nbf > 0 && ipair_first >= 1 &&
+  MGW_REQUIRE(index_basis && npair > 0 && mpair > 0 && nbf > 0 && ipair_first >= 1 &&
                   ipair_first - 1 + mpair <= npair,
               MGW_ERR_ARG, "mgw_build_eri_3center: invalid arguments");
+  const bool resident_tmp = eri_3center_tmp == nullptr;
+  if (resident_tmp) {
+    MGW_REQUIRE(c->has_ri_tmp3 && c->ri_tmp3_npair == npair && c->ri_tmp3_naux == c->naux_bas, MGW_ERR_STATE,
+                "mgw_build_eri_3center: no resident 3-center integrals of this shape (mgw_eri_3center_ao)");
+  }
   MGW_REQUIRE(where == MGW_MEM_HOST || where == MGW_MEM_DEVICE, MGW_ERR_ARG, "invalid memory kind");
   const int nb = c->naux_bas, nk = c->naux_kept;
   MGW_REQUIRE((int64_t)nk * nbf * (int64_t)nbf < (int64_t)1 << 40, MGW_ERR_ARG, "RI tensor too large");
@@ -1416,7 +1447,11 @@ int mgw_build_eri_3center(mgw_ctx* c, const double* eri_3center_tmp, int64_t ipa
     MGW_REQUIRE(dense, MGW_ERR_STATE, "mgw_build_eri_3center: the tensor of the first batch is gone");
   }
   const double* tmp = eri_3center_tmp;
-  if (where == MGW_MEM_HOST) {
+  int64_t ld_tmp = mpair;
+  if (resident_tmp) {
+    tmp = find_buf(c, "ri_tmp3") + (ipair_first - 1);
+    ld_tmp = npair;
+  } else if (where == MGW_MEM_HOST) {
     double* d = nullptr;
     MGW_TRY(upload(c, "ri_tmp", eri_3center_tmp, (size_t)mpair * nb, &d));
     tmp = d;
@@ -1427,7 +1462,7 @@ int mgw_build_eri_3center(mgw_ctx* c, const double* eri_3center_tmp, int64_t ipa
   // (src/m_eri_calculate.f90:1889-1892); FP64
   GemmF64 g;
   g.M = (int)mpair; g.N = nk; g.K = nb;
-  g.A = tmp; g.lda = mpair;
+  g.A = tmp; g.lda = ld_tmp;
   g.B = find_buf(c, "ri_sqrtinv"); g.ldb = nb;
   g.C = packed; g.ldc = mpair;
   g.klass = MGW_KCLASS_GEMM_OTHER;
@@ -1457,7 +1492,357 @@ int mgw_build_eri_3center(mgw_ctx* c, const double* eri_3center_tmp, int64_t ipa
     c->has_ri = true;
     c->build_next = 0;
     invalidate_derived(c);
+    if (resident_tmp) {
+      MGW_TRY(release(c, "ri_tmp3"));
+      c->has_ri_tmp3 = false;
+    }
   }
+  return MGW_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ integral front end
+namespace {
+
+int ncart_of(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Checks a caller's shell set and copies it to the device under name prefix.
+int upload_shells(mgw_ctx* c, const std::string& prefix, const mgw_shells* sh, EriShells& out, int& nfunc) {
+  MGW_REQUIRE(sh && sh->nshell > 0 && sh->am && sh->x0 && sh->prim_offset && sh->alpha && sh->coeff &&
+                  sh->transform_offset && sh->transform && sh->function_offset,
+              MGW_ERR_ARG, "mgw_shells: null field or no shells");
+  const int n = sh->nshell;
+  MGW_REQUIRE(sh->prim_offset[0] == 0 && sh->function_offset[0] == 0, MGW_ERR_ARG,
+              "mgw_shells: offsets must start at 0");
+  int64_t ntmat = 0;
+  for (int k = 0; k < n; ++k) {
+    const int l = sh->am[k];
+    MGW_REQUIRE(l >= 0 && l <= 6, MGW_ERR_ARG, "mgw_shells: angular momentum out of range at shell " + std::to_string(k));
+    MGW_REQUIRE(sh->prim_offset[k + 1] > sh->prim_offset[k], MGW_ERR_ARG,
+                "mgw_shells: shell " + std::to_string(k) + " has no primitive");
+    const int nf = sh->function_offset[k + 1] - sh->function_offset[k];
+    MGW_REQUIRE(nf >= 1 && nf <= ncart_of(l), MGW_ERR_ARG,
+                "mgw_shells: function count of shell " + std::to_string(k) + " out of range");
+    MGW_REQUIRE(sh->transform_offset[k] >= 0, MGW_ERR_ARG, "mgw_shells: negative transform offset");
+    ntmat = std::max<int64_t>(ntmat, (int64_t)sh->transform_offset[k] + (int64_t)nf * ncart_of(l));
+  }
+  const int nprim = sh->prim_offset[n];
+  nfunc = sh->function_offset[n];
+  int32_t *l_d, *poff_d, *toff_d, *foff_d;
+  double *cen_d, *exp_d, *coef_d, *tm_d;
+  MGW_TRY(upload(c, prefix + "_l", sh->am, n, &l_d));
+  MGW_TRY(upload(c, prefix + "_x0", sh->x0, 3 * (size_t)n, &cen_d));
+  MGW_TRY(upload(c, prefix + "_poff", sh->prim_offset, n + 1, &poff_d));
+  MGW_TRY(upload(c, prefix + "_alpha", sh->alpha, nprim, &exp_d));
+  MGW_TRY(upload(c, prefix + "_coeff", sh->coeff, nprim, &coef_d));
+  MGW_TRY(upload(c, prefix + "_toff", sh->transform_offset, n, &toff_d));
+  MGW_TRY(upload(c, prefix + "_tmat", sh->transform, (size_t)ntmat, &tm_d));
+  MGW_TRY(upload(c, prefix + "_foff", sh->function_offset, n + 1, &foff_d));
+  out.l = l_d; out.center = cen_d; out.poff = poff_d; out.exps = exp_d; out.coefs = coef_d;
+  out.toff = toff_d; out.tmat = tm_d; out.foff = foff_d;
+  return MGW_OK;
+}
+
+int release_shells(mgw_ctx* c, const std::string& prefix) {
+  for (const char* f : {"_l", "_x0", "_poff", "_alpha", "_coeff", "_toff", "_tmat", "_foff"})
+    MGW_TRY(release(c, prefix + f));
+  return MGW_OK;
+}
+
+int boys_table(mgw_ctx* c, const double** out) {
+  double* d = find_buf(c, "boys_table");
+  if (!d) {
+    std::vector<double> h(boys_table_doubles());
+    boys_table_fill(h.data());
+    MGW_TRY(upload(c, "boys_table", h.data(), h.size(), &d));
+  }
+  *out = d;
+  return MGW_OK;
+}
+
+// ket functions (shell, function) grouped by angular momentum: kfun holds
+// them class by class; first[l] .. first[l + 1] is class l
+void ket_functions(const mgw_shells* sh, std::vector<int32_t>& kfun, std::vector<int>& first) {
+  first.assign(8, 0);
+  kfun.clear();
+  for (int l = 0; l <= 6; ++l) {
+    first[l] = (int)kfun.size() / 2;
+    for (int k = 0; k < sh->nshell; ++k) {
+      if (sh->am[k] != l) continue;
+      const int nf = sh->function_offset[k + 1] - sh->function_offset[k];
+      for (int f = 0; f < nf; ++f) {
+        kfun.push_back(k);
+        kfun.push_back(f);
+      }
+    }
+  }
+  first[7] = (int)kfun.size() / 2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgw_eri_2center_ao(mgw_ctx* c, const mgw_shells* auxil, int where, double* eri_2center) {
+  MGW_CTX(c);
+  MGW_REQUIRE(eri_2center, MGW_ERR_ARG, "mgw_eri_2center_ao: null output");
+  MGW_REQUIRE(where == MGW_MEM_HOST || where == MGW_MEM_DEVICE, MGW_ERR_ARG, "invalid memory kind");
+  EriShells A;
+  int naux = 0;
+  MGW_TRY(upload_shells(c, "int_aux", auxil, A, naux));
+  const int n = auxil->nshell;
+  for (int k = 0; k < n; ++k)
+    MGW_REQUIRE(eri_class_supported(auxil->am[k], 0, auxil->am[k]), MGW_ERR_ARG,
+                "mgw_eri_2center_ao: auxiliary angular momentum above 4 is not supported on the GPU");
+  // the bra's second shell: a unit s function (exponent 0) at each shell's centre
+  std::vector<int32_t> ul(n, 0), upoff(n + 1), utoff(n), ufoff(n + 1), pairs;
+  std::vector<double> uexp(n, 0.0), ucoef(n, 1.0), utm(n, 1.0);
+  for (int k = 0; k <= n; ++k) upoff[k] = ufoff[k] = k;
+  for (int k = 0; k < n; ++k) utoff[k] = k;
+  const mgw_shells unit{n, ul.data(), auxil->x0, upoff.data(), uexp.data(), ucoef.data(), utoff.data(), utm.data(),
+                        ufoff.data()};
+  EriShells U;
+  int nunit = 0;
+  MGW_TRY(upload_shells(c, "int_unit", &unit, U, nunit));
+  std::vector<int32_t> kfun;
+  std::vector<int> first;
+  ket_functions(auxil, kfun, first);
+  // bra pairs (P shell, its unit shell) grouped by l_P, in the same order
+  std::vector<int32_t> kshell_pairs;
+  std::vector<int> pfirst(8, 0);
+  for (int l = 0; l <= 6; ++l) {
+    pfirst[l] = (int)kshell_pairs.size() / 2;
+    for (int k = 0; k < n; ++k)
+      if (auxil->am[k] == l) {
+        kshell_pairs.push_back(k);
+        kshell_pairs.push_back(k);
+      }
+  }
+  pfirst[7] = (int)kshell_pairs.size() / 2;
+  int32_t *kf_d, *pr_d;
+  MGW_TRY(upload(c, "int_kfun", kfun.data(), kfun.size(), &kf_d));
+  MGW_TRY(upload(c, "int_pairs", kshell_pairs.data(), kshell_pairs.size(), &pr_d));
+  const double* boys = nullptr;
+  MGW_TRY(boys_table(c, &boys));
+  double* out = eri_2center;
+  if (where == MGW_MEM_HOST) MGW_TRY(dbuf(c, "int_out2", (size_t)naux * naux, &out));
+  for (int la = 0; la <= 4; ++la)
+    for (int lc = 0; lc <= 4; ++lc) {
+      EriLaunch e;
+      e.la = la; e.lb = 0; e.lc = lc;
+      e.A = A; e.B = U; e.C = A;
+      e.pairs = pr_d + 2 * (int64_t)pfirst[la];
+      e.npairs = pfirst[la + 1] - pfirst[la];
+      e.kfun = kf_d + 2 * (int64_t)first[lc];
+      e.nkfun = first[lc + 1] - first[lc];
+      e.boys = boys;
+      e.metric = 1;
+      e.naux = naux;
+      e.out = out;
+      MGW_TRY(launch_eri_class(e, c->stream));
+    }
+  if (where == MGW_MEM_HOST) MGW_TRY(download(c, eri_2center, out, (size_t)naux * naux));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  for (const char* b : {"int_kfun", "int_pairs", "int_out2"}) MGW_TRY(release(c, b));
+  MGW_TRY(release_shells(c, "int_aux"));
+  MGW_TRY(release_shells(c, "int_unit"));
+  return MGW_OK;
+}
+
+int mgw_eri_3center_ao(mgw_ctx* c, const mgw_shells* basis, const mgw_shells* auxil, int64_t npair,
+                       const int32_t* index_basis, int where, double* eri_3center_tmp) {
+  MGW_CTX(c);
+  MGW_REQUIRE(index_basis && npair > 0, MGW_ERR_ARG, "mgw_eri_3center_ao: invalid pair list");
+  MGW_REQUIRE(where == MGW_MEM_HOST || where == MGW_MEM_DEVICE, MGW_ERR_ARG, "invalid memory kind");
+  EriShells S, A;
+  int nbf = 0, naux = 0;
+  MGW_TRY(upload_shells(c, "int_bas", basis, S, nbf));
+  MGW_TRY(upload_shells(c, "int_aux", auxil, A, naux));
+  MGW_REQUIRE(npair <= (int64_t)nbf * (nbf + 1) / 2, MGW_ERR_ARG, "mgw_eri_3center_ao: more pairs than nbf(nbf+1)/2");
+  // pair index of every (mu, nu), both orders
+  std::vector<int32_t> pmap((size_t)nbf * nbf, -1);
+  for (int64_t p = 0; p < npair; ++p) {
+    const int k = index_basis[2 * p] - 1, l = index_basis[2 * p + 1] - 1;
+    MGW_REQUIRE(k >= 0 && k < nbf && l >= 0 && l < nbf, MGW_ERR_ARG,
+                "index_basis entry out of range at pair " + std::to_string(p + 1));
+    int32_t& a = pmap[(size_t)k * nbf + l];
+    MGW_REQUIRE(a < 0, MGW_ERR_ARG, "index_basis lists a pair twice at pair " + std::to_string(p + 1));
+    a = (int32_t)p;
+    pmap[(size_t)l * nbf + k] = (int32_t)p;
+  }
+  // shell pairs (s1, s2) with l1 >= l2, grouped by (l1, l2)
+  const int n = basis->nshell;
+  std::vector<std::vector<int32_t>> cls(64);
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b <= a; ++b) {
+      int s1 = a, s2 = b;
+      if (basis->am[s1] < basis->am[s2]) std::swap(s1, s2);
+      const int l1 = basis->am[s1], l2 = basis->am[s2];
+      MGW_REQUIRE(l1 <= 3, MGW_ERR_ARG, "mgw_eri_3center_ao: orbital angular momentum above 3 is not supported on the GPU");
+      cls[l1 * 8 + l2].push_back(s1);
+      cls[l1 * 8 + l2].push_back(s2);
+    }
+  for (int k = 0; k < auxil->nshell; ++k)
+    MGW_REQUIRE(auxil->am[k] <= 4, MGW_ERR_ARG,
+                "mgw_eri_3center_ao: auxiliary angular momentum above 4 is not supported on the GPU");
+  std::vector<int32_t> pairs;
+  std::vector<int64_t> cfirst(65, 0);
+  for (int k = 0; k < 64; ++k) {
+    cfirst[k] = (int64_t)pairs.size() / 2;
+    pairs.insert(pairs.end(), cls[k].begin(), cls[k].end());
+  }
+  cfirst[64] = (int64_t)pairs.size() / 2;
+  std::vector<int32_t> kfun;
+  std::vector<int> first;
+  ket_functions(auxil, kfun, first);
+  int32_t *pm_d, *pr_d, *kf_d;
+  MGW_TRY(upload(c, "int_pairmap", pmap.data(), pmap.size(), &pm_d));
+  MGW_TRY(upload(c, "int_pairs", pairs.data(), pairs.size(), &pr_d));
+  MGW_TRY(upload(c, "int_kfun", kfun.data(), kfun.size(), &kf_d));
+  const double* boys = nullptr;
+  MGW_TRY(boys_table(c, &boys));
+  double* out = eri_3center_tmp;
+  if (!eri_3center_tmp) {
+    MGW_TRY(dbuf(c, "ri_tmp3", (size_t)npair * naux, &out));
+  } else if (where == MGW_MEM_HOST) {
+    MGW_TRY(dbuf(c, "int_out3", (size_t)npair * naux, &out));
+  }
+  {
+    ProfScope ps(&c->prof, MGW_KCLASS_OTHER, 0.0, 8.0 * (double)npair * naux, c->stream);
+    for (int l1 = 0; l1 <= 3; ++l1)
+      for (int l2 = 0; l2 <= l1; ++l2)
+        for (int lc = 0; lc <= 4; ++lc) {
+          EriLaunch e;
+          e.la = l1; e.lb = l2; e.lc = lc;
+          e.A = S; e.B = S; e.C = A;
+          e.pairs = pr_d + 2 * cfirst[l1 * 8 + l2];
+          e.npairs = cfirst[l1 * 8 + l2 + 1] - cfirst[l1 * 8 + l2];
+          e.kfun = kf_d + 2 * (int64_t)first[lc];
+          e.nkfun = first[lc + 1] - first[lc];
+          e.boys = boys;
+          e.metric = 0;
+          e.pairmap = pm_d;
+          e.nbf = nbf;
+          e.npair = npair;
+          e.out = out;
+          MGW_TRY(launch_eri_class(e, c->stream));
+        }
+  }
+  if (eri_3center_tmp && where == MGW_MEM_HOST) MGW_TRY(download(c, eri_3center_tmp, out, (size_t)npair * naux));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  for (const char* b : {"int_pairmap", "int_pairs", "int_kfun", "int_out3"}) MGW_TRY(release(c, b));
+  MGW_TRY(release_shells(c, "int_bas"));
+  MGW_TRY(release_shells(c, "int_aux"));
+  if (!eri_3center_tmp) {
+    c->has_ri_tmp3 = true;
+    c->ri_tmp3_npair = npair;
+    c->ri_tmp3_naux = naux;
+  }
+  return MGW_OK;
+}
+
+int mgw_exchange_m_vxc(mgw_ctx* c, const double* fock, double* exchange_m_vxc, double* diag) {
+  MGW_CTX(c);
+  MGW_REQUIRE(fock, MGW_ERR_ARG, "mgw_exchange_m_vxc: null hamiltonian_fock");
+  MGW_REQUIRE(c->has_mo, MGW_ERR_STATE, "MO data not set (mgw_set_mo)");
+  const int nbf = c->mo_nbf, ns = c->nstate;
+  double *f, *t, *m;
+  MGW_TRY(dbuf(c, "xc_fock", (size_t)nbf * nbf, &f));
+  MGW_TRY(dbuf(c, "xc_fc", (size_t)nbf * ns, &t));
+  MGW_TRY(dbuf(c, "xc_mo", (size_t)ns * ns, &m));
+  std::vector<double> h((size_t)ns * ns);
+  for (int sp = 0; sp < c->nspin; ++sp) {
+    const double* C = find_buf(c, "c_matrix") + (int64_t)sp * nbf * ns;
+    MGW_HIP(hipMemcpyAsync(f, fock + (size_t)sp * nbf * nbf, sizeof(double) * nbf * nbf, hipMemcpyHostToDevice,
+                           c->stream));
+    // (F C), then C^T (F C)
+    GemmF64 a;
+    a.M = nbf; a.N = ns; a.K = nbf;
+    a.A = f; a.lda = nbf;
+    a.B = C; a.ldb = nbf;
+    a.C = t; a.ldc = nbf;
+    MGW_TRY(gemm_f64(a, c->stream));
+    GemmF64 b;
+    b.M = ns; b.N = ns; b.K = nbf;
+    b.transA = true;
+    b.A = C; b.lda = nbf;
+    b.B = t; b.ldb = nbf;
+    b.C = m; b.ldc = ns;
+    MGW_TRY(gemm_f64(b, c->stream));
+    MGW_TRY(download(c, h.data(), m, h.size()));
+    for (int p = 0; p < ns; ++p) h[(size_t)p * ns + p] -= c->energy[(size_t)sp * ns + p];
+    if (exchange_m_vxc) std::memcpy(exchange_m_vxc + (size_t)sp * ns * ns, h.data(), sizeof(double) * h.size());
+    if (diag)
+      for (int p = 0; p < ns; ++p) diag[(size_t)sp * ns + p] = h[(size_t)p * ns + p];
+  }
+  for (const char* b : {"xc_fock", "xc_fc", "xc_mo"}) MGW_TRY(scratch_done(c, b));
+  return MGW_OK;
+}
+
+int mgw_hartree_exchange_ri(mgw_ctx* c, const double* x, int nbf, int nx, double spin_fact, double* hartree_ao,
+                            double* exchange_ao) {
+  MGW_CTX(c);
+  MGW_REQUIRE(c->has_ri, MGW_ERR_STATE, "RI tensor not set (mgw_set_eri_3center / mgw_build_eri_3center)");
+  MGW_REQUIRE(x && nbf == c->nbf && nx >= 1, MGW_ERR_ARG, "mgw_hartree_exchange_ri: x must be nbf x nx with nbf of the RI tensor");
+  const int naux = c->naux;
+  MGW_REQUIRE((int64_t)naux * nx < ((int64_t)1 << 31) && (int64_t)nbf * nbf < ((int64_t)1 << 31), MGW_ERR_ARG,
+              "mgw_hartree_exchange_ri: problem too large");
+  const double* B = find_buf(c, "ri_dense");
+  double *xd, *xt, *h, *gam, *jv, *kv;
+  MGW_TRY(upload(c, "hf_x", x, (size_t)nbf * nx, &xd));
+  std::vector<double> xth((size_t)nx * nbf);   // xt[i + nx k] = x(k, i)
+  for (int k = 0; k < nbf; ++k)
+    for (int i = 0; i < nx; ++i) xth[(size_t)i + (size_t)nx * k] = x[(size_t)k + (size_t)nbf * i];
+  MGW_TRY(upload(c, "hf_xt", xth.data(), xth.size(), &xt));
+  MGW_TRY(dbuf(c, "hf_h", (size_t)naux * nx * nbf, &h));
+  MGW_TRY(dbuf(c, "hf_gamma", naux, &gam));
+  MGW_TRY(dbuf(c, "hf_j", (size_t)nbf * nbf, &jv));
+  MGW_TRY(dbuf(c, "hf_k", (size_t)nbf * nbf, &kv));
+  {
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_OTHER,
+                 2.0 * naux * (double)nbf * nbf * nx * 2.0 + 4.0 * naux * (double)nbf * nbf,
+                 8.0 * ((double)naux * nbf * nbf * 2.0 + 2.0 * naux * nx * nbf), c->stream);
+    // H(P, i, k) = sum_l B(P, k, l) x(l, i): one product per k (B(:, :, k) is contiguous)
+    GemmF64 g;
+    g.M = naux; g.N = nx; g.K = nbf;
+    g.A = B; g.lda = naux; g.strideA = (int64_t)naux * nbf;
+    g.B = xd; g.ldb = nbf; g.strideB = 0;
+    g.C = h; g.ldc = naux; g.strideC = (int64_t)naux * nx;
+    g.batch = nbf;
+    MGW_TRY(gemm_f64(g, c->stream));
+    // exchange = -H^T H over the rows (P, i)
+    GemmF64 k;
+    k.M = nbf; k.N = nbf; k.K = naux * nx;
+    k.transA = true;
+    k.A = h; k.lda = (int64_t)naux * nx;
+    k.B = h; k.ldb = (int64_t)naux * nx;
+    k.C = kv; k.ldc = nbf;
+    k.alpha = -1.0;
+    k.lower_only = true;
+    MGW_TRY(gemm_f64(k, c->stream));
+    // gamma(P) = spin_fact sum_{i,k} H(P, i, k) x(k, i)
+    GemmF64 y;
+    y.M = naux; y.N = 1; y.K = nx * nbf;
+    y.A = h; y.lda = naux;
+    y.B = xt; y.ldb = (int64_t)nx * nbf;
+    y.C = gam; y.ldc = naux;
+    y.alpha = spin_fact;
+    MGW_TRY(gemm_f64(y, c->stream));
+    // hartree(k, l) = sum_P B(P, k, l) gamma(P)
+    GemmF64 jj;
+    jj.M = nbf * nbf; jj.N = 1; jj.K = naux;
+    jj.transA = true;
+    jj.A = B; jj.lda = naux;
+    jj.B = gam; jj.ldb = naux;
+    jj.C = jv; jj.ldc = (int64_t)nbf * nbf;
+    MGW_TRY(gemm_f64(jj, c->stream));
+  }
+  if (hartree_ao) MGW_TRY(download(c, hartree_ao, jv, (size_t)nbf * nbf));
+  if (exchange_ao) MGW_TRY(download(c, exchange_ao, kv, (size_t)nbf * nbf));
+  MGW_HIP(hipStreamSynchronize(c->stream));
+  for (const char* b : {"hf_x", "hf_xt", "hf_gamma", "hf_j", "hf_k"}) MGW_TRY(scratch_done(c, b));
+  MGW_TRY(scratch_done(c, "hf_h"));
   return MGW_OK;
 }
 

@@ -158,6 +158,47 @@ int launch_lower_frob2(const double* m, int n, int nbatch, double* partial, hipS
 constexpr int FROB_BLOCKS = 64;
 int gemm_autotune_mode(int mode); // -1 query; returns the previous mode
 
+// ---------------------------------------------------------------- integrals
+// Device view of a set of contracted shells (struct of arrays): angular
+// momentum, centre (3 per shell), primitive range [poff[s], poff[s+1]) into
+// exps/coefs (weights of the unnormalised Cartesian primitives), the
+// (nfunc x ncart) row-major map to the final functions at toff[s], and the
+// first final function foff[s] (foff has nshell + 1 entries).
+struct EriShells {
+  const int32_t* l = nullptr;
+  const double* center = nullptr;
+  const int32_t* poff = nullptr;
+  const double* exps = nullptr;
+  const double* coefs = nullptr;
+  const int32_t* toff = nullptr;
+  const double* tmat = nullptr;
+  const int32_t* foff = nullptr;
+};
+// One class (la, lb | lc) of Coulomb integrals: pairs (sa in A, sb in B) as
+// int32 pairs, ket functions (sc in C, function f of sc) as int32 pairs.
+// metric = 0: out(ipair, P) of packed pairs (pairmap: nbf x nbf pair index
+// or -1, diagonal pairs x1/2, npair rows); metric = 1: out(P, Q) (naux x
+// naux) with B the unit shells and pairs (s, s).
+struct EriLaunch {
+  int la = 0, lb = 0, lc = 0;
+  EriShells A, B, C;
+  const int32_t* pairs = nullptr;
+  int64_t npairs = 0;
+  const int32_t* kfun = nullptr;
+  int nkfun = 0;
+  const double* boys = nullptr;
+  int metric = 0;
+  const int32_t* pairmap = nullptr;
+  int nbf = 0;
+  int64_t npair = 0;
+  int naux = 0;
+  double* out = nullptr;
+};
+bool eri_class_supported(int la, int lb, int lc);
+size_t boys_table_doubles();
+void boys_table_fill(double* host);
+int launch_eri_class(const EriLaunch& e, hipStream_t s);
+
 // ---------------------------------------------------------------- kernels
 // zero_dense false: only the pairs listed are written (a later batch of a
 // tensor whose buffer was cleared before the first batch)

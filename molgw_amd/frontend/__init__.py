@@ -2,6 +2,8 @@
 format, Gaussian integrals, and the closed-shell RI-HF that produces the
 inputs of the G0W0 path."""
 from .basis import BasisSet, read_basis_file
-from .scf import Molecule, RIHFResult, alkane, frozen_core_states, run_rihf
+from .scf import (DeviceRIHF, Molecule, RIHFResult, alkane, frozen_core_states, run_rihf, run_rihf_device,
+                  water_cluster)
 
-__all__ = ['BasisSet', 'read_basis_file', 'Molecule', 'RIHFResult', 'alkane', 'frozen_core_states', 'run_rihf']
+__all__ = ['BasisSet', 'read_basis_file', 'DeviceRIHF', 'Molecule', 'RIHFResult', 'alkane', 'frozen_core_states',
+           'run_rihf', 'run_rihf_device', 'water_cluster']

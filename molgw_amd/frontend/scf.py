@@ -184,6 +184,161 @@ def run_rihf(mol: Molecule, basis: str, auxil_basis: str, basis_dir: str, gaussi
     return RIHFResult(float(energy), comps, c, eps, occ, _pack(e3, ib), m2, ib, bs.nbf, it)
 
 
+@dataclass
+class DeviceRIHF:
+    """Closed-shell RI-HF whose RI tensor was built and stays resident on the
+    GPU (Engine): the G0W0 inputs without any host copy of the 3-centre
+    integrals."""
+    energy: float
+    components: dict
+    c_matrix: np.ndarray        # (nbf, nstate)
+    energy_mo: np.ndarray       # (nstate,)
+    occupation: np.ndarray      # (nstate,)
+    hamiltonian_fock: np.ndarray  # (nbf, nbf): h + J + exchange at the final density
+    s_matrix: np.ndarray        # (nbf, nbf)
+    index_basis: np.ndarray     # (2, npair), 1-based
+    nbf: int
+    naux: int                   # kept auxiliary functions
+    naux_bas: int
+    iterations: int
+
+    @property
+    def nstate(self) -> int:
+        return self.c_matrix.shape[1]
+
+    @property
+    def nocc(self) -> int:
+        return int(np.count_nonzero(self.occupation > 1e-5))
+
+
+def run_rihf_device(mol: Molecule, basis: str, auxil_basis: str, basis_dir: str, engine, gaussian_type: str = 'PURE',
+                    tol: float = 1e-10, max_iter: int = 100, diis: int = 8, eri_3center_out: int | None = None):
+    """Closed-shell RI-HF with the integrals and the Hartree/exchange builds
+    on the GPU: (P|Q) and (mu nu|P) by the device integral kernels, V^-1/2
+    and B = (mu nu|Q) V^-1/2 on the device (B then stays resident in
+    ``engine``, a molgw_amd.gw.Engine), and J and K through the resident B
+    at every iteration (setup_hartree_ri / setup_exchange_ri). The
+    one-electron integrals, the orthogonalisation, the diagonalisations and
+    DIIS run on the host (nbf x nbf). The same SCF as run_rihf, so at
+    convergence the energy and orbitals are those of the host route.
+    ``eri_3center_out`` (optional device address) receives the packed RI
+    tensor eri_3center(npair, naux_kept)."""
+    if mol.nelectron % 2:
+        raise ValueError('closed shell only')
+    from . import native
+    bs = BasisSet(mol.atoms, mol.coords, basis, basis_dir, gaussian_type)
+    aux = BasisSet(mol.atoms, mol.coords, auxil_basis, basis_dir, gaussian_type)
+    charges = [float(z) for z in mol.atoms]
+    s, t, v = native.one_electron(bs, charges, mol.coords)
+    enuc = I.nuclear_repulsion(charges, mol.coords)
+    m2 = engine.eri_2center_ao(aux)
+    kept, _, _ = engine.inverse_sqrt_eri_2center(m2, copy=False)
+    ib = upper_pairs(bs.nbf)
+    engine.eri_3center_ao(bs, aux, ib, copy=False)
+    engine.build_eri_3center_resident(ib, bs.nbf, out_ptr=eri_3center_out)
+    sw, su = np.linalg.eigh(s)
+    xk = sw > MIN_OVERLAP
+    x = su[:, xk] / np.sqrt(sw[xk])
+    nstate = x.shape[1]
+    nocc = mol.nelectron // 2
+    h = t + v
+
+    def fock(c):
+        j, kx = engine.hartree_exchange_ri(c[:, :nocc], spin_fact=2.0)   # sqrt(occupation / spin_fact) = 1
+        return h + j + kx, j, kx
+
+    def solve(f):
+        e, cp = np.linalg.eigh(x.T @ f @ x)
+        return e, x @ cp
+
+    eps, c = solve(h)
+    e_old = 0.0
+    fs, errs = [], []
+    it = 0
+    for it in range(1, max_iter + 1):
+        d = 2.0 * c[:, :nocc] @ c[:, :nocc].T
+        f, j, kx = fock(c)
+        energy = 0.5 * np.sum(d * (h + f)) + enuc
+        err = x.T @ (f @ d @ s - s @ d @ f) @ x
+        fs.append(f)
+        errs.append(err)
+        if len(fs) > diis:
+            fs.pop(0)
+            errs.pop(0)
+        if len(fs) > 1:
+            nd = len(fs)
+            bm = -np.ones((nd + 1, nd + 1))
+            bm[nd, nd] = 0.0
+            for a_ in range(nd):
+                for b_ in range(nd):
+                    bm[a_, b_] = np.sum(errs[a_] * errs[b_])
+            rhs = np.zeros(nd + 1)
+            rhs[nd] = -1.0
+            coef = np.linalg.solve(bm, rhs)[:nd]
+            f_use = sum(cf * ff for cf, ff in zip(coef, fs))
+        else:
+            f_use = f
+        eps, c = solve(f_use)
+        if abs(energy - e_old) < tol and np.abs(err).max() < 1e-7:
+            break
+        e_old = energy
+    # final orbitals from the Fock matrix of the converged density itself
+    f, j, kx = fock(c)
+    eps, c = solve(f)
+    d = 2.0 * c[:, :nocc] @ c[:, :nocc].T
+    f, j, kx = fock(c)
+    energy = 0.5 * np.sum(d * (h + f)) + enuc
+    comps = {'kinetic': float(np.sum(d * t)), 'nucleus': float(np.sum(d * v)),
+             'hartree': float(0.5 * np.sum(d * j)), 'exchange': float(0.5 * np.sum(d * kx)),
+             'nucleus_nucleus': enuc}
+    occ = np.zeros(nstate)
+    occ[:nocc] = 2.0
+    return DeviceRIHF(float(energy), comps, c, eps, occ, f, s, ib, bs.nbf, kept, aux.nbf, it)
+
+
+def water_cluster(m: int, seed: int = 0, oo_min: float = 2.7, volume_per_molecule: float = 45.0,
+                  inter_min: float = 1.5) -> Molecule:
+    """(H2O)m from rigid monomers of the water geometry O (0, 0, 0),
+    H (+-0.757, 0.587, 0) Angstrom, each with a seeded random orientation
+    (a uniformly distributed rotation), with O centres placed by seeded
+    rejection sampling in a sphere of volume m * volume_per_molecule A^3
+    with every O-O distance >= oo_min. A trial placement is also rejected
+    when an atom comes within inter_min Angstrom of an atom of another
+    monomer (no hydrogen closer than a hydrogen bond allows)."""
+    rng = np.random.default_rng(seed)
+    mono = np.array([[0.0, 0.0, 0.0], [0.757, 0.587, 0.0], [-0.757, 0.587, 0.0]])
+    radius = (3.0 * m * volume_per_molecule / (4.0 * math.pi)) ** (1.0 / 3.0)
+    placed = []
+    tries = 0
+    while len(placed) < m:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError(f'water_cluster: could not place {m} molecules')
+        o = rng.uniform(-radius, radius, 3)
+        if np.dot(o, o) > radius * radius:
+            continue
+        q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+        rot = q * np.sign(np.diag(r))
+        if np.linalg.det(rot) < 0:
+            rot[:, 0] = -rot[:, 0]
+        xyz = o + mono @ rot.T
+        ok = True
+        for other in placed:
+            if np.linalg.norm(other[0] - o) < oo_min:
+                ok = False
+                break
+            if np.min(np.linalg.norm(other[:, None, :] - xyz[None, :, :], axis=2)) < inter_min:
+                ok = False
+                break
+        if ok:
+            placed.append(xyz)
+    atoms, coords = [], []
+    for xyz in placed:
+        atoms += [8, 1, 1]
+        coords += list(xyz)
+    return Molecule(atoms, np.array(coords) * ANGSTROM_BOHR)
+
+
 def alkane(n: int, cc: float = 1.54, ch: float = 1.09) -> Molecule:
     """All-trans n-alkane CnH2n+2 from bond rules: C-C cc, C-H ch (Angstrom),
     tetrahedral angles, the carbon zigzag in the xy plane."""

@@ -196,6 +196,12 @@ class Engine:
         check(self._lib.mgw_device_pool_report(self.device, 1 if reset_high else 0, C.byref(cur), C.byref(high)))
         return cur.value, high.value
 
+    def pool_limit(self, nbytes: int) -> None:
+        """Release threshold of the device's memory pool (shared by every
+        engine of the process): above it, unused memory goes back to the
+        device at each synchronisation."""
+        check(self._lib.mgw_device_pool_limit(self.device, int(nbytes)))
+
     def set_energy_g(self, energy_g=None) -> None:
         """Energies in G for the following self-energy calls (W is kept);
         None returns to the energies of set_mo."""
@@ -342,6 +348,94 @@ class Engine:
                                               _lib.MGW_MEM_DEVICE, None if out_ptr is None else C.c_void_p(out_ptr)))
         if ipair_first - 1 + mpair == npair:
             self.nbf, self.naux = int(nbf), self._naux_kept
+
+    # ------------------------------------------------------------ integral front end
+    @staticmethod
+    def _shells(bs):
+        """(keep-alive arrays, mgw_shells) of a frontend BasisSet."""
+        sh = bs.shells
+        l = np.array([x.l for x in sh], dtype=np.int32)
+        x0 = np.ascontiguousarray(np.array([x.center for x in sh], dtype=np.float64).reshape(-1))
+        poff = np.cumsum([0] + [len(x.exps) for x in sh]).astype(np.int32)
+        al = np.ascontiguousarray(np.concatenate([x.exps for x in sh]), dtype=np.float64)
+        co = np.ascontiguousarray(np.concatenate([x.coefs for x in sh]), dtype=np.float64)
+        tm = [np.ascontiguousarray(x.transform, dtype=np.float64).reshape(-1) for x in sh]
+        toff = np.cumsum([0] + [len(t) for t in tm[:-1]]).astype(np.int32)
+        tmat = np.ascontiguousarray(np.concatenate(tm))
+        foff = np.asarray(bs.offsets, dtype=np.int32)
+        keep = (l, x0, poff, al, co, toff, tmat, foff)
+        ip, dp = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+        st = _lib.Shells(len(sh), l.ctypes.data_as(ip), x0.ctypes.data_as(dp), poff.ctypes.data_as(ip),
+                         al.ctypes.data_as(dp), co.ctypes.data_as(dp), toff.ctypes.data_as(ip),
+                         tmat.ctypes.data_as(dp), foff.ctypes.data_as(ip))
+        return keep, st
+
+    def eri_2center_ao(self, auxil):
+        """(P|Q) over the final functions of a BasisSet, computed on the GPU
+        (calculate_integrals_eri_2center_scalapack, src/m_eri_calculate.f90:703-945)."""
+        keep, st = self._shells(auxil)
+        out = np.zeros((auxil.nbf, auxil.nbf), order='F')
+        check(self._lib.mgw_eri_2center_ao(self._h, C.byref(st), _lib.MGW_MEM_HOST, out.ctypes.data))
+        return out
+
+    def eri_3center_ao(self, basis, auxil, index_basis, copy: bool = True):
+        """eri_3center_tmp(npair, naux_bas) = (mu nu|P) of the pairs index_basis
+        (2, npair), diagonal pairs x1/2, computed on the GPU
+        (calculate_integrals_eri_3center_scalapack, src/m_eri_calculate.f90:1266-1580).
+        With copy False the tensor stays resident for build_eri_3center_resident."""
+        kb, sb = self._shells(basis)
+        ka, sa = self._shells(auxil)
+        ib = np.asfortranarray(np.asarray(index_basis, dtype=np.int32))
+        npair = ib.shape[1]
+        out = np.zeros((npair, auxil.nbf), order='F') if copy else None
+        check(self._lib.mgw_eri_3center_ao(self._h, C.byref(sb), C.byref(sa), npair,
+                                           ib.ctypes.data_as(C.POINTER(C.c_int32)), _lib.MGW_MEM_HOST,
+                                           out.ctypes.data if copy else None))
+        return out
+
+    def build_eri_3center_resident(self, index_basis, nbf: int, out_ptr: int | None = None,
+                                   copy: bool = False):
+        """build_eri_3center on the 3-center integrals that eri_3center_ao left
+        resident: B becomes the resident RI tensor. ``out_ptr`` (device or
+        host address) receives the packed eri_3center(npair, nauxil_kept);
+        with copy set a host array is returned instead."""
+        if not self._naux_kept:
+            raise RuntimeError('no 2-center metric: call inverse_sqrt_eri_2center first')
+        ib = np.asfortranarray(np.asarray(index_basis, dtype=np.int32))
+        npair = ib.shape[1]
+        out = np.zeros((npair, self._naux_kept), order='F') if copy else None
+        dst = out.ctypes.data if copy else (None if out_ptr is None else C.c_void_p(out_ptr))
+        check(self._lib.mgw_build_eri_3center(self._h, None, 1, npair, npair, ib.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              int(nbf), _lib.MGW_MEM_HOST, dst))
+        self.nbf, self.naux = int(nbf), self._naux_kept
+        return out
+
+    def hartree_exchange_ri(self, x, spin_fact: float = 2.0):
+        """(hartree_ao, exchange_ao) of the density spin_fact x x^T through the
+        resident RI tensor (setup_hartree_ri / setup_exchange_ri,
+        src/m_hamiltonian_twobodies.f90:280-381, 707-786); x (nbf, nx) are the
+        occupied orbitals times sqrt(occupation / spin_fact)."""
+        xf = _f64(x)
+        nbf, nx = xf.shape
+        j = np.zeros((nbf, nbf), order='F')
+        k = np.zeros((nbf, nbf), order='F')
+        check(self._lib.mgw_hartree_exchange_ri(self._h, _ptr(xf), nbf, nx, float(spin_fact), _ptr(j), _ptr(k)))
+        return j, k
+
+    def exchange_m_vxc(self, hamiltonian_fock):
+        """setup_exchange_m_vxc (src/m_selfenergy_tools.f90:720-802): (C^T F C -
+        diag(energy)) of every spin channel for the resident MO data; F is
+        (nbf, nbf) or (nbf, nbf, nspin). Returns (matrix (nstate, nstate[, nspin]),
+        diagonal (nstate[, nspin]))."""
+        f = _f64(np.asarray(hamiltonian_fock, dtype=np.float64).reshape(self.nbf, self.nbf, -1))
+        if f.shape[2] != self.nspin:
+            raise ValueError('hamiltonian_fock must hold one nbf x nbf matrix per spin channel')
+        mat = np.zeros((self.nstate, self.nstate, self.nspin), order='F')
+        dg = np.zeros((self.nstate, self.nspin), order='F')
+        check(self._lib.mgw_exchange_m_vxc(self._h, _ptr(f), _ptr(mat), _ptr(dg)))
+        if self.nspin == 1:
+            return mat[:, :, 0], dg[:, 0]
+        return mat, dg
 
     def set_mo(self, c_matrix, energy, occupation) -> None:
         """MO coefficients c_matrix (nbf, nstate) or (nbf, nstate, nspin), energies

@@ -80,6 +80,9 @@ class Server:
         for e in self.slots:
             e.graphs(graphs)
         self.budget = hbm_budget if hbm_budget is not None else int(0.80 * device_memory_bytes(device))
+        # the device pool that every slot allocates from gives memory back
+        # above the budget at each synchronisation
+        self.slots[0].pool_limit(self.budget)
         self.trims = 0       # cached-buffer releases forced by admission
         self._jobs = [queue.Queue() for _ in range(nslots)]
         self._done = queue.Queue()
@@ -128,13 +131,20 @@ class Server:
     def _used(self) -> int:
         return sum(self._slot_bytes(k) for k in range(len(self.slots)))
 
+    def _others(self, k: int) -> int:
+        """HBM held beside slot k: the other slots' ledgers, or the device
+        pool's reserved memory (freed buffers the pool still holds count
+        too) less slot k's own, whichever is larger."""
+        ledgers = sum(self._slot_bytes(j) for j in range(len(self.slots)) if j != k)
+        reserved, _ = self.slots[k].pool_memory()
+        return max(ledgers, reserved - self.slots[k].memory()[0])
+
     def _trim_until(self, k: int, req) -> None:
         """Free the cached buffers of idle slots, largest first, until ``req``
         fits on slot ``k``. Freeing device memory waits for the whole device,
         so every in-flight solve stalls on it: free no more than needed."""
         def fits():
-            others = sum(self._slot_bytes(j) for j in range(len(self.slots)) if j != k)
-            return others + max(req.footprint, self.slots[k].memory()[0]) <= self.budget
+            return self._others(k) + max(req.footprint, self.slots[k].memory()[0]) <= self.budget
         idle = sorted((j for j in range(len(self.slots)) if not self._busy[j] and j != k),
                       key=lambda j: -self.slots[j].memory()[0])
         for j in [k] + idle:
@@ -143,6 +153,10 @@ class Server:
             if self.slots[j].memory()[0] > 0:
                 self.slots[j].trim()
                 self.trims += 1
+        if not fits():
+            # freed buffers that the device pool still holds
+            self.slots[k].trim()
+            self.trims += 1
 
     # ------------------------------------------------------------ run
     def run(self, requests: list[Request], realtime: bool = True) -> list[Result]:
@@ -167,11 +181,11 @@ class Server:
                 if not free:
                     break
                 k = free[0]
-                others = sum(self._slot_bytes(j) for j in range(len(self.slots)) if j != k)
+                others = self._others(k)
                 need = max(req.footprint, self.slots[k].memory()[0])
                 if others + need > self.budget:
                     self._trim_until(k, req)
-                    others = sum(self._slot_bytes(j) for j in range(len(self.slots)) if j != k)
+                    others = self._others(k)
                     need = max(req.footprint, self.slots[k].memory()[0])
                 if others + need > self.budget:
                     continue

@@ -20,6 +20,9 @@ lives in ``tests/test_oracle.py``:
   * ``build_eri_3center`` -- from (mu nu|Q) = B V^1/2 (``scipy.linalg.sqrtm``)
     the build returns B up to a rotation (B B^T), and the G0W0 QP energies
     of the built tensor equal those of B
+  * ``setup_exchange_m_vxc`` -- zero at a converged RI-HF of the front end
+    (F C = S C e); a planted Fock matrix F = S C diag(e') C^T S gives
+    diag(e' - e) and nothing off the diagonal
   * ``coeffs_gausslegint``  -- numpy ``leggauss``; integral of 2a/(w^2+a^2)
   * ``calculate_eri_3center_mo`` -- brute-force dense einsum; CC4S (11|11)
   * ``g0w0_casida`` end to end -- the reference's lih_gw.in golden (G0W0@HF
@@ -266,6 +269,16 @@ def calculate_eri_3center_mo(eri_3center: np.ndarray, index_basis: np.ndarray,
     del b_full
     out = np.tensordot(cm, tmp1, axes=([0], [1]))          # (m, n, P)
     return np.ascontiguousarray(out.transpose(2, 0, 1))
+
+
+def setup_exchange_m_vxc(c_matrix: np.ndarray, hamiltonian_fock: np.ndarray, energy: np.ndarray) -> np.ndarray:
+    """setup_exchange_m_vxc without a DFT core-valence split
+    (src/m_selfenergy_tools.f90:786-797), one spin channel: <p|Sigma_x - Vxc|q>
+    = <p|F - H|q> = (C^T F C)(p, q) - delta_pq e_p (h_ao_to_mo, then the
+    diagonal minus the energies)."""
+    m = c_matrix.T @ hamiltonian_fock @ c_matrix
+    m[np.diag_indices_from(m)] -= energy
+    return m
 
 
 # ---------------------------------------------------------------------------

@@ -568,3 +568,27 @@ def test_contour_closed_shell_as_two_channels_equals_one_channel():
     for sp in range(2):
         np.testing.assert_allclose(got['energy_qp'][:, sp], ref['energy_qp'][:, 0], rtol=0, atol=1e-11)
         np.testing.assert_allclose(got['sigma'][:, :, sp], ref['sigma'][:, :, 0], rtol=0, atol=1e-11)
+
+
+# ---------------------------------------------------------------- A10
+def test_exchange_m_vxc_zero_at_converged_hf_and_planted_fock():
+    """Sigma_x - Vxc = C^T F C - diag(e) vanishes at a converged RI-HF (the
+    front end's water, F C = S C e), and a planted F = S C diag(e') C^T S
+    gives exactly diag(e' - e)."""
+    from molgw_amd.frontend import Molecule, run_rihf
+    from molgw_amd.frontend.basis import BasisSet
+    from molgw_amd.frontend import native
+    basis_dir = os.path.join(os.path.dirname(__file__), 'golden', 'basis')
+    m = Molecule.from_lines(['O 0.0 0.0 0.0', 'H 0.757 0.587 0.0', 'H -0.757 0.587 0.0'])
+    r = run_rihf(m, 'cc-pVDZ', 'cc-pVDZ-RI', basis_dir)
+    bs = BasisSet(m.atoms, m.coords, 'cc-pVDZ', basis_dir)
+    s, t, v = native.one_electron(bs, [8.0, 1.0, 1.0], m.coords)
+    # the converged Fock matrix rebuilt from its own orbitals: F = S C diag(e) C^T S
+    f = s @ r.c_matrix @ np.diag(r.energy_mo) @ r.c_matrix.T @ s
+    x = O.setup_exchange_m_vxc(r.c_matrix, f, r.energy_mo)
+    assert np.abs(x).max() < 1e-10
+    shift = np.linspace(-0.1, 0.2, len(r.energy_mo))
+    f2 = s @ r.c_matrix @ np.diag(r.energy_mo + shift) @ r.c_matrix.T @ s
+    x2 = O.setup_exchange_m_vxc(r.c_matrix, f2, r.energy_mo)
+    np.testing.assert_allclose(np.diag(x2), shift, atol=1e-11)
+    assert np.abs(x2 - np.diag(np.diag(x2))).max() < 1e-11
