@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -68,29 +69,36 @@ def stream_bench(workload, args):
     precision, concurrent slots, continuous batching). A step is one request;
     the warm-up requests are served before the timed stream."""
     import torch
+    from molgw_amd.frontend import alkane, water_cluster
+    from molgw_amd.gw import Engine
     from molgw_amd.serving import Request, Server, summarize, poisson_stream, alkane_batch
-    from molgw_amd.workloads import synthetic_system_device, alkane_shape, water_cluster_shape
+    from molgw_amd.workloads import molecular_system_device
     if workload == 'serving':
         spec = poisson_stream(args.requests, args.rate, seed=args.seed)
         realtime = True
     else:
         spec = [(0.0, 'alkane', n) for n in alkane_batch(args.requests, seed=args.seed)]
         realtime = False
-    shapes = {}
-    for _, kind, size in spec:
-        key = (kind, size)
-        if key not in shapes:
-            nbf, naux, nocc = alkane_shape(size) if kind == 'alkane' else water_cluster_shape(size)
-            shapes[key] = synthetic_system_device(nbf, naux, nocc, seed=1000 + 97 * size + (kind == 'water'),
-                                                  name=f'{kind}{size}', scale=0.3)
+    # the molecules, from the GPU front end (not timed): alkanes from bond
+    # rules, water clusters seeded per request (rigid monomers, random
+    # orientations, O-O >= 2.7 A in a sphere)
+    t0 = time.perf_counter()
+    systems, cache = [], {}
+    with Engine(0) as fe:
+        for k, (_, kind, size) in enumerate(spec):
+            key = (kind, size) if kind == 'alkane' else (kind, size, args.seed * 100003 + k)
+            if key not in cache:
+                mol = alkane(size) if kind == 'alkane' else water_cluster(size, seed=key[2])
+                cache[key] = molecular_system_device(mol, f'{kind}{size}', engine=fe)
+            systems.append(cache[key])
     torch.cuda.synchronize()
     free, _ = torch.cuda.mem_get_info()
     budget = int(0.85 * free)
-    print(f'# {workload}: {len(spec)} requests, {len(shapes)} shapes resident, HBM budget {budget / 2**30:.0f} GiB',
-          file=sys.stderr, flush=True)
+    print(f'# {workload}: {len(spec)} requests, {len(cache)} molecules from the GPU front end in '
+          f'{time.perf_counter() - t0:.1f}s, HBM budget {budget / 2**30:.0f} GiB', file=sys.stderr, flush=True)
 
     def requests(precision):
-        return [Request(k, shapes[(kind, size)], arrival=t, technique='imaginary_pade', precision=precision,
+        return [Request(k, systems[k], arrival=t, technique='imaginary_pade', precision=precision,
                         nomega_chi_imag=32, nomega_sigma_calc=16) for k, (t, kind, size) in enumerate(spec)]
 
     with Server(nslots=1, hbm_budget=budget) as srv:
@@ -151,7 +159,7 @@ def stream_bench(workload, args):
     cpu = None
     if not args.no_cpu_baseline:
         print('# cpu baseline (oracle on a sample of the stream) ...', file=sys.stderr, flush=True)
-        cpu = cpu_baseline_stream(spec, shapes)
+        cpu = cpu_baseline_stream(spec, systems)
         print(f'# cpu: {cpu["value"]:.4f} solves/s; {cpu["sample"]}', file=sys.stderr, flush=True)
     sm['cpu_baseline'] = cpu
     return sm, ref_sum, err * 27211.38505, spec
@@ -396,47 +404,84 @@ def _stream_flops(nbf, naux, nocc, nomega=32, nse=16):
             + 4.0 * naux ** 3 * nomega + 2.0 * naux * naux * nstate * nse * nomega)
 
 
-def cpu_baseline_stream(spec, shapes, budget_s=15.0, window=8):
-    """CPU oracle (imaginary axis + Pade, FP64) on a bounded sample of the
-    stream's own requests, scaled to the whole stream by its operation count.
+def _blas_info():
+    """BLAS library and version numpy calls (the oracle's dense algebra)."""
+    try:
+        cfg = np.show_config(mode='dicts')
+        b = cfg['Build Dependencies']['blas']
+        return f"{b.get('name', '?')} {b.get('version', '')}".strip()
+    except Exception:                       # older numpy: no dict mode
+        return 'numpy BLAS (unknown)'
 
-    Distinct request shapes are timed from the median-size one upwards until
-    about ``budget_s`` seconds of CPU work; the measured rate (operations per
-    second over the sample) converts the stream's total count into seconds.
-    Value: requests per second served one at a time on the host."""
+
+def _timed(fn, repeats):
+    """Median wall time of ``repeats`` runs after one untimed warm-up run."""
+    fn()
+    ts = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline_stream(spec, systems, window=8, repeats=3):
+    """CPU oracle (imaginary axis + Pade, FP64; numpy/OpenBLAS) on three of
+    the stream's own molecules: the smallest, the median and the largest
+    request by operation count. Each is timed after one warm-up run as the
+    median of ``repeats`` runs. The smallest and the median run the whole
+    32-frequency solve; the largest is timed at 1 and 2 frequencies and its
+    32-frequency time follows from the measured cost per frequency (every
+    frequency repeats the same chi0 build, inversion and self-energy work).
+    The stream's time is the sum over its requests of the measured time of
+    the nearest timed molecule in operation count, scaled by the operation
+    count (AO->MO, chi0, 4 naux^3 per frequency, W.B). Value: requests per
+    second served one at a time on the host."""
     from oracle import gw_oracle as O
-    keys = sorted(shapes, key=lambda k: _stream_flops(shapes[k].nbf, shapes[k].naux, shapes[k].nocc))
-    start = len(keys) // 2
-    timed, t_sum, f_sum = [], 0.0, 0.0
-    for key in keys[start:]:
-        s = shapes[key]
+    order = sorted(range(len(systems)), key=lambda k: _stream_flops(systems[k].nbf, systems[k].naux,
+                                                                     systems[k].nocc))
+    picks = [order[0], order[len(order) // 2], order[-1]]
+    measured = []      # (flops, seconds for 32 frequencies, description)
+    for rank, k in enumerate(picks):
+        s = systems[k]
         f = _stream_flops(s.nbf, s.naux, s.nocc)
-        if timed and t_sum / f_sum * f > 2.0 * budget_s:
-            break
-        eri = s.eri_3center_dev.cpu().numpy().T
+        if measured and f == measured[-1][0]:
+            continue
+        eri = np.asfortranarray(s.eri_3center_dev.cpu().numpy().T)
         rng = O.selfenergy_set_state_range(s.occupation[:, None])
         rng.nsemin = max(rng.nsemin, rng.nhomo_G - window + 1)
         rng.nsemax = min(rng.nsemax, rng.nhomo_G + window)
-        t0 = time.perf_counter()
-        O.g0w0_pade(eri, s.index_basis, s.c_matrix, s.energy, s.occupation, 32, nomega_sigma_calc=16,
-                    ranges=rng)
-        dt = time.perf_counter() - t0
-        timed.append(f'{key[0]}{key[1]} (nbf {s.nbf}, naux {s.naux}) {dt:.1f}s')
-        t_sum += dt
-        f_sum += f
-        if t_sum >= budget_s:
-            break
-    total_f = sum(_stream_flops(shapes[(k, n)].nbf, shapes[(k, n)].naux, shapes[(k, n)].nocc) for _, k, n in spec)
-    seconds = total_f * t_sum / f_sum
+
+        def solve(nw):
+            return lambda: O.g0w0_pade(eri, s.index_basis, s.c_matrix, s.energy, s.occupation, nw,
+                                       nomega_sigma_calc=16, ranges=rng, exchange_m_vxc=s.exchange_m_vxc)
+        if rank < 2:
+            t, ts = _timed(solve(32), repeats)
+            measured.append((f, t, f'{s.name} (nbf {s.nbf}, naux {s.naux}) 32 frequencies: median {t:.2f}s of '
+                                   + ', '.join(f'{x:.2f}' for x in ts)))
+        else:
+            t1, _ = _timed(solve(1), repeats)
+            t2, _ = _timed(solve(2), repeats)
+            t = t1 + 31.0 * (t2 - t1)
+            measured.append((f, t, f'{s.name} (nbf {s.nbf}, naux {s.naux}): median {t1:.2f}s at 1 frequency, '
+                                   f'{t2:.2f}s at 2, so {t:.1f}s at 32'))
+        print(f'# cpu: {measured[-1][2]}', file=sys.stderr, flush=True)
+        del eri
+    seconds = 0.0
+    for sy in systems:
+        f = _stream_flops(sy.nbf, sy.naux, sy.nocc)
+        fm, tm, _ = min(measured, key=lambda m: abs(math.log(m[0] / f)))
+        seconds += tm * f / fm
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count()
-    threads = int(os.environ.get('OMP_NUM_THREADS', cores))
+    threads = int(os.environ.get('OPENBLAS_NUM_THREADS', os.environ.get('OMP_NUM_THREADS', cores)))
     return {'value': len(spec) / seconds, 'unit': 'solves/s', 'cores': min(cores, threads), 'kind': 'port',
-            'sample': ('oracle (numpy/OpenBLAS FP64, imaginary axis + Pade, 32 frequencies, window HOMO-7..LUMO+8) '
-                       'timed on ' + ', '.join(timed) + f'; the stream\'s {len(spec)} requests scaled by their '
-                       'operation count (AO->MO, chi0, 4 naux^3 per frequency, W.B)'),
+            'sample': ('oracle (numpy FP64 with ' + _blas_info() + f', {min(cores, threads)} threads; imaginary '
+                       'axis + Pade, window HOMO-7..LUMO+8) on three of the stream\'s molecules, one warm-up run and '
+                       f'the median of {repeats}: ' + '; '.join(m[2] for m in measured)
+                       + f'; the stream\'s {len(spec)} requests scaled from the nearest of them by operation count'),
             'seconds_for_stream': seconds}
 
 
@@ -572,7 +617,7 @@ def main():
             'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
             'dtype': f'{prec} products, f64 accumulation; f64 inversion and QP',
             'qp_error_vs_fp64_mev': err,
-            'data': 'synthetic RI tensors at the molecules\' shapes (seeded, molecular magnitudes)',
+            'data': 'real molecules: integrals, RI tensor and RI-HF orbitals from the GPU front end (no synthetic data)',
             'config': {'workload': desc, 'technique': 'imaginary_pade', 'nomega_chi_imag': 32,
                        'nomega_sigma_calc': 16, 'states': 'HOMO-7..LUMO+8', 'slots': args.slots,
                        'precision': prec,

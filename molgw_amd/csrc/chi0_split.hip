@@ -367,6 +367,209 @@ chi_ksplit_reduce_kernel(const float* __restrict__ part, int nkc, int tiles, int
   if (m > n) out[n + (int64_t)m * naux] = 0.0;
 }
 
+// ---------------------------------------------------------------- fused split
+// The two-plane modes (FP16X3, BF16X3) split inside the SYRK instead of in a
+// pre-pass per frequency: X is converted once to FP32 in the k-tiled layout
+// xf[kt][P][32], and every workgroup (frequency f, tile, K chunk) scales the
+// FP32 rows of its slab by w_f(t) = sqrt(s_f(t)) 2^e_f (a per-frequency FP32
+// table), splits y = x w into hi = E(y) and lo = E(y - hi) in registers and
+// stores the two planes to LDS. The arithmetic is the planes path's (FP32
+// product, then the split), so the results are the same bit for bit; what
+// goes away is writing and reading two planes per frequency through HBM
+// (4 bytes per element and frequency). Workgroups are ordered frequency
+// fastest: the frequencies of one tile and K chunk read the same X slabs and
+// share them in L2.
+__global__ void __launch_bounds__(256)
+chi_xf32_kernel(const double* __restrict__ x, int naux, int npole, int ppad, float* __restrict__ xf) {
+  __shared__ double tile[KT][65];
+  const int tid = threadIdx.x;
+  const int p0 = blockIdx.x * 64, kt = blockIdx.y, t0 = kt * KT;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int tl = (tid >> 6) + 4 * j, p = tid & 63;
+    const int P = p0 + p, t = t0 + tl;
+    tile[tl][p] = (P < naux && t < npole) ? x[P + (int64_t)t * naux] : 0.0;
+  }
+  __syncthreads();
+  const int p = tid >> 2, c8 = tid & 3;
+  f4 lo4, hi4;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    lo4[e] = (float)tile[8 * c8 + e][p];
+    hi4[e] = (float)tile[8 * c8 + 4 + e][p];
+  }
+  float* __restrict__ dst = xf + ((int64_t)kt * ppad + p0 + p) * KT + 8 * c8;
+  *reinterpret_cast<f4*>(dst) = lo4;
+  *reinterpret_cast<f4*>(dst + 4) = hi4;
+}
+
+// wf[f][t] = sqrt(s_f(t)) scale_f in FP32 (zero for t >= npole, up to nkt * 32)
+__global__ void __launch_bounds__(256)
+chi_wf_kernel(const double* __restrict__ sw, const double* __restrict__ scale, int npole, int ntpad,
+              float* __restrict__ wf) {
+  const int t = blockIdx.x * 256 + threadIdx.x, f = blockIdx.y;
+  if (t >= ntpad) return;
+  double w = 0.0;
+  if (t < npole) w = sqrt(sw[(int64_t)f * npole + t]) * (scale ? scale[f] : 1.0);
+  wf[(int64_t)f * ntpad + t] = (float)w;
+}
+
+struct FusedArgs {
+  const float* xf;       // [kt][ppad][32]
+  const float* wf;       // [f][nkt * 32]
+  int ppad, nkt, naux, nf, tiles, tm;
+  double* C;             // chi + f0 * naux^2
+  int64_t strideC;
+  int64_t nblocks;
+  int kc_slabs, nkc;     // K split (nkc == 1: none)
+  float* part;
+  const double* unscale; // per frequency of the chunk (FP16X3), else null
+};
+
+template <bool H, int BM, int WM, int WN, int FLUSH, bool KSPLIT>
+__global__ void __launch_bounds__(64 * WM * WN, KSPLIT ? 2 : 1)
+chi_fsyrk_kernel(FusedArgs g) {
+  typedef typename PlaneT<H>::E E;
+  typedef typename PlaneT<H>::V V;
+  constexpr int NS = 2;
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = BM / WM, TN = BM / WN;
+  constexpr int FM = TM / 16, FN = TN / 16;
+  constexpr int CH = BM * KT / 8;         // 8-k chunks per operand slab
+  constexpr int LPT = CH / NT;
+  static_assert(CH % NT == 0 && (NT % 4) == 0, "chunks per thread");
+  __shared__ __attribute__((aligned(16))) E As[NS][BM][KT];
+  __shared__ __attribute__((aligned(16))) E Bs[NS][BM][KT];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int64_t L = xcd_logical(blockIdx.x, g.nblocks);
+  const int f = (int)(L % g.nf);
+  const int64_t rest = L / g.nf;
+  const int tile = (int)(rest % g.tiles);
+  const int kc = KSPLIT ? (int)(rest / g.tiles) : 0;
+  int bm, bn;
+  tri_grouped(tile, g.tm, bm, bn);
+  const int m0 = bm * BM, n0 = bn * BM;
+  const float* __restrict__ Xa = g.xf + (int64_t)m0 * KT;
+  const float* __restrict__ Xb = g.xf + (int64_t)n0 * KT;
+  const float* __restrict__ W = g.wf + (int64_t)f * g.nkt * KT + 8 * (tid & 3);
+  const int64_t slab = (int64_t)g.ppad * KT;
+
+  f4 ra[LPT][2], rb[LPT][2], rw[2];
+  auto load = [&](int s) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      const int c = tid + i * NT;
+      const int64_t off = (int64_t)s * slab + (c >> 2) * KT + 8 * (c & 3);
+      ra[i][0] = *reinterpret_cast<const f4*>(Xa + off);
+      ra[i][1] = *reinterpret_cast<const f4*>(Xa + off + 4);
+      rb[i][0] = *reinterpret_cast<const f4*>(Xb + off);
+      rb[i][1] = *reinterpret_cast<const f4*>(Xb + off + 4);
+    }
+    rw[0] = *reinterpret_cast<const f4*>(W + (int64_t)s * KT);
+    rw[1] = *reinterpret_cast<const f4*>(W + (int64_t)s * KT + 4);
+  };
+  auto split_store = [&](f4 (&r)[LPT][2], E (&S)[NS][BM][KT]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      const int c = tid + i * NT, row = c >> 2, j = c & 3;
+      V hi, lo;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float y = r[i][e >> 2][e & 3] * rw[e >> 2][e & 3];
+        const E h = (E)y;
+        hi[e] = h;
+        lo[e] = (E)(y - (float)h);
+      }
+      *reinterpret_cast<V*>(&S[0][row][8 * lds_chunk(row, j)]) = hi;
+      *reinterpret_cast<V*>(&S[1][row][8 * lds_chunk(row, j)]) = lo;
+    }
+  };
+
+  double acc[FM][FN][4];
+  f4 part[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      part[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
+    }
+  const int fr = lane & 15, fq = lane >> 4;
+  const int nslab = g.nkt;
+  const int s0 = KSPLIT ? kc * g.kc_slabs : 0;
+  const int s1 = KSPLIT ? min(nslab, s0 + g.kc_slabs) : nslab;
+  load(s0);
+  for (int s = s0; s < s1; ++s) {
+    __syncthreads();                   // previous slab's fragments consumed
+    split_store(ra, As);
+    split_store(rb, Bs);
+    __syncthreads();
+    if (s + 1 < s1) load(s + 1);
+    V a[NS][FM], b[NS][FN];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        a[q][i] = *reinterpret_cast<const V*>(&As[q][wm * TM + i * 16 + fr][8 * lds_chunk(fr, fq)]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        b[q][j] = *reinterpret_cast<const V*>(&Bs[q][wn * TN + j * 16 + fr][8 * lds_chunk(fr, fq)]);
+    }
+    // hi lo + lo hi + hi hi, product-major (small products first)
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const int qa = pr == 0 ? 1 : 0, qb = pr == 1 ? 1 : 0;
+          part[i][j] = mfma16x16x32<H>(b[qb][j], a[qa][i], part[i][j]);
+        }
+    if (!KSPLIT && ((s + 1) % FLUSH == 0 || s + 1 == nslab)) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[i][j][r] += (double)part[i][j][r];
+          part[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+  }
+
+  if constexpr (KSPLIT) {
+    float* __restrict__ P = g.part + ((int64_t)(f * g.tiles + tile) * g.nkc + kc) * (BM * BM);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          P[(wm * TM + i * 16 + fr) + (wn * TN + j * 16 + 4 * fq + r) * BM] = part[i][j][r];
+    return;
+  }
+  double* __restrict__ C = g.C + (int64_t)f * g.strideC;
+  const double us = g.unscale ? g.unscale[f] : 1.0;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int m = m0 + wm * TM + i * 16 + fr;
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn * TN + j * 16 + 4 * fq + r;
+        if (m >= g.naux || n >= g.naux || m < n) continue;
+        double v = acc[i][j][r] * us;
+        if (m == n) v += 1.0;
+        C[m + (int64_t)n * g.naux] = v;
+        if (m > n) C[n + (int64_t)m * g.naux] = 0.0;
+      }
+  }
+}
+
 // Pipeline of the SYRK. Measured on MI355X (1200/3600/120, 32 frequencies):
 // two LDS buffers for two planes (BF16x3, 731 vs 850 ms with two register
 // sets), two register sets for three planes (BF16x6, 1392 vs 1419 ms).
@@ -552,6 +755,121 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
   return MGW_OK;
 }
 
+// Fused split (two planes): layout and K chunking. The K chunk keeps the
+// FP32 accumulation short (at most KC_MAX slabs) and the grid large enough to
+// fill the device; a problem with enough (tile, frequency) workgroups runs
+// without a K split and with the FP64 flush every FUSED_FLUSH slabs, which
+// writes no partial tiles. MGW_CHI_FUSED_KSPLIT = 0 / 1 forces one of the two.
+constexpr int FUSED_FLUSH = 4;
+constexpr int KC_MAX = 256;
+constexpr int64_t FILL_BLOCKS = 2048;     // 256 CUs x 8 workgroups
+
+struct FusedLayout {
+  int ppad, nkt, tm, tiles, kc_slabs, nkc;
+  bool ksplit;
+  size_t xf_bytes, wf_bytes, part_per_f;
+};
+
+// read at every call (a getenv per chi0 build), so that a process can compare the variants
+int fused_ksplit_forced() {
+  const char* e = std::getenv("MGW_CHI_FUSED_KSPLIT");
+  return e ? std::atoi(e) : -1;
+}
+
+FusedLayout fused_layout(int naux, int npole, int nomega) {
+  constexpr int bm = 128;
+  FusedLayout L;
+  L.ppad = (naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
+  L.nkt = (npole + KT - 1) / KT;
+  L.tm = (naux + bm - 1) / bm;
+  L.tiles = L.tm * (L.tm + 1) / 2;
+  const int64_t base = (int64_t)L.tiles * nomega;
+  const int forced = fused_ksplit_forced();
+  L.ksplit = forced >= 0 ? forced != 0 : base < FILL_BLOCKS;
+  if (L.ksplit) {
+    const int64_t want = (FILL_BLOCKS + base - 1) / base;
+    int kc = (int)std::max<int64_t>(KC_SLABS, (L.nkt + want - 1) / want);
+    kc = std::min(kc, KC_MAX);
+    L.kc_slabs = kc;
+    L.nkc = (L.nkt + kc - 1) / kc;
+  } else {
+    L.kc_slabs = L.nkt;
+    L.nkc = 1;
+  }
+  L.xf_bytes = ((size_t)L.nkt * L.ppad * KT * sizeof(float) + 255) / 256 * 256;
+  L.wf_bytes = ((size_t)nomega * L.nkt * KT * sizeof(float) + 255) / 256 * 256;
+  L.part_per_f = L.ksplit ? (size_t)L.tiles * L.nkc * bm * bm * sizeof(float) : 0;
+  return L;
+}
+
+// MGW_CHI_FUSED = 0 selects the per-frequency planes for the two-plane modes
+bool fused_enabled() {
+  const char* e = std::getenv("MGW_CHI_FUSED");
+  return e ? std::atoi(e) != 0 : true;
+}
+
+size_t fused_workspace(int naux, int npole, int nomega) {
+  const FusedLayout L = fused_layout(naux, npole, nomega);
+  size_t nf = (size_t)nomega;
+  if (L.part_per_f) {
+    const size_t budget = std::max<size_t>((size_t)32 * naux * npole, (size_t)2 << 30);
+    nf = std::max<size_t>(1, std::min<size_t>((size_t)nomega, budget / L.part_per_f));
+  }
+  return scale_header_bytes(nomega) + L.xf_bytes + L.wf_bytes + nf * L.part_per_f + 512;
+}
+
+template <bool H>
+int fused_split_syrk(const double* x, int naux, int npole, const double* sw, int nomega, void* work,
+                     size_t work_bytes, double* chi, hipStream_t stream) {
+  constexpr int BM = 128;
+  const FusedLayout L = fused_layout(naux, npole, nomega);
+  const size_t head = scale_header_bytes(nomega);
+  MGW_REQUIRE(work_bytes >= head + L.xf_bytes + L.wf_bytes + 256, MGW_ERR_ARG, "chi0 fused: workspace too small");
+  size_t nf_max = (size_t)nomega;
+  if (L.part_per_f) nf_max = std::min(nf_max, (work_bytes - head - L.xf_bytes - L.wf_bytes - 256) / L.part_per_f);
+  MGW_REQUIRE(nf_max >= 1, MGW_ERR_ARG, "chi0 fused: workspace smaller than one frequency");
+  double* scl = static_cast<double*>(work);
+  double* uns = scl + nomega;
+  double* xpart = uns + nomega;
+  float* xf = reinterpret_cast<float*>(static_cast<char*>(work) + head);
+  float* wf = reinterpret_cast<float*>(static_cast<char*>(work) + head + L.xf_bytes);
+  float* part = reinterpret_cast<float*>(static_cast<char*>(work) + head + L.xf_bytes + L.wf_bytes);
+  if (H) {
+    MGW_TRY(launch_absmax(x, (int64_t)naux * npole, xpart, stream));
+    hipLaunchKernelGGL(chi_scale_kernel, dim3(nomega), dim3(256), 0, stream, xpart, NB_MAX, sw, npole, scl, uns);
+    MGW_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(chi_xf32_kernel, dim3(L.ppad / 64, L.nkt), dim3(256), 0, stream, x, naux, npole, L.ppad, xf);
+  MGW_HIP(hipGetLastError());
+  const int ntpad = L.nkt * KT;
+  hipLaunchKernelGGL(chi_wf_kernel, dim3((ntpad + 255) / 256, nomega), dim3(256), 0, stream, sw,
+                     H ? scl : nullptr, npole, ntpad, wf);
+  MGW_HIP(hipGetLastError());
+  for (int f0 = 0; f0 < nomega; f0 += (int)nf_max) {
+    const int nf = std::min((int)nf_max, nomega - f0);
+    FusedArgs g;
+    g.xf = xf;
+    g.wf = wf + (int64_t)f0 * ntpad;
+    g.ppad = L.ppad; g.nkt = L.nkt; g.naux = naux; g.nf = nf; g.tiles = L.tiles; g.tm = L.tm;
+    g.C = chi + (int64_t)f0 * naux * naux; g.strideC = (int64_t)naux * naux;
+    g.kc_slabs = L.kc_slabs; g.nkc = L.nkc; g.part = part;
+    g.unscale = H ? uns + f0 : nullptr;
+    g.nblocks = (int64_t)L.tiles * L.nkc * nf;
+    MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chi0 fused: too many tiles");
+    const dim3 grid((unsigned)g.nblocks), block(256);
+    if (L.ksplit) {
+      hipLaunchKernelGGL((chi_fsyrk_kernel<H, BM, 2, 2, FUSED_FLUSH, true>), grid, block, 0, stream, g);
+      MGW_HIP(hipGetLastError());
+      hipLaunchKernelGGL((chi_ksplit_reduce_kernel<BM>), dim3(BM * BM / 256, L.tiles, nf), block, 0, stream, part,
+                         L.nkc, L.tiles, L.tm, naux, g.unscale, g.C, g.strideC);
+    } else {
+      hipLaunchKernelGGL((chi_fsyrk_kernel<H, BM, 2, 2, FUSED_FLUSH, false>), grid, block, 0, stream, g);
+    }
+    MGW_HIP(hipGetLastError());
+  }
+  return MGW_OK;
+}
+
 }  // namespace
 
 size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision) {
@@ -563,6 +881,7 @@ size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision) {
 // when they fit in max(4 x the FP64 operand, 2 GiB), else as many as fit (at
 // least one).
 size_t chi0_split_workspace(int naux, int npole, int nomega, int precision) {
+  if (precision != MGW_BF16X6 && fused_enabled()) return fused_workspace(naux, npole, nomega);
   const size_t per_f = chi0_split_bytes_per_frequency(naux, npole, precision);
   const size_t budget = std::max<size_t>((size_t)32 * naux * npole, (size_t)2 << 30);
   const size_t nf = std::max<size_t>(1, std::min<size_t>((size_t)nomega, budget / per_f));
@@ -573,6 +892,10 @@ int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int 
                     void* work, size_t work_bytes, double* chi, hipStream_t stream) {
   if (naux <= 0 || nomega <= 0) return MGW_OK;
   MGW_REQUIRE(npole > 0, MGW_ERR_ARG, "chi0 split: empty transition space");
+  if (precision == MGW_BF16X3 && fused_enabled())
+    return fused_split_syrk<false>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
+  if (precision == MGW_FP16X3 && fused_enabled())
+    return fused_split_syrk<true>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
   if (precision == MGW_BF16X3)
     return split_and_syrk<2, false>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
   if (precision == MGW_BF16X6)

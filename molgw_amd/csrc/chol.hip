@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace mgw {
 namespace {
@@ -99,6 +100,85 @@ chol_diag_kernel(double* __restrict__ A, int n, int64_t strideA, int k, int kb, 
   }
 }
 
+// The same factorisation and inverse with four waves per matrix and the
+// block in LDS: every step of the right-looking elimination (and of the
+// column sweep of the inverse) is one barrier and a few FMAs per thread.
+// Elimination without the square roots: at step j every element (i, l),
+// j < l <= i, loses A(i, j) A(l, j) / A(j, j); at the end L(i, j) =
+// A(i, j) / sqrt(A(j, j)). The inverse works on U = T diag(L) (rows of T
+// times their diagonal L), so that row l is final, unscaled, when step l
+// reads it: U(i, j) -= L(i, l) U(l, j) / L(l, l) for i > l, j <= l, and
+// T(i, j) = U(i, j) / L(i, i) at the end. A non-positive pivot is recorded
+// and replaced by 1, as in chol_diag_kernel.
+constexpr int LP = NB + 1;   // LDS row pitch (doubles)
+
+__global__ void __launch_bounds__(256)
+chol_leaf_kernel(double* __restrict__ A, int n, int64_t strideA, int k, int kb, double* __restrict__ logdet,
+                 int32_t* __restrict__ info) {
+  __shared__ double a[NB * LP];     // a[i + j LP]: the block, then L
+  __shared__ double u[NB * LP];     // U, then T
+  __shared__ double piv[NB];
+  __shared__ int fail;
+  const int f = blockIdx.x, tid = threadIdx.x;
+  double* __restrict__ g = A + (int64_t)f * strideA + k + (int64_t)k * n;
+  if (tid == 0) fail = 0;
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e % NB, j = e / NB;
+    a[i + j * LP] = (i < kb && j < kb) ? (j <= i ? g[i + (int64_t)j * n] : 0.0) : (i == j ? 1.0 : 0.0);
+    u[i + j * LP] = i == j ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < kb; ++j) {
+    double p = a[j + j * LP];
+    if (!(p > 0.0)) {
+      if (tid == 0 && fail == 0) fail = k + j + 1;
+      p = 1.0;
+    }
+    const double rp = 1.0 / p;
+    const int m = kb - 1 - j;              // trailing size
+    for (int e = tid; e < m * m; e += 256) {
+      const int i = j + 1 + e % m, l = j + 1 + e / m;
+      if (i >= l) a[i + l * LP] -= a[i + j * LP] * a[l + j * LP] * rp;
+    }
+    if (tid == 0) piv[j] = p;
+    __syncthreads();
+  }
+  // L(i, j) = A(i, j) / sqrt(pivot j)
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e % NB, j = e / NB;
+    if (j < kb && i >= j && i < kb) a[i + j * LP] = (i == j) ? sqrt(piv[j]) : a[i + j * LP] / sqrt(piv[j]);
+  }
+  __syncthreads();
+  for (int l = 0; l < kb; ++l) {
+    const double rl = 1.0 / a[l + l * LP];
+    const int rows = kb - 1 - l, cols = l + 1;
+    for (int e = tid; e < rows * cols; e += 256) {
+      const int i = l + 1 + e % rows, j = e / rows;
+      u[i + j * LP] -= a[i + l * LP] * u[l + j * LP] * rl;
+    }
+    __syncthreads();
+  }
+  double ld = 0.0;
+  if (tid == 0)
+    for (int j = 0; j < kb; ++j) ld += log(a[j + j * LP]);
+  for (int e = tid; e < kb * kb; e += 256) {
+    const int i = e % kb, j = e / kb;
+    g[i + (int64_t)j * n] = (j <= i) ? u[i + j * LP] / a[i + i * LP] : 0.0;
+  }
+  if (tid == 0) {
+    logdet[f] = (k == 0 ? 0.0 : logdet[f]) + ld;
+    if (k == 0) info[f] = 0;
+    if (fail && info[f] == 0) info[f] = fail;
+  }
+}
+
+// Leaf kernel: MGW_CHOL_LEAF = 0 (one wave, registers) or 1 (four waves,
+// LDS); read at every factorisation.
+bool chol_leaf_lds() {
+  const char* e = std::getenv("MGW_CHOL_LEAF");
+  return e ? std::atoi(e) != 0 : true;
+}
+
 int gemm_chol(const GemmF64& g0, hipStream_t stream) {
   GemmF64 g = g0;
   g.klass = MGW_KCLASS_CHOLESKY;
@@ -174,8 +254,12 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   const int n = c.n;
   const int64_t s2 = (int64_t)n * n;
   if (m <= NB) {
-    hipLaunchKernelGGL(chol_diag_kernel, dim3(c.nbatch), dim3(64), 0, c.stream, c.A, n, s2, k0, m, c.logdet,
-                       c.info);
+    if (chol_leaf_lds())
+      hipLaunchKernelGGL(chol_leaf_kernel, dim3(c.nbatch), dim3(256), 0, c.stream, c.A, n, s2, k0, m, c.logdet,
+                         c.info);
+    else
+      hipLaunchKernelGGL(chol_diag_kernel, dim3(c.nbatch), dim3(64), 0, c.stream, c.A, n, s2, k0, m, c.logdet,
+                         c.info);
     MGW_HIP(hipGetLastError());
     return MGW_OK;
   }

@@ -54,7 +54,7 @@ module m_mgw
   public :: mgw_polarizability_rpa, mgw_polarizability_imag
   public :: mgw_selfenergy_poles, mgw_selfenergy_imag, mgw_selfenergy_contour, mgw_pade
   public :: mgw_qp_energy, mgw_one_shot
-  public :: mgw_selfenergy_qs, mgw_gw_ingredients
+  public :: mgw_selfenergy_qs, mgw_gw_ingredients, mgw_exchange_m_vxc
 
   type(c_ptr), save :: ctx = c_null_ptr
 
@@ -288,6 +288,15 @@ module m_mgw
       real(c_double) :: w(*), v(*)
       integer(c_int) :: ierr
     end function c_gw_ingredients
+
+    function c_exchange_m_vxc(handle, hamiltonian_fock, exchange_m_vxc, diag) &
+        bind(C, name='mgw_exchange_m_vxc') result(ierr)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      real(c_double), intent(in) :: hamiltonian_fock(*)
+      real(c_double) :: exchange_m_vxc(*), diag(*)
+      integer(c_int) :: ierr
+    end function c_exchange_m_vxc
 
     function c_strlen(s) bind(C, name='strlen') result(n)
       import :: c_ptr, c_size_t
@@ -554,5 +563,18 @@ subroutine mgw_gw_ingredients(ranges, w, v)
   real(dp), intent(out) :: w(:, :, :), v(:, :, :)
   call mgw_check(c_gw_ingredients(ctx, ranges, w, v), 'mgw_gw_ingredients')
 end subroutine mgw_gw_ingredients
+
+!=========================================================================
+! setup_exchange_m_vxc (src/m_selfenergy_tools.f90:720-802), the branch without
+! a DFT core-valence split: exchange_m_vxc(:,:,ispin) = C^T F C - diag(energy)
+! for the uploaded MO data (mgw_upload_mo / mgw_upload_mo_spin)
+subroutine mgw_exchange_m_vxc(hamiltonian_fock, exchange_m_vxc)
+  real(dp), intent(in)  :: hamiltonian_fock(:, :, :)
+  real(dp), intent(out) :: exchange_m_vxc(:, :, :)
+  real(dp), allocatable :: diag(:, :)
+  allocate(diag(SIZE(exchange_m_vxc, DIM=1), SIZE(exchange_m_vxc, DIM=3)))
+  call mgw_check(c_exchange_m_vxc(ctx, hamiltonian_fock, exchange_m_vxc, diag), 'mgw_exchange_m_vxc')
+  deallocate(diag)
+end subroutine mgw_exchange_m_vxc
 
 end module m_mgw

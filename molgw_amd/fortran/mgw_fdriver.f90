@@ -15,7 +15,9 @@
 !                  selfenergy_mo(nstate, nstate), selfenergy_ao(nbf, nbf) (the
 !                  QSGW matrix on the Casida W, S = identity),
 !                  w(npole, nstate, nstate), v(nauxil, nstate, nstate)
-!                  (dump_gw_ingredients arrays, all states), w_s(nauxil, npole)
+!                  (dump_gw_ingredients arrays, all states), w_s(nauxil, npole),
+!                  exchange_m_vxc(nstate, nstate) of the Fock matrix
+!                  C diag(energy(p) + 0.001 p) C^T
 !=========================================================================
 program mgw_fdriver
   use m_mgw
@@ -31,7 +33,7 @@ program mgw_fdriver
   complex(dp), allocatable :: sigma(:, :), sigma2(:, :), sigma3(:, :, :)
   real(dp) :: en_rpa, en_gw, energies(2)
   real(dp), allocatable :: s_matrix(:, :), sigma_qs_mo(:, :), sigma_qs_ao(:, :), w_ing(:, :, :), v_ing(:, :, :)
-  real(dp), allocatable :: w_s(:, :)
+  real(dp), allocatable :: w_s(:, :), fock(:, :, :), x_m_vxc(:, :, :)
   type(mgw_ranges) :: ranges
   type(mgw_g0w0_params) :: params
   character(len=512) :: fin, fout
@@ -90,6 +92,16 @@ program mgw_fdriver
   allocate(w_ing(SIZE(pole), nstate, nstate), v_ing(nauxil, nstate, nstate))
   call mgw_gw_ingredients(ranges, w_ing, v_ing)
 
+  ! Sigma_x - Vxc of a Fock matrix C diag(energy + 0.001 p) C^T (orthonormal C, S = 1)
+  allocate(fock(nbf, nbf, 1), x_m_vxc(nstate, nstate, 1))
+  fock(:, :, 1) = 0.0_dp
+  do istate = 1, nstate
+    do k = 1, nbf
+      fock(:, k, 1) = fock(:, k, 1) + c_matrix(:, istate) * (energy(istate) + 0.001_dp * istate) * c_matrix(k, istate)
+    end do
+  end do
+  call mgw_exchange_m_vxc(fock, x_m_vxc)
+
   ! the same solve in one call
   allocate(eqp2(nstate), zz2(nstate), elin2(nstate), sigma2(nomega_sigma, nstate))
   call mgw_one_shot(params, ranges, exc, eqp2, zz2, elin2, sigma2, energies)
@@ -103,7 +115,7 @@ program mgw_fdriver
 
   open(newunit=iu, file=TRIM(fout), access='stream', form='unformatted', status='replace', action='write')
   write(iu) energy_qp, zz, energy_qp_lin, en_rpa, en_gw, eqp2, eqp3
-  write(iu) sigma_qs_mo, sigma_qs_ao, w_ing, v_ing, w_s
+  write(iu) sigma_qs_mo, sigma_qs_ao, w_ing, v_ing, w_s, x_m_vxc
   close(iu)
   write(*, '(a,i0,a,f16.10)') ' mgw_fdriver: nstate=', nstate, ' E_RPA=', en_rpa
 end program mgw_fdriver

@@ -119,7 +119,7 @@ class Server:
         rng.nsemax = min(rng.nsemax, rng.nhomo_G + req.window)
         out = eng.g0w0(rng, technique=req.technique, precision=req.precision,
                        nomega_chi_imag=req.nomega_chi_imag, nomega_sigma_calc=req.nomega_sigma_calc,
-                       return_sigma=False)
+                       exchange_m_vxc=getattr(s, 'exchange_m_vxc', None), return_sigma=False)
         end = time.perf_counter() - t0
         return Result(req.rid, out.energy_qp, out.zz, out.en_rpa, out.en_gw, req.arrival, start, end, k,
                       rng.nsemin, rng.nsemax)

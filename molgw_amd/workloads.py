@@ -18,6 +18,7 @@ by the benchmark are given by ``alkane_shape`` and ``water_cluster_shape``
 """
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -119,6 +120,45 @@ def dense_ri(sys: System) -> np.ndarray:
     return out
 
 
+BASIS_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'tests', 'golden', 'basis')
+
+
+def molecular_system_device(mol, name: str, basis: str = 'cc-pVDZ', auxil_basis: str = 'cc-pVDZ-RI',
+                            basis_dir: str | None = None, engine=None):
+    """A real molecule's G0W0 inputs from the GPU front end (device
+    integrals, RI-HF with J and K through the resident RI tensor,
+    Sigma_x - Vxc from its Fock matrix): a DeviceSystem whose packed RI tensor
+    eri_3center(npair, naux_kept) stays in HBM (a torch tensor used as
+    device memory), with the orbitals, energies, occupations and the
+    diagonal of Sigma_x - Vxc on the host."""
+    import torch
+    from .frontend import run_rihf_device
+    from .frontend.basis import BasisSet
+    from .gw import Engine
+    bdir = basis_dir or BASIS_DIR
+    own = engine is None
+    eng = Engine(0) if own else engine
+    try:
+        nbf = BasisSet(mol.atoms, mol.coords, basis, bdir).nbf
+        npair = nbf * (nbf + 1) // 2
+        # the kept auxiliary count is known after the metric; allocate for all
+        naux_bas = BasisSet(mol.atoms, mol.coords, auxil_basis, bdir).nbf
+        buf = torch.empty((naux_bas, npair), dtype=torch.float64, device='cuda')
+        hf = run_rihf_device(mol, basis, auxil_basis, bdir, eng, eri_3center_out=buf.data_ptr())
+        eri = buf[:hf.naux] if hf.naux < naux_bas else buf
+        eng.set_mo(hf.c_matrix, hf.energy_mo, hf.occupation)
+        _, exc = eng.exchange_m_vxc(hf.hamiltonian_fock)
+    finally:
+        if own:
+            eng.close()
+    torch.cuda.synchronize()
+    sysd = DeviceSystem(name, hf.nbf, hf.naux, hf.nocc, eri, hf.index_basis, np.asfortranarray(hf.c_matrix),
+                        hf.energy_mo, hf.occupation)
+    sysd.exchange_m_vxc = exc
+    sysd.scf_energy = hf.energy
+    return sysd
+
+
 @dataclass
 class DeviceSystem:
     """A synthetic system whose packed RI tensor lives in device memory (torch
@@ -132,6 +172,8 @@ class DeviceSystem:
     c_matrix: np.ndarray
     energy: np.ndarray
     occupation: np.ndarray
+    exchange_m_vxc: np.ndarray | None = None   # diagonal of Sigma_x - Vxc (None: converged HF, zero)
+    scf_energy: float | None = None
 
     @property
     def nstate(self) -> int:

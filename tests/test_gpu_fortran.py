@@ -60,7 +60,12 @@ def test_fortran_driver_matches_oracle(tmp_path):
     v = raw[off:off + s.naux * n * n].reshape((s.naux, n, n), order='F')
     off += s.naux * n * n
     ws = raw[off:off + s.naux * npole].reshape((s.naux, npole), order='F')
-    assert off + s.naux * npole == raw.size
+    off += s.naux * npole
+    xmv = raw[off:off + n * n].reshape((n, n), order='F')
+    assert off + n * n == raw.size
+    # Sigma_x - Vxc through its wrapper: the planted shifts on the diagonal, nothing off it
+    np.testing.assert_allclose(np.diag(xmv), 0.001 * np.arange(1, n + 1), atol=1e-12)
+    assert np.abs(xmv - np.diag(np.diag(xmv))).max() < 1e-12
     rng = O.selfenergy_set_state_range(s.occupation[:, None])
     eri = O.calculate_eri_3center_mo(s.eri_3center, s.index_basis, s.c_matrix, 1, n, 1, n)
     ref_qs = O.gw_selfenergy_qs(eri, ws, ref.pole, s.energy, s.occupation, rng)
