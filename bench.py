@@ -68,9 +68,8 @@ def stream_bench(workload, args):
     FP64 drop-in, also the QP reference), then the serving pass (mixed
     precision, concurrent slots, continuous batching). A step is one request;
     the warm-up requests are served before the timed stream."""
-    import torch
     from molgw_amd.frontend import alkane, water_cluster
-    from molgw_amd.gw import Engine
+    from molgw_amd.gw import Engine, device_memory
     from molgw_amd.serving import Request, Server, summarize, poisson_stream, alkane_batch
     from molgw_amd.workloads import molecular_system_device
     if workload == 'serving':
@@ -91,8 +90,7 @@ def stream_bench(workload, args):
                 mol = alkane(size) if kind == 'alkane' else water_cluster(size, seed=key[2])
                 cache[key] = molecular_system_device(mol, f'{kind}{size}', engine=fe)
             systems.append(cache[key])
-    torch.cuda.synchronize()
-    free, _ = torch.cuda.mem_get_info()
+    free, _ = device_memory(0)
     budget = int(0.85 * free)
     print(f'# {workload}: {len(spec)} requests, {len(cache)} molecules from the GPU front end in '
           f'{time.perf_counter() - t0:.1f}s, HBM budget {budget / 2**30:.0f} GiB', file=sys.stderr, flush=True)
@@ -111,10 +109,8 @@ def stream_bench(workload, args):
             srv.run(requests(args.precision)[:args.warmup], realtime=False)  # untimed warm-up requests
         # capacity: the whole stream available at t = 0, as in the FP64 pass
         srv.profile(True)
-        torch.cuda.synchronize()
         w0 = (time.clock_gettime_ns(time.CLOCK_MONOTONIC), time.clock_gettime_ns(time.CLOCK_BOOTTIME))
         res = srv.run(requests(args.precision), realtime=False)
-        torch.cuda.synchronize()
         w1 = (time.clock_gettime_ns(time.CLOCK_MONOTONIC), time.clock_gettime_ns(time.CLOCK_BOOTTIME))
         # the capacity pass's clock window, to select its kernels in a kernel trace
         print(f'# capacity window ns: monotonic {w0[0]} {w1[0]} boottime {w0[1]} {w1[1]}', file=sys.stderr, flush=True)
@@ -123,7 +119,6 @@ def stream_bench(workload, args):
         # latency: the same requests arriving as the Poisson stream
         trims_cap = srv.trims
         live = srv.run(requests(args.precision), realtime=True) if realtime else res
-        torch.cuda.synchronize()
         esc = srv.escalations()
         print(f'# admission trims: capacity pass {trims_cap}, live pass {srv.trims - trims_cap}', file=sys.stderr,
               flush=True)
@@ -133,10 +128,8 @@ def stream_bench(workload, args):
     with Server(nslots=1, hbm_budget=budget) as srv:
         srv.run(requests(args.precision), realtime=False)   # warm: buffers of every shape allocated
         srv.profile(True)
-        torch.cuda.synchronize()
         i0 = time.clock_gettime_ns(time.CLOCK_MONOTONIC)
         srv.run(requests(args.precision), realtime=False)
-        torch.cuda.synchronize()
         i1 = time.clock_gettime_ns(time.CLOCK_MONOTONIC)
         print(f'# isolated window ns: monotonic {i0} {i1}', file=sys.stderr, flush=True)
         prof_iso = srv.profile_read()
@@ -175,9 +168,8 @@ def gpu_bench(system, technique, steps, warmup, kw, precision='fp64'):
     if hasattr(system, 'eri_3center_dev'):
         eng.set_eri_3center_device(system.eri_3center_dev.data_ptr(), system.npair, system.naux,
                                    system.index_basis, system.nbf)
-        system.eri_3center_dev = None                    # the engine holds its own dense copy
-        import torch
-        torch.cuda.empty_cache()
+        system.eri_3center_dev.free()                    # the engine holds its own dense copy
+        system.eri_3center_dev = None
     else:
         eng.set_eri_3center(system.eri_3center, system.index_basis, system.nbf)
     eng.set_mo(system.c_matrix, system.energy, system.occupation)
@@ -447,7 +439,7 @@ def cpu_baseline_stream(spec, systems, window=8, repeats=3):
         f = _stream_flops(s.nbf, s.naux, s.nocc)
         if measured and f == measured[-1][0]:
             continue
-        eri = np.asfortranarray(s.eri_3center_dev.cpu().numpy().T)
+        eri = s.eri_3center_host()
         rng = O.selfenergy_set_state_range(s.occupation[:, None])
         rng.nsemin = max(rng.nsemin, rng.nhomo_G - window + 1)
         rng.nsemax = min(rng.nsemax, rng.nhomo_G + window)
@@ -494,8 +486,7 @@ def ri_build_bench(args):
     """Metric V^-1/2 and B build (src/m_eri_calculate.f90:1049-1262, :1877-1892)
     at the octane shape, (mu nu|Q) resident in HBM. A step is one metric
     eigendecomposition plus one B build over all pairs."""
-    import torch
-    from molgw_amd.gw import Engine
+    from molgw_amd.gw import DeviceBuffer, Engine
     nbf, naux, desc = RI_BUILD
     k, l = np.triu_indices(nbf)
     ib = np.vstack([k + 1, l + 1]).astype(np.int32)
@@ -506,10 +497,9 @@ def ri_build_bench(args):
     lam = np.geomspace(1e-8, 1e2, naux)
     v = np.asfortranarray((q * lam) @ q.T)
     v = 0.5 * (v + v.T)
-    g = torch.Generator(device='cuda:0')
-    g.manual_seed(args.seed)
-    tmp = torch.randn((naux, npair), device='cuda:0', dtype=torch.float64, generator=g)  # column-major (npair, naux)
-    tmp *= 0.1
+    tmp_h = np.asfortranarray(0.1 * rng.standard_normal((npair, naux)))     # (mu nu|Q), column-major
+    tmp = DeviceBuffer(tmp_h.nbytes)
+    tmp.upload(tmp_h)
     eng = Engine(0)
     for _ in range(args.warmup):
         eng.inverse_sqrt_eri_2center(v, copy=False)
@@ -543,7 +533,7 @@ def ri_build_bench(args):
         # oracle on the full metric and on a sample of the pairs, scaled by npair / sample
         from oracle import gw_oracle as O
         sample = 16384
-        tmp_h = tmp[:, :sample].T.cpu().numpy()
+        tmp_h = np.ascontiguousarray(tmp_h[:sample])
         c0 = time.perf_counter()
         sq, _ = O.inverse_sqrt_eri_2center(v)
         c1 = time.perf_counter()

@@ -135,6 +135,17 @@ int mgw_device_pool_report(int device, int reset_high, int64_t* reserved_bytes, 
  * budget). Contexts start with no limit (the pool keeps freed memory for
  * the next allocation). No reference counterpart. */
 int mgw_device_pool_limit(int device, int64_t bytes);
+/* Caller-side device memory, for callers without a HIP runtime of their own
+ * (the Python host side): a plain device allocation, its release, copies
+ * (kind 0 host to device, 1 device to host, 2 device to device; dst and src
+ * at byte offsets the caller computed) and the device's free and total
+ * memory. Buffers from here may be passed wherever an entry point takes
+ * `where` = MGW_MEM_DEVICE. The Fortran caller's counterpart is its own
+ * device allocation. No reference counterpart. */
+int mgw_device_alloc(int device, int64_t bytes, void** ptr);
+int mgw_device_free(void* ptr);
+int mgw_device_copy(void* dst, const void* src, int64_t bytes, int kind);
+int mgw_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
 /* Frequencies whose reduced-precision 1 - chi0 lost positive definiteness
  * in the Cholesky factorisation and were rebuilt in FP64 on the device,
  * counted since the context was created. No reference counterpart: the

@@ -56,6 +56,10 @@ SIGNATURES = {
     'mgw_memory_report': (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'mgw_device_pool_report': (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'mgw_device_pool_limit': (C.c_int, [C.c_int, C.c_int64]),
+    'mgw_device_alloc': (C.c_int, [C.c_int, C.c_int64, C.POINTER(C.c_void_p)]),
+    'mgw_device_free': (C.c_int, [C.c_void_p]),
+    'mgw_device_copy': (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
+    'mgw_device_memory': (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'mgw_precision_escalations': (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     'mgw_footprint_bytes': (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     'mgw_eri_2center_ao': (C.c_int, [_vp, C.POINTER(Shells), C.c_int, _vp]),

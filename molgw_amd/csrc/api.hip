@@ -1258,6 +1258,51 @@ int mgw_device_pool_limit(int device, int64_t bytes) {
   return MGW_OK;
 }
 
+int mgw_device_alloc(int device, int64_t bytes, void** ptr) {
+  MGW_REQUIRE(ptr && bytes >= 0, MGW_ERR_ARG, "mgw_device_alloc: invalid arguments");
+  int ndev = 0;
+  MGW_HIP(hipGetDeviceCount(&ndev));
+  MGW_REQUIRE(device >= 0 && device < ndev, MGW_ERR_ARG, "invalid device " + std::to_string(device));
+  MGW_HIP(hipSetDevice(device));
+  *ptr = nullptr;
+  if (bytes == 0) return MGW_OK;
+  const hipError_t e = hipMalloc(ptr, (size_t)bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    set_error("mgw_device_alloc: " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? MGW_ERR_NOMEM : MGW_ERR_HIP;
+  }
+  return MGW_OK;
+}
+
+int mgw_device_free(void* ptr) {
+  if (ptr) MGW_HIP(hipFree(ptr));
+  return MGW_OK;
+}
+
+int mgw_device_copy(void* dst, const void* src, int64_t bytes, int kind) {
+  MGW_REQUIRE(bytes >= 0 && kind >= 0 && kind <= 2, MGW_ERR_ARG, "mgw_device_copy: invalid arguments");
+  if (bytes == 0) return MGW_OK;
+  MGW_REQUIRE(dst && src, MGW_ERR_ARG, "mgw_device_copy: null pointer");
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                          : hipMemcpyDeviceToDevice;
+  MGW_HIP(hipMemcpy(dst, src, (size_t)bytes, k));
+  return MGW_OK;
+}
+
+int mgw_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes) {
+  int ndev = 0;
+  MGW_HIP(hipGetDeviceCount(&ndev));
+  MGW_REQUIRE(device >= 0 && device < ndev, MGW_ERR_ARG, "invalid device " + std::to_string(device));
+  MGW_HIP(hipSetDevice(device));
+  size_t f = 0, t = 0;
+  MGW_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = (int64_t)f;
+  if (total_bytes) *total_bytes = (int64_t)t;
+  return MGW_OK;
+}
+
 int mgw_memory_report(mgw_ctx* c, int64_t* current_bytes, int64_t* peak_bytes) {
   MGW_REQUIRE(c, MGW_ERR_ARG, "null context");
   if (current_bytes) *current_bytes = c->ledger.current;

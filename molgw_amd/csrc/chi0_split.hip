@@ -802,10 +802,15 @@ FusedLayout fused_layout(int naux, int npole, int nomega) {
   return L;
 }
 
-// MGW_CHI_FUSED = 0 selects the per-frequency planes for the two-plane modes
+// MGW_CHI_FUSED = 1 selects the fused split for the two-plane modes. Off by
+// default: measured on MI355X over the serving stream (real molecules, one
+// solve at a time), the chi0 stage took 22.3 ms per solve fused against
+// 13.5 ms with the per-frequency planes; the in-kernel scale and split of
+// both operands of every (tile, frequency) workgroup costs more VALU time
+// than the plane traffic it saves.
 bool fused_enabled() {
   const char* e = std::getenv("MGW_CHI_FUSED");
-  return e ? std::atoi(e) != 0 : true;
+  return e ? std::atoi(e) != 0 : false;
 }
 
 size_t fused_workspace(int naux, int npole, int nomega) {

@@ -58,6 +58,62 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+class DeviceBuffer:
+    """Device memory from libmgw (mgw_device_alloc), for arrays the host side
+    hands to the entry points with where = device. ``data_ptr()`` is the
+    device address; the buffer is freed by ``free()`` or when collected."""
+
+    def __init__(self, nbytes: int, device: int = 0):
+        self._lib = _lib.load()
+        p = C.c_void_p()
+        check(self._lib.mgw_device_alloc(int(device), int(nbytes), C.byref(p)))
+        self.ptr = p.value or 0
+        self.nbytes = int(nbytes)
+        self.device = device
+
+    def data_ptr(self, offset: int = 0) -> int:
+        if not self.ptr:
+            raise ValueError('device buffer already freed')
+        if not 0 <= offset <= self.nbytes:
+            raise ValueError(f'offset {offset} outside the buffer of {self.nbytes} bytes')
+        return self.ptr + offset
+
+    def upload(self, host: np.ndarray, offset: int = 0) -> None:
+        """Copy a contiguous host array to byte ``offset`` of the buffer."""
+        a = np.ascontiguousarray(host) if not (host.flags.c_contiguous or host.flags.f_contiguous) else host
+        if offset + a.nbytes > self.nbytes:
+            raise ValueError('upload past the end of the device buffer')
+        check(self._lib.mgw_device_copy(C.c_void_p(self.data_ptr(offset)), C.c_void_p(a.ctypes.data), a.nbytes, 0))
+
+    def download(self, out: np.ndarray, offset: int = 0) -> np.ndarray:
+        """Copy bytes from ``offset`` into the contiguous host array ``out``."""
+        if not (out.flags.c_contiguous or out.flags.f_contiguous):
+            raise ValueError('download needs a contiguous host array')
+        if offset + out.nbytes > self.nbytes:
+            raise ValueError('download past the end of the device buffer')
+        check(self._lib.mgw_device_copy(C.c_void_p(out.ctypes.data), C.c_void_p(self.data_ptr(offset)), out.nbytes, 1))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            check(self._lib.mgw_device_free(C.c_void_p(self.ptr)))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_memory(device: int = 0) -> tuple[int, int]:
+    """(free, total) bytes of the device's memory."""
+    lib = _lib.load()
+    f, t = C.c_int64(), C.c_int64()
+    check(lib.mgw_device_memory(int(device), C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
 @dataclass
 class StateRanges:
     """1-based state ranges of the reference (nsemin, ncore_G ...)."""

@@ -69,8 +69,8 @@ def footprint_bytes(req: Request) -> int:
 
 
 def device_memory_bytes(device: int = 0) -> int:
-    import torch
-    return int(torch.cuda.get_device_properties(device).total_memory)
+    from .gw import device_memory
+    return device_memory(device)[1]
 
 
 class Server:
@@ -132,12 +132,14 @@ class Server:
         return sum(self._slot_bytes(k) for k in range(len(self.slots)))
 
     def _others(self, k: int) -> int:
-        """HBM held beside slot k: the other slots' ledgers, or the device
-        pool's reserved memory (freed buffers the pool still holds count
-        too) less slot k's own, whichever is larger."""
-        ledgers = sum(self._slot_bytes(j) for j in range(len(self.slots)) if j != k)
-        reserved, _ = self.slots[k].pool_memory()
-        return max(ledgers, reserved - self.slots[k].memory()[0])
+        """HBM the other slots hold or have been promised: their ledgers, or
+        the footprints of the requests they run, whichever is larger. Freed
+        buffers that the device pool keeps for reuse are not counted: any slot
+        can allocate from them, and the pool's release threshold (set to the
+        budget) hands what exceeds the budget back to the device at each
+        synchronisation. Memory of the process outside this server (other
+        contexts, other libraries) is outside the budget."""
+        return sum(self._slot_bytes(j) for j in range(len(self.slots)) if j != k)
 
     def _trim_until(self, k: int, req) -> None:
         """Free the cached buffers of idle slots, largest first, until ``req``
@@ -153,10 +155,6 @@ class Server:
             if self.slots[j].memory()[0] > 0:
                 self.slots[j].trim()
                 self.trims += 1
-        if not fits():
-            # freed buffers that the device pool still holds
-            self.slots[k].trim()
-            self.trims += 1
 
     # ------------------------------------------------------------ run
     def run(self, requests: list[Request], realtime: bool = True) -> list[Result]:

@@ -128,13 +128,12 @@ def molecular_system_device(mol, name: str, basis: str = 'cc-pVDZ', auxil_basis:
     """A real molecule's G0W0 inputs from the GPU front end (device
     integrals, RI-HF with J and K through the resident RI tensor,
     Sigma_x - Vxc from its Fock matrix): a DeviceSystem whose packed RI tensor
-    eri_3center(npair, naux_kept) stays in HBM (a torch tensor used as
-    device memory), with the orbitals, energies, occupations and the
-    diagonal of Sigma_x - Vxc on the host."""
-    import torch
+    eri_3center(npair, naux_kept) stays in HBM (a libmgw device buffer), with
+    the orbitals, energies, occupations and the diagonal of Sigma_x - Vxc on
+    the host."""
     from .frontend import run_rihf_device
     from .frontend.basis import BasisSet
-    from .gw import Engine
+    from .gw import DeviceBuffer, Engine
     bdir = basis_dir or BASIS_DIR
     own = engine is None
     eng = Engine(0) if own else engine
@@ -142,17 +141,17 @@ def molecular_system_device(mol, name: str, basis: str = 'cc-pVDZ', auxil_basis:
         nbf = BasisSet(mol.atoms, mol.coords, basis, bdir).nbf
         npair = nbf * (nbf + 1) // 2
         # the kept auxiliary count is known after the metric; allocate for all
+        # (column-major (npair, naux): the kept columns come first)
         naux_bas = BasisSet(mol.atoms, mol.coords, auxil_basis, bdir).nbf
-        buf = torch.empty((naux_bas, npair), dtype=torch.float64, device='cuda')
+        buf = DeviceBuffer(8 * naux_bas * npair, eng.device)
         hf = run_rihf_device(mol, basis, auxil_basis, bdir, eng, eri_3center_out=buf.data_ptr())
-        eri = buf[:hf.naux] if hf.naux < naux_bas else buf
         eng.set_mo(hf.c_matrix, hf.energy_mo, hf.occupation)
         _, exc = eng.exchange_m_vxc(hf.hamiltonian_fock)
+        eng.synchronize()
     finally:
         if own:
             eng.close()
-    torch.cuda.synchronize()
-    sysd = DeviceSystem(name, hf.nbf, hf.naux, hf.nocc, eri, hf.index_basis, np.asfortranarray(hf.c_matrix),
+    sysd = DeviceSystem(name, hf.nbf, hf.naux, hf.nocc, buf, hf.index_basis, np.asfortranarray(hf.c_matrix),
                         hf.energy_mo, hf.occupation)
     sysd.exchange_m_vxc = exc
     sysd.scf_energy = hf.energy
@@ -161,13 +160,14 @@ def molecular_system_device(mol, name: str, basis: str = 'cc-pVDZ', auxil_basis:
 
 @dataclass
 class DeviceSystem:
-    """A synthetic system whose packed RI tensor lives in device memory (torch
-    tensor of shape (naux, npair), i.e. column-major (npair, naux))."""
+    """A system whose packed RI tensor lives in device memory: a libmgw
+    DeviceBuffer whose first 8 npair naux bytes hold eri_3center(npair, naux)
+    column-major."""
     name: str
     nbf: int
     naux: int
     nocc: int
-    eri_3center_dev: object      # torch.Tensor on cuda, (naux, npair) contiguous
+    eri_3center_dev: object      # molgw_amd.gw.DeviceBuffer
     index_basis: np.ndarray
     c_matrix: np.ndarray
     energy: np.ndarray
@@ -183,15 +183,20 @@ class DeviceSystem:
     def npair(self) -> int:
         return self.index_basis.shape[1]
 
+    def eri_3center_host(self) -> np.ndarray:
+        """eri_3center(npair, naux) copied to the host (Fortran order)."""
+        out = np.empty((self.npair, self.naux), order='F')
+        return self.eri_3center_dev.download(out)
+
 
 def synthetic_system_device(nbf: int, naux: int, nocc: int, seed: int = 0, name: str = 'synthetic',
-                            scale: float = 1.0, chunk: int = 65536) -> DeviceSystem:
-    """Same recipe as ``synthetic_system`` with the large factor G and the product
-    B = L G drawn and formed on the GPU (torch is used as device-memory plumbing),
-    for sizes whose RI tensor does not fit the host comfortably. C, energies and
-    L come from the same numpy stream as ``synthetic_system``; G comes from a
-    seeded torch generator, so values differ from the host recipe."""
-    import torch
+                            scale: float = 1.0, chunk: int = 32768) -> DeviceSystem:
+    """Same recipe as ``synthetic_system`` for sizes whose RI tensor does not
+    fit the host comfortably: B = L G is formed on the host a block of pairs
+    at a time (G drawn per block from the same seeded stream, so values
+    differ from the host recipe) and copied into a device buffer column by
+    column of the (npair, naux) layout."""
+    from .gw import DeviceBuffer
     rng = np.random.default_rng(seed)
     c = orthogonal(rng, nbf)
     e_occ = np.sort(rng.uniform(-1.2, -0.3, nocc))
@@ -201,20 +206,17 @@ def synthetic_system_device(nbf: int, naux: int, nocc: int, seed: int = 0, name:
     u = orthogonal(rng, naux)
     v = orthogonal(rng, naux)
     sv = 1.0 / (1.0 + np.arange(naux)) ** 1.5
-    lmat = torch.from_numpy((u * sv[None, :]) @ v.T).cuda()
+    lmat = (u * sv[None, :]) @ v.T
     ib = upper_pairs(nbf)
     npair = ib.shape[1]
-    gen = torch.Generator(device='cuda')
-    gen.manual_seed(seed)
-    b = torch.empty((naux, npair), dtype=torch.float64, device='cuda')
-    diag = torch.from_numpy(ib[0] == ib[1]).cuda()
+    diag = ib[0] == ib[1]
+    buf = DeviceBuffer(8 * naux * npair)
     for p0 in range(0, npair, chunk):
         p1 = min(npair, p0 + chunk)
-        g = torch.randn((naux, p1 - p0), generator=gen, dtype=torch.float64, device='cuda')
-        blk = lmat @ g
+        blk = lmat @ rng.standard_normal((naux, p1 - p0))      # (naux, pairs) C order
         if scale != 1.0:
             blk *= scale
         blk[:, diag[p0:p1]] *= 0.5
-        b[:, p0:p1] = blk
-    torch.cuda.synchronize()
-    return DeviceSystem(name, nbf, naux, nocc, b, ib, np.asfortranarray(c), energy, occupation)
+        for q in range(naux):                                  # column q of (npair, naux)
+            buf.upload(blk[q], 8 * (p0 + q * npair))
+    return DeviceSystem(name, nbf, naux, nocc, buf, ib, np.asfortranarray(c), energy, occupation)

@@ -5,9 +5,20 @@ its two K schedules against each other and against FP64.
 * With the K split (FP32 partial tiles of 64 slabs summed in FP64) the fused
   kernel does the same arithmetic in the same order as the planes path: the
   result is identical bit for bit.
-* Without the K split (FP32 MFMA sums flushed into FP64 every 4 slabs) it
-  agrees with the K-split result to 1e-6 relative and with the FP64 build
-  to the FP16x3 accuracy (1e-5 relative to the largest element of chi0).
+* Without the K split (FP32 MFMA sums flushed into FP64 every 4 slabs) the
+  products are the same and only the FP32 sums are shorter, so it must be at
+  least as accurate against the FP64 build as the planes path (within 1.5x,
+  a margin for the statistics of the rounding). chi = (1 - chi0)^-1 chi0
+  amplifies chi0's rounding by the condition number of 1 - chi0, so the
+  errors are stated on chi0 = 1 - (1 + chi)^-1, recovered on the host in
+  FP64. Bounds, relative to max |chi0| (a diagonal element, whose 576
+  transition products all have one sign): a two-plane split represents each
+  operand to u^2 (u = 2^-11 fp16, 2^-8 bf16), the three products drop lo*lo,
+  so a product is off by at most ~3 u^2 (7e-7 fp16, 4.6e-5 bf16); y = x w is
+  rounded to FP32 (2e-7); and each MFMA accumulation into an FP32 register
+  may round by 2^-23 of the running sum, 3 products x 18 slabs = 54 of them
+  before the FP64 sum (6.4e-6): FP16x3 within 8e-6, BF16x3 within 1e-4.
+  (Measured on MI355X: 3.2e-6 and 3.9e-6.)
 """
 import os
 
@@ -48,13 +59,22 @@ def test_fused_split_equals_the_planes(precision):
         e64, chi64 = _chi(eng, s, 'fp64', {})
     np.testing.assert_array_equal(chi_fused, chi_planes)
     assert e_fused == e_planes
-    scale = np.max(np.abs(chi64))
-    print(f'{precision}: flush vs K split {np.max(np.abs(chi_flush - chi_fused)) / scale:.2e}, '
-          f'vs FP64 {np.max(np.abs(chi_flush - chi64)) / scale:.2e} (relative to max |chi|)')
-    assert np.max(np.abs(chi_flush - chi_fused)) < 1e-6 * scale
-    tol = 1e-5 if precision == 'fp16x3' else 1e-3
-    assert np.max(np.abs(chi_flush - chi64)) < tol * scale
-    assert np.max(np.abs(chi_fused - chi64)) < tol * scale
+    c64, cpl, cfl = _chi0(chi64), _chi0(chi_planes), _chi0(chi_flush)
+    scale = np.max(np.abs(c64))
+    err_planes = np.max(np.abs(cpl - c64)) / scale
+    err_flush = np.max(np.abs(cfl - c64)) / scale
+    print(f'{precision}: chi0 vs FP64 (relative to max |chi0|): planes / fused K split {err_planes:.2e}, '
+          f'fused FP64 flush {err_flush:.2e}; E_RPA {e_planes - e64:.2e} / {e_flush - e64:.2e} Ha')
+    tol = 8e-6 if precision == 'fp16x3' else 1e-4
+    assert err_planes < tol and err_flush < tol
+    assert err_flush <= 1.5 * err_planes + 1e-15
+
+
+def _chi0(chi):
+    """chi0 = 1 - (1 + chi)^-1 for every frequency (chi is naux x naux x nomega)."""
+    n = chi.shape[0]
+    eye = np.eye(n)
+    return np.stack([eye - np.linalg.inv(eye + chi[:, :, f]) for f in range(chi.shape[2])], axis=2)
 
 
 def test_cholesky_leaf_kernels_agree():

@@ -11,8 +11,15 @@ and the same flow at the same basis on propane against the oracle:
     at 48 points (the quadrature error stated in the output);
   * 1/2 Tr[Sigma_c G] of the pole self-energy equals the Galitskii-Migdal
     E_GW of the Casida step, 1e-9 relative;
-  * the Casida and the imaginary-axis routes give the same HOMO/LUMO QP
-    energies within the stated Pade continuation error (1e-3 Ha);
+  * the imaginary-axis self-energy Sigma_c(z) at the Pade input points
+    z = E_F + 0.03 k i (k = 0..15; the reference's grid,
+    src/m_selfenergy_tools.f90:610-614), built from chi(iw) on 48
+    Gauss-Legendre points, equals the analytic pole sum of the Casida poles
+    and residues at the same complex z (the identity that pins the oracle's
+    imaginary-axis route), 1e-7 Ha: the quadrature error measured with the
+    oracle on propane cc-pVDZ is 4e-9 Ha at 32 points and 1e-12 Ha at 48.
+    The QP energies of the two routes are printed, not compared: the
+    difference is the Pade continuation's, which nothing bounds;
   * propane cc-pVTZ: QP energies against the oracle's g0w0_casida, 1e-8 Ha.
 
 Serving stream, extreme shapes: (H2O)32 (nbf 768, naux 2688) and C24H50
@@ -49,7 +56,7 @@ def _window(s, half=8):
 
 def _packed_host(s):
     """eri_3center(npair, naux) of a DeviceSystem on the host (Fortran order)."""
-    return np.asfortranarray(s.eri_3center_dev.cpu().numpy().T)
+    return s.eri_3center_host()
 
 
 def _load(eng, s):
@@ -82,16 +89,32 @@ def test_octane_casida_identities(octane):
         erpa_im, _, _ = eng.polarizability_grid(rng, om, wt)
         print(f'  E_RPA imaginary axis (48 points) {erpa_im:.10f}: difference {erpa_im - en_rpa:.2e} Ha', flush=True)
         assert abs(erpa_im - en_rpa) < 1e-6
-        # the imaginary axis + Pade route on the frontier states
+        # Sigma_c(z) on the imaginary axis (W from the 48-point grid, resident)
         win = _window(s, 2)
+        efermi = 0.5 * (s.energy[s.nocc - 1] + s.energy[s.nocc])
+        zc = efermi + 0.03j * np.arange(16)
+        sig_im = eng.gw_selfenergy_imag(win, zc)                      # (16, nse)
+        # the pole sum of the Casida W at the same z
+        _, _, pole, ws = eng.polarizability(rng, return_ws=True)
+        eri = eng.calculate_eri_3center_mo(win.nsemin, win.nsemax, 1, s.nstate)   # (P, m, i)
+        ref = np.zeros_like(sig_im)
+        for i in range(s.nstate):
+            w = ws.T @ eri[:, :, i]                                      # (npole, nse)
+            f = s.occupation[i] / 2.0
+            g = f / (zc[:, None] - s.energy[i] + pole[None, :]) + (1 - f) / (zc[:, None] - s.energy[i] - pole[None, :])
+            ref += g @ (w * w)
+        dsig = np.max(np.abs(sig_im - ref))
+        print(f'  Sigma_c(E_F + 0.03k i), k < 16, states {win.nsemin}..{win.nsemax}: imaginary axis vs Casida '
+              f'poles {dsig:.2e} Ha (max |Sigma_c| {np.max(np.abs(ref)):.3f})', flush=True)
+        assert dsig < 1e-7
+        # the imaginary axis + Pade route on the frontier states (information)
         pade = eng.g0w0(win, technique='imaginary_pade', nomega_chi_imag=48, nomega_sigma_calc=16,
                         exchange_m_vxc=s.exchange_m_vxc, return_sigma=False)
     sl = slice(win.nsemin - 1, win.nsemax)
-    d = np.abs(pade.energy_qp[sl] - casida.energy_qp[sl])
+    d = pade.energy_qp[sl] - casida.energy_qp[sl]
     gap = (casida.energy_qp[s.nocc] - casida.energy_qp[s.nocc - 1]) * HA_MEV / 1e3
-    print(f'  Casida QP gap {gap:.4f} eV; Pade route differs by at most {d.max():.2e} Ha on states '
-          f'{win.nsemin}..{win.nsemax}', flush=True)
-    assert d.max() < 1e-3
+    print(f'  Casida QP gap {gap:.4f} eV; Pade route minus Casida on states {win.nsemin}..{win.nsemax}: '
+          + ', '.join(f'{x * HA_MEV:.1f}' for x in d) + ' meV', flush=True)
 
 
 def test_propane_cc_pvtz_against_the_oracle():

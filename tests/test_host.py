@@ -99,6 +99,18 @@ def test_product_does_not_import_oracle():
                 assert 'import oracle' not in text and 'from oracle' not in text, f
 
 
+def test_product_does_not_use_torch():
+    """The product's device memory comes from libmgw (mgw_device_alloc): a
+    second HIP runtime in the process (torch's bundled one) cannot see the GPU
+    once libmgw's runtime holds it."""
+    paths = [os.path.join(ROOT, 'bench.py'), os.path.join(ROOT, '__graft_entry__.py')]
+    for dirpath, _, files in os.walk(os.path.join(ROOT, 'molgw_amd')):
+        paths += [os.path.join(dirpath, f) for f in files if f.endswith('.py')]
+    for path in paths:
+        text = open(path, errors='ignore').read()
+        assert 'import torch' not in text, path
+
+
 def test_fortran_interface_compiles(tmp_path):
     """m_mgw (the iso_c_binding module MOLGW would use) compiles and a program
     using it links against libmgw.so."""
