@@ -300,6 +300,23 @@ int mgw_calculate_eri_3center_mo(mgw_ctx* ctx, int mstate_min, int mstate_max,
                                  int nstate_min, int nstate_max, double* eri_3center_mo);
 /* destroy_eri_3center_mo (src/m_eri_ao_mo.f90:1018). */
 int mgw_destroy_eri_3center_mo(mgw_ctx* ctx);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the transform of mgw_calculate_eri_3center_mo for the
+ * active spin channel, FP64, written straight into the caller's device
+ * buffer eri_3center_mo_dev (nauxil x (mmax-mmin+1) x (nmax-nmin+1),
+ * column-major, P fastest). Nothing is allocated and nothing is left
+ * resident: the half-transformed tensor (nauxil x nbf x min(m range,
+ * n range) doubles) goes to a fixed device scratch array of
+ * MGW_NOALLOC_SCRATCH_DOUBLES doubles (64 MiB) per GPU, held until the
+ * process ends; larger blocks are refused with MGW_ERR_ARG. The launches
+ * are queued on the context's stream and the call returns without waiting
+ * (mgw_ctx_synchronize waits). Every context on a GPU shares its scratch
+ * array: calls that overlap on one GPU race. Needs the resident RI tensor
+ * and MO data (mgw_set_eri_3center or mgw_build_eri_3center, mgw_set_mo). */
+#define MGW_NOALLOC_SCRATCH_DOUBLES (8 * 1024 * 1024)
+int mgw_calculate_eri_3center_mo_noalloc(mgw_ctx* ctx, int mstate_min, int mstate_max, int nstate_min,
+                                         int nstate_max, double* eri_3center_mo_dev);
 
 /* -------------------------------------------- A3/A4: Casida RPA (analytic) */
 /* Number of resonant transitions of the transition table

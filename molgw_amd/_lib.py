@@ -76,6 +76,7 @@ SIGNATURES = {
     'mgw_set_spin': (C.c_int, [_vp, C.c_int]),
     'mgw_calculate_eri_3center_mo': (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     'mgw_destroy_eri_3center_mo': (C.c_int, [_vp]),
+    'mgw_calculate_eri_3center_mo_noalloc': (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]),
     'mgw_npole': (C.c_int, [_vp, C.POINTER(Ranges), C.POINTER(C.c_int)]),
     'mgw_set_spectral_function': (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int]),
     'mgw_gw_selfenergy_qs': (C.c_int, [_vp, C.POINTER(Ranges), C.c_double, C.c_double, _dp, _dp, _dp]),

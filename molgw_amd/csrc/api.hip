@@ -336,7 +336,11 @@ bool rectangular(const std::vector<int32_t>& ia, int& i0, int& i1, int& a0, int&
 // is contracted first: H(P, k, f) = B(P k, l) C(l, f) is one GEMM over the
 // dense resident tensor, then a GEMM batched over f contracts k and writes
 // straight into the (P, m, n) layout of the reference.
-int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int precision = MGW_FP64, int spin = -1) {
+// `half` (nullable): caller-provided space for the half-transformed tensor
+// (naux x nbf x min(nm, nn) doubles); null takes a context buffer. With it,
+// the call allocates nothing and times no GEMM candidates.
+int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int precision = MGW_FP64, int spin = -1,
+          double* half = nullptr) {
   MGW_TRY(check_mo_matches_ri(c));
   MGW_REQUIRE(mlo >= 1 && mhi <= c->nstate && mlo <= mhi && nlo >= 1 && nhi <= c->nstate && nlo <= nhi,
               MGW_ERR_ARG, "ao2mo: invalid state range");
@@ -350,9 +354,10 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int preci
   const int flo = first_n ? nlo : mlo;
   const int no = first_n ? nm : nn;
   const int olo = first_n ? mlo : nlo;
-  double* H = nullptr;
-  MGW_TRY(dbuf(c, "ao2mo_half", (size_t)naux * nbf * nf, &H));
+  double* H = half;
+  if (!H) MGW_TRY(dbuf(c, "ao2mo_half", (size_t)naux * nbf * nf, &H));
   GemmF64 g1;
+  g1.no_tune = half != nullptr;
   g1.M = naux * nbf; g1.N = nf; g1.K = nbf;
   g1.A = B; g1.lda = (int64_t)naux * nbf;
   g1.B = C + (int64_t)(flo - 1) * nbf; g1.ldb = nbf;
@@ -365,6 +370,7 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int preci
     MGW_TRY(gemm_f64(g1, c->stream));
   }
   GemmF64 g2;
+  g2.no_tune = half != nullptr;
   g2.M = naux; g2.N = no; g2.K = nbf;
   g2.A = H; g2.lda = naux; g2.strideA = (int64_t)naux * nbf;
   g2.B = C + (int64_t)(olo - 1) * nbf; g2.ldb = nbf; g2.strideB = 0;
@@ -1972,6 +1978,55 @@ int mgw_calculate_eri_3center_mo(mgw_ctx* c, int mstate_min, int mstate_max, int
   }
   MGW_HIP(hipStreamSynchronize(c->stream));
   return MGW_OK;
+}
+
+// ---------------------------------------------------------------- no-allocation AO -> MO
+// The half-transformed tensor of mgw_calculate_eri_3center_mo_noalloc lives
+// in this fixed array (one copy per GPU, held until the process ends). Host
+// code never passes `noalloc_scratch` itself to a kernel: its device address
+// is looked up per device with hipGetSymbolAddress on the first call there.
+constexpr int64_t NOALLOC_SCRATCH_DOUBLES = MGW_NOALLOC_SCRATCH_DOUBLES;
+static __device__ __align__(256) double noalloc_scratch[NOALLOC_SCRATCH_DOUBLES];
+
+int noalloc_scratch_ptr(int device, double** out) {
+  constexpr int MAXDEV = 64;
+  static std::mutex m;
+  static double* addr[MAXDEV] = {};
+  MGW_REQUIRE(device >= 0 && device < MAXDEV, MGW_ERR_ARG, "no-allocation scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(m);
+  if (!addr[device]) {
+    MGW_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      set_error(std::string("no-allocation scratch: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+      return MGW_ERR_HIP;
+    }
+    addr[device] = static_cast<double*>(p);
+  }
+  *out = addr[device];
+  return MGW_OK;
+}
+
+int mgw_calculate_eri_3center_mo_noalloc(mgw_ctx* c, int mstate_min, int mstate_max, int nstate_min,
+                                         int nstate_max, double* eri_3center_mo_dev) {
+  MGW_CTX(c);
+  MGW_REQUIRE(eri_3center_mo_dev, MGW_ERR_ARG, "mgw_calculate_eri_3center_mo_noalloc: null output");
+  MGW_REQUIRE(c->has_ri && c->has_mo, MGW_ERR_STATE,
+              "mgw_calculate_eri_3center_mo_noalloc: set the RI tensor and the MO data first");
+  MGW_REQUIRE(mstate_min >= 1 && mstate_max <= c->nstate && mstate_min <= mstate_max && nstate_min >= 1 &&
+                  nstate_max <= c->nstate && nstate_min <= nstate_max,
+              MGW_ERR_ARG, "mgw_calculate_eri_3center_mo_noalloc: invalid state range");
+  const int64_t nf = std::min(mstate_max - mstate_min + 1, nstate_max - nstate_min + 1);
+  const int64_t need = (int64_t)c->naux * c->nbf * nf;
+  MGW_REQUIRE(need <= NOALLOC_SCRATCH_DOUBLES, MGW_ERR_ARG,
+              "mgw_calculate_eri_3center_mo_noalloc: nauxil x nbf x min(m range, n range) = " +
+                  std::to_string(need) + " doubles exceeds the scratch of " +
+                  std::to_string(NOALLOC_SCRATCH_DOUBLES));
+  double* scratch = nullptr;
+  MGW_TRY(noalloc_scratch_ptr(c->device, &scratch));
+  return ao2mo(c, mstate_min, mstate_max, nstate_min, nstate_max, eri_3center_mo_dev, MGW_FP64, -1, scratch);
 }
 
 int mgw_destroy_eri_3center_mo(mgw_ctx* c) {

@@ -458,7 +458,7 @@ int gemm_f64(const GemmF64& h, hipStream_t stream) {
     load_table(t);
     auto it = t.table.find(key);
     if (it != t.table.end()) cfg = it->second;
-    else need_tune = t.mode == 1 && h.beta == 0.0;   // candidates overwrite C: only pure products
+    else need_tune = t.mode == 1 && h.beta == 0.0 && !h.no_tune;   // candidates overwrite C: only pure products
   }
   if (need_tune) {
     int best = cfg;

@@ -126,6 +126,9 @@ struct GemmF64 {
   // (FP64 operands rounded on the way to LDS, FP64 accumulation); the
   // triangular and no-mirror options are FP64 only
   int precision = MGW_FP64;
+  // use the tuned or default tiles without timing candidates (the timing
+  // creates events and synchronises the stream)
+  bool no_tune = false;
 };
 int gemm_f64(const GemmF64& g, hipStream_t stream);
 int gemm_lowp(const GemmF64& g, int precision, hipStream_t stream);

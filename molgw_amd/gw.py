@@ -535,6 +535,17 @@ class Engine:
     def destroy_eri_3center_mo(self) -> None:
         check(self._lib.mgw_destroy_eri_3center_mo(self._h))
 
+    def calculate_eri_3center_mo_noalloc(self, mstate_min, mstate_max, nstate_min, nstate_max,
+                                         out: DeviceBuffer) -> None:
+        """The same transform into the device buffer ``out`` (nauxil x m x n,
+        column-major) with no device allocation (mgw_calculate_eri_3center_mo_noalloc);
+        queued on the context's stream: call synchronize() before reading ``out``."""
+        need = 8 * self.naux * (mstate_max - mstate_min + 1) * (nstate_max - nstate_min + 1)
+        if out.nbytes < need:
+            raise ValueError(f'output buffer holds {out.nbytes} bytes, the block needs {need}')
+        check(self._lib.mgw_calculate_eri_3center_mo_noalloc(self._h, mstate_min, mstate_max, nstate_min,
+                                                             nstate_max, C.c_void_p(out.data_ptr())))
+
     # ------------------------------------------------------------ A4
     def npole(self, rng: StateRanges) -> int:
         n = C.c_int()
