@@ -207,7 +207,7 @@ LOWP_CLASSES = {'fp32': {'gemm_ao2mo': 'fp32', 'gemm_chi0': 'fp32', 'gemm_casida
                 'bf16x3': {'gemm_ao2mo': 'bf16x3', 'gemm_chi0': 'bf16x3', 'gemm_casida': 'bf16x3',
                            'gemm_wb': 'bf16x3'},
                 'bf16x6': {'gemm_chi0': 'bf16x6', 'gemm_wb': 'bf16x6'},
-                'fp16x3': {'gemm_chi0': 'fp16x3', 'gemm_wb': 'fp16x3'}}
+                'fp16x3': {'gemm_ao2mo': 'fp16x3', 'gemm_chi0': 'fp16x3', 'gemm_wb': 'fp16x3'}}
 
 
 def roofline_of(prof, precision='fp64', traffic_key=None, by='ms'):
@@ -624,6 +624,7 @@ def main():
                                       **{'pass': 'the same requests one at a time after the timed passes'}),
             'cpu_baseline': sm['cpu_baseline'],
             'stages_device_ms_summed_over_slots': {k: v['ms'] for k, v in sm['profile'].items()},
+            'stages_ms_per_solve_isolated': {k: round(v['ms'] / len(spec), 3) for k, v in sm['profile_isolated'].items()},
             'fp64_rebuilt_frequencies': sm['escalations'],
         }
         print(json.dumps(line), flush=True)

@@ -51,9 +51,11 @@ enum { MGW_MEM_HOST = 0, MGW_MEM_DEVICE = 1 };
  *   MGW_BF16X3 two-way bf16 split, 3 products (~2^-17 relative)
  *   MGW_BF16X6 three-way bf16 split for chi0 and the self-energy product,
  *              6 products (~2^-24, FP32 level); the transforms stay FP64
- *   MGW_FP16X3 two-way fp16 split of power-of-two-scaled operands for chi0
- *              and the self-energy product, 3 products (~2^-22); the
- *              transforms stay FP64                                       */
+ *   MGW_FP16X3 two-way fp16 split of power-of-two-scaled operands for chi0,
+ *              the self-energy product and the AO->MO transforms (FP32
+ *              accumulation over K = nbf), 3 products (~2^-22); the
+ *              environment variable MGW_AO2MO_FP16X3=0 keeps the transforms
+ *              FP64                                                       */
 enum { MGW_FP64 = 0, MGW_FP32 = 1, MGW_BF16X3 = 2, MGW_BF16X6 = 3, MGW_FP16X3 = 4 };
 
 enum { MGW_TECH_ONE_SHOT = 0, MGW_TECH_IMAGINARY_PADE = 1, MGW_TECH_CONTOUR = 2 };

@@ -237,7 +237,7 @@ int scratch_done(mgw_ctx* c, const std::string& name) {
 
 // Buffers that hold the caller's resident inputs survive a trim.
 bool is_resident_input(const std::string& name) {
-  return name == "ri_dense" || name == "ri_sqrtinv" || name == "c_matrix" || name == "energy" ||
+  return name == "ri_dense" || name == "ri_absmax" || name == "ri_sqrtinv" || name == "c_matrix" || name == "energy" ||
          name == "occupation";
 }
 
@@ -336,6 +336,16 @@ bool rectangular(const std::vector<int32_t>& ia, int& i0, int& i1, int& a0, int&
 // is contracted first: H(P, k, f) = B(P k, l) C(l, f) is one GEMM over the
 // dense resident tensor, then a GEMM batched over f contracts k and writes
 // straight into the (P, m, n) layout of the reference.
+// FP16x3 AO->MO transforms in MGW_FP16X3 solves (default); MGW_AO2MO_FP16X3
+// = 0 (read at every transform) keeps the transforms of that mode FP64.
+// Measured on MI355X over the serving stream (real molecules, one solve at a
+// time): AO->MO 4.5 -> 3.5 ms per solve, 42.0 -> 43.3-44.1 solves/s, the
+// largest QP error against FP64 0.0013 -> 0.0014 meV.
+bool ao2mo_fp16x3() {
+  const char* e = std::getenv("MGW_AO2MO_FP16X3");
+  return e ? std::atoi(e) != 0 : true;
+}
+
 // `half` (nullable): caller-provided space for the half-transformed tensor
 // (naux x nbf x min(nm, nn) doubles); null takes a context buffer. With it,
 // the call allocates nothing and times no GEMM candidates.
@@ -356,8 +366,24 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int preci
   const int olo = first_n ? mlo : nlo;
   double* H = half;
   if (!H) MGW_TRY(dbuf(c, "ao2mo_half", (size_t)naux * nbf * nf, &H));
+  // FP16x3 transforms: operand maxima on the device (max |B| from the unpack,
+  // max |C| of this channel, max |H| from the first product's epilogue)
+  const bool f16 = precision == MGW_FP16X3 && ao2mo_fp16x3() && find_buf(c, "ri_absmax");
+  double* mx = nullptr;
+  if (f16) {
+    double* part = nullptr;
+    MGW_TRY(dbuf(c, "ao2mo_max", 2, &mx));
+    MGW_TRY(dbuf(c, "ao2mo_maxpart", ABSMAX_BLOCKS, &part));
+    MGW_TRY(launch_absmax_scalar(C, (int64_t)nbf * c->nstate, part, mx, c->stream));
+    MGW_TRY(launch_fill(mx + 1, 1, 0.0, c->stream));
+  }
   GemmF64 g1;
   g1.no_tune = half != nullptr;
+  if (f16) {
+    g1.amax = find_buf(c, "ri_absmax");
+    g1.bmax = mx;
+    g1.cmax = mx + 1;
+  }
   g1.M = naux * nbf; g1.N = nf; g1.K = nbf;
   g1.A = B; g1.lda = (int64_t)naux * nbf;
   g1.B = C + (int64_t)(flo - 1) * nbf; g1.ldb = nbf;
@@ -371,6 +397,10 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int preci
   }
   GemmF64 g2;
   g2.no_tune = half != nullptr;
+  if (f16) {
+    g2.amax = mx + 1;
+    g2.bmax = mx;
+  }
   g2.M = naux; g2.N = no; g2.K = nbf;
   g2.A = H; g2.lda = naux; g2.strideA = (int64_t)naux * nbf;
   g2.B = C + (int64_t)(olo - 1) * nbf; g2.ldb = nbf; g2.strideB = 0;
@@ -1389,7 +1419,10 @@ int mgw_set_eri_3center(mgw_ctx* c, const double* eri, int64_t npair, int nauxil
   MGW_TRY(upload(c, "ri_kl", kl.data(), kl.size(), &kl_d));
   double* dense = nullptr;
   MGW_TRY(dbuf(c, "ri_dense", (size_t)nauxil * nbf * nbf, &dense));
-  MGW_TRY(launch_unpack_ri(packed, npair, nauxil, kl_d, nbf, dense, c->stream, !complete));
+  double* amax = nullptr;                     // max |B|: the scale of reduced-precision transforms
+  MGW_TRY(dbuf(c, "ri_absmax", 1, &amax));
+  MGW_TRY(launch_fill(amax, 1, 0.0, c->stream));
+  MGW_TRY(launch_unpack_ri(packed, npair, nauxil, kl_d, nbf, dense, c->stream, !complete, amax));
   MGW_HIP(hipStreamSynchronize(c->stream));
   MGW_TRY(release(c, "ri_packed"));
   MGW_TRY(release(c, "ri_kl"));
@@ -1491,6 +1524,9 @@ int mgw_build_eri_3center(mgw_ctx* c, const double* eri_3center_tmp, int64_t ipa
     c->has_eri_mo = false;
     MGW_TRY(dbuf(c, "ri_dense", (size_t)nk * nbf * nbf, &dense));
     MGW_TRY(launch_fill(dense, (int64_t)nk * nbf * nbf, 0.0, c->stream));
+    double* amax = nullptr;
+    MGW_TRY(dbuf(c, "ri_absmax", 1, &amax));
+    MGW_TRY(launch_fill(amax, 1, 0.0, c->stream));
     c->build_npair = npair;
     c->build_nbf = nbf;
   } else {
@@ -1525,7 +1561,7 @@ int mgw_build_eri_3center(mgw_ctx* c, const double* eri_3center_tmp, int64_t ipa
   }
   int32_t* kl_d = nullptr;
   MGW_TRY(upload(c, "ri_kl", kl.data(), kl.size(), &kl_d));
-  MGW_TRY(launch_unpack_ri(packed, mpair, nk, kl_d, nbf, dense, c->stream, false));
+  MGW_TRY(launch_unpack_ri(packed, mpair, nk, kl_d, nbf, dense, c->stream, false, find_buf(c, "ri_absmax")));
   if (eri_3center_out) {
     // rows ipair_first.. of the caller's eri_3center(npair, nauxil_kept)
     MGW_HIP(hipMemcpy2DAsync(eri_3center_out + (ipair_first - 1), (size_t)npair * sizeof(double), packed,

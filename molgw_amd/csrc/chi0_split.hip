@@ -403,20 +403,98 @@ chi_xf32_kernel(const double* __restrict__ x, int naux, int npole, int ppad, flo
   *reinterpret_cast<f4*>(dst + 4) = hi4;
 }
 
-// wf[f][t] = sqrt(s_f(t)) scale_f in FP32 (zero for t >= npole, up to nkt * 32)
+// wf[f][t] = sqrt(s_f(t)) scale_f in FP32 (zero for t >= npole, up to nkt * 32);
+// with `whole`, s_f(t) scale_f (the one-sided split carries the whole weight)
 __global__ void __launch_bounds__(256)
 chi_wf_kernel(const double* __restrict__ sw, const double* __restrict__ scale, int npole, int ntpad,
-              float* __restrict__ wf) {
+              float* __restrict__ wf, int whole) {
   const int t = blockIdx.x * 256 + threadIdx.x, f = blockIdx.y;
   if (t >= ntpad) return;
   double w = 0.0;
-  if (t < npole) w = sqrt(sw[(int64_t)f * npole + t]) * (scale ? scale[f] : 1.0);
+  if (t < npole) {
+    const double sv = sw[(int64_t)f * npole + t];
+    w = (whole ? sv : sqrt(sv)) * (scale ? scale[f] : 1.0);
+  }
   wf[(int64_t)f * ntpad + t] = (float)w;
+}
+
+// One-sided split: the frequency-independent operand 2^eA X split once into
+// hi and lo planes in the k-tiled layout ap[q][kt][P][32] (zeros in the
+// padding); scale (nullable) holds 2^eA.
+template <bool H>
+__global__ void __launch_bounds__(256)
+chi_xplanes_kernel(const double* __restrict__ x, int naux, int npole, int ppad, const double* __restrict__ scale,
+                   typename PlaneT<H>::E* __restrict__ ap, int64_t aplane) {
+  typedef typename PlaneT<H>::E E;
+  typedef typename PlaneT<H>::V V;
+  __shared__ double tile[KT][65];
+  const int tid = threadIdx.x;
+  const int p0 = blockIdx.x * 64, kt = blockIdx.y, t0 = kt * KT;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int tl = (tid >> 6) + 4 * j, p = tid & 63;
+    const int P = p0 + p, t = t0 + tl;
+    tile[tl][p] = (P < naux && t < npole) ? x[P + (int64_t)t * naux] : 0.0;
+  }
+  __syncthreads();
+  const float sc = scale ? (float)scale[0] : 1.0f;
+  const int p = tid >> 2, c8 = tid & 3;
+  V hi, lo;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const float y = (float)tile[8 * c8 + e][p] * sc;
+    const E h = (E)y;
+    hi[e] = h;
+    lo[e] = (E)(y - (float)h);
+  }
+  E* __restrict__ dst = ap + ((int64_t)kt * ppad + p0 + p) * KT + 8 * c8;
+  *reinterpret_cast<V*>(dst) = hi;
+  *reinterpret_cast<V*>(dst + aplane) = lo;
+}
+
+// Scales of the one-sided split (FP16X3): 2^eA brings max|X| into
+// [2^13, 2^14) (scale[nomega] holds it); 2^eB_f brings the bound
+// max|X| max_t s_f(t) of X diag(s_f) there; unscale_f = 2^-(eA + eB_f) up to
+// the 2^26 the two targets carry.
+__global__ void __launch_bounds__(256)
+chi_scale_half_kernel(const double* __restrict__ xpart, int nb, const double* __restrict__ sw, int npole,
+                      int nomega, double* __restrict__ scale, double* __restrict__ unscale) {
+  __shared__ double red[256];
+  const int f = blockIdx.x, tid = threadIdx.x;
+  const double* __restrict__ s = sw + (int64_t)f * npole;
+  double m = 0.0;
+  for (int t = tid; t < npole; t += 256) m = fmax(m, s[t]);
+  double xm = 0.0;
+  for (int b = tid; b < nb; b += 256) xm = fmax(xm, xpart[b]);
+  red[tid] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+    __syncthreads();
+  }
+  const double smax = red[0];
+  __syncthreads();
+  red[tid] = xm;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double xb = red[0], bb = red[0] * smax;
+    const int ea = (xb > 0.0 && xb <= 1.0e300) ? ilogb(xb) : 0;
+    const int eb = (bb > 0.0 && bb <= 1.0e300) ? ilogb(bb) : 0;
+    scale[f] = ldexp(1.0, 13 - eb);
+    unscale[f] = ldexp(1.0, ea + eb - 26);
+    if (f == 0) scale[nomega] = ldexp(1.0, 13 - ea);
+  }
 }
 
 struct FusedArgs {
   const float* xf;       // [kt][ppad][32]
   const float* wf;       // [f][nkt * 32]
+  const void* ap;        // one-sided split: A planes [q][kt][ppad][32] (bf16 or fp16), else null
+  int64_t aplane;        // elements between the A planes
   int ppad, nkt, naux, nf, tiles, tm;
   double* C;             // chi + f0 * naux^2
   int64_t strideC;
@@ -426,7 +504,12 @@ struct FusedArgs {
   const double* unscale; // per frequency of the chunk (FP16X3), else null
 };
 
-template <bool H, int BM, int WM, int WN, int FLUSH, bool KSPLIT>
+// HALF: the one-sided split. The A operand (rows m) comes from the
+// frequency-independent planes of 2^eA X (copied to LDS as they are); only
+// the B operand (rows n) is scaled by 2^eB_f s_f(t) and split in the kernel:
+// y_m y_n = x_m (x_n s_f), half the VALU work of the two-sided split and the
+// same two-plane accuracy.
+template <bool H, int BM, int WM, int WN, int FLUSH, bool KSPLIT, bool HALF = false>
 __global__ void __launch_bounds__(64 * WM * WN, KSPLIT ? 2 : 1)
 chi_fsyrk_kernel(FusedArgs g) {
   typedef typename PlaneT<H>::E E;
@@ -457,18 +540,33 @@ chi_fsyrk_kernel(FusedArgs g) {
   const int64_t slab = (int64_t)g.ppad * KT;
 
   f4 ra[LPT][2], rb[LPT][2], rw[2];
+  u4 rp[NS][LPT];                          // HALF: the A planes' chunks
+  const E* __restrict__ Pa = static_cast<const E*>(g.ap) + (int64_t)m0 * KT;
   auto load = [&](int s) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < LPT; ++i) {
       const int c = tid + i * NT;
       const int64_t off = (int64_t)s * slab + (c >> 2) * KT + 8 * (c & 3);
-      ra[i][0] = *reinterpret_cast<const f4*>(Xa + off);
-      ra[i][1] = *reinterpret_cast<const f4*>(Xa + off + 4);
+      if constexpr (HALF) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) rp[q][i] = *reinterpret_cast<const u4*>(Pa + q * g.aplane + off);
+      } else {
+        ra[i][0] = *reinterpret_cast<const f4*>(Xa + off);
+        ra[i][1] = *reinterpret_cast<const f4*>(Xa + off + 4);
+      }
       rb[i][0] = *reinterpret_cast<const f4*>(Xb + off);
       rb[i][1] = *reinterpret_cast<const f4*>(Xb + off + 4);
     }
     rw[0] = *reinterpret_cast<const f4*>(W + (int64_t)s * KT);
     rw[1] = *reinterpret_cast<const f4*>(W + (int64_t)s * KT + 4);
+  };
+  auto copy_store = [&](E (&S)[NS][BM][KT]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      const int c = tid + i * NT, row = c >> 2, j = c & 3;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) *reinterpret_cast<u4*>(&S[q][row][8 * lds_chunk(row, j)]) = rp[q][i];
+    }
   };
   auto split_store = [&](f4 (&r)[LPT][2], E (&S)[NS][BM][KT]) __attribute__((always_inline)) {
 #pragma unroll
@@ -504,7 +602,8 @@ chi_fsyrk_kernel(FusedArgs g) {
   load(s0);
   for (int s = s0; s < s1; ++s) {
     __syncthreads();                   // previous slab's fragments consumed
-    split_store(ra, As);
+    if constexpr (HALF) copy_store(As);
+    else split_store(ra, As);
     split_store(rb, Bs);
     __syncthreads();
     if (s + 1 < s1) load(s + 1);
@@ -623,6 +722,14 @@ int chi_kc_slabs() {
   return forced ? forced : KC_SLABS;
 }
 
+// Waves of the K-split SYRK workgroup (128 x 128 tile): 4 (64 x 64 per wave,
+// default) or 2 (64 x 128 per wave: a quarter fewer LDS fragment reads per
+// MFMA, more registers per wave). MGW_CHI_WAVES, read at every chi0 build.
+int chi_syrk_waves() {
+  const char* e = std::getenv("MGW_CHI_WAVES");
+  return (e && std::atoi(e) == 2) ? 2 : 4;
+}
+
 struct SplitLayout {
   int ppad, nkt, tm, tiles, kc_slabs, nkc;
   int64_t per_plane_f;   // elements of one plane of one frequency
@@ -630,14 +737,31 @@ struct SplitLayout {
   size_t part_per_f;     // bytes of the FP32 partial tiles of one frequency
 };
 
-SplitLayout split_layout(int naux, int npole, int ns, bool ksplit) {
+// Largest K chunk (slabs) that the grid-filling rule of split_layout may
+// choose: MGW_CHI_KC_CAP (read at every chi0 build), default KC_SLABS.
+int chi_kc_cap() {
+  const char* v = std::getenv("MGW_CHI_KC_CAP");
+  const int k = v ? std::atoi(v) : 0;
+  return k >= KC_SLABS ? k : KC_SLABS;
+}
+
+SplitLayout split_layout(int naux, int npole, int ns, bool ksplit, int nomega = 1) {
   constexpr int bm = 128;
+  constexpr int64_t FILL = 4096;   // workgroups that keep 256 CUs busy at 2-3 per CU, several waves deep
   SplitLayout L;
   L.ppad = (naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
   L.nkt = ((npole + KT - 1) / KT + 1) / 2 * 2;   // even: slabs of one or two k tiles
   L.tm = (naux + bm - 1) / bm;
   L.tiles = L.tm * (L.tm + 1) / 2;
   L.kc_slabs = chi_kc_slabs();
+  if (!std::getenv("MGW_CHI_KC")) {
+    // the fewest K chunks that still fill the device: each chunk costs an
+    // FP32 partial tile written and read back
+    const int64_t base = (int64_t)L.tiles * std::max(1, nomega);
+    const int64_t want = std::max<int64_t>(1, (FILL + base - 1) / base);
+    const int kc = (int)std::max<int64_t>(KC_SLABS, (L.nkt + want - 1) / want);
+    L.kc_slabs = std::min(kc, chi_kc_cap());
+  }
   L.nkc = (L.nkt + L.kc_slabs - 1) / L.kc_slabs;
   L.per_plane_f = (int64_t)L.nkt * L.ppad * KT;
   L.planes_per_f = (size_t)L.per_plane_f * ns * sizeof(__bf16);
@@ -692,7 +816,7 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
   typedef typename PlaneT<H>::E E;
   constexpr int BM = 128;
   const bool ksplit = chi_ksplit();
-  const SplitLayout L = split_layout(naux, npole, NS, ksplit);
+  const SplitLayout L = split_layout(naux, npole, NS, ksplit, nomega);
   const size_t per_f = L.planes_per_f + L.part_per_f;
   const size_t head = scale_header_bytes(nomega);
   MGW_REQUIRE(work_bytes > head + 256, MGW_ERR_ARG, "chi0 split: workspace too small");
@@ -731,7 +855,10 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
       // VGPRs, two or three waves per SIMD; a second register set spills
       // (two LDS buffers with one barrier per slab, or two k tiles per slab:
       // 64 KB of LDS and two waves per SIMD, measured slower)
-      hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
+      if (chi_syrk_waves() == 2)
+        hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 1, 4, 1, 1, true>), grid, dim3(128), 0, stream, g);
+      else
+        hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
       MGW_HIP(hipGetLastError());
       hipLaunchKernelGGL((chi_ksplit_reduce_kernel<BM>), dim3(BM * BM / 256, L.tiles, nf), block, 0, stream, part,
                          L.nkc, L.tiles, L.tm, naux, g.unscale, g.C, g.strideC);
@@ -766,8 +893,8 @@ constexpr int64_t FILL_BLOCKS = 2048;     // 256 CUs x 8 workgroups
 
 struct FusedLayout {
   int ppad, nkt, tm, tiles, kc_slabs, nkc;
-  bool ksplit;
-  size_t xf_bytes, wf_bytes, part_per_f;
+  bool ksplit, half;
+  size_t head, xf_bytes, wf_bytes, ap_bytes, part_per_f;
 };
 
 // read at every call (a getenv per chi0 build), so that a process can compare the variants
@@ -776,9 +903,11 @@ int fused_ksplit_forced() {
   return e ? std::atoi(e) : -1;
 }
 
-FusedLayout fused_layout(int naux, int npole, int nomega) {
+FusedLayout fused_layout(int naux, int npole, int nomega, bool half) {
   constexpr int bm = 128;
   FusedLayout L;
+  L.half = half;
+  L.head = scale_header_bytes(nomega + 1);   // scales and unscales of nomega + 1 (the one-sided A scale)
   L.ppad = (naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
   L.nkt = (npole + KT - 1) / KT;
   L.tm = (naux + bm - 1) / bm;
@@ -798,63 +927,84 @@ FusedLayout fused_layout(int naux, int npole, int nomega) {
   }
   L.xf_bytes = ((size_t)L.nkt * L.ppad * KT * sizeof(float) + 255) / 256 * 256;
   L.wf_bytes = ((size_t)nomega * L.nkt * KT * sizeof(float) + 255) / 256 * 256;
+  L.ap_bytes = half ? ((size_t)2 * L.nkt * L.ppad * KT * sizeof(_Float16) + 255) / 256 * 256 : 0;
   L.part_per_f = L.ksplit ? (size_t)L.tiles * L.nkc * bm * bm * sizeof(float) : 0;
   return L;
 }
 
-// MGW_CHI_FUSED = 1 selects the fused split for the two-plane modes. Off by
-// default: measured on MI355X over the serving stream (real molecules, one
-// solve at a time), the chi0 stage took 22.3 ms per solve fused against
-// 13.5 ms with the per-frequency planes; the in-kernel scale and split of
+// The split of the two-plane modes: MGW_CHI_FUSED = 0 per-frequency planes
+// (pre-pass), 1 both operands split in the SYRK, 2 one-sided (A planes split
+// once, B scaled and split in the SYRK). Read at every chi0 build. Measured
+// on MI355X over the serving stream (real molecules, one solve at a time),
+// the chi0 stage took 22.3 ms per solve with both operands split in the
+// kernel against 13.5 ms with the planes: the in-kernel scale and split of
 // both operands of every (tile, frequency) workgroup costs more VALU time
 // than the plane traffic it saves.
-bool fused_enabled() {
+int fused_mode() {
   const char* e = std::getenv("MGW_CHI_FUSED");
-  return e ? std::atoi(e) != 0 : false;
+  const int v = e ? std::atoi(e) : 0;
+  return (v >= 0 && v <= 2) ? v : 0;
 }
 
+bool fused_enabled() { return fused_mode() != 0; }
+
 size_t fused_workspace(int naux, int npole, int nomega) {
-  const FusedLayout L = fused_layout(naux, npole, nomega);
+  const FusedLayout L = fused_layout(naux, npole, nomega, fused_mode() == 2);
   size_t nf = (size_t)nomega;
   if (L.part_per_f) {
     const size_t budget = std::max<size_t>((size_t)32 * naux * npole, (size_t)2 << 30);
     nf = std::max<size_t>(1, std::min<size_t>((size_t)nomega, budget / L.part_per_f));
   }
-  return scale_header_bytes(nomega) + L.xf_bytes + L.wf_bytes + nf * L.part_per_f + 512;
+  return L.head + L.xf_bytes + L.wf_bytes + L.ap_bytes + nf * L.part_per_f + 512;
 }
 
-template <bool H>
+template <bool H, bool HALF>
 int fused_split_syrk(const double* x, int naux, int npole, const double* sw, int nomega, void* work,
                      size_t work_bytes, double* chi, hipStream_t stream) {
+  typedef typename PlaneT<H>::E E;
   constexpr int BM = 128;
-  const FusedLayout L = fused_layout(naux, npole, nomega);
-  const size_t head = scale_header_bytes(nomega);
-  MGW_REQUIRE(work_bytes >= head + L.xf_bytes + L.wf_bytes + 256, MGW_ERR_ARG, "chi0 fused: workspace too small");
+  const FusedLayout L = fused_layout(naux, npole, nomega, HALF);
+  const size_t fixed = L.head + L.xf_bytes + L.wf_bytes + L.ap_bytes;
+  MGW_REQUIRE(work_bytes >= fixed + 256, MGW_ERR_ARG, "chi0 fused: workspace too small");
   size_t nf_max = (size_t)nomega;
-  if (L.part_per_f) nf_max = std::min(nf_max, (work_bytes - head - L.xf_bytes - L.wf_bytes - 256) / L.part_per_f);
+  if (L.part_per_f) nf_max = std::min(nf_max, (work_bytes - fixed - 256) / L.part_per_f);
   MGW_REQUIRE(nf_max >= 1, MGW_ERR_ARG, "chi0 fused: workspace smaller than one frequency");
-  double* scl = static_cast<double*>(work);
-  double* uns = scl + nomega;
-  double* xpart = uns + nomega;
-  float* xf = reinterpret_cast<float*>(static_cast<char*>(work) + head);
-  float* wf = reinterpret_cast<float*>(static_cast<char*>(work) + head + L.xf_bytes);
-  float* part = reinterpret_cast<float*>(static_cast<char*>(work) + head + L.xf_bytes + L.wf_bytes);
+  double* scl = static_cast<double*>(work);      // nomega + 1
+  double* uns = scl + nomega + 1;
+  double* xpart = uns + nomega + 1;
+  char* w8 = static_cast<char*>(work);
+  float* xf = reinterpret_cast<float*>(w8 + L.head);
+  float* wf = reinterpret_cast<float*>(w8 + L.head + L.xf_bytes);
+  E* ap = reinterpret_cast<E*>(w8 + L.head + L.xf_bytes + L.wf_bytes);
+  float* part = reinterpret_cast<float*>(w8 + fixed);
+  const int64_t aplane = (int64_t)L.nkt * L.ppad * KT;
   if (H) {
     MGW_TRY(launch_absmax(x, (int64_t)naux * npole, xpart, stream));
-    hipLaunchKernelGGL(chi_scale_kernel, dim3(nomega), dim3(256), 0, stream, xpart, NB_MAX, sw, npole, scl, uns);
+    if (HALF)
+      hipLaunchKernelGGL(chi_scale_half_kernel, dim3(nomega), dim3(256), 0, stream, xpart, NB_MAX, sw, npole, nomega,
+                         scl, uns);
+    else
+      hipLaunchKernelGGL(chi_scale_kernel, dim3(nomega), dim3(256), 0, stream, xpart, NB_MAX, sw, npole, scl, uns);
     MGW_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(chi_xf32_kernel, dim3(L.ppad / 64, L.nkt), dim3(256), 0, stream, x, naux, npole, L.ppad, xf);
   MGW_HIP(hipGetLastError());
+  if (HALF) {
+    hipLaunchKernelGGL((chi_xplanes_kernel<H>), dim3(L.ppad / 64, L.nkt), dim3(256), 0, stream, x, naux, npole,
+                       L.ppad, H ? scl + nomega : nullptr, ap, aplane);
+    MGW_HIP(hipGetLastError());
+  }
   const int ntpad = L.nkt * KT;
   hipLaunchKernelGGL(chi_wf_kernel, dim3((ntpad + 255) / 256, nomega), dim3(256), 0, stream, sw,
-                     H ? scl : nullptr, npole, ntpad, wf);
+                     H ? scl : nullptr, npole, ntpad, wf, HALF ? 1 : 0);
   MGW_HIP(hipGetLastError());
   for (int f0 = 0; f0 < nomega; f0 += (int)nf_max) {
     const int nf = std::min((int)nf_max, nomega - f0);
     FusedArgs g;
     g.xf = xf;
     g.wf = wf + (int64_t)f0 * ntpad;
+    g.ap = HALF ? ap : nullptr;
+    g.aplane = aplane;
     g.ppad = L.ppad; g.nkt = L.nkt; g.naux = naux; g.nf = nf; g.tiles = L.tiles; g.tm = L.tm;
     g.C = chi + (int64_t)f0 * naux * naux; g.strideC = (int64_t)naux * naux;
     g.kc_slabs = L.kc_slabs; g.nkc = L.nkc; g.part = part;
@@ -863,12 +1013,12 @@ int fused_split_syrk(const double* x, int naux, int npole, const double* sw, int
     MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chi0 fused: too many tiles");
     const dim3 grid((unsigned)g.nblocks), block(256);
     if (L.ksplit) {
-      hipLaunchKernelGGL((chi_fsyrk_kernel<H, BM, 2, 2, FUSED_FLUSH, true>), grid, block, 0, stream, g);
+      hipLaunchKernelGGL((chi_fsyrk_kernel<H, BM, 2, 2, FUSED_FLUSH, true, HALF>), grid, block, 0, stream, g);
       MGW_HIP(hipGetLastError());
       hipLaunchKernelGGL((chi_ksplit_reduce_kernel<BM>), dim3(BM * BM / 256, L.tiles, nf), block, 0, stream, part,
                          L.nkc, L.tiles, L.tm, naux, g.unscale, g.C, g.strideC);
     } else {
-      hipLaunchKernelGGL((chi_fsyrk_kernel<H, BM, 2, 2, FUSED_FLUSH, false>), grid, block, 0, stream, g);
+      hipLaunchKernelGGL((chi_fsyrk_kernel<H, BM, 2, 2, FUSED_FLUSH, false, HALF>), grid, block, 0, stream, g);
     }
     MGW_HIP(hipGetLastError());
   }
@@ -877,8 +1027,8 @@ int fused_split_syrk(const double* x, int naux, int npole, const double* sw, int
 
 }  // namespace
 
-size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision) {
-  const SplitLayout L = split_layout(naux, npole, precision == MGW_BF16X6 ? 3 : 2, chi_ksplit());
+size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision, int nomega) {
+  const SplitLayout L = split_layout(naux, npole, precision == MGW_BF16X6 ? 3 : 2, chi_ksplit(), nomega);
   return L.planes_per_f + L.part_per_f;
 }
 
@@ -887,7 +1037,7 @@ size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision) {
 // least one).
 size_t chi0_split_workspace(int naux, int npole, int nomega, int precision) {
   if (precision != MGW_BF16X6 && fused_enabled()) return fused_workspace(naux, npole, nomega);
-  const size_t per_f = chi0_split_bytes_per_frequency(naux, npole, precision);
+  const size_t per_f = chi0_split_bytes_per_frequency(naux, npole, precision, nomega);
   const size_t budget = std::max<size_t>((size_t)32 * naux * npole, (size_t)2 << 30);
   const size_t nf = std::max<size_t>(1, std::min<size_t>((size_t)nomega, budget / per_f));
   return scale_header_bytes(nomega) + nf * per_f + 512;
@@ -897,10 +1047,15 @@ int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int 
                     void* work, size_t work_bytes, double* chi, hipStream_t stream) {
   if (naux <= 0 || nomega <= 0) return MGW_OK;
   MGW_REQUIRE(npole > 0, MGW_ERR_ARG, "chi0 split: empty transition space");
-  if (precision == MGW_BF16X3 && fused_enabled())
-    return fused_split_syrk<false>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
-  if (precision == MGW_FP16X3 && fused_enabled())
-    return fused_split_syrk<true>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
+  const int fm = fused_mode();
+  if (precision == MGW_BF16X3 && fm == 1)
+    return fused_split_syrk<false, false>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
+  if (precision == MGW_FP16X3 && fm == 1)
+    return fused_split_syrk<true, false>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
+  if (precision == MGW_BF16X3 && fm == 2)
+    return fused_split_syrk<false, true>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
+  if (precision == MGW_FP16X3 && fm == 2)
+    return fused_split_syrk<true, true>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
   if (precision == MGW_BF16X3)
     return split_and_syrk<2, false>(x, naux, npole, sw, nomega, work, work_bytes, chi, stream);
   if (precision == MGW_BF16X6)

@@ -100,75 +100,102 @@ chol_diag_kernel(double* __restrict__ A, int n, int64_t strideA, int k, int kb, 
   }
 }
 
-// The same factorisation and inverse with four waves per matrix and the
-// block in LDS: every step of the right-looking elimination (and of the
-// column sweep of the inverse) is one barrier and a few FMAs per thread.
-// Elimination without the square roots: at step j every element (i, l),
-// j < l <= i, loses A(i, j) A(l, j) / A(j, j); at the end L(i, j) =
-// A(i, j) / sqrt(A(j, j)). The inverse works on U = T diag(L) (rows of T
-// times their diagonal L), so that row l is final, unscaled, when step l
-// reads it: U(i, j) -= L(i, l) U(l, j) / L(l, l) for i > l, j <= l, and
-// T(i, j) = U(i, j) / L(i, i) at the end. A non-positive pivot is recorded
-// and replaced by 1, as in chol_diag_kernel.
-constexpr int LP = NB + 1;   // LDS row pitch (doubles)
-
+// The same factorisation and inverse with four waves per matrix, row owned:
+// thread t holds row i = t / 4 of the block at the columns l = 4 c + (t % 4),
+// c = 0 .. 15, in registers (the four threads of a row are neighbouring lanes
+// of one wave). Step j of the right-looking factorisation publishes column j
+// (its owners write A(i, j), i >= j, to an LDS vector), one barrier, and every
+// thread updates its own columns l, j < l <= i:
+//   A(i, l) -= A(i, j) A(l, j) / A(j, j),   L(i, j) = A(i, j) / sqrt(A(j, j)).
+// The inverse T = L^-1 goes by rows: step l divides the row l of T by L(l, l),
+// publishes it, one barrier, and every row i > l subtracts L(i, l) T(l, :),
+// L(i, l) coming from the row's owner lane by a cross-lane read. The two LDS
+// vectors alternate, so a step's writes never meet the previous step's reads
+// and one barrier per step suffices. A non-positive pivot is recorded and
+// replaced by 1, as in chol_diag_kernel.
 __global__ void __launch_bounds__(256)
 chol_leaf_kernel(double* __restrict__ A, int n, int64_t strideA, int k, int kb, double* __restrict__ logdet,
                  int32_t* __restrict__ info) {
-  __shared__ double a[NB * LP];     // a[i + j LP]: the block, then L
-  __shared__ double u[NB * LP];     // U, then T
-  __shared__ double piv[NB];
-  __shared__ int fail;
-  const int f = blockIdx.x, tid = threadIdx.x;
+  constexpr int NC = NB / 4;        // columns per thread
+  __shared__ double vec[2][NB];     // published column (factorisation) / row (inverse)
+  __shared__ double red[4];
+  const int f = blockIdx.x, t = threadIdx.x;
+  const int i = t >> 2, q = t & 3, lane = t & 63;
   double* __restrict__ g = A + (int64_t)f * strideA + k + (int64_t)k * n;
-  if (tid == 0) fail = 0;
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int i = e % NB, j = e / NB;
-    a[i + j * LP] = (i < kb && j < kb) ? (j <= i ? g[i + (int64_t)j * n] : 0.0) : (i == j ? 1.0 : 0.0);
-    u[i + j * LP] = i == j ? 1.0 : 0.0;
+  double a[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int l = 4 * c + q;
+    a[c] = (i < kb && l < kb) ? (l <= i ? g[i + (int64_t)l * n] : 0.0) : (i == l ? 1.0 : 0.0);
   }
-  __syncthreads();
-  for (int j = 0; j < kb; ++j) {
-    double p = a[j + j * LP];
+  int failed = 0;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    double* __restrict__ v = vec[j & 1];
+    if (q == (j & 3) && i >= j) v[i] = a[j >> 2];
+    __syncthreads();
+    double p = v[j];
     if (!(p > 0.0)) {
-      if (tid == 0 && fail == 0) fail = k + j + 1;
+      if (!failed && j < kb) failed = k + j + 1;
       p = 1.0;
     }
-    const double rp = 1.0 / p;
-    const int m = kb - 1 - j;              // trailing size
-    for (int e = tid; e < m * m; e += 256) {
-      const int i = j + 1 + e % m, l = j + 1 + e / m;
-      if (i >= l) a[i + l * LP] -= a[i + j * LP] * a[l + j * LP] * rp;
+    const double d = sqrt(p), rd = 1.0 / d;
+    if (i > j) {
+      const double lij = v[i] * rd;
+#pragma unroll
+      for (int c = j >> 2; c < NC; ++c) {
+        const int l = 4 * c + q;
+        if (l > j && l <= i) a[c] -= lij * (v[l] * rd);
+      }
+      if (q == (j & 3)) a[j >> 2] = lij;
+    } else if (i == j && q == (j & 3)) {
+      a[j >> 2] = d;
     }
-    if (tid == 0) piv[j] = p;
-    __syncthreads();
   }
-  // L(i, j) = A(i, j) / sqrt(pivot j)
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int i = e % NB, j = e / NB;
-    if (j < kb && i >= j && i < kb) a[i + j * LP] = (i == j) ? sqrt(piv[j]) : a[i + j * LP] / sqrt(piv[j]);
-  }
-  __syncthreads();
-  for (int l = 0; l < kb; ++l) {
-    const double rl = 1.0 / a[l + l * LP];
-    const int rows = kb - 1 - l, cols = l + 1;
-    for (int e = tid; e < rows * cols; e += 256) {
-      const int i = l + 1 + e % rows, j = e / rows;
-      u[i + j * LP] -= a[i + l * LP] * u[l + j * LP] * rl;
-    }
-    __syncthreads();
-  }
+  // ln det = 2 sum ln L(i, i): the diagonal owner of each row, then a block sum
   double ld = 0.0;
-  if (tid == 0)
-    for (int j = 0; j < kb; ++j) ld += log(a[j + j * LP]);
-  for (int e = tid; e < kb * kb; e += 256) {
-    const int i = e % kb, j = e / kb;
-    g[i + (int64_t)j * n] = (j <= i) ? u[i + j * LP] / a[i + i * LP] : 0.0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    if (4 * c + q == i && i < kb) ld = log(a[c]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) ld += __shfl_down(ld, off, 64);
+  if (lane == 0) red[t >> 6] = ld;
+  // inverse by rows
+  double tt[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) tt[c] = (4 * c + q == i) ? 1.0 : 0.0;
+#pragma unroll
+  for (int l = 0; l < NB; ++l) {
+    double* __restrict__ v = vec[l & 1];
+    // L(i, l) from the lane of row i that owns column l
+    const double lil = __shfl(a[l >> 2], (lane & ~3) | (l & 3), 64);
+    if (i == l) {
+      const double r = 1.0 / lil;
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (4 * c + q <= l) {
+          tt[c] *= r;
+          v[4 * c + q] = tt[c];
+        }
+    }
+    __syncthreads();
+    if (i > l) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (4 * c + q <= l) tt[c] -= lil * v[4 * c + q];
+    }
   }
-  if (tid == 0) {
-    logdet[f] = (k == 0 ? 0.0 : logdet[f]) + ld;
+  if (i < kb) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int l = 4 * c + q;
+      if (l < kb) g[i + (int64_t)l * n] = l <= i ? tt[c] : 0.0;
+    }
+  }
+  if (t == 0) {
+    logdet[f] = (k == 0 ? 0.0 : logdet[f]) + red[0] + red[1] + red[2] + red[3];
     if (k == 0) info[f] = 0;
-    if (fail && info[f] == 0) info[f] = fail;
+    if (failed && info[f] == 0) info[f] = failed;
   }
 }
 

@@ -427,6 +427,8 @@ int gemm_f64(const GemmF64& h, hipStream_t stream) {
   // of those modes stays FP64
   MGW_REQUIRE(h.dot_w == nullptr || h.precision == MGW_FP64, MGW_ERR_ARG, "gemm_f64: row-weighted dots are FP64 only");
   if (h.precision == MGW_FP32 || h.precision == MGW_BF16X3) return gemm_lowp(h, h.precision, stream);
+  // FP16x3 products need the operand maxima; without them the product stays FP64
+  if (h.precision == MGW_FP16X3 && h.amax && h.bmax) return gemm_lowp(h, h.precision, stream);
   MGW_REQUIRE(h.K >= 0, MGW_ERR_ARG, "gemm_f64: negative K");
   MGW_REQUIRE(!h.lower_only || h.M == h.N, MGW_ERR_ARG, "gemm_f64: lower_only needs M == N");
   MGW_REQUIRE(h.dotpart == nullptr || !h.lower_only, MGW_ERR_ARG, "gemm_f64: dot epilogue needs a full tile grid");

@@ -353,6 +353,225 @@ gemm_f32acc64_kernel(LowpDev g) {
   store_tile<FM, FN, TM, TN>(g, acc, batch, m0, n0, wm, wn, lane);
 }
 
+// ----------------------------------------------------------------- FP16x3
+// Plain products C = alpha A B (+ beta C), A (M x K) and B (K x N) both
+// column-major and not transposed, batched by strides: the AO->MO
+// transforms. Each operand is scaled by an exact power of two that brings
+// its maximum (the device scalars amax, bmax) into [2^13, 2^14) and split
+// into two fp16 parts, x 2^e = hi + lo with hi = fp16(y), lo = fp16(y - hi)
+// (y the FP32 value); a b ~ hi hi + hi lo + lo hi on
+// v_mfma_f32_16x16x32_f16 (~2^-22 relative per product, against 2^-16 for
+// bf16), accumulated in FP32 over the whole K (K = nbf <= a few thousand:
+// 3 products per 32-k slab) and unscaled exactly in FP64 in the epilogue.
+// The split happens as the tile goes to LDS: A, the large operand (the RI
+// tensor or the half-transformed one), is split once per column tile of C,
+// i.e. about once per element for the narrow N of these products.
+// LDS rows hold 32 k (64 bytes) with the 16-byte chunks swizzled as in the
+// chi0 kernels. cmax (nullable) receives max |C| of the stored results, the
+// scale input of the next transform.
+__device__ inline int lds_chunk16(int r, int j) { return j ^ ((r >> 1) & 3); }
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+
+struct F16Dev {
+  int M, N, K;
+  const double* A; int64_t lda, strideA;
+  const double* B; int64_t ldb, strideB;
+  double* C; int64_t ldc, strideC;
+  double alpha, beta;
+  const double* amax;
+  const double* bmax;
+  double* cmax;
+  int tiles_m, tiles_n, batch;
+  int64_t nblocks;
+};
+
+__device__ inline double pow2_scale(double mx) {
+  return (mx > 0.0 && mx <= 1.0e300) ? ldexp(1.0, 13 - ilogb(mx)) : 1.0;
+}
+
+template <int BM, int BN, int WM, int WN, int KLASS>
+__global__ void __launch_bounds__(64 * WM * WN)
+gemm_f16x3_kernel(F16Dev g) {
+  constexpr int BK = 32;
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = BM / WM, TN = BN / WN;
+  constexpr int FM = TM / 16, FN = TN / 16;
+  // A: thread = (m, k octet); B: thread = (k octet, n)
+  constexpr int AU = BM * (BK / 8) / NT;      // A units (8 k of one m) per thread
+  constexpr int BU = (BN * (BK / 8) + NT - 1) / NT;
+  static_assert(BM * (BK / 8) % NT == 0 && NT % BM == 0, "A walk");
+  __shared__ __attribute__((aligned(16))) _Float16 As[2][BM][BK];
+  __shared__ __attribute__((aligned(16))) _Float16 Bs[2][BN][BK];
+  __shared__ double cred[NT / 64];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int64_t L = xcd_logical(blockIdx.x, g.nblocks);
+  const int batch = (int)(L % g.batch);
+  const int tile = (int)(L / g.batch);
+  // groups of 8 row tiles walked column by column: concurrent workgroups share B
+  constexpr int GROUP_M = 8;
+  const int per_group = GROUP_M * g.tiles_n;
+  const int group = tile / per_group;
+  const int first_m = group * GROUP_M;
+  const int gsize = min(g.tiles_m - first_m, GROUP_M);
+  const int in_group = tile - group * per_group;
+  const int m0 = (first_m + in_group % gsize) * BM, n0 = (in_group / gsize) * BN;
+
+  const double* __restrict__ A = g.A + (int64_t)batch * g.strideA;
+  const double* __restrict__ B = g.B + (int64_t)batch * g.strideB;
+  const double sa = pow2_scale(g.amax[0]), sb = pow2_scale(g.bmax[0]);
+  const double unscale = 1.0 / (sa * sb);     // exact: a power of two
+
+  const int am = tid % BM, ako = tid / BM;    // A: row m, first k octet
+  constexpr int A_KO_STEP = NT / BM;
+  double ra[AU][8], rb[BU][8];
+  auto load = [&](int k0) __attribute__((always_inline)) {
+    const int gm = m0 + am;
+#pragma unroll
+    for (int u = 0; u < AU; ++u) {
+      const int kb = k0 + 8 * (ako + u * A_KO_STEP);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int gk = kb + e;
+        ra[u][e] = (gm < g.M && gk < g.K) ? A[gm + (int64_t)gk * g.lda] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < BU; ++u) {
+      const int c = tid + u * NT;             // (n, k octet) with the octet fastest
+      const int n = c >> 2, ko = c & 3;
+      const int gn = n0 + n, kb = k0 + 8 * ko;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int gk = kb + e;
+        rb[u][e] = (c < BN * 4 && gn < g.N && gk < g.K) ? B[gk + (int64_t)gn * g.ldb] : 0.0;
+      }
+    }
+  };
+  auto store = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < AU; ++u) {
+      const int ko = ako + u * A_KO_STEP;
+      h8v hi, lo;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float y = (float)(ra[u][e] * sa);
+        const _Float16 h = (_Float16)y;
+        hi[e] = h;
+        lo[e] = (_Float16)(y - (float)h);
+      }
+      *reinterpret_cast<h8v*>(&As[0][am][8 * lds_chunk16(am, ko)]) = hi;
+      *reinterpret_cast<h8v*>(&As[1][am][8 * lds_chunk16(am, ko)]) = lo;
+    }
+#pragma unroll
+    for (int u = 0; u < BU; ++u) {
+      const int c = tid + u * NT;
+      if (c >= BN * 4) break;
+      const int n = c >> 2, ko = c & 3;
+      h8v hi, lo;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float y = (float)(rb[u][e] * sb);
+        const _Float16 h = (_Float16)y;
+        hi[e] = h;
+        lo[e] = (_Float16)(y - (float)h);
+      }
+      *reinterpret_cast<h8v*>(&Bs[0][n][8 * lds_chunk16(n, ko)]) = hi;
+      *reinterpret_cast<h8v*>(&Bs[1][n][8 * lds_chunk16(n, ko)]) = lo;
+    }
+  };
+
+  f4 part[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) part[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const int fr = lane & 15, fq = lane >> 4;
+  const int nslab = (g.K + BK - 1) / BK;
+  load(0);
+  for (int s = 0; s < nslab; ++s) {
+    __syncthreads();                 // previous slab's fragments consumed
+    store();
+    __syncthreads();
+    if (s + 1 < nslab) load((s + 1) * BK);
+    h8v a[2][FM], b[2][FN];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int r = wm * TM + i * 16 + fr;
+        a[q][i] = *reinterpret_cast<const h8v*>(&As[q][r][8 * lds_chunk16(r, fq)]);
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int r = wn * TN + j * 16 + fr;
+        b[q][j] = *reinterpret_cast<const h8v*>(&Bs[q][r][8 * lds_chunk16(r, fq)]);
+      }
+    }
+    // lo hi + hi lo + hi hi, product-major (consecutive MFMAs write different accumulators)
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const int qa = pr == 1 ? 1 : 0, qb = pr == 0 ? 1 : 0;
+          part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[qb][j], a[qa][i], part[i][j], 0, 0, 0);
+        }
+  }
+  // epilogue: lane l, register r holds C(m = m_base + (l & 15), n = n_base + 4 (l >> 4) + r)
+  double* __restrict__ C = g.C + (int64_t)batch * g.strideC;
+  double cm = 0.0;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int m = m0 + wm * TM + i * 16 + fr;
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn * TN + j * 16 + 4 * fq + r;
+        if (m >= g.M || n >= g.N) continue;
+        const int64_t idx = m + (int64_t)n * g.ldc;
+        double v = g.alpha * ((double)part[i][j][r] * unscale);
+        if (g.beta != 0.0) v += g.beta * C[idx];
+        C[idx] = v;
+        cm = fmax(cm, fabs(v));
+      }
+  }
+  if (g.cmax) {
+    for (int off = 32; off > 0; off >>= 1) cm = fmax(cm, __shfl_down(cm, off, 64));
+    if (lane == 0) cred[wave] = cm;
+    __syncthreads();
+    if (tid == 0) {
+      double b = 0.0;
+#pragma unroll
+      for (int w = 0; w < NT / 64; ++w) b = fmax(b, cred[w]);
+      atomicMax(reinterpret_cast<unsigned long long*>(g.cmax), (unsigned long long)__double_as_longlong(b));
+    }
+  }
+}
+
+template <int BM, int BN, int WM, int WN, int KLASS>
+int launch_f16x3(const GemmF64& h, hipStream_t stream) {
+  F16Dev g;
+  g.M = h.M; g.N = h.N; g.K = h.K;
+  g.A = h.A; g.lda = h.lda; g.strideA = h.strideA;
+  g.B = h.B; g.ldb = h.ldb; g.strideB = h.strideB;
+  g.C = h.C; g.ldc = h.ldc; g.strideC = h.strideC;
+  g.alpha = h.alpha; g.beta = h.beta;
+  g.amax = h.amax; g.bmax = h.bmax; g.cmax = h.cmax;
+  g.tiles_m = (h.M + BM - 1) / BM;
+  g.tiles_n = (h.N + BN - 1) / BN;
+  g.batch = h.batch;
+  g.nblocks = (int64_t)g.tiles_m * g.tiles_n * h.batch;
+  MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "gemm_lowp: too many tiles");
+  hipLaunchKernelGGL((gemm_f16x3_kernel<BM, BN, WM, WN, KLASS>), dim3((unsigned)g.nblocks), dim3(64 * WM * WN), 0,
+                     stream, g);
+  MGW_HIP(hipGetLastError());
+  return MGW_OK;
+}
+
 template <int BM, int BN, int WM, int WN, int FLUSH, int KLASS, bool BF16>
 int launch_lowp(const GemmF64& h, hipStream_t stream) {
   LowpDev g;
@@ -403,6 +622,14 @@ int gemm_lowp(const GemmF64& h, int precision, hipStream_t stream) {
   if (precision == MGW_BF16X3) {
     if (h.klass == MGW_KCLASS_GEMM_CHI0) return launch_lowp<128, 128, 4, 2, 4, MGW_KCLASS_GEMM_CHI0, true>(h, stream);
     return launch_lowp<128, 128, 4, 2, 4, MGW_KCLASS_GEMM_AO2MO, true>(h, stream);
+  }
+  if (precision == MGW_FP16X3) {
+    MGW_REQUIRE(!h.transA && !h.transB && !h.lower_only && !h.kscale && !h.rowscale && !h.colscale && !h.diag_add &&
+                    h.amax && h.bmax,
+                MGW_ERR_ARG, "gemm_lowp: FP16x3 takes plain non-transposed products with operand maxima");
+    // narrow N (the first transform's occupied or window columns): 64-wide column tiles
+    if (h.N <= 64) return launch_f16x3<128, 64, 2, 2, MGW_KCLASS_GEMM_AO2MO>(h, stream);
+    return launch_f16x3<128, 128, 2, 2, MGW_KCLASS_GEMM_AO2MO>(h, stream);
   }
   if (precision == MGW_FP32) {
     if (h.klass == MGW_KCLASS_GEMM_CHI0) return launch_lowp<128, 128, 4, 2, 8, MGW_KCLASS_GEMM_CHI0, false>(h, stream);

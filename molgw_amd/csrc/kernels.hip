@@ -40,8 +40,9 @@ int launch_fill(double* p, int64_t n, double v, hipStream_t s) {
 // the stored value (src/m_eri_ao_mo.f90:517-522 adds both scatter branches).
 __global__ void __launch_bounds__(256)
 unpack_ri_kernel(const double* __restrict__ packed, int64_t npair, int naux,
-                 const int32_t* __restrict__ kl, int nbf, double* __restrict__ dense) {
+                 const int32_t* __restrict__ kl, int nbf, double* __restrict__ dense, double* __restrict__ amax) {
   __shared__ double tile[64][65];
+  __shared__ double red[4];
   const int t = threadIdx.x;
   const int64_t p0 = (int64_t)blockIdx.x * 64;
   const int P0 = blockIdx.y * 64;
@@ -57,28 +58,42 @@ unpack_ri_kernel(const double* __restrict__ packed, int64_t npair, int naux,
   __syncthreads();
   const int Pl = t & 63;
   const int P = P0 + Pl;
-  if (P >= naux) return;
-  for (int i = 0; i < 16; ++i) {
-    const int pair_l = (t >> 6) + 4 * i;
-    const int64_t pair = p0 + pair_l;
-    if (pair >= npair) break;
-    const int k = kl[2 * pair];
-    const int l = kl[2 * pair + 1];
-    const double v = tile[Pl][pair_l];
-    if (k == l) {
-      dense[P + (int64_t)naux * (k + (int64_t)nbf * l)] = 2.0 * v;
-    } else {
-      dense[P + (int64_t)naux * (k + (int64_t)nbf * l)] = v;
-      dense[P + (int64_t)naux * (l + (int64_t)nbf * k)] = v;
+  double m = 0.0;                            // max |value written|
+  if (P < naux) {
+    for (int i = 0; i < 16; ++i) {
+      const int pair_l = (t >> 6) + 4 * i;
+      const int64_t pair = p0 + pair_l;
+      if (pair >= npair) break;
+      const int k = kl[2 * pair];
+      const int l = kl[2 * pair + 1];
+      const double v = tile[Pl][pair_l];
+      if (k == l) {
+        dense[P + (int64_t)naux * (k + (int64_t)nbf * l)] = 2.0 * v;
+        m = fmax(m, 2.0 * fabs(v));
+      } else {
+        dense[P + (int64_t)naux * (k + (int64_t)nbf * l)] = v;
+        dense[P + (int64_t)naux * (l + (int64_t)nbf * k)] = v;
+        m = fmax(m, fabs(v));
+      }
     }
+  }
+  if (!amax) return;
+  // block maximum, then one atomic per block: the bit patterns of
+  // non-negative doubles order as unsigned integers
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  if ((t & 63) == 0) red[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    const double b = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    atomicMax(reinterpret_cast<unsigned long long*>(amax), (unsigned long long)__double_as_longlong(b));
   }
 }
 
 int launch_unpack_ri(const double* packed, int64_t npair, int naux, const int32_t* kl, int nbf,
-                     double* dense, hipStream_t s, bool zero_dense) {
+                     double* dense, hipStream_t s, bool zero_dense, double* amax) {
   if (zero_dense) MGW_TRY(launch_fill(dense, (int64_t)naux * nbf * nbf, 0.0, s));
   dim3 grid(nblocks(npair, 64), nblocks(naux, 64));
-  hipLaunchKernelGGL(unpack_ri_kernel, grid, dim3(256), 0, s, packed, npair, naux, kl, nbf, dense);
+  hipLaunchKernelGGL(unpack_ri_kernel, grid, dim3(256), 0, s, packed, npair, naux, kl, nbf, dense, amax);
   MGW_LAUNCH_CHECK();
   return MGW_OK;
 }
@@ -159,6 +174,28 @@ absmax_kernel(const double* __restrict__ x, int64_t n, double* __restrict__ part
 
 int launch_absmax(const double* x, int64_t n, double* part, hipStream_t s) {
   hipLaunchKernelGGL(absmax_kernel, dim3(ABSMAX_BLOCKS), dim3(256), 0, s, x, n, part);
+  MGW_LAUNCH_CHECK();
+  return MGW_OK;
+}
+
+// out[0] = max of the ABSMAX_BLOCKS partial maxima
+__global__ void __launch_bounds__(256) absmax_finish_kernel(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  double m = 0.0;
+  for (int b = tid; b < ABSMAX_BLOCKS; b += 256) m = fmax(m, part[b]);
+  red[tid] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) out[0] = red[0];
+}
+
+int launch_absmax_scalar(const double* x, int64_t n, double* part, double* out, hipStream_t s) {
+  MGW_TRY(launch_absmax(x, n, part, s));
+  hipLaunchKernelGGL(absmax_finish_kernel, dim3(1), dim3(256), 0, s, part, out);
   MGW_LAUNCH_CHECK();
   return MGW_OK;
 }

@@ -129,13 +129,19 @@ struct GemmF64 {
   // use the tuned or default tiles without timing candidates (the timing
   // creates events and synchronises the stream)
   bool no_tune = false;
+  // MGW_FP16X3 (gemm_lowp, plain non-transposed products): device scalars
+  // max|A| and max|B| that set the power-of-two operand scales; cmax
+  // (nullable, zeroed by the caller) receives max|C| of the stored results
+  const double* amax = nullptr;
+  const double* bmax = nullptr;
+  double* cmax = nullptr;
 };
 int gemm_f64(const GemmF64& g, hipStream_t stream);
 int gemm_lowp(const GemmF64& g, int precision, hipStream_t stream);
 // 1 - chi0(iw_f) = I + X diag(s_f) X^T (lower triangle, zeros above it) for all
 // nomega frequencies from bf16-split operands (precision MGW_BF16X3 or
 // MGW_BF16X6); needs s_f >= 0 and a workspace of chi0_split_workspace bytes.
-size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision);
+size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision, int nomega);
 size_t chi0_split_workspace(int naux, int npole, int nomega, int precision);
 int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int nomega, int precision,
                     void* work, size_t work_bytes, double* chi, hipStream_t stream);
@@ -205,8 +211,10 @@ int launch_eri_class(const EriLaunch& e, hipStream_t s);
 // ---------------------------------------------------------------- kernels
 // zero_dense false: only the pairs listed are written (a later batch of a
 // tensor whose buffer was cleared before the first batch)
+// amax (nullable, zeroed by the caller): atomically raised to max |dense| of
+// the elements written
 int launch_unpack_ri(const double* packed, int64_t npair, int naux, const int32_t* kl,
-                     int nbf, double* dense, hipStream_t s, bool zero_dense = true);
+                     int nbf, double* dense, hipStream_t s, bool zero_dense = true, double* amax = nullptr);
 // out(:, j) = u(:, j + neglect) / sqrt(eig[j + neglect]), j < kept (u n x n)
 int launch_invsqrt_select(const double* u, int n, const double* eig, int neglect, int kept, double* out,
                           hipStream_t s);
@@ -219,6 +227,8 @@ int launch_symmetrize(double* m, int n, hipStream_t s);
 // part[b] = max |x_i| over the elements of block b, b < ABSMAX_BLOCKS
 constexpr int ABSMAX_BLOCKS = 1024;
 int launch_absmax(const double* x, int64_t n, double* part, hipStream_t s);
+// out[0] = max |x_i| (part: ABSMAX_BLOCKS doubles of workspace)
+int launch_absmax_scalar(const double* x, int64_t n, double* part, double* out, hipStream_t s);
 // m_f(i, i) += v for the nbatch n x n matrices m_f (stride n^2)
 int launch_diag_add(double* m, int n, int nbatch, double v, hipStream_t s);
 int launch_gather_ia(const double* eri, int naux, int nm, int nn, const int32_t* ia_dev, int npole,

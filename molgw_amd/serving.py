@@ -16,6 +16,7 @@ end - arrival (queueing included).
 """
 from __future__ import annotations
 
+import os
 import queue
 import threading
 import time
@@ -157,14 +158,26 @@ class Server:
                 self.trims += 1
 
     # ------------------------------------------------------------ run
-    def run(self, requests: list[Request], realtime: bool = True) -> list[Result]:
+    def run(self, requests: list[Request], realtime: bool = True, order: str | None = None) -> list[Result]:
         """Serve ``requests`` (arrival times in seconds). With ``realtime``
-        False every request is available at t = 0 (a batch)."""
+        False every request is available at t = 0 (a batch). ``order`` of the
+        waiting queue: 'arrival' (FIFO) or 'largest_first' (by HBM footprint,
+        a proxy of the work: the longest solves start first, so the end of a
+        batch is not one large solve running alone). Default: 'arrival' for
+        realtime streams, the MGW_BATCH_ORDER environment variable or
+        'arrival' for batches."""
+        if order is None:
+            order = 'arrival' if realtime else os.environ.get('MGW_BATCH_ORDER', 'arrival')
+        if order not in ('arrival', 'largest_first'):
+            raise ValueError(f'unknown order {order!r}')
         for r in requests:
             r.footprint = footprint_bytes(r)
             if r.footprint > self.budget:
                 raise ValueError(f'request {r.rid}: footprint {r.footprint} exceeds the HBM budget {self.budget}')
-        pending = sorted(requests, key=lambda r: r.arrival if realtime else 0.0)
+        if order == 'largest_first':
+            pending = sorted(requests, key=lambda r: (r.arrival if realtime else 0.0, -r.footprint))
+        else:
+            pending = sorted(requests, key=lambda r: r.arrival if realtime else 0.0)
         waiting: list[Request] = []
         results: list[Result] = []
         t0 = time.perf_counter()

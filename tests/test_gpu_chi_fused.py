@@ -70,6 +70,31 @@ def test_fused_split_equals_the_planes(precision):
     assert err_flush <= 1.5 * err_planes + 1e-15
 
 
+@pytest.mark.parametrize('precision', ['fp16x3', 'bf16x3'])
+@pytest.mark.parametrize('ksplit', ['1', '0'])
+def test_one_sided_split_against_fp64(precision, ksplit):
+    """The one-sided split (A = X split once, B = X diag(s_f) split in the
+    SYRK): other roundings than the planes path (no square root of the
+    weight, scales per operand), the same two-plane accuracy: chi0 within the
+    bounds above and within 1.5x the planes path's error against FP64."""
+    s = synthetic_system(60, 300, 12, seed=41, scale=0.3)
+    with Engine(0) as eng:
+        eng.set_eri_3center(s.eri_3center, s.index_basis, s.nbf)
+        eng.set_mo(s.c_matrix, s.energy, s.occupation)
+        e_planes, chi_planes = _chi(eng, s, precision, {'MGW_CHI_FUSED': '0'})
+        e_half, chi_half = _chi(eng, s, precision, {'MGW_CHI_FUSED': '2', 'MGW_CHI_FUSED_KSPLIT': ksplit})
+        e64, chi64 = _chi(eng, s, 'fp64', {})
+    c64 = _chi0(chi64)
+    scale = np.max(np.abs(c64))
+    err_planes = np.max(np.abs(_chi0(chi_planes) - c64)) / scale
+    err_half = np.max(np.abs(_chi0(chi_half) - c64)) / scale
+    print(f'{precision} K split {ksplit}: chi0 vs FP64 (relative to max |chi0|): planes {err_planes:.2e}, '
+          f'one-sided {err_half:.2e}; E_RPA {e_planes - e64:.2e} / {e_half - e64:.2e} Ha')
+    tol = 8e-6 if precision == 'fp16x3' else 1e-4
+    assert err_half < tol
+    assert err_half <= 1.5 * err_planes + 1e-15
+
+
 def _chi0(chi):
     """chi0 = 1 - (1 + chi)^-1 for every frequency (chi is naux x naux x nomega)."""
     n = chi.shape[0]
