@@ -120,6 +120,13 @@ struct mgw_ctx {
   const std::vector<double>& energy_in_g() const { return energy_g.empty() ? energy : energy_g; }
   // offset of the active channel in the spin-major vectors
   int so() const { return spin * nstate; }
+  // the half-transformed tensor H(P k, f) = B(P k, l) C(l, f) in buffer
+  // "ao2mo_half" for states f = half_lo .. half_hi of channel half_spin, made
+  // with arithmetic half_prec (and, in FP16x3, its maximum in "ao2mo_max"):
+  // a later transform whose first states lie in that range skips the first
+  // product
+  bool half_valid = false;
+  int half_lo = 0, half_hi = 0, half_spin = 0, half_prec = 0;
   // A2 module state eri_3center_mo(P, m, n)
   bool has_eri_mo = false;
   int mlo = 0, mhi = 0, nlo = 0, nhi = 0;
@@ -159,6 +166,7 @@ bool split_precision(int p) { return p == MGW_BF16X3 || p == MGW_BF16X6 || p == 
 
 // Everything computed from the RI tensor and the MO data: new inputs make it stale.
 void invalidate_derived(mgw_ctx* c) {
+  c->half_valid = false;
   c->has_eri_mo = false;
   c->has_wpol = false;
   c->has_chi = false;
@@ -346,11 +354,37 @@ bool ao2mo_fp16x3() {
   return e ? std::atoi(e) != 0 : true;
 }
 
+// H(P k, f) = sum_l B(P k, l) C(l, f) for the nf1 states from flo (the first
+// product of the transform).
+int ao2mo_first(mgw_ctx* c, const double* B, const double* C, int flo, int nf1, double* H, int precision, bool f16,
+                double* mx, bool no_tune) {
+  const int nbf = c->nbf, naux = c->naux;
+  GemmF64 g1;
+  g1.no_tune = no_tune;
+  if (f16) {
+    g1.amax = find_buf(c, "ri_absmax");
+    g1.bmax = mx;
+    g1.cmax = mx + 1;
+  }
+  g1.M = naux * nbf; g1.N = nf1; g1.K = nbf;
+  g1.A = B; g1.lda = (int64_t)naux * nbf;
+  g1.B = C + (int64_t)(flo - 1) * nbf; g1.ldb = nbf;
+  g1.C = H; g1.ldc = (int64_t)naux * nbf;
+  g1.klass = MGW_KCLASS_GEMM_AO2MO;
+  g1.precision = precision;
+  ProfScope ps(&c->prof, MGW_KCLASS_GEMM_AO2MO, 2.0 * g1.M * (double)g1.N * g1.K,
+               8.0 * ((double)g1.M * g1.K + (double)g1.K * g1.N + (double)g1.M * g1.N), c->stream);
+  return gemm_f64(g1, c->stream);
+}
+
 // `half` (nullable): caller-provided space for the half-transformed tensor
 // (naux x nbf x min(nm, nn) doubles); null takes a context buffer. With it,
-// the call allocates nothing and times no GEMM candidates.
+// the call allocates nothing, times no GEMM candidates and leaves nothing
+// cached. `extend_hi` (context buffer only): the first product also covers
+// the first-index states up to extend_hi, so that a later transform of those
+// (the self-energy window after the W transform) reuses it.
 int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int precision = MGW_FP64, int spin = -1,
-          double* half = nullptr) {
+          double* half = nullptr, int extend_hi = 0) {
   MGW_TRY(check_mo_matches_ri(c));
   MGW_REQUIRE(mlo >= 1 && mhi <= c->nstate && mlo <= mhi && nlo >= 1 && nhi <= c->nstate && nlo <= nhi,
               MGW_ERR_ARG, "ao2mo: invalid state range");
@@ -364,36 +398,40 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int preci
   const int flo = first_n ? nlo : mlo;
   const int no = first_n ? nm : nn;
   const int olo = first_n ? mlo : nlo;
-  double* H = half;
-  if (!H) MGW_TRY(dbuf(c, "ao2mo_half", (size_t)naux * nbf * nf, &H));
   // FP16x3 transforms: operand maxima on the device (max |B| from the unpack,
   // max |C| of this channel, max |H| from the first product's epilogue)
   const bool f16 = precision == MGW_FP16X3 && ao2mo_fp16x3() && find_buf(c, "ri_absmax");
+  const int sp = spin < 0 ? c->spin : spin;
+  const int prec_key = f16 ? -1 : precision;
+  const int fhi = flo + nf - 1;
+  const bool reuse = !half && c->half_valid && c->half_spin == sp && c->half_prec == prec_key && flo >= c->half_lo &&
+                     fhi <= c->half_hi;
+  double* H = half;
   double* mx = nullptr;
-  if (f16) {
-    double* part = nullptr;
-    MGW_TRY(dbuf(c, "ao2mo_max", 2, &mx));
-    MGW_TRY(dbuf(c, "ao2mo_maxpart", ABSMAX_BLOCKS, &part));
-    MGW_TRY(launch_absmax_scalar(C, (int64_t)nbf * c->nstate, part, mx, c->stream));
-    MGW_TRY(launch_fill(mx + 1, 1, 0.0, c->stream));
-  }
-  GemmF64 g1;
-  g1.no_tune = half != nullptr;
-  if (f16) {
-    g1.amax = find_buf(c, "ri_absmax");
-    g1.bmax = mx;
-    g1.cmax = mx + 1;
-  }
-  g1.M = naux * nbf; g1.N = nf; g1.K = nbf;
-  g1.A = B; g1.lda = (int64_t)naux * nbf;
-  g1.B = C + (int64_t)(flo - 1) * nbf; g1.ldb = nbf;
-  g1.C = H; g1.ldc = (int64_t)naux * nbf;
-  g1.klass = MGW_KCLASS_GEMM_AO2MO;
-  g1.precision = precision;
-  {
-    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_AO2MO, 2.0 * g1.M * (double)g1.N * g1.K,
-                 8.0 * ((double)g1.M * g1.K + (double)g1.K * g1.N + (double)g1.M * g1.N), c->stream);
-    MGW_TRY(gemm_f64(g1, c->stream));
+  if (reuse) {
+    H = find_buf(c, "ao2mo_half") + (int64_t)(flo - c->half_lo) * naux * nbf;
+    if (f16) mx = find_buf(c, "ao2mo_max");
+  } else {
+    // the first product's states: the requested ones, extended up to extend_hi
+    const int f1hi = (!half && extend_hi > fhi) ? std::min(extend_hi, c->nstate) : fhi;
+    const int nf1 = f1hi - flo + 1;
+    c->half_valid = false;
+    if (!H) MGW_TRY(dbuf(c, "ao2mo_half", (size_t)naux * nbf * nf1, &H));
+    if (f16) {
+      double* part = nullptr;
+      MGW_TRY(dbuf(c, "ao2mo_max", 2, &mx));
+      MGW_TRY(dbuf(c, "ao2mo_maxpart", ABSMAX_BLOCKS, &part));
+      MGW_TRY(launch_absmax_scalar(C, (int64_t)nbf * c->nstate, part, mx, c->stream));
+      MGW_TRY(launch_fill(mx + 1, 1, 0.0, c->stream));
+    }
+    MGW_TRY(ao2mo_first(c, B, C, flo, nf1, H, precision, f16, mx, half != nullptr));
+    if (!half) {
+      c->half_valid = true;
+      c->half_lo = flo;
+      c->half_hi = f1hi;
+      c->half_spin = sp;
+      c->half_prec = prec_key;
+    }
   }
   GemmF64 g2;
   g2.no_tune = half != nullptr;
@@ -443,7 +481,9 @@ int build_x_ia(mgw_ctx* c, const mgw_ranges* r, double spin_fact, std::vector<in
     int i0, i1, a0, a1;
     if (rectangular(sub, i0, i1, a0, a1)) {
       // (P, a, i) with a inner is exactly X(P, t), t = (i outer, a inner)
-      MGW_TRY(ao2mo(c, a0 + 1, a1 + 1, i0 + 1, i1 + 1, xs, precision, s));
+      // the first product also covers the self-energy window (nsemax), which
+      // the self-energy's transform of this channel then reuses
+      MGW_TRY(ao2mo(c, a0 + 1, a1 + 1, i0 + 1, i1 + 1, xs, precision, s, nullptr, r->nsemax));
     } else {
       const int wlo = r->ncore_W + 1, whi = r->nvirtual_W - 1;
       const int nw = whi - wlo + 1;

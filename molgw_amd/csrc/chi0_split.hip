@@ -344,6 +344,153 @@ chi_syrk_kernel(SyrkArgs g) {
 #undef MGW_SYRK_LOAD
 #undef MGW_SYRK_STORE
 
+// The K-split SYRK with the slabs copied from the planes straight into LDS
+// (global_load_lds_dwordx4: each lane's 16 bytes land at the wave's LDS base
+// + 16 x lane), so no prefetch registers: more waves fit a SIMD, or a wave
+// can hold a 64 x 128 tile (WN = 1, a quarter fewer LDS fragment reads per
+// MFMA than 64 x 64). Lane p of a 1 KB window of 16 rows writes LDS chunk
+// p, i.e. row p / 4 and slot p % 4, which holds the swizzled chunk
+// j = slot ^ ((row >> 1) & 3) of that row, so the lane loads that global
+// chunk. DB: two LDS buffers (the next slab's copies run during this slab's
+// MFMAs, one barrier per slab) or one (copy, wait, multiply, two barriers
+// per slab). Same products in the same order per accumulator as
+// chi_syrk_kernel: identical results.
+template <bool H, int BM, int WM, int WN, bool DB>
+__global__ void __launch_bounds__(64 * WM * WN)
+chi_syrk_lds_kernel(SyrkArgs g) {
+  typedef typename PlaneT<H>::E E;
+  typedef typename PlaneT<H>::V V;
+  constexpr int NS = 2;
+  constexpr int NW = WM * WN;
+  constexpr int TM = BM / WM, TN = BM / WN;
+  constexpr int FM = TM / 16, FN = TN / 16;
+  constexpr int WINS = BM / 16;                 // 1 KB windows per plane per operand
+  constexpr int NBUF = DB ? 2 : 1;
+  static_assert((2 * NS * WINS) % NW == 0, "windows per wave");
+  __shared__ __attribute__((aligned(16))) E As[NBUF][NS][BM][KT];
+  __shared__ __attribute__((aligned(16))) E Bs[NBUF][NS][BM][KT];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int64_t L = xcd_logical(blockIdx.x, g.nblocks);
+  const int64_t per_f = (int64_t)g.nkc * g.tiles;
+  const int f = (int)(L / per_f);
+  const int kc = (int)((L % per_f) / g.tiles);
+  const int tile = (int)(L % g.tiles);
+  int bm, bn;
+  tri_grouped(tile, g.tm, bm, bn);
+  const int m0 = bm * BM, n0 = bn * BM;
+  const E* __restrict__ Ya = static_cast<const E*>(g.y) + (int64_t)f * g.fstride + (int64_t)m0 * KT;
+  const E* __restrict__ Yb = static_cast<const E*>(g.y) + (int64_t)f * g.fstride + (int64_t)n0 * KT;
+  const int64_t slab = (int64_t)g.ppad * KT;
+  // this lane's chunk within a window: row rl = lane / 4 (of 16), logical chunk j
+  const int rl = lane >> 2;
+  // the XOR of lds_chunk depends on the row's bits 1-2: the same for row
+  // 16 w + rl as for rl, since windows start at multiples of 16
+  const int jl = (lane & 3) ^ ((rl >> 1) & 3);
+  auto issue = [&](int s, int buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < 2 * NS * WINS / NW; ++u) {
+      const int w = wave + u * NW;             // window index over (operand, plane, 16-row block)
+      const int op = w / (NS * WINS), q = (w / WINS) % NS, win = w % WINS;
+      const int64_t off = q * g.plane + (int64_t)s * slab + (int64_t)(16 * win + rl) * KT + 8 * jl;
+      // operands cast to non-dependent types: with the element type a template
+      // parameter, the host pass drops the kernel's launch stub
+      if (op == 0)
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned*>(Ya + off),
+                                         reinterpret_cast<unsigned*>(&As[buf][q][16 * win][0]), 16, 0, 0);
+      else
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned*>(Yb + off),
+                                         reinterpret_cast<unsigned*>(&Bs[buf][q][16 * win][0]), 16, 0, 0);
+    }
+  };
+
+  f4 part[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) part[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const int fr = lane & 15, fq = lane >> 4;
+  auto compute = [&](int buf) __attribute__((always_inline)) {
+    V a[NS][FM], b[NS][FN];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        a[q][i] = *reinterpret_cast<const V*>(&As[buf][q][wm * TM + i * 16 + fr][8 * lds_chunk(fr, fq)]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        b[q][j] = *reinterpret_cast<const V*>(&Bs[buf][q][wn * TN + j * 16 + fr][8 * lds_chunk(fr, fq)]);
+    }
+    constexpr int QA[3] = {1, 0, 0};
+    constexpr int QB[3] = {0, 1, 0};
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) part[i][j] = mfma16x16x32<H>(b[QB[pr]][j], a[QA[pr]][i], part[i][j]);
+  };
+
+  const int nslab = g.nkt;
+  const int s0 = kc * g.kc_slabs;
+  const int s1 = min(nslab, s0 + g.kc_slabs);
+  if constexpr (DB) {
+    issue(s0, 0);
+    for (int s = s0; s < s1; ++s) {
+      const int cur = (s - s0) & 1;
+      __builtin_amdgcn_s_waitcnt(0);           // this wave's copies of slab s have landed
+      __syncthreads();                         // everyone's; and slab s - 1's buffer is free
+      if (s + 1 < s1) issue(s + 1, cur ^ 1);
+      compute(cur);
+    }
+  } else {
+    for (int s = s0; s < s1; ++s) {
+      issue(s, 0);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      compute(0);
+      __syncthreads();                         // fragments read before the next copies
+    }
+  }
+
+  float* __restrict__ P = g.part + ((int64_t)(f * g.tiles + tile) * g.nkc + kc) * (BM * BM);
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        P[(wm * TM + i * 16 + fr) + (wn * TN + j * 16 + 4 * fq + r) * BM] = part[i][j][r];
+}
+
+// SYRK kernel of the K-split planes path: MGW_CHI_SYRK = 0 (register
+// prefetch, default), 1 (LDS copies, one buffer, 4 waves), 2 (two buffers,
+// 4 waves), 3 (one buffer, 2 waves of 64 x 128), 4 (two buffers, 2 waves).
+// Read at every chi0 build.
+int chi_syrk_variant() {
+  const char* e = std::getenv("MGW_CHI_SYRK");
+  const int v = e ? std::atoi(e) : 0;
+  return (v >= 0 && v <= 4) ? v : 0;
+}
+
+int launch_syrk_lds(bool h, int var, dim3 grid, const SyrkArgs& g, hipStream_t stream) {
+  constexpr int BM = 128;
+  if (h) {
+    if (var == 1) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 2, false>), grid, dim3(256), 0, stream, g);
+    else if (var == 2) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 2, true>), grid, dim3(256), 0, stream, g);
+    else if (var == 3) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 1, false>), grid, dim3(128), 0, stream, g);
+    else hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 1, true>), grid, dim3(128), 0, stream, g);
+  } else {
+    if (var == 1) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 2, false>), grid, dim3(256), 0, stream, g);
+    else if (var == 2) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 2, true>), grid, dim3(256), 0, stream, g);
+    else if (var == 3) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 1, false>), grid, dim3(128), 0, stream, g);
+    else hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 1, true>), grid, dim3(128), 0, stream, g);
+  }
+  MGW_HIP(hipGetLastError());
+  return MGW_OK;
+}
+
 // 1 - chi0 = I + sum over the K chunks of the FP32 partial tiles, summed in
 // FP64 in chunk order; lower triangle, zeros above it.
 template <int BM>
@@ -722,14 +869,6 @@ int chi_kc_slabs() {
   return forced ? forced : KC_SLABS;
 }
 
-// Waves of the K-split SYRK workgroup (128 x 128 tile): 4 (64 x 64 per wave,
-// default) or 2 (64 x 128 per wave: a quarter fewer LDS fragment reads per
-// MFMA, more registers per wave). MGW_CHI_WAVES, read at every chi0 build.
-int chi_syrk_waves() {
-  const char* e = std::getenv("MGW_CHI_WAVES");
-  return (e && std::atoi(e) == 2) ? 2 : 4;
-}
-
 struct SplitLayout {
   int ppad, nkt, tm, tiles, kc_slabs, nkc;
   int64_t per_plane_f;   // elements of one plane of one frequency
@@ -855,10 +994,9 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
       // VGPRs, two or three waves per SIMD; a second register set spills
       // (two LDS buffers with one barrier per slab, or two k tiles per slab:
       // 64 KB of LDS and two waves per SIMD, measured slower)
-      if (chi_syrk_waves() == 2)
-        hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 1, 4, 1, 1, true>), grid, dim3(128), 0, stream, g);
-      else
-        hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
+      const int var = NS == 2 ? chi_syrk_variant() : 0;
+      if (var) MGW_TRY(launch_syrk_lds(H, var, grid, g, stream));
+      else hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
       MGW_HIP(hipGetLastError());
       hipLaunchKernelGGL((chi_ksplit_reduce_kernel<BM>), dim3(BM * BM / 256, L.tiles, nf), block, 0, stream, part,
                          L.nkc, L.tiles, L.tm, naux, g.unscale, g.C, g.strideC);

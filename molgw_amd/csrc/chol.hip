@@ -100,24 +100,37 @@ chol_diag_kernel(double* __restrict__ A, int n, int64_t strideA, int k, int kb, 
   }
 }
 
-// The same factorisation and inverse with four waves per matrix, row owned:
-// thread t holds row i = t / 4 of the block at the columns l = 4 c + (t % 4),
-// c = 0 .. 15, in registers (the four threads of a row are neighbouring lanes
-// of one wave). Step j of the right-looking factorisation publishes column j
-// (its owners write A(i, j), i >= j, to an LDS vector), one barrier, and every
-// thread updates its own columns l, j < l <= i:
-//   A(i, l) -= A(i, j) A(l, j) / A(j, j),   L(i, j) = A(i, j) / sqrt(A(j, j)).
-// The inverse T = L^-1 goes by rows: step l divides the row l of T by L(l, l),
-// publishes it, one barrier, and every row i > l subtracts L(i, l) T(l, :),
-// L(i, l) coming from the row's owner lane by a cross-lane read. The two LDS
-// vectors alternate, so a step's writes never meet the previous step's reads
-// and one barrier per step suffices. A non-positive pivot is recorded and
-// replaced by 1, as in chol_diag_kernel.
+// FP64 reciprocal: the hardware estimate refined by two Newton steps
+// (each doubles the correct bits), so the dependent chain of a leaf step is a
+// few FMAs instead of the division sequence.
+__device__ inline double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return r;
+}
+
+// The same factor and inverse with four waves per matrix, row owned, as
+// L D L^T with L unit lower (no square root or division on the sequential
+// path): thread t holds row i = t / 4 of the block at the columns
+// l = 4 c + (t % 4), c = 0 .. 15, in registers (the four threads of a row are
+// neighbouring lanes of one wave). Step j of the elimination publishes
+// column j (its owners write A(i, j), i >= j, to an LDS vector), one
+// barrier, d_j = A(j, j), and every thread updates its own columns j < l <= i:
+//   A(i, l) -= (A(i, j) / d_j) A(l, j),   L(i, j) = A(i, j) / d_j.
+// The unit lower L is inverted by rows without divisions: step l publishes
+// row l of T = L^-1, one barrier, and every row i > l subtracts
+// L(i, l) T(l, :), L(i, l) coming from the row's owner lane by a cross-lane
+// read. Then the Cholesky factor's inverse is D^-1/2 T, row i scaled by
+// d_i^-1/2, and ln det = sum ln d_j, all in parallel. The two LDS vectors
+// alternate, so a step's writes never meet the previous step's reads and one
+// barrier per step suffices. A non-positive pivot is recorded and replaced
+// by 1, as in chol_diag_kernel.
 __global__ void __launch_bounds__(256)
 chol_leaf_kernel(double* __restrict__ A, int n, int64_t strideA, int k, int kb, double* __restrict__ logdet,
                  int32_t* __restrict__ info) {
   constexpr int NC = NB / 4;        // columns per thread
-  __shared__ double vec[2][NB];     // published column (factorisation) / row (inverse)
+  __shared__ double vec[2][NB];     // published column (elimination) / row (inverse)
   __shared__ double red[4];
   const int f = blockIdx.x, t = threadIdx.x;
   const int i = t >> 2, q = t & 3, lane = t & 63;
@@ -134,49 +147,45 @@ chol_leaf_kernel(double* __restrict__ A, int n, int64_t strideA, int k, int kb, 
     double* __restrict__ v = vec[j & 1];
     if (q == (j & 3) && i >= j) v[i] = a[j >> 2];
     __syncthreads();
-    double p = v[j];
-    if (!(p > 0.0)) {
+    double d = v[j];
+    if (!(d > 0.0)) {
       if (!failed && j < kb) failed = k + j + 1;
-      p = 1.0;
+      d = 1.0;
     }
-    const double d = sqrt(p), rd = 1.0 / d;
     if (i > j) {
-      const double lij = v[i] * rd;
+      const double lij = v[i] * rcp_nr(d);
 #pragma unroll
       for (int c = j >> 2; c < NC; ++c) {
         const int l = 4 * c + q;
-        if (l > j && l <= i) a[c] -= lij * (v[l] * rd);
+        if (l > j && l <= i) a[c] -= lij * v[l];
       }
       if (q == (j & 3)) a[j >> 2] = lij;
     } else if (i == j && q == (j & 3)) {
-      a[j >> 2] = d;
+      a[j >> 2] = d;                 // D on the diagonal
     }
   }
-  // ln det = 2 sum ln L(i, i): the diagonal owner of each row, then a block sum
-  double ld = 0.0;
+  // d_i of this row from its owner lane; ln det(block) / 2 = sum ln d_i / 2
+  double di = 0.0;
 #pragma unroll
   for (int c = 0; c < NC; ++c)
-    if (4 * c + q == i && i < kb) ld = log(a[c]);
+    if (4 * c + q == i) di = a[c];
+  di = __shfl(di, (lane & ~3) | (i & 3), 64);
+  double ld = (q == 0 && i < kb) ? 0.5 * log(di) : 0.0;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) ld += __shfl_down(ld, off, 64);
   if (lane == 0) red[t >> 6] = ld;
-  // inverse by rows
+  // T = L^-1 by rows (unit diagonal: no divisions)
   double tt[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) tt[c] = (4 * c + q == i) ? 1.0 : 0.0;
 #pragma unroll
   for (int l = 0; l < NB; ++l) {
     double* __restrict__ v = vec[l & 1];
-    // L(i, l) from the lane of row i that owns column l
-    const double lil = __shfl(a[l >> 2], (lane & ~3) | (l & 3), 64);
+    const double lil = __shfl(a[l >> 2], (lane & ~3) | (l & 3), 64);   // L(i, l)
     if (i == l) {
-      const double r = 1.0 / lil;
 #pragma unroll
       for (int c = 0; c < NC; ++c)
-        if (4 * c + q <= l) {
-          tt[c] *= r;
-          v[4 * c + q] = tt[c];
-        }
+        if (4 * c + q <= l) v[4 * c + q] = tt[c];
     }
     __syncthreads();
     if (i > l) {
@@ -186,10 +195,11 @@ chol_leaf_kernel(double* __restrict__ A, int n, int64_t strideA, int k, int kb, 
     }
   }
   if (i < kb) {
+    const double rs = 1.0 / sqrt(di);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int l = 4 * c + q;
-      if (l < kb) g[i + (int64_t)l * n] = l <= i ? tt[c] : 0.0;
+      if (l < kb) g[i + (int64_t)l * n] = l <= i ? tt[c] * rs : 0.0;
     }
   }
   if (t == 0) {

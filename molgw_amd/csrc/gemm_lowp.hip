@@ -627,8 +627,10 @@ int gemm_lowp(const GemmF64& h, int precision, hipStream_t stream) {
     MGW_REQUIRE(!h.transA && !h.transB && !h.lower_only && !h.kscale && !h.rowscale && !h.colscale && !h.diag_add &&
                     h.amax && h.bmax,
                 MGW_ERR_ARG, "gemm_lowp: FP16x3 takes plain non-transposed products with operand maxima");
-    // narrow N (the first transform's occupied or window columns): 64-wide column tiles
+    // one column tile when N (the first transform's occupied and window
+    // columns) fits 64, 128 or 192, so that the large operand A is read once
     if (h.N <= 64) return launch_f16x3<128, 64, 2, 2, MGW_KCLASS_GEMM_AO2MO>(h, stream);
+    if (h.N > 128 && h.N <= 192) return launch_f16x3<128, 192, 2, 2, MGW_KCLASS_GEMM_AO2MO>(h, stream);
     return launch_f16x3<128, 128, 2, 2, MGW_KCLASS_GEMM_AO2MO>(h, stream);
   }
   if (precision == MGW_FP32) {

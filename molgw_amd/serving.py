@@ -87,7 +87,13 @@ class Server:
         self.trims = 0       # cached-buffer releases forced by admission
         self._jobs = [queue.Queue() for _ in range(nslots)]
         self._done = queue.Queue()
-        self._threads = [threading.Thread(target=self._worker, args=(k,), daemon=True) for k in range(nslots)]
+        # slot threads with 64 MiB stacks: the library's calls (and a profiler's
+        # callbacks on these threads) run on them
+        old_stack = threading.stack_size(64 << 20)
+        try:
+            self._threads = [threading.Thread(target=self._worker, args=(k,), daemon=True) for k in range(nslots)]
+        finally:
+            threading.stack_size(old_stack)
         for t in self._threads:
             t.start()
         self._reserved = [0] * nslots

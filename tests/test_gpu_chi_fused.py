@@ -117,3 +117,20 @@ def test_cholesky_leaf_kernels_agree():
           f'E_RPA {abs(e_lds - e_wave):.2e}')
     assert np.max(np.abs(chi_lds - chi_wave)) < 1e-12 * scale
     assert abs(e_lds - e_wave) < 1e-12 * abs(e_wave)
+
+
+@pytest.mark.parametrize('variant', ['1', '2', '3', '4'])
+def test_syrk_lds_copies_equal_the_register_prefetch(variant):
+    """The SYRK variants that copy slabs straight into LDS (one or two
+    buffers, 4 waves of 64 x 64 or 2 waves of 64 x 128) do the same products
+    in the same order per accumulator as the register-prefetch kernel: chi
+    is identical bit for bit, FP16x3 and BF16x3."""
+    s = synthetic_system(60, 300, 12, seed=41, scale=0.3)
+    with Engine(0) as eng:
+        eng.set_eri_3center(s.eri_3center, s.index_basis, s.nbf)
+        eng.set_mo(s.c_matrix, s.energy, s.occupation)
+        for precision in ('fp16x3', 'bf16x3'):
+            e0, chi0 = _chi(eng, s, precision, {'MGW_CHI_SYRK': '0'})
+            e1, chi1 = _chi(eng, s, precision, {'MGW_CHI_SYRK': variant})
+            np.testing.assert_array_equal(chi1, chi0)
+            assert e1 == e0
