@@ -129,6 +129,61 @@ chi_split_kernel(const double* __restrict__ x, int naux, int npole, const double
   }
 }
 
+// The two-plane split from the k-tiled FP32 copy of X (xf[kt][P][32], made
+// once per build by chi_xf32_kernel): a thread's 8 transitions of one row are
+// two 16-byte loads, so the block needs no LDS transpose, and each frequency
+// chunk reads 4 bytes per element of X instead of 8. Same arithmetic as
+// chi_split_kernel<2, H> (FP32 product, then the split): identical planes.
+template <bool H>
+__global__ void __launch_bounds__(256)
+chi_split_xf_kernel(const float* __restrict__ xf, int npole, const double* __restrict__ sw,
+                    const double* __restrict__ scale, int f0, int nf, int ppad, int nkt,
+                    typename PlaneT<H>::E* __restrict__ y, int64_t plane) {
+  typedef typename PlaneT<H>::E E;
+  typedef typename PlaneT<H>::V V;
+  constexpr int FB = 32;
+  __shared__ float wt[FB][KT];
+  const int tid = threadIdx.x;
+  const int p0 = blockIdx.x * 64, kt = blockIdx.y, t0 = kt * KT;
+  const int p = tid >> 2, c8 = tid & 3;
+  const int64_t row = (int64_t)(p0 + p) * KT + 8 * c8;
+  const float* __restrict__ src = xf + (int64_t)kt * ppad * KT + row;
+  const f4 x0 = *reinterpret_cast<const f4*>(src), x1 = *reinterpret_cast<const f4*>(src + 4);
+  float xv[8];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    xv[e] = x0[e];
+    xv[4 + e] = x1[e];
+  }
+  for (int fb = 0; fb < nf; fb += FB) {
+    const int nb = min(FB, nf - fb);
+    __syncthreads();                        // previous block's weights consumed
+    for (int e = tid; e < FB * KT; e += 256) {
+      const int fl = e / KT, tl = e % KT, t = t0 + tl;
+      double w = 0.0;
+      if (fl < nb && t < npole) {
+        const int f = f0 + fb + fl;
+        w = sqrt(sw[(int64_t)f * npole + t]) * (scale ? scale[f] : 1.0);
+      }
+      wt[fl][tl] = (float)w;
+    }
+    __syncthreads();
+    for (int fl = 0; fl < nb; ++fl) {
+      V hi, lo;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float v = xv[e] * wt[fl][8 * c8 + e];
+        const E h = (E)v;
+        hi[e] = h;
+        lo[e] = (E)(v - (float)h);
+      }
+      const int64_t off = ((int64_t)(fb + fl) * nkt + kt) * (int64_t)ppad * KT + row;
+      *reinterpret_cast<V*>(y + off) = hi;
+      *reinterpret_cast<V*>(y + plane + off) = lo;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- SYRK
 struct SyrkArgs {
   const void* y;       // planes (bf16 or fp16)
@@ -908,6 +963,11 @@ SplitLayout split_layout(int naux, int npole, int ns, bool ksplit, int nomega = 
   return L;
 }
 
+// bytes of the FP32 k-tiled copy of X that the two-plane split reads
+size_t split_xf_bytes(const SplitLayout& L, int ns) {
+  return ns == 2 ? ((size_t)L.nkt * L.ppad * KT * sizeof(float) + 255) / 256 * 256 : 0;
+}
+
 // ---------------------------------------------------------------- scales
 constexpr int NB_MAX = ABSMAX_BLOCKS;   // blocks of the |X| max reduction
 
@@ -958,8 +1018,9 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
   const SplitLayout L = split_layout(naux, npole, NS, ksplit, nomega);
   const size_t per_f = L.planes_per_f + L.part_per_f;
   const size_t head = scale_header_bytes(nomega);
-  MGW_REQUIRE(work_bytes > head + 256, MGW_ERR_ARG, "chi0 split: workspace too small");
-  const int nf_max = (int)std::min<size_t>((size_t)nomega, (work_bytes - head - 256) / per_f);
+  const size_t xf_bytes = split_xf_bytes(L, NS);
+  MGW_REQUIRE(work_bytes > head + xf_bytes + 256, MGW_ERR_ARG, "chi0 split: workspace too small");
+  const int nf_max = (int)std::min<size_t>((size_t)nomega, (work_bytes - head - xf_bytes - 256) / per_f);
   MGW_REQUIRE(nf_max >= 1, MGW_ERR_ARG, "chi0 split: workspace smaller than one frequency");
   double* scl = static_cast<double*>(work);
   double* uns = scl + nomega;
@@ -967,16 +1028,28 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
   E* planes = reinterpret_cast<E*>(static_cast<char*>(work) + head);
   float* part = reinterpret_cast<float*>(static_cast<char*>(work) + head +
                                          (L.planes_per_f * nf_max + 255) / 256 * 256);
+  // the FP32 k-tiled copy of X behind the partial tiles (two planes only)
+  float* xf = reinterpret_cast<float*>(static_cast<char*>(work) + head +
+                                       (L.planes_per_f * nf_max + 255) / 256 * 256 +
+                                       (L.part_per_f * nf_max + 255) / 256 * 256);
   if (H) {
     MGW_TRY(launch_absmax(x, (int64_t)naux * npole, xpart, stream));
     hipLaunchKernelGGL(chi_scale_kernel, dim3(nomega), dim3(256), 0, stream, xpart, NB_MAX, sw, npole, scl, uns);
     MGW_HIP(hipGetLastError());
   }
+  if (NS == 2) {
+    hipLaunchKernelGGL(chi_xf32_kernel, dim3(L.ppad / 64, L.nkt), dim3(256), 0, stream, x, naux, npole, L.ppad, xf);
+    MGW_HIP(hipGetLastError());
+  }
   for (int f0 = 0; f0 < nomega; f0 += nf_max) {
     const int nf = std::min(nf_max, nomega - f0);
     const int64_t plane = L.per_plane_f * nf;
-    hipLaunchKernelGGL((chi_split_kernel<NS, H>), dim3(L.ppad / 64, L.nkt), dim3(256), 0, stream, x, naux, npole,
-                       sw, H ? scl : nullptr, f0, nf, L.ppad, L.nkt, planes, plane);
+    if constexpr (NS == 2)
+      hipLaunchKernelGGL((chi_split_xf_kernel<H>), dim3(L.ppad / 64, L.nkt), dim3(256), 0, stream, xf, npole, sw,
+                         H ? scl : nullptr, f0, nf, L.ppad, L.nkt, planes, plane);
+    else
+      hipLaunchKernelGGL((chi_split_kernel<NS, H>), dim3(L.ppad / 64, L.nkt), dim3(256), 0, stream, x, naux, npole,
+                         sw, H ? scl : nullptr, f0, nf, L.ppad, L.nkt, planes, plane);
     MGW_HIP(hipGetLastError());
     SyrkArgs g;
     g.y = planes; g.plane = plane; g.fstride = L.per_plane_f;
@@ -1175,10 +1248,12 @@ size_t chi0_split_bytes_per_frequency(int naux, int npole, int precision, int no
 // least one).
 size_t chi0_split_workspace(int naux, int npole, int nomega, int precision) {
   if (precision != MGW_BF16X6 && fused_enabled()) return fused_workspace(naux, npole, nomega);
-  const size_t per_f = chi0_split_bytes_per_frequency(naux, npole, precision, nomega);
+  const int ns = precision == MGW_BF16X6 ? 3 : 2;
+  const SplitLayout L = split_layout(naux, npole, ns, chi_ksplit(), nomega);
+  const size_t per_f = L.planes_per_f + L.part_per_f;
   const size_t budget = std::max<size_t>((size_t)32 * naux * npole, (size_t)2 << 30);
   const size_t nf = std::max<size_t>(1, std::min<size_t>((size_t)nomega, budget / per_f));
-  return scale_header_bytes(nomega) + nf * per_f + 512;
+  return scale_header_bytes(nomega) + nf * per_f + split_xf_bytes(L, ns) + 1024;
 }
 
 int chi0_split_syrk(const double* x, int naux, int npole, const double* sw, int nomega, int precision,
