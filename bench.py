@@ -573,6 +573,8 @@ def main():
     ap.add_argument('--requests', type=int, default=None, help='stream / batch length (same as --steps)')
     ap.add_argument('--rate', type=float, default=20.0, help='Poisson arrival rate, requests/s (serving)')
     ap.add_argument('--slots', type=int, default=16, help='concurrent solve slots (serving, batch64)')
+    ap.add_argument('--autotune', action='store_true',
+                    help='time the FP64 GEMM tile candidates of each new shape class on first use (mgw_autotune)')
     ap.add_argument('--hw-queues', type=int, default=16,
                     help='hardware queues of this process (GPU_MAX_HW_QUEUES, set before the HIP runtime starts; '
                          'HIP defaults to 4, which the slots\' streams then share)')
@@ -588,6 +590,9 @@ def main():
     # which is the first libmgw call below
     if args.hw_queues > 0:
         os.environ['GPU_MAX_HW_QUEUES'] = str(min(args.hw_queues, 32))
+    if args.autotune:
+        from molgw_amd.gw import autotune
+        autotune(True)
     if args.gpus != 1:
         sys.exit('bench.py measures one GPU: every box here has one, and the path has no multi-GPU code; '
                  'run with --gpus 1')
