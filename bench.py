@@ -619,7 +619,8 @@ def main():
             'metric': METRIC, 'value': sm['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
             'steps': sm['solves'], 'warmup': args.warmup, 'ms_per_step': 1e3 / sm['solves_per_s'],
             'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-            'dtype': f'{prec} products, f64 accumulation; f64 inversion and QP',
+            'dtype': (f'{prec} products, f64 accumulation; f64 Cholesky leaves and QP' if prec == 'fp16x3'
+                      else f'{prec} products, f64 accumulation; f64 inversion and QP'),
             'qp_error_vs_fp64_mev': err,
             'data': 'real molecules: integrals, RI tensor and RI-HF orbitals from the GPU front end (no synthetic data)',
             'config': {'workload': desc, 'technique': 'imaginary_pade', 'nomega_chi_imag': 32,
@@ -695,7 +696,8 @@ def main():
         'dtype': {'fp64': 'f64', 'fp32': 'f32 products, f64 accumulation',
                   'bf16x3': 'bf16x3 products, f64 accumulation',
                   'bf16x6': 'bf16x6 products (chi0, W.B), f64 transforms, f64 accumulation',
-                  'fp16x3': 'fp16x3 products (chi0, W.B), f64 transforms, f64 accumulation'}[args.precision],
+                  'fp16x3': 'fp16x3 products (transforms, chi0, Cholesky products, W.B), f64 accumulation'}[
+                      args.precision],
         'qp_error_vs_fp64_mev': err_mev,
         'data': 'synthetic (seeded RI tensor, orthonormal C and energies at the molecule\'s shape; '
                 'no integral engine in the image)',

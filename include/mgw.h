@@ -45,16 +45,20 @@ enum {
 enum { MGW_MEM_HOST = 0, MGW_MEM_DEVICE = 1 };
 
 /* Arithmetic of the transforms, the chi0 / Casida-matrix build and the
- * imaginary-axis self-energy product (the metric, the (1 - chi0)
- * factorisation, the eigensolve and the QP solve are FP64 in every mode):
+ * imaginary-axis self-energy product (the metric, the eigensolve and the QP
+ * solve are FP64 in every mode; the (1 - chi0) factorisation is FP64 except
+ * as stated for MGW_FP16X3):
  *   MGW_FP32   FP32 products on MFMA, FP64 accumulation
  *   MGW_BF16X3 two-way bf16 split, 3 products (~2^-17 relative)
  *   MGW_BF16X6 three-way bf16 split for chi0 and the self-energy product,
  *              6 products (~2^-24, FP32 level); the transforms stay FP64
  *   MGW_FP16X3 two-way fp16 split of power-of-two-scaled operands for chi0,
- *              the self-energy product and the AO->MO transforms (FP32
- *              accumulation over K = nbf), 3 products (~2^-22); the
- *              environment variable MGW_AO2MO_FP16X3=0 keeps the transforms
+ *              the self-energy product, the AO->MO transforms (FP32
+ *              accumulation over K = nbf) and the products of the batched
+ *              (1 - chi0) factorisation with all dimensions >= 256 (FP64
+ *              64 x 64 leaves, FP64 sums across products), 3 products
+ *              (~2^-22); the environment variables MGW_AO2MO_FP16X3=0 and
+ *              MGW_CHOL_F16_MIN=0 keep the transforms and the factorisation
  *              FP64                                                       */
 enum { MGW_FP64 = 0, MGW_FP32 = 1, MGW_BF16X3 = 2, MGW_BF16X6 = 3, MGW_FP16X3 = 4 };
 
@@ -89,10 +93,11 @@ typedef struct mgw_g0w0_params {
   double  step_sigma_calc;    /* Ha (Pade technique only)                     */
   double  spin_fact;          /* 2 for restricted closed shell                */
   int32_t precision;          /* MGW_FP64 | MGW_FP32 | MGW_BF16X3 | MGW_BF16X6
-                                 | MGW_FP16X3 (transforms, chi0 and the
-                                 imaginary-axis self-energy product; metric,
-                                 inversion, eigensolve and the QP solve are
-                                 always FP64)                                 */
+                                 | MGW_FP16X3 (transforms, chi0, the
+                                 imaginary-axis self-energy product and, for
+                                 MGW_FP16X3, the large products of the
+                                 (1 - chi0) factorisation; metric, eigensolve
+                                 and the QP solve are always FP64)            */
   int32_t nomega_chi_real;    /* real-axis W grid, >= 2 (contour technique;
                                  input keyword nomega_chi_real, default 2)    */
 } mgw_g0w0_params;

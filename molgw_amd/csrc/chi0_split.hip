@@ -521,16 +521,33 @@ chi_syrk_lds_kernel(SyrkArgs g) {
 
 // SYRK kernel of the K-split planes path: MGW_CHI_SYRK = 0 (register
 // prefetch, default), 1 (LDS copies, one buffer, 4 waves), 2 (two buffers,
-// 4 waves), 3 (one buffer, 2 waves of 64 x 128), 4 (two buffers, 2 waves).
-// Read at every chi0 build.
+// 4 waves), 3 (one buffer, 2 waves of 64 x 128), 4 (two buffers, 2 waves),
+// 5 (256 x 256 tiles, LDS copies, one buffer, 8 waves of 64 x 128), 6 (the
+// same with two buffers). A 256-wide tile reads each row and column panel
+// half as often as a 128-wide one. Read at every chi0 build.
 int chi_syrk_variant() {
   const char* e = std::getenv("MGW_CHI_SYRK");
   const int v = e ? std::atoi(e) : 0;
-  return (v >= 0 && v <= 4) ? v : 0;
+  return (v >= 0 && v <= 6) ? v : 0;
+}
+
+// Output tile of the planes path's SYRK (two planes, K split): 256 for the
+// variants with 256-wide tiles, else 128.
+int chi_syrk_bm(int ns, bool ksplit) {
+  return (ns == 2 && ksplit && chi_syrk_variant() >= 5) ? 256 : 128;
 }
 
 int launch_syrk_lds(bool h, int var, dim3 grid, const SyrkArgs& g, hipStream_t stream) {
   constexpr int BM = 128;
+  if (var >= 5) {
+    const bool db = var == 6;
+    if (h && db) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, 256, 4, 2, true>), grid, dim3(512), 0, stream, g);
+    else if (h) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, 256, 4, 2, false>), grid, dim3(512), 0, stream, g);
+    else if (db) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, 256, 4, 2, true>), grid, dim3(512), 0, stream, g);
+    else hipLaunchKernelGGL((chi_syrk_lds_kernel<false, 256, 4, 2, false>), grid, dim3(512), 0, stream, g);
+    MGW_HIP(hipGetLastError());
+    return MGW_OK;
+  }
   if (h) {
     if (var == 1) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 2, false>), grid, dim3(256), 0, stream, g);
     else if (var == 2) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 2, true>), grid, dim3(256), 0, stream, g);
@@ -925,7 +942,7 @@ int chi_kc_slabs() {
 }
 
 struct SplitLayout {
-  int ppad, nkt, tm, tiles, kc_slabs, nkc;
+  int bm, ppad, nkt, tm, tiles, kc_slabs, nkc;
   int64_t per_plane_f;   // elements of one plane of one frequency
   size_t planes_per_f;   // bytes of all planes of one frequency
   size_t part_per_f;     // bytes of the FP32 partial tiles of one frequency
@@ -940,10 +957,13 @@ int chi_kc_cap() {
 }
 
 SplitLayout split_layout(int naux, int npole, int ns, bool ksplit, int nomega = 1) {
-  constexpr int bm = 128;
+  const int bm = chi_syrk_bm(ns, ksplit);
   constexpr int64_t FILL = 4096;   // workgroups that keep 256 CUs busy at 2-3 per CU, several waves deep
   SplitLayout L;
-  L.ppad = (naux + PAD_ROWS - 1) / PAD_ROWS * PAD_ROWS;
+  L.bm = bm;
+  // rows padded to whole tiles: the SYRK reads full BM-row panels
+  const int pad = std::max(PAD_ROWS, bm);
+  L.ppad = (naux + pad - 1) / pad * pad;
   L.nkt = ((npole + KT - 1) / KT + 1) / 2 * 2;   // even: slabs of one or two k tiles
   L.tm = (naux + bm - 1) / bm;
   L.tiles = L.tm * (L.tm + 1) / 2;
@@ -1068,11 +1088,16 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
       // (two LDS buffers with one barrier per slab, or two k tiles per slab:
       // 64 KB of LDS and two waves per SIMD, measured slower)
       const int var = NS == 2 ? chi_syrk_variant() : 0;
+      MGW_REQUIRE((L.bm == 256) == (var >= 5), MGW_ERR_ARG, "chi0 split: tile size and SYRK variant disagree");
       if (var) MGW_TRY(launch_syrk_lds(H, var, grid, g, stream));
       else hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
       MGW_HIP(hipGetLastError());
-      hipLaunchKernelGGL((chi_ksplit_reduce_kernel<BM>), dim3(BM * BM / 256, L.tiles, nf), block, 0, stream, part,
-                         L.nkc, L.tiles, L.tm, naux, g.unscale, g.C, g.strideC);
+      if (L.bm == 256)
+        hipLaunchKernelGGL((chi_ksplit_reduce_kernel<256>), dim3(256 * 256 / 256, L.tiles, nf), block, 0, stream,
+                           part, L.nkc, L.tiles, L.tm, naux, g.unscale, g.C, g.strideC);
+      else
+        hipLaunchKernelGGL((chi_ksplit_reduce_kernel<BM>), dim3(BM * BM / 256, L.tiles, nf), block, 0, stream, part,
+                           L.nkc, L.tiles, L.tm, naux, g.unscale, g.C, g.strideC);
       MGW_HIP(hipGetLastError());
       continue;
     }

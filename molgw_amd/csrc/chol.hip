@@ -220,7 +220,58 @@ int gemm_chol(const GemmF64& g0, hipStream_t stream) {
   GemmF64 g = g0;
   g.klass = MGW_KCLASS_CHOLESKY;
   g.batch_outer = true;   // the matrices of the batch share no operand
+  if (g.amax) return gemm_f16x3_chol(g, stream);
   return gemm_f64(g, stream);
+}
+
+// Operand bounds of the FP16x3 products. With A >= I (so ||L^-1||_2 <= 1)
+// and d = max_f,i A_f(i, i), in exact arithmetic at every level:
+//   |A21|, |A22 - L21 L21^T| <= d        (entries of a positive definite matrix
+//                                          are bounded by its largest diagonal
+//                                          element, and Schur complements do
+//                                          not grow the diagonal),
+//   |L21|, |P = L21 (L^-1)11| <= sqrt(d)  (rows of L have norm sqrt(A_ii);
+//                                          columns of L^-1 have norm <= 1),
+//   |L^-1| <= 1.
+// bounds = {d, sqrt(d), 1}. The FP16x3 scale puts a bound in [2^13, 2^14),
+// so rounding that pushes a value past its bound stays far inside the fp16
+// range (65504), and a bound that overestimates costs precision only for
+// elements below 2^-17 of it.
+// part[f] = max_i A_f(i, i), one workgroup per matrix
+__global__ void __launch_bounds__(256)
+chol_diagmax_kernel(const double* __restrict__ A, int n, int64_t strideA, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int f = blockIdx.x;
+  const double* __restrict__ a = A + (int64_t)f * strideA;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, a[i + (int64_t)i * n]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[f] = red[0];
+}
+
+__global__ void __launch_bounds__(256)
+chol_bounds_kernel(const double* __restrict__ part, int nbatch, double* __restrict__ bounds) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int f = threadIdx.x; f < nbatch; f += 256) m = fmax(m, part[f]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double d = fmax(red[0], 1.0);
+    bounds[0] = d;
+    bounds[1] = sqrt(d);
+    bounds[2] = 1.0;
+    bounds[3] = 0.0;
+  }
 }
 
 // ---------------------------------------------------------------- traces
@@ -268,6 +319,15 @@ size_t chol_need(int m) {
 
 size_t chol_work_doubles(int n) { return std::max<size_t>(chol_need(n), 1); }
 
+// Products of the recursion with min(M, N, K) at least this many run in
+// FP16x3 when the factorisation is given operand bounds (smaller ones are
+// latency-bound, where the split on load does not pay): MGW_CHOL_F16_MIN,
+// read at every factorisation; 0 keeps every product in FP64.
+int chol_f16_min() {
+  const char* e = std::getenv("MGW_CHOL_F16_MIN");
+  return e ? std::atoi(e) : 256;
+}
+
 namespace {
 
 struct CholCtx {
@@ -278,7 +338,18 @@ struct CholCtx {
   double* work;
   int64_t sw;           // work doubles per matrix
   hipStream_t stream;
+  const double* bounds; // device {d, sqrt(d), 1} for FP16x3 products, or null (FP64)
+  int f16_min;          // smallest min(M, N, K) of an FP16x3 product
 };
+
+// Operand bounds of one product of the recursion, FP16x3 when the
+// factorisation has bounds and the product is large enough.
+void chol_lowp(const CholCtx& c, GemmF64& g, int a_bound, int b_bound) {
+  if (!c.bounds || c.f16_min <= 0 || std::min(g.M, std::min(g.N, g.K)) < c.f16_min) return;
+  g.amax = c.bounds + a_bound;
+  g.bmax = c.bounds + b_bound;
+}
+constexpr int BND_D = 0, BND_SQRT_D = 1, BND_ONE = 2;
 
 // Recursive inverse Cholesky factor of the diagonal block [k0, k0 + m):
 //   [A11    ]   factor and invert A11 (recursion), L21 = A21 L11^-T,
@@ -314,6 +385,7 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   p.B = a11; p.ldb = n; p.strideB = s2; p.transB = true;
   p.C = l21; p.ldc = m2; p.strideC = c.sw;
   p.batch = c.nbatch;
+  chol_lowp(c, p, BND_D, BND_ONE);
   MGW_TRY(gemm_chol(p, c.stream));
   // A22 -= L21 L21^T (lower triangle; the upper stays zero)
   GemmF64 u;
@@ -324,6 +396,7 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   u.alpha = -1.0; u.beta = 1.0;
   u.lower_only = true; u.no_mirror = true;
   u.batch = c.nbatch;
+  chol_lowp(c, u, BND_SQRT_D, BND_SQRT_D);
   MGW_TRY(gemm_chol(u, c.stream));
   MGW_TRY(chol_rec(c, k0 + m1, m2, rest));
   // (L^-1)21 = -(L^-1)22 P,  P = L21 (L^-1)11
@@ -333,6 +406,7 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   q.B = a11; q.ldb = n; q.strideB = s2;
   q.C = rest; q.ldc = m2; q.strideC = c.sw;
   q.batch = c.nbatch;
+  chol_lowp(c, q, BND_SQRT_D, BND_ONE);
   MGW_TRY(gemm_chol(q, c.stream));
   GemmF64 t;
   t.M = m2; t.N = m1; t.K = m2;
@@ -341,16 +415,24 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   t.C = a21; t.ldc = n; t.strideC = s2;
   t.alpha = -1.0;
   t.batch = c.nbatch;
+  chol_lowp(c, t, BND_ONE, BND_SQRT_D);
   return gemm_chol(t, c.stream);
 }
 
 }  // namespace
 
 int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* info, double* work,
-                         hipStream_t stream) {
+                         hipStream_t stream, double* bounds) {
   if (n <= 0 || nbatch <= 0) return MGW_OK;
   MGW_REQUIRE(nbatch <= 65535, MGW_ERR_ARG, "chol_inverse_batched: batch too large");
-  const CholCtx c{A, n, nbatch, logdet, info, work, (int64_t)chol_work_doubles(n), stream};
+  const int f16_min = bounds ? chol_f16_min() : 0;
+  if (bounds && f16_min > 0) {
+    hipLaunchKernelGGL(chol_diagmax_kernel, dim3(nbatch), dim3(256), 0, stream, A, n, (int64_t)n * n, bounds + 4);
+    hipLaunchKernelGGL(chol_bounds_kernel, dim3(1), dim3(256), 0, stream, bounds + 4, nbatch, bounds);
+    MGW_HIP(hipGetLastError());
+  }
+  const CholCtx c{A, n, nbatch, logdet, info, work, (int64_t)chol_work_doubles(n), stream,
+                  f16_min > 0 ? bounds : nullptr, f16_min};
   return chol_rec(c, 0, n, work);
 }
 

@@ -119,12 +119,14 @@ def test_cholesky_leaf_kernels_agree():
     assert abs(e_lds - e_wave) < 1e-12 * abs(e_wave)
 
 
-@pytest.mark.parametrize('variant', ['1', '2', '3', '4'])
+@pytest.mark.parametrize('variant', ['1', '2', '3', '4', '5', '6'])
 def test_syrk_lds_copies_equal_the_register_prefetch(variant):
     """The SYRK variants that copy slabs straight into LDS (one or two
-    buffers, 4 waves of 64 x 64 or 2 waves of 64 x 128) do the same products
-    in the same order per accumulator as the register-prefetch kernel: chi
-    is identical bit for bit, FP16x3 and BF16x3."""
+    buffers; 128-wide tiles on 4 waves of 64 x 64 or 2 waves of 64 x 128,
+    or 256-wide tiles on 8 waves of 64 x 128) do the same products in the
+    same order per accumulator, over the same K chunks, as the
+    register-prefetch kernel: chi is identical bit for bit, FP16x3 and
+    BF16x3 (naux 300: one full and two partial 256-wide tiles)."""
     s = synthetic_system(60, 300, 12, seed=41, scale=0.3)
     with Engine(0) as eng:
         eng.set_eri_3center(s.eri_3center, s.index_basis, s.nbf)
@@ -134,3 +136,31 @@ def test_syrk_lds_copies_equal_the_register_prefetch(variant):
             e1, chi1 = _chi(eng, s, precision, {'MGW_CHI_SYRK': variant})
             np.testing.assert_array_equal(chi1, chi0)
             assert e1 == e0
+
+
+def test_fp16x3_cholesky_products_against_fp64():
+    """The batched inverse Cholesky of an FP16x3 solve with its large products
+    in FP16x3 (operand bounds from max diag(1 - chi0): |L^-1| <= 1,
+    |L21| <= sqrt(d), |A21| <= d), forced down to 64-wide products, against
+    the same factorisation in FP64 on the same FP16x3 chi0 (bitwise the same
+    input): chi0 = 1 - (1 + chi)^-1 recovered on the host, relative to
+    max |chi0|, within the FP16x3 bound of the chi0 build itself (8e-6), so
+    the factorisation adds no more than the build's own rounding; E_RPA
+    within 1e-6 relative."""
+    s = synthetic_system(60, 330, 12, seed=43, scale=0.3)
+    with Engine(0) as eng:
+        eng.set_eri_3center(s.eri_3center, s.index_basis, s.nbf)
+        eng.set_mo(s.c_matrix, s.energy, s.occupation)
+        e_lo, chi_lo = _chi(eng, s, 'fp16x3', {'MGW_CHOL_F16_MIN': '64'})
+        e_hi, chi_hi = _chi(eng, s, 'fp16x3', {'MGW_CHOL_F16_MIN': '0'})
+        e64, chi64 = _chi(eng, s, 'fp64', {})
+    c_lo, c_hi, c64 = _chi0(chi_lo), _chi0(chi_hi), _chi0(chi64)
+    scale = np.max(np.abs(c_hi))
+    d_chol = np.max(np.abs(c_lo - c_hi)) / scale
+    d_lo64 = np.max(np.abs(c_lo - c64)) / scale
+    d_hi64 = np.max(np.abs(c_hi - c64)) / scale
+    print(f'FP16x3 Cholesky products: chi0 vs FP64 factorisation {d_chol:.2e} (relative to max |chi0|); '
+          f'against the FP64 solve {d_lo64:.2e} (FP64 factorisation {d_hi64:.2e}); '
+          f'E_RPA {e_lo - e_hi:.2e} Ha (E_RPA {e_hi:.6f})')
+    assert d_chol < 8e-6
+    assert abs(e_lo - e_hi) < 1e-6 * abs(e_hi)
