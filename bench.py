@@ -99,12 +99,12 @@ def stream_bench(workload, args):
         return [Request(k, systems[k], arrival=t, technique='imaginary_pade', precision=precision,
                         nomega_chi_imag=32, nomega_sigma_calc=16) for k, (t, kind, size) in enumerate(spec)]
 
-    with Server(nslots=1, hbm_budget=budget) as srv:
+    with Server(nslots=1, hbm_budget=budget, graphs=not args.no_graphs) as srv:
         srv.run(requests('fp64')[:1], realtime=False)                 # warm-up
         ref = srv.run(requests('fp64'), realtime=False)
     ref_sum = summarize(ref)
     print(f'# fp64 one at a time: {ref_sum}', file=sys.stderr, flush=True)
-    with Server(nslots=args.slots, hbm_budget=budget) as srv:
+    with Server(nslots=args.slots, hbm_budget=budget, graphs=not args.no_graphs) as srv:
         if args.warmup:
             srv.run(requests(args.precision)[:args.warmup], realtime=False)  # untimed warm-up requests
         # capacity: the whole stream available at t = 0, as in the FP64 pass
@@ -125,7 +125,7 @@ def stream_bench(workload, args):
     # the same requests one at a time, after the timed passes: each kernel
     # class alone on the device, so its HIP-event time is its kernels' time
     # (as a kernel trace of the pass shows them)
-    with Server(nslots=1, hbm_budget=budget) as srv:
+    with Server(nslots=1, hbm_budget=budget, graphs=not args.no_graphs) as srv:
         srv.run(requests(args.precision), realtime=False)   # warm: buffers of every shape allocated
         srv.profile(True)
         i0 = time.clock_gettime_ns(time.CLOCK_MONOTONIC)
@@ -579,6 +579,9 @@ def main():
                     help='hardware queues of this process (GPU_MAX_HW_QUEUES, set before the HIP runtime starts; '
                          'HIP defaults to 4, which the slots\' streams then share)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-graphs', action='store_true',
+                    help='launch the Cholesky sequence eagerly instead of replaying its HIP graph (serving slots); '
+                         'for profiler runs')
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--precision', default=None, choices=['fp64', 'fp32', 'bf16x3', 'bf16x6', 'fp16x3'],
                     help='arithmetic of the AO->MO transforms, the chi0/Casida build and the '

@@ -150,11 +150,11 @@ struct mgw_ctx {
   // HIP graphs of the batched Cholesky sequence, one per shape and buffer set
   struct GraphKey {
     int n, nbatch;
-    const void *a, *logdet, *info, *work, *bounds;
+    const void *a, *logdet, *info, *work, *bounds, *planes;
     int f16_min;
     bool operator<(const GraphKey& o) const {
-      return std::tie(n, nbatch, a, logdet, info, work, bounds, f16_min) <
-             std::tie(o.n, o.nbatch, o.a, o.logdet, o.info, o.work, o.bounds, o.f16_min);
+      return std::tie(n, nbatch, a, logdet, info, work, bounds, planes, f16_min) <
+             std::tie(o.n, o.nbatch, o.a, o.logdet, o.info, o.work, o.bounds, o.planes, o.f16_min);
     }
   };
   bool use_graphs = false;
@@ -670,14 +670,14 @@ void clear_graphs(mgw_ctx* c) {
 }
 
 int run_chol(mgw_ctx* c, double* A, int n, int nbatch, double* logdet, int32_t* info, double* work,
-             double* bounds) {
+             double* bounds, void* planes) {
   if (!c->use_graphs || gemm_autotune_mode(-1) == 1)
-    return chol_inverse_batched(A, n, nbatch, logdet, info, work, c->stream, bounds);
-  const mgw_ctx::GraphKey key{n, nbatch, A, logdet, info, work, bounds, bounds ? chol_f16_min() : 0};
+    return chol_inverse_batched(A, n, nbatch, logdet, info, work, c->stream, bounds, planes);
+  const mgw_ctx::GraphKey key{n, nbatch, A, logdet, info, work, bounds, planes, bounds ? chol_f16_min() : 0};
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     MGW_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = chol_inverse_batched(A, n, nbatch, logdet, info, work, c->stream, bounds);
+    const int rc = chol_inverse_batched(A, n, nbatch, logdet, info, work, c->stream, bounds, planes);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc != MGW_OK) {
@@ -764,14 +764,18 @@ int polarizability_grid(mgw_ctx* c, const mgw_ranges* r, double spin_fact, int n
   // FP16x3 solves factor with FP16x3 products at the large levels: 1 - chi0
   // >= I holds there (every weight s >= 0), which the operand bounds need
   double* bounds = nullptr;
-  if (precision == MGW_FP16X3 && nonneg && chol_f16_min() > 0)
+  double* planes = nullptr;
+  if (precision == MGW_FP16X3 && nonneg && chol_f16_min() > 0) {
     MGW_TRY(dbuf(c, "chol_bounds", 4 + (size_t)nomega, &bounds));
+    MGW_TRY(dbuf(c, "chol_planes", std::max<size_t>(1, (chol_plane_bytes(naux, nomega) + 7) / 8), &planes));
+  }
   const double chol_flops = 2.0 / 3.0 * (double)naux * naux * naux;
   {
     ProfScope chol(&c->prof, MGW_KCLASS_CHOLESKY, chol_flops * nomega, 16.0 * (double)naux * naux * nomega,
                    c->stream);
-    MGW_TRY(run_chol(c, chi, naux, nomega, logdet, info_d, work, bounds));
+    MGW_TRY(run_chol(c, chi, naux, nomega, logdet, info_d, work, bounds, planes));
   }
+  if (planes) MGW_TRY(scratch_done(c, "chol_planes"));
   std::vector<int32_t> info(nomega);
   MGW_TRY(download(c, info.data(), info_d, nomega));
   bool checked_x = false;
@@ -1412,6 +1416,8 @@ int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int techn
   } else {
     stage_w = na * nb * no + na * npole + na * na * (int64_t)nomega_chi_imag + npole * nomega_chi_imag +
               (int64_t)mgw::chol_work_doubles((int)na) * nomega_chi_imag + 4 + nomega_chi_imag;
+    if (precision == MGW_FP16X3)
+      stage_w += (int64_t)((mgw::chol_plane_bytes((int)na, nomega_chi_imag) + 7) / 8);
     if (split_precision(precision))
       stage_w += (int64_t)(mgw::chi0_split_workspace((int)na, (int)npole, nomega_chi_imag, precision) / 8);
   }

@@ -17,6 +17,7 @@
 // zero (the 1 - chi0 builds write zeros there); the recursion keeps it zero,
 // so the inverted diagonal blocks are plain operands of the next products.
 #include "mgw_internal.h"
+#include "tile_order.h"
 
 #include <algorithm>
 #include <cmath>
@@ -220,7 +221,6 @@ int gemm_chol(const GemmF64& g0, hipStream_t stream) {
   GemmF64 g = g0;
   g.klass = MGW_KCLASS_CHOLESKY;
   g.batch_outer = true;   // the matrices of the batch share no operand
-  if (g.amax) return gemm_f16x3_chol(g, stream);
   return gemm_f64(g, stream);
 }
 
@@ -271,6 +271,222 @@ chol_bounds_kernel(const double* __restrict__ part, int nbatch, double* __restri
     bounds[1] = sqrt(d);
     bounds[2] = 1.0;
     bounds[3] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------- FP16x3 products on planes
+// The large products of the recursion multiply operands split once into
+// two fp16 planes (hi, lo) of 2^e x, in the k-tiled layout
+// plane[f][q][kt][row][32] (q = 0 hi, 1 lo; rows padded to PL_ROWS, k to
+// 32, zeros in the padding), so a row block's 32-k slab is one contiguous
+// run. An operand is read in FP64 once per split instead of once per tile
+// of its row or column, and the product kernel does no conversions.
+constexpr int PL_K = 32;       // k per tile of the plane layout
+constexpr int PL_ROWS = 128;   // rows padded to the product's tile
+typedef float pf4 __attribute__((ext_vector_type(4)));
+typedef _Float16 ph8 __attribute__((ext_vector_type(8)));
+typedef unsigned int pu4 __attribute__((ext_vector_type(4)));
+
+__device__ inline double plane_scale(double bound) {
+  return (bound > 0.0 && bound <= 1.0e300) ? ldexp(1.0, 13 - ilogb(bound)) : 1.0;
+}
+// LDS rows of 64 bytes; 16-byte chunk j of row r stored at chunk j ^ ((r >> 1) & 3)
+__device__ inline int pl_chunk(int r, int j) { return j ^ ((r >> 1) & 3); }
+
+struct PlaneDims {
+  int rows, k;           // logical operand: rows x k
+  int rpad, nkt;         // padded rows, k tiles
+  int64_t per_q;         // elements of one plane of one matrix
+  int64_t per_f;         // elements of both planes of one matrix
+};
+inline PlaneDims plane_dims(int rows, int k) {
+  PlaneDims d;
+  d.rows = rows; d.k = k;
+  d.rpad = (rows + PL_ROWS - 1) / PL_ROWS * PL_ROWS;
+  d.nkt = (k + PL_K - 1) / PL_K;
+  d.per_q = (int64_t)d.nkt * d.rpad * PL_K;
+  d.per_f = 2 * d.per_q;
+  return d;
+}
+
+// Split op(X) (rows x k) of every matrix f into the two planes. trans = 0:
+// op(X)(r, k) = X[r + k ld] (rows contiguous: staged through LDS so that a
+// thread writes 8 consecutive k); trans = 1: op(X)(r, k) = X[k + r ld].
+// Block: 64 rows x 32 k of one matrix.
+__global__ void __launch_bounds__(256)
+chol_split_kernel(const double* __restrict__ X, int64_t ld, int64_t strideX, int trans, int rows, int k,
+                  int rpad, int nkt, const double* __restrict__ bound, _Float16* __restrict__ planes) {
+  __shared__ double tile[PL_K][65];
+  const int tid = threadIdx.x;
+  const int r0 = blockIdx.x * 64, kt = blockIdx.y, f = blockIdx.z, k0 = kt * PL_K;
+  const double* __restrict__ x = X + (int64_t)f * strideX;
+  const double sc = plane_scale(bound[0]);
+  const int r = tid >> 2, c8 = tid & 3;
+  double v[8];
+  if (!trans) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int kl = (tid >> 6) + 4 * j, rl = tid & 63;
+      const int R = r0 + rl, K = k0 + kl;
+      tile[kl][rl] = (R < rows && K < k) ? x[R + (int64_t)K * ld] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = tile[8 * c8 + e][r];
+  } else {
+    const int R = r0 + r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int K = k0 + 8 * c8 + e;
+      v[e] = (R < rows && K < k) ? x[K + (int64_t)R * ld] : 0.0;
+    }
+  }
+  ph8 hi, lo;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const float y = (float)(v[e] * sc);
+    const _Float16 h = (_Float16)y;
+    hi[e] = h;
+    lo[e] = (_Float16)(y - (float)h);
+  }
+  const int64_t per_q = (int64_t)nkt * rpad * PL_K;
+  _Float16* __restrict__ dst = planes + (int64_t)f * 2 * per_q + ((int64_t)kt * rpad + r0 + r) * PL_K + 8 * c8;
+  *reinterpret_cast<ph8*>(dst) = hi;
+  *reinterpret_cast<ph8*>(dst + per_q) = lo;
+}
+
+struct PlaneGemmArgs {
+  const _Float16* a;  int64_t a_q, a_f; int a_rpad;   // A planes (M rows), plane and matrix strides
+  const _Float16* b;  int64_t b_q, b_f; int b_rpad;   // B^T planes (N rows)
+  int M, N, nkt;
+  // K range of a tile in k tiles: A lower triangular (k <= m) stops the K
+  // loop at the tile's last row; B^T(n, k) zero for k > n (kb_hi) or for
+  // k < n (kb_lo) trims it at the tile's columns
+  int tri_a, kb_hi, kb_lo;
+  const double* amax; const double* bmax;
+  double alpha, beta;
+  double* C; int64_t ldc, strideC;
+  int tiles_m, tiles_n;
+  int64_t tiles, nblocks;
+};
+
+// C(m, n) = alpha sum_k A(m, k) B^T(n, k) + beta C(m, n) for every matrix,
+// 128 x 128 tiles, 4 waves of 64 x 64, one 32-k slab per step (register
+// prefetch of the next slab, one LDS buffer); hi hi + hi lo + lo hi on
+// v_mfma_f32_16x16x32_f16 accumulated in FP32 over the tile's K range,
+// unscaled in FP64 in the epilogue. LOWER: lower-triangle tiles (M == N),
+// entries with m < n not written. Tiles of one matrix before the next.
+template <bool LOWER>
+__global__ void __launch_bounds__(256, 2)
+chol_plane_gemm_kernel(PlaneGemmArgs g) {
+  constexpr int BM = 128, WM = 2, WN = 2, NT = 256;
+  constexpr int TM = BM / WM, TN = BM / WN, FM = TM / 16, FN = TN / 16;
+  constexpr int CH = BM * PL_K / 8;        // 16-byte chunks per plane per operand slab
+  constexpr int LPT = CH / NT;
+  __shared__ __attribute__((aligned(16))) _Float16 As[2][BM][PL_K];
+  __shared__ __attribute__((aligned(16))) _Float16 Bs[2][BM][PL_K];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int64_t L = xcd_logical(blockIdx.x, g.nblocks);
+  const int f = (int)(L / g.tiles);
+  const int tile = (int)(L % g.tiles);
+  int bm, bn;
+  if constexpr (LOWER) {
+    tri_grouped(tile, g.tiles_m, bm, bn);
+  } else {
+    constexpr int GROUP_M = 8;
+    const int per_group = GROUP_M * g.tiles_n;
+    const int group = tile / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(g.tiles_m - first_m, GROUP_M);
+    const int in_group = tile - group * per_group;
+    bm = first_m + in_group % gsize;
+    bn = in_group / gsize;
+  }
+  const int m0 = bm * BM, n0 = bn * BM;
+  int kt0 = 0, kt1 = g.nkt;
+  if (g.tri_a) kt1 = min(kt1, (m0 + BM + PL_K - 1) / PL_K);
+  if (g.kb_hi) kt1 = min(kt1, (n0 + BM + PL_K - 1) / PL_K);
+  if (g.kb_lo) kt0 = n0 / PL_K;
+  const _Float16* __restrict__ Ya = g.a + (int64_t)f * g.a_f + (int64_t)m0 * PL_K;
+  const _Float16* __restrict__ Yb = g.b + (int64_t)f * g.b_f + (int64_t)n0 * PL_K;
+  const int64_t sa = (int64_t)g.a_rpad * PL_K, sb = (int64_t)g.b_rpad * PL_K;
+  pu4 ra[2][LPT], rb[2][LPT];
+  auto load = [&](int s) __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int i = 0; i < LPT; ++i) {
+        const int c = tid + i * NT;
+        ra[q][i] = *reinterpret_cast<const pu4*>(Ya + q * g.a_q + (int64_t)s * sa + 8 * c);
+        rb[q][i] = *reinterpret_cast<const pu4*>(Yb + q * g.b_q + (int64_t)s * sb + 8 * c);
+      }
+  };
+  auto store = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int i = 0; i < LPT; ++i) {
+        const int c = tid + i * NT;
+        *reinterpret_cast<pu4*>(&As[q][c >> 2][8 * pl_chunk(c >> 2, c & 3)]) = ra[q][i];
+        *reinterpret_cast<pu4*>(&Bs[q][c >> 2][8 * pl_chunk(c >> 2, c & 3)]) = rb[q][i];
+      }
+  };
+  pf4 part[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) part[i][j] = pf4{0.f, 0.f, 0.f, 0.f};
+  const int fr = lane & 15, fq = lane >> 4;
+  if (kt0 < kt1) load(kt0);
+  for (int s = kt0; s < kt1; ++s) {
+    __syncthreads();                   // previous slab's fragments consumed
+    store();
+    __syncthreads();
+    if (s + 1 < kt1) load(s + 1);
+    ph8 a[2][FM], b[2][FN];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int r = wm * TM + i * 16 + fr;
+        a[q][i] = *reinterpret_cast<const ph8*>(&As[q][r][8 * pl_chunk(r, fq)]);
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int r = wn * TN + j * 16 + fr;
+        b[q][j] = *reinterpret_cast<const ph8*>(&Bs[q][r][8 * pl_chunk(r, fq)]);
+      }
+    }
+    // lo hi + hi lo + hi hi, product-major (consecutive MFMAs write different accumulators)
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const int qa = pr == 1 ? 1 : 0, qb = pr == 0 ? 1 : 0;
+          part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[qb][j], a[qa][i], part[i][j], 0, 0, 0);
+        }
+  }
+  // lane l, register r holds C(m = m_base + (l & 15), n = n_base + 4 (l >> 4) + r)
+  const double unscale = 1.0 / (plane_scale(g.amax[0]) * plane_scale(g.bmax[0]));   // a power of two
+  double* __restrict__ C = g.C + (int64_t)f * g.strideC;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int m = m0 + wm * TM + i * 16 + fr;
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn * TN + j * 16 + 4 * fq + r;
+        if (m >= g.M || n >= g.N) continue;
+        if (LOWER && m < n) continue;
+        const int64_t idx = m + (int64_t)n * g.ldc;
+        double v = g.alpha * ((double)part[i][j][r] * unscale);
+        if (g.beta != 0.0) v += g.beta * C[idx];
+        C[idx] = v;
+      }
   }
 }
 
@@ -339,25 +555,65 @@ struct CholCtx {
   int64_t sw;           // work doubles per matrix
   hipStream_t stream;
   const double* bounds; // device {d, sqrt(d), 1} for FP16x3 products, or null (FP64)
-  int f16_min;          // smallest min(M, N, K) of an FP16x3 product
+  int f16_min;          // smallest dimension of a level whose products run in FP16x3
+  _Float16* pa;         // plane regions of the two operands of a product
+  _Float16* pb;
+  int64_t plane_elems;  // capacity of each region, elements per matrix
+};
+constexpr int BND_D = 0, BND_SQRT_D = 1, BND_ONE = 2;
+
+// Split op(X) of every matrix into the planes at dst (layout d).
+int split_planes(const CholCtx& c, const double* X, int64_t ld, int64_t strideX, int trans, int bnd,
+                 const PlaneDims& d, _Float16* dst) {
+  MGW_REQUIRE(d.per_f <= c.plane_elems, MGW_ERR_ARG, "chol: plane workspace too small");
+  hipLaunchKernelGGL(chol_split_kernel, dim3(d.rpad / 64, d.nkt, c.nbatch), dim3(256), 0, c.stream, X, ld, strideX,
+                     trans, d.rows, d.k, d.rpad, d.nkt, c.bounds + bnd, dst);
+  MGW_HIP(hipGetLastError());
+  return MGW_OK;
+}
+
+struct PlaneProduct {
+  int M, N;
+  double* C; int64_t ldc, strideC;
+  double alpha = 1.0, beta = 0.0;
+  bool lower = false;
+  int tri_a = 0, kb_hi = 0, kb_lo = 0;
 };
 
-// Operand bounds of one product of the recursion, FP16x3 when the
-// factorisation has bounds and the product is large enough.
-void chol_lowp(const CholCtx& c, GemmF64& g, int a_bound, int b_bound) {
-  if (!c.bounds || c.f16_min <= 0 || std::min(g.M, std::min(g.N, g.K)) < c.f16_min) return;
-  g.amax = c.bounds + a_bound;
-  g.bmax = c.bounds + b_bound;
+int plane_gemm(const CholCtx& c, const PlaneDims& da, const _Float16* pa, int abnd, const PlaneDims& db,
+               const _Float16* pb, int bbnd, const PlaneProduct& p) {
+  MGW_REQUIRE(da.nkt == db.nkt && da.rows == p.M && db.rows == p.N && (!p.lower || p.M == p.N), MGW_ERR_ARG,
+              "chol: plane product shapes disagree");
+  PlaneGemmArgs g;
+  g.a = pa; g.a_q = da.per_q; g.a_f = da.per_f; g.a_rpad = da.rpad;
+  g.b = pb; g.b_q = db.per_q; g.b_f = db.per_f; g.b_rpad = db.rpad;
+  g.M = p.M; g.N = p.N; g.nkt = da.nkt;
+  g.tri_a = p.tri_a; g.kb_hi = p.kb_hi; g.kb_lo = p.kb_lo;
+  g.amax = c.bounds + abnd; g.bmax = c.bounds + bbnd;
+  g.alpha = p.alpha; g.beta = p.beta;
+  g.C = p.C; g.ldc = p.ldc; g.strideC = p.strideC;
+  g.tiles_m = da.rpad / PL_ROWS;
+  g.tiles_n = db.rpad / PL_ROWS;
+  g.tiles = p.lower ? (int64_t)g.tiles_m * (g.tiles_m + 1) / 2 : (int64_t)g.tiles_m * g.tiles_n;
+  g.nblocks = g.tiles * c.nbatch;
+  MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chol: too many tiles");
+  if (p.lower)
+    hipLaunchKernelGGL(chol_plane_gemm_kernel<true>, dim3((unsigned)g.nblocks), dim3(256), 0, c.stream, g);
+  else
+    hipLaunchKernelGGL(chol_plane_gemm_kernel<false>, dim3((unsigned)g.nblocks), dim3(256), 0, c.stream, g);
+  MGW_HIP(hipGetLastError());
+  return MGW_OK;
 }
-constexpr int BND_D = 0, BND_SQRT_D = 1, BND_ONE = 2;
 
 // Recursive inverse Cholesky factor of the diagonal block [k0, k0 + m):
 //   [A11    ]   factor and invert A11 (recursion), L21 = A21 L11^-T,
 //   [A21 A22]   A22 -= L21 L21^T, factor and invert A22 (recursion),
 //               (L^-1)21 = -(L^-1)22 L21 (L^-1)11.
-// The products are batched MFMA GEMMs; the 64 x 64 leaves are factored and
-// inverted in registers by chol_diag_kernel. `w` is this block's work
-// (chol_need(m) doubles per matrix, matrix stride c.sw).
+// The products are batched MFMA GEMMs (FP64, or FP16x3 on planes for the
+// large levels when the factorisation has operand bounds); the 64 x 64
+// leaves are factored and inverted by chol_leaf_kernel (or
+// chol_diag_kernel). `w` is this block's work (chol_need(m) doubles per
+// matrix, matrix stride c.sw).
 int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   const int n = c.n;
   const int64_t s2 = (int64_t)n * n;
@@ -377,36 +633,70 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   double* a22 = a11 + m1 + (int64_t)m1 * n;
   double* l21 = w;                          // m2 x m1, leading dimension m2
   double* rest = w + (int64_t)m2 * m1;      // A22's work, then P
+  const bool lowp = c.bounds && c.f16_min > 0 && std::min(m1, m2) >= c.f16_min;
   MGW_TRY(chol_rec(c, k0, m1, w));
   // L21 = A21 (L^-1)11^T   ((L^-1)11 has exact zeros above the diagonal)
-  GemmF64 p;
-  p.M = m2; p.N = m1; p.K = m1;
-  p.A = a21; p.lda = n; p.strideA = s2;
-  p.B = a11; p.ldb = n; p.strideB = s2; p.transB = true;
-  p.C = l21; p.ldc = m2; p.strideC = c.sw;
-  p.batch = c.nbatch;
-  chol_lowp(c, p, BND_D, BND_ONE);
-  MGW_TRY(gemm_chol(p, c.stream));
+  if (lowp) {
+    const PlaneDims da = plane_dims(m2, m1), db = plane_dims(m1, m1);
+    MGW_TRY(split_planes(c, a21, n, s2, 0, BND_D, da, c.pa));
+    MGW_TRY(split_planes(c, a11, n, s2, 0, BND_ONE, db, c.pb));
+    PlaneProduct p;
+    p.M = m2; p.N = m1; p.C = l21; p.ldc = m2; p.strideC = c.sw;
+    p.kb_hi = 1;                            // (L^-1)11(n, k) = 0 for k > n
+    MGW_TRY(plane_gemm(c, da, c.pa, BND_D, db, c.pb, BND_ONE, p));
+  } else {
+    GemmF64 p;
+    p.M = m2; p.N = m1; p.K = m1;
+    p.A = a21; p.lda = n; p.strideA = s2;
+    p.B = a11; p.ldb = n; p.strideB = s2; p.transB = true;
+    p.C = l21; p.ldc = m2; p.strideC = c.sw;
+    p.batch = c.nbatch;
+    MGW_TRY(gemm_chol(p, c.stream));
+  }
   // A22 -= L21 L21^T (lower triangle; the upper stays zero)
-  GemmF64 u;
-  u.M = m2; u.N = m2; u.K = m1;
-  u.A = l21; u.lda = m2; u.strideA = c.sw;
-  u.B = l21; u.ldb = m2; u.strideB = c.sw; u.transB = true;
-  u.C = a22; u.ldc = n; u.strideC = s2;
-  u.alpha = -1.0; u.beta = 1.0;
-  u.lower_only = true; u.no_mirror = true;
-  u.batch = c.nbatch;
-  chol_lowp(c, u, BND_SQRT_D, BND_SQRT_D);
-  MGW_TRY(gemm_chol(u, c.stream));
+  if (lowp) {
+    const PlaneDims dl = plane_dims(m2, m1);
+    MGW_TRY(split_planes(c, l21, m2, c.sw, 0, BND_SQRT_D, dl, c.pa));
+    PlaneProduct u;
+    u.M = m2; u.N = m2; u.C = a22; u.ldc = n; u.strideC = s2;
+    u.alpha = -1.0; u.beta = 1.0; u.lower = true;
+    MGW_TRY(plane_gemm(c, dl, c.pa, BND_SQRT_D, dl, c.pa, BND_SQRT_D, u));
+  } else {
+    GemmF64 u;
+    u.M = m2; u.N = m2; u.K = m1;
+    u.A = l21; u.lda = m2; u.strideA = c.sw;
+    u.B = l21; u.ldb = m2; u.strideB = c.sw; u.transB = true;
+    u.C = a22; u.ldc = n; u.strideC = s2;
+    u.alpha = -1.0; u.beta = 1.0;
+    u.lower_only = true; u.no_mirror = true;
+    u.batch = c.nbatch;
+    MGW_TRY(gemm_chol(u, c.stream));
+  }
   MGW_TRY(chol_rec(c, k0 + m1, m2, rest));
   // (L^-1)21 = -(L^-1)22 P,  P = L21 (L^-1)11
+  if (lowp) {
+    // the recursion on A22 reused the plane regions: split L21 again
+    const PlaneDims dl = plane_dims(m2, m1), dt = plane_dims(m1, m1);
+    MGW_TRY(split_planes(c, l21, m2, c.sw, 0, BND_SQRT_D, dl, c.pa));
+    MGW_TRY(split_planes(c, a11, n, s2, 1, BND_ONE, dt, c.pb));      // B^T(n, k) = (L^-1)11(k, n)
+    PlaneProduct q;
+    q.M = m2; q.N = m1; q.C = rest; q.ldc = m2; q.strideC = c.sw;
+    q.kb_lo = 1;                            // (L^-1)11(k, n) = 0 for k < n
+    MGW_TRY(plane_gemm(c, dl, c.pa, BND_SQRT_D, dt, c.pb, BND_ONE, q));
+    const PlaneDims da = plane_dims(m2, m2), dp = plane_dims(m1, m2);
+    MGW_TRY(split_planes(c, a22, n, s2, 0, BND_ONE, da, c.pa));
+    MGW_TRY(split_planes(c, rest, m2, c.sw, 1, BND_SQRT_D, dp, c.pb));  // B^T(n, k) = P(k, n)
+    PlaneProduct t;
+    t.M = m2; t.N = m1; t.C = a21; t.ldc = n; t.strideC = s2;
+    t.alpha = -1.0; t.tri_a = 1;
+    return plane_gemm(c, da, c.pa, BND_ONE, dp, c.pb, BND_SQRT_D, t);
+  }
   GemmF64 q;
   q.M = m2; q.N = m1; q.K = m1;
   q.A = l21; q.lda = m2; q.strideA = c.sw;
   q.B = a11; q.ldb = n; q.strideB = s2;
   q.C = rest; q.ldc = m2; q.strideC = c.sw;
   q.batch = c.nbatch;
-  chol_lowp(c, q, BND_SQRT_D, BND_ONE);
   MGW_TRY(gemm_chol(q, c.stream));
   GemmF64 t;
   t.M = m2; t.N = m1; t.K = m2;
@@ -415,24 +705,38 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   t.C = a21; t.ldc = n; t.strideC = s2;
   t.alpha = -1.0;
   t.batch = c.nbatch;
-  chol_lowp(c, t, BND_ONE, BND_SQRT_D);
   return gemm_chol(t, c.stream);
 }
 
 }  // namespace
 
+// fp16 elements per matrix of each of the two plane regions: the top
+// level's operands are the largest
+int64_t chol_plane_elems(int n) {
+  if (n <= NB) return 0;
+  const int m1 = chol_split(n), mx = std::max(m1, n - m1);
+  return plane_dims(mx, mx).per_f;
+}
+
+size_t chol_plane_bytes(int n, int nbatch) {
+  return (size_t)2 * chol_plane_elems(n) * nbatch * sizeof(_Float16);
+}
+
 int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* info, double* work,
-                         hipStream_t stream, double* bounds) {
+                         hipStream_t stream, double* bounds, void* planes) {
   if (n <= 0 || nbatch <= 0) return MGW_OK;
   MGW_REQUIRE(nbatch <= 65535, MGW_ERR_ARG, "chol_inverse_batched: batch too large");
+  MGW_REQUIRE(!bounds || planes || n <= NB, MGW_ERR_ARG, "chol_inverse_batched: FP16x3 products need planes");
   const int f16_min = bounds ? chol_f16_min() : 0;
   if (bounds && f16_min > 0) {
     hipLaunchKernelGGL(chol_diagmax_kernel, dim3(nbatch), dim3(256), 0, stream, A, n, (int64_t)n * n, bounds + 4);
     hipLaunchKernelGGL(chol_bounds_kernel, dim3(1), dim3(256), 0, stream, bounds + 4, nbatch, bounds);
     MGW_HIP(hipGetLastError());
   }
+  const int64_t pe = chol_plane_elems(n);
+  _Float16* pa = static_cast<_Float16*>(planes);
   const CholCtx c{A, n, nbatch, logdet, info, work, (int64_t)chol_work_doubles(n), stream,
-                  f16_min > 0 ? bounds : nullptr, f16_min};
+                  f16_min > 0 ? bounds : nullptr, f16_min, pa, pa ? pa + pe * nbatch : nullptr, pe};
   return chol_rec(c, 0, n, work);
 }
 

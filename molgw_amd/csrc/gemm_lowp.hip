@@ -382,9 +382,6 @@ struct F16Dev {
   const double* bmax;
   double* cmax;
   int tiles_m, tiles_n, batch;
-  int tri_a;         // A (M == K) lower triangular: the K loop of a row block stops at its last row
-  int batch_outer;   // all tiles of one batch entry before the next (entries share no operand)
-  int64_t tiles;     // tiles per batch entry
   int64_t nblocks;
 };
 
@@ -392,11 +389,7 @@ __device__ inline double pow2_scale(double mx) {
   return (mx > 0.0 && mx <= 1.0e300) ? ldexp(1.0, 13 - ilogb(mx)) : 1.0;
 }
 
-// The batched inverse Cholesky of 1 - chi0 uses the same kernel with B
-// transposed (TB: B^T(k, n) = B[n + k ldb]), with lower-triangle tiles only
-// (LOWER: M == N, BM == BN, grouped triangular walk, results with m < n not
-// written) and with a lower-triangular A (tri_a).
-template <int BM, int BN, int WM, int WN, int KLASS, bool TB = false, bool LOWER = false>
+template <int BM, int BN, int WM, int WN, int KLASS>
 __global__ void __launch_bounds__(64 * WM * WN)
 gemm_f16x3_kernel(F16Dev g) {
   constexpr int BK = 32;
@@ -414,26 +407,16 @@ gemm_f16x3_kernel(F16Dev g) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   const int64_t L = xcd_logical(blockIdx.x, g.nblocks);
-  const int batch = g.batch_outer ? (int)(L / g.tiles) : (int)(L % g.batch);
-  const int tile = g.batch_outer ? (int)(L % g.tiles) : (int)(L / g.batch);
-  int m0, n0;
-  if constexpr (LOWER) {
-    static_assert(BM == BN, "lower-triangle tiles are square");
-    int bm, bn;
-    tri_grouped(tile, g.tiles_m, bm, bn);
-    m0 = bm * BM;
-    n0 = bn * BN;
-  } else {
-    // groups of 8 row tiles walked column by column: concurrent workgroups share B
-    constexpr int GROUP_M = 8;
-    const int per_group = GROUP_M * g.tiles_n;
-    const int group = tile / per_group;
-    const int first_m = group * GROUP_M;
-    const int gsize = min(g.tiles_m - first_m, GROUP_M);
-    const int in_group = tile - group * per_group;
-    m0 = (first_m + in_group % gsize) * BM;
-    n0 = (in_group / gsize) * BN;
-  }
+  const int batch = (int)(L % g.batch);
+  const int tile = (int)(L / g.batch);
+  // groups of 8 row tiles walked column by column: concurrent workgroups share B
+  constexpr int GROUP_M = 8;
+  const int per_group = GROUP_M * g.tiles_n;
+  const int group = tile / per_group;
+  const int first_m = group * GROUP_M;
+  const int gsize = min(g.tiles_m - first_m, GROUP_M);
+  const int in_group = tile - group * per_group;
+  const int m0 = (first_m + in_group % gsize) * BM, n0 = (in_group / gsize) * BN;
 
   const double* __restrict__ A = g.A + (int64_t)batch * g.strideA;
   const double* __restrict__ B = g.B + (int64_t)batch * g.strideB;
@@ -462,8 +445,7 @@ gemm_f16x3_kernel(F16Dev g) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int gk = kb + e;
-        const int64_t off = TB ? gn + (int64_t)gk * g.ldb : gk + (int64_t)gn * g.ldb;
-        rb[u][e] = (c < BN * 4 && gn < g.N && gk < g.K) ? B[off] : 0.0;
+        rb[u][e] = (c < BN * 4 && gn < g.N && gk < g.K) ? B[gk + (int64_t)gn * g.ldb] : 0.0;
       }
     }
   };
@@ -506,8 +488,7 @@ gemm_f16x3_kernel(F16Dev g) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) part[i][j] = f4{0.f, 0.f, 0.f, 0.f};
   const int fr = lane & 15, fq = lane >> 4;
-  const int kend = g.tri_a ? min(g.K, m0 + BM) : g.K;
-  const int nslab = (kend + BK - 1) / BK;
+  const int nslab = (g.K + BK - 1) / BK;
   load(0);
   for (int s = 0; s < nslab; ++s) {
     __syncthreads();                 // previous slab's fragments consumed
@@ -551,7 +532,6 @@ gemm_f16x3_kernel(F16Dev g) {
       for (int r = 0; r < 4; ++r) {
         const int n = n0 + wn * TN + j * 16 + 4 * fq + r;
         if (m >= g.M || n >= g.N) continue;
-        if (LOWER && m < n) continue;
         const int64_t idx = m + (int64_t)n * g.ldc;
         double v = g.alpha * ((double)part[i][j][r] * unscale);
         if (g.beta != 0.0) v += g.beta * C[idx];
@@ -572,7 +552,7 @@ gemm_f16x3_kernel(F16Dev g) {
   }
 }
 
-template <int BM, int BN, int WM, int WN, int KLASS, bool TB = false, bool LOWER = false>
+template <int BM, int BN, int WM, int WN, int KLASS>
 int launch_f16x3(const GemmF64& h, hipStream_t stream) {
   F16Dev g;
   g.M = h.M; g.N = h.N; g.K = h.K;
@@ -584,13 +564,10 @@ int launch_f16x3(const GemmF64& h, hipStream_t stream) {
   g.tiles_m = (h.M + BM - 1) / BM;
   g.tiles_n = (h.N + BN - 1) / BN;
   g.batch = h.batch;
-  g.tri_a = h.tri_a ? 1 : 0;
-  g.batch_outer = h.batch_outer ? 1 : 0;
-  g.tiles = LOWER ? (int64_t)g.tiles_m * (g.tiles_m + 1) / 2 : (int64_t)g.tiles_m * g.tiles_n;
-  g.nblocks = g.tiles * h.batch;
+  g.nblocks = (int64_t)g.tiles_m * g.tiles_n * h.batch;
   MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "gemm_lowp: too many tiles");
-  hipLaunchKernelGGL((gemm_f16x3_kernel<BM, BN, WM, WN, KLASS, TB, LOWER>), dim3((unsigned)g.nblocks),
-                     dim3(64 * WM * WN), 0, stream, g);
+  hipLaunchKernelGGL((gemm_f16x3_kernel<BM, BN, WM, WN, KLASS>), dim3((unsigned)g.nblocks), dim3(64 * WM * WN), 0,
+                     stream, g);
   MGW_HIP(hipGetLastError());
   return MGW_OK;
 }
@@ -648,7 +625,7 @@ int gemm_lowp(const GemmF64& h, int precision, hipStream_t stream) {
   }
   if (precision == MGW_FP16X3) {
     MGW_REQUIRE(!h.transA && !h.transB && !h.lower_only && !h.kscale && !h.rowscale && !h.colscale && !h.diag_add &&
-                    h.amax && h.bmax && !h.batch_outer,
+                    h.amax && h.bmax,
                 MGW_ERR_ARG, "gemm_lowp: FP16x3 takes plain non-transposed products with operand maxima");
     // one column tile when N (the first transform's occupied and window
     // columns) fits 64, 128 or 192, so that the large operand A is read once
@@ -662,21 +639,6 @@ int gemm_lowp(const GemmF64& h, int precision, hipStream_t stream) {
   }
   set_error("gemm_lowp: unknown precision");
   return MGW_ERR_ARG;
-}
-
-int gemm_f16x3_chol(const GemmF64& h, hipStream_t stream) {
-  if (h.M <= 0 || h.N <= 0 || h.batch <= 0) return MGW_OK;
-  MGW_REQUIRE(h.C != nullptr && h.dotpart == nullptr && !h.transA && !h.kscale && !h.rowscale && !h.colscale &&
-                  !h.diag_add && h.amax && h.bmax && !h.upper_zero && (!h.lower_only || h.no_mirror) &&
-                  (!h.lower_only || h.M == h.N) && (!h.tri_a || (h.M == h.K && !h.transB)),
-              MGW_ERR_ARG, "gemm_f16x3_chol: unsupported product");
-  constexpr int K = MGW_KCLASS_CHOLESKY;
-  if (h.lower_only) {
-    MGW_REQUIRE(h.transB, MGW_ERR_ARG, "gemm_f16x3_chol: the lower-triangle product is A B^T");
-    return launch_f16x3<128, 128, 2, 2, K, true, true>(h, stream);
-  }
-  if (h.transB) return launch_f16x3<128, 128, 2, 2, K, true, false>(h, stream);
-  return launch_f16x3<128, 128, 2, 2, K, false, false>(h, stream);
 }
 
 }  // namespace mgw

@@ -138,11 +138,6 @@ struct GemmF64 {
 };
 int gemm_f64(const GemmF64& g, hipStream_t stream);
 int gemm_lowp(const GemmF64& g, int precision, hipStream_t stream);
-// FP16x3 products of the batched inverse Cholesky (chol.hip): A B, A B^T,
-// the lower triangle of A B^T (no mirror) and lower-triangular A, with the
-// device operand bounds amax, bmax; tiles of one batch entry before the next
-// when batch_outer.
-int gemm_f16x3_chol(const GemmF64& g, hipStream_t stream);
 // 1 - chi0(iw_f) = I + X diag(s_f) X^T (lower triangle, zeros above it) for all
 // nomega frequencies from bf16-split operands (precision MGW_BF16X3 or
 // MGW_BF16X6); needs s_f >= 0 and a workspace of chi0_split_workspace bytes.
@@ -164,14 +159,17 @@ int wb_split_dot(const double* d, int naux, int nomega, const double* b, int64_t
 // sum_i ln L_f(i, i); info[f] the 1-based column of the first non-positive
 // pivot (0 on success). work: chol_work_doubles(n) doubles per matrix.
 // bounds (nullable; 4 + nbatch doubles of device memory): when given, the matrices
-// must satisfy A_f >= I (1 - chi0 with chi0 <= 0), and the large products
-// of the recursion run in FP16x3 with operand bounds derived from max_i
-// A_f(i, i), which a first kernel writes to bounds.
+// must satisfy A_f >= I (1 - chi0 with chi0 <= 0), and the products of the
+// large levels of the recursion run in FP16x3 on operands split into fp16
+// planes (planes: chol_plane_bytes(n, nbatch) of device memory) with
+// operand bounds derived from max_i A_f(i, i), which a first kernel writes
+// to bounds.
 constexpr int CHOL_NB = 64;
 size_t chol_work_doubles(int n);
+size_t chol_plane_bytes(int n, int nbatch);
 int chol_f16_min();
 int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* info, double* work,
-                         hipStream_t stream, double* bounds = nullptr);
+                         hipStream_t stream, double* bounds = nullptr, void* planes = nullptr);
 // tr[f] = sum of squares of the lower triangle of M_f (n x n, stride n^2).
 int launch_lower_frob2(const double* m, int n, int nbatch, double* partial, hipStream_t s);
 constexpr int FROB_BLOCKS = 64;
