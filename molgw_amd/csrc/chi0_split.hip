@@ -406,11 +406,18 @@ chi_syrk_kernel(SyrkArgs g) {
 // MFMA than 64 x 64). Lane p of a 1 KB window of 16 rows writes LDS chunk
 // p, i.e. row p / 4 and slot p % 4, which holds the swizzled chunk
 // j = slot ^ ((row >> 1) & 3) of that row, so the lane loads that global
-// chunk. DB: two LDS buffers (the next slab's copies run during this slab's
-// MFMAs, one barrier per slab) or one (copy, wait, multiply, two barriers
-// per slab). Same products in the same order per accumulator as
-// chi_syrk_kernel: identical results.
-template <bool H, int BM, int WM, int WN, bool DB>
+// chunk. NBUF: with 2 to 4 LDS buffers the copies of the next NBUF - 1
+// slabs run during this slab's MFMAs (one barrier per slab); with one, each
+// slab is copied, waited for and multiplied (two barriers per slab). Same
+// products in the same order per accumulator as chi_syrk_kernel: identical
+// results.
+
+// s_waitcnt immediate that waits until at most v vector-memory operations of
+// this wave are outstanding (expcnt and lgkmcnt not waited on): vmcnt bits
+// [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]
+constexpr int waitcnt_vm(int v) { return (v & 15) | (7 << 4) | (15 << 8) | ((v >> 4) << 14); }
+
+template <bool H, int BM, int WM, int WN, int NBUF>
 __global__ void __launch_bounds__(64 * WM * WN)
 chi_syrk_lds_kernel(SyrkArgs g) {
   typedef typename PlaneT<H>::E E;
@@ -420,8 +427,9 @@ chi_syrk_lds_kernel(SyrkArgs g) {
   constexpr int TM = BM / WM, TN = BM / WN;
   constexpr int FM = TM / 16, FN = TN / 16;
   constexpr int WINS = BM / 16;                 // 1 KB windows per plane per operand
-  constexpr int NBUF = DB ? 2 : 1;
+  constexpr int LPW = 2 * NS * WINS / NW;       // copies per wave per slab
   static_assert((2 * NS * WINS) % NW == 0, "windows per wave");
+  static_assert(NBUF == 1 || NBUF == 2, "LDS buffers");
   __shared__ __attribute__((aligned(16))) E As[NBUF][NS][BM][KT];
   __shared__ __attribute__((aligned(16))) E Bs[NBUF][NS][BM][KT];
 
@@ -490,13 +498,21 @@ chi_syrk_lds_kernel(SyrkArgs g) {
   const int nslab = g.nkt;
   const int s0 = kc * g.kc_slabs;
   const int s1 = min(nslab, s0 + g.kc_slabs);
-  if constexpr (DB) {
-    issue(s0, 0);
+  if constexpr (NBUF >= 2) {
+    // slab s + D is copied into the buffer of slab s - 1 while slab s is
+    // multiplied: D = NBUF - 1 slabs in flight
+    constexpr int D = NBUF - 1;
+#pragma unroll
+    for (int p = 0; p < D; ++p)
+      if (s0 + p < s1) issue(s0 + p, p);
     for (int s = s0; s < s1; ++s) {
-      const int cur = (s - s0) & 1;
-      __builtin_amdgcn_s_waitcnt(0);           // this wave's copies of slab s have landed
+      const int cur = (s - s0) % NBUF;
+      // this wave's copies of slab s have landed once at most the copies of
+      // the D - 1 later slabs in flight are outstanding (all, near the end)
+      if (s + D - 1 < s1) __builtin_amdgcn_s_waitcnt(waitcnt_vm((D - 1) * LPW));
+      else __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();                         // everyone's; and slab s - 1's buffer is free
-      if (s + 1 < s1) issue(s + 1, cur ^ 1);
+      if (s + D < s1) issue(s + D, (s + D - s0) % NBUF);
       compute(cur);
     }
   } else {
@@ -519,45 +535,178 @@ chi_syrk_lds_kernel(SyrkArgs g) {
         P[(wm * TM + i * 16 + fr) + (wn * TN + j * 16 + 4 * fq + r) * BM] = part[i][j][r];
 }
 
+// The K-split SYRK (128 x 128 tiles, 4 waves of 64 x 64) with NBUF LDS
+// buffers, each its own __shared__ array, and NBUF - 1 slabs copied ahead
+// by global_load_lds. A slab is waited for by an explicit vmcnt that leaves
+// the later slabs' copies in flight, and the workgroup then meets at a bare
+// s_barrier: __syncthreads() would also wait for every outstanding copy,
+// and so would the fragment reads of a single array that the copies may
+// alias. With one array per buffer and the slab loop unrolled by NBUF, each
+// fragment read names its buffer, so it waits only for that buffer's copies.
+// Same products in the same order per accumulator as chi_syrk_kernel.
+template <bool H, int NBUF>
+__global__ void __launch_bounds__(256)
+chi_syrk_pipe_kernel(SyrkArgs g) {
+  typedef typename PlaneT<H>::E E;
+  typedef typename PlaneT<H>::V V;
+  constexpr int BM = 128, NS = 2, WM = 2, NW = 4;
+  constexpr int TM = 64, TN = 64, FM = 4, FN = 4;
+  constexpr int WINS = BM / 16;                 // 1 KB windows per plane per operand
+  constexpr int LPW = 2 * NS * WINS / NW;       // copies per wave per slab
+  constexpr int D = NBUF - 1;                   // slabs in flight
+  static_assert(NBUF == 3 || NBUF == 4, "LDS buffers");
+  static_assert((D - 1) * LPW <= 63, "vmcnt range");
+  // buffer u: [operand][plane][row][k]
+  __shared__ __attribute__((aligned(16))) E S0[2][NS][BM][KT];
+  __shared__ __attribute__((aligned(16))) E S1[2][NS][BM][KT];
+  __shared__ __attribute__((aligned(16))) E S2[2][NS][BM][KT];
+  __shared__ __attribute__((aligned(16))) E S3[2][NS][BM][KT];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int64_t L = xcd_logical(blockIdx.x, g.nblocks);
+  const int64_t per_f = (int64_t)g.nkc * g.tiles;
+  const int f = (int)(L / per_f);
+  const int kc = (int)((L % per_f) / g.tiles);
+  const int tile = (int)(L % g.tiles);
+  int bm, bn;
+  tri_grouped(tile, g.tm, bm, bn);
+  const int m0 = bm * BM, n0 = bn * BM;
+  const E* __restrict__ Ya = static_cast<const E*>(g.y) + (int64_t)f * g.fstride + (int64_t)m0 * KT;
+  const E* __restrict__ Yb = static_cast<const E*>(g.y) + (int64_t)f * g.fstride + (int64_t)n0 * KT;
+  const int64_t slab = (int64_t)g.ppad * KT;
+  const int rl = lane >> 2;
+  const int jl = (lane & 3) ^ ((rl >> 1) & 3);
+  const int fr = lane & 15, fq = lane >> 4;
+
+  f4 part[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) part[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // the buffers are named directly (no pointer chosen at run time), so
+  // every LDS access carries its array's alias scope
+#define MGW_PIPE_ISSUE(S_, SB)                                                                         \
+  _Pragma("unroll") for (int c = 0; c < LPW; ++c) {                                                     \
+    const int w = wave + c * NW;                                                                        \
+    const int op = w / (NS * WINS), q = (w / WINS) % NS, win = w % WINS;                                \
+    const int64_t off = q * g.plane + (int64_t)(S_) * slab + (int64_t)(16 * win + rl) * KT + 8 * jl;    \
+    __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned*>((op == 0 ? Ya : Yb) + off),      \
+                                     reinterpret_cast<unsigned*>(&SB[op][q][16 * win][0]), 16, 0, 0);   \
+  }
+#define MGW_PIPE_COMPUTE(SB)                                                                           \
+  {                                                                                                     \
+    V a[NS][FM], b[NS][FN];                                                                             \
+    _Pragma("unroll") for (int q = 0; q < NS; ++q) {                                                    \
+      _Pragma("unroll") for (int i = 0; i < FM; ++i)                                                    \
+        a[q][i] = *reinterpret_cast<const V*>(&SB[0][q][wm * TM + i * 16 + fr][8 * lds_chunk(fr, fq)]); \
+      _Pragma("unroll") for (int j = 0; j < FN; ++j)                                                    \
+        b[q][j] = *reinterpret_cast<const V*>(&SB[1][q][wn * TN + j * 16 + fr][8 * lds_chunk(fr, fq)]); \
+    }                                                                                                   \
+    constexpr int QA[3] = {1, 0, 0};                                                                    \
+    constexpr int QB[3] = {0, 1, 0};                                                                    \
+    _Pragma("unroll") for (int pr = 0; pr < 3; ++pr)                                                    \
+      _Pragma("unroll") for (int i = 0; i < FM; ++i)                                                    \
+        _Pragma("unroll") for (int j = 0; j < FN; ++j)                                                  \
+          part[i][j] = mfma16x16x32<H>(b[QB[pr]][j], a[QA[pr]][i], part[i][j]);                         \
+  }
+  // slab ss = s + U sits in SCUR; slab ss + D goes to SNEXT, the buffer of
+  // slab ss - 1. This wave's copies of slab ss have landed once at most
+  // those of the D - 1 later slabs in flight are outstanding (all, near the
+  // end); then the bare barrier: everyone's copies have landed and every
+  // wave is done with slab ss - 1. The empty asm statements keep the
+  // compiler from moving LDS accesses across it.
+#define MGW_PIPE_SUB(U, SCUR, SNEXT)                                                                   \
+  if (s + (U) < s1) {                                                                                   \
+    if (s + (U) + D - 1 < s1) __builtin_amdgcn_s_waitcnt(waitcnt_vm((D - 1) * LPW));                    \
+    else __builtin_amdgcn_s_waitcnt(0);                                                                 \
+    __asm__ __volatile__("" ::: "memory");                                                              \
+    __builtin_amdgcn_s_barrier();                                                                       \
+    __asm__ __volatile__("" ::: "memory");                                                              \
+    if (s + (U) + D < s1) { MGW_PIPE_ISSUE(s + (U) + D, SNEXT) }                                        \
+    MGW_PIPE_COMPUTE(SCUR)                                                                              \
+  }
+
+  const int s0 = kc * g.kc_slabs;
+  const int s1 = min(g.nkt, s0 + g.kc_slabs);
+  if (s0 < s1) { MGW_PIPE_ISSUE(s0, S0) }
+  if (s0 + 1 < s1) { MGW_PIPE_ISSUE(s0 + 1, S1) }
+  if constexpr (NBUF == 4) {
+    if (s0 + 2 < s1) { MGW_PIPE_ISSUE(s0 + 2, S2) }
+  }
+  for (int s = s0; s < s1; s += NBUF) {
+    if constexpr (NBUF == 3) {
+      MGW_PIPE_SUB(0, S0, S2)
+      MGW_PIPE_SUB(1, S1, S0)
+      MGW_PIPE_SUB(2, S2, S1)
+    } else {
+      MGW_PIPE_SUB(0, S0, S3)
+      MGW_PIPE_SUB(1, S1, S0)
+      MGW_PIPE_SUB(2, S2, S1)
+      MGW_PIPE_SUB(3, S3, S2)
+    }
+  }
+#undef MGW_PIPE_SUB
+#undef MGW_PIPE_COMPUTE
+#undef MGW_PIPE_ISSUE
+
+  float* __restrict__ P = g.part + ((int64_t)(f * g.tiles + tile) * g.nkc + kc) * (BM * BM);
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        P[(wm * TM + i * 16 + fr) + (wn * TN + j * 16 + 4 * fq + r) * BM] = part[i][j][r];
+}
+
 // SYRK kernel of the K-split planes path: MGW_CHI_SYRK = 0 (register
 // prefetch, default), 1 (LDS copies, one buffer, 4 waves), 2 (two buffers,
 // 4 waves), 3 (one buffer, 2 waves of 64 x 128), 4 (two buffers, 2 waves),
 // 5 (256 x 256 tiles, LDS copies, one buffer, 8 waves of 64 x 128), 6 (the
-// same with two buffers). A 256-wide tile reads each row and column panel
-// half as often as a 128-wide one. Read at every chi0 build.
+// same with two buffers), 7 and 8 (128 x 128 tiles, 4 waves, three or four
+// LDS buffers: two or three slabs in flight, for the latency of panel reads
+// that miss L2). A 256-wide tile reads each row and column panel half as
+// often as a 128-wide one. Read at every chi0 build.
 int chi_syrk_variant() {
   const char* e = std::getenv("MGW_CHI_SYRK");
   const int v = e ? std::atoi(e) : 0;
-  return (v >= 0 && v <= 6) ? v : 0;
+  return (v >= 0 && v <= 8) ? v : 0;
 }
 
 // Output tile of the planes path's SYRK (two planes, K split): 256 for the
 // variants with 256-wide tiles, else 128.
 int chi_syrk_bm(int ns, bool ksplit) {
-  return (ns == 2 && ksplit && chi_syrk_variant() >= 5) ? 256 : 128;
+  const int v = chi_syrk_variant();
+  return (ns == 2 && ksplit && (v == 5 || v == 6)) ? 256 : 128;
 }
 
 int launch_syrk_lds(bool h, int var, dim3 grid, const SyrkArgs& g, hipStream_t stream) {
   constexpr int BM = 128;
-  if (var >= 5) {
+  if (var == 5 || var == 6) {
     const bool db = var == 6;
-    if (h && db) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, 256, 4, 2, true>), grid, dim3(512), 0, stream, g);
-    else if (h) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, 256, 4, 2, false>), grid, dim3(512), 0, stream, g);
-    else if (db) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, 256, 4, 2, true>), grid, dim3(512), 0, stream, g);
-    else hipLaunchKernelGGL((chi_syrk_lds_kernel<false, 256, 4, 2, false>), grid, dim3(512), 0, stream, g);
+    if (h && db) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, 256, 4, 2, 2>), grid, dim3(512), 0, stream, g);
+    else if (h) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, 256, 4, 2, 1>), grid, dim3(512), 0, stream, g);
+    else if (db) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, 256, 4, 2, 2>), grid, dim3(512), 0, stream, g);
+    else hipLaunchKernelGGL((chi_syrk_lds_kernel<false, 256, 4, 2, 1>), grid, dim3(512), 0, stream, g);
     MGW_HIP(hipGetLastError());
     return MGW_OK;
   }
   if (h) {
-    if (var == 1) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 2, false>), grid, dim3(256), 0, stream, g);
-    else if (var == 2) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 2, true>), grid, dim3(256), 0, stream, g);
-    else if (var == 3) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 1, false>), grid, dim3(128), 0, stream, g);
-    else hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 1, true>), grid, dim3(128), 0, stream, g);
+    if (var == 1) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 2, 1>), grid, dim3(256), 0, stream, g);
+    else if (var == 2) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 2, 2>), grid, dim3(256), 0, stream, g);
+    else if (var == 3) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 1, 1>), grid, dim3(128), 0, stream, g);
+    else if (var == 4) hipLaunchKernelGGL((chi_syrk_lds_kernel<true, BM, 2, 1, 2>), grid, dim3(128), 0, stream, g);
+    else if (var == 7) hipLaunchKernelGGL((chi_syrk_pipe_kernel<true, 3>), grid, dim3(256), 0, stream, g);
+    else hipLaunchKernelGGL((chi_syrk_pipe_kernel<true, 4>), grid, dim3(256), 0, stream, g);
   } else {
-    if (var == 1) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 2, false>), grid, dim3(256), 0, stream, g);
-    else if (var == 2) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 2, true>), grid, dim3(256), 0, stream, g);
-    else if (var == 3) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 1, false>), grid, dim3(128), 0, stream, g);
-    else hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 1, true>), grid, dim3(128), 0, stream, g);
+    if (var == 1) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 2, 1>), grid, dim3(256), 0, stream, g);
+    else if (var == 2) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 2, 2>), grid, dim3(256), 0, stream, g);
+    else if (var == 3) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 1, 1>), grid, dim3(128), 0, stream, g);
+    else if (var == 4) hipLaunchKernelGGL((chi_syrk_lds_kernel<false, BM, 2, 1, 2>), grid, dim3(128), 0, stream, g);
+    else if (var == 7) hipLaunchKernelGGL((chi_syrk_pipe_kernel<false, 3>), grid, dim3(256), 0, stream, g);
+    else hipLaunchKernelGGL((chi_syrk_pipe_kernel<false, 4>), grid, dim3(256), 0, stream, g);
   }
   MGW_HIP(hipGetLastError());
   return MGW_OK;
@@ -1088,7 +1237,8 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
       // (two LDS buffers with one barrier per slab, or two k tiles per slab:
       // 64 KB of LDS and two waves per SIMD, measured slower)
       const int var = NS == 2 ? chi_syrk_variant() : 0;
-      MGW_REQUIRE((L.bm == 256) == (var >= 5), MGW_ERR_ARG, "chi0 split: tile size and SYRK variant disagree");
+      MGW_REQUIRE((L.bm == 256) == (var == 5 || var == 6), MGW_ERR_ARG,
+                  "chi0 split: tile size and SYRK variant disagree");
       if (var) MGW_TRY(launch_syrk_lds(H, var, grid, g, stream));
       else hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
       MGW_HIP(hipGetLastError());

@@ -119,14 +119,16 @@ def test_cholesky_leaf_kernels_agree():
     assert abs(e_lds - e_wave) < 1e-12 * abs(e_wave)
 
 
-@pytest.mark.parametrize('variant', ['1', '2', '3', '4', '5', '6'])
+@pytest.mark.parametrize('variant', ['1', '2', '3', '4', '5', '6', '7', '8'])
 def test_syrk_lds_copies_equal_the_register_prefetch(variant):
     """The SYRK variants that copy slabs straight into LDS (one or two
     buffers; 128-wide tiles on 4 waves of 64 x 64 or 2 waves of 64 x 128,
-    or 256-wide tiles on 8 waves of 64 x 128) do the same products in the
-    same order per accumulator, over the same K chunks, as the
-    register-prefetch kernel: chi is identical bit for bit, FP16x3 and
-    BF16x3 (naux 300: one full and two partial 256-wide tiles)."""
+    or 256-wide tiles on 8 waves of 64 x 128; three or four buffers with
+    two or three slabs in flight) do the same products in the same order
+    per accumulator, over the same K chunks, as the register-prefetch
+    kernel: chi is identical bit for bit, FP16x3 and BF16x3 (naux 300: one
+    full and two partial 256-wide tiles; one K chunk of 18 slabs, so the
+    pipelines' tails are exercised)."""
     s = synthetic_system(60, 300, 12, seed=41, scale=0.3)
     with Engine(0) as eng:
         eng.set_eri_3center(s.eri_3center, s.index_basis, s.nbf)
