@@ -55,7 +55,7 @@ enum { MGW_MEM_HOST = 0, MGW_MEM_DEVICE = 1 };
  *   MGW_FP16X3 two-way fp16 split of power-of-two-scaled operands for chi0,
  *              the self-energy product, the AO->MO transforms (FP32
  *              accumulation over K = nbf) and the products of the batched
- *              (1 - chi0) factorisation with all dimensions >= 256 (FP64
+ *              (1 - chi0) factorisation at levels >= 128 wide (FP64
  *              64 x 64 leaves, FP64 sums across products), 3 products
  *              (~2^-22); the environment variables MGW_AO2MO_FP16X3=0 and
  *              MGW_CHOL_F16_MIN=0 keep the transforms and the factorisation

@@ -371,15 +371,19 @@ struct PlaneGemmArgs {
 };
 
 // C(m, n) = alpha sum_k A(m, k) B^T(n, k) + beta C(m, n) for every matrix,
-// 128 x 128 tiles, 4 waves of 64 x 64, one 32-k slab per step (register
-// prefetch of the next slab, one LDS buffer); hi hi + hi lo + lo hi on
-// v_mfma_f32_16x16x32_f16 accumulated in FP32 over the tile's K range,
-// unscaled in FP64 in the epilogue. LOWER: lower-triangle tiles (M == N),
-// entries with m < n not written. Tiles of one matrix before the next.
-template <bool LOWER>
+// BM x BM tiles (128, or 64 for the middle levels, whose products would
+// otherwise have too few workgroups), 4 waves, one 32-k slab per step
+// (register prefetch of the next slab, one LDS buffer); hi hi + hi lo +
+// lo hi on v_mfma_f32_16x16x32_f16 accumulated in FP32 over the tile's K
+// range, unscaled in FP64 in the epilogue. LOWER: lower-triangle tiles
+// (M == N), entries with m < n not written. Tiles of one matrix before the
+// next.
+template <bool LOWER, int BM>
 __global__ void __launch_bounds__(256, 2)
 chol_plane_gemm_kernel(PlaneGemmArgs g) {
-  constexpr int BM = 128, WM = 2, WN = 2, NT = 256;
+  constexpr int WM = 2, WN = 2, NT = 256;
+  static_assert(BM == 64 || BM == 128, "tile");
+  static_assert(PL_ROWS % BM == 0, "plane row padding covers whole tiles");
   constexpr int TM = BM / WM, TN = BM / WN, FM = TM / 16, FN = TN / 16;
   constexpr int CH = BM * PL_K / 8;        // 16-byte chunks per plane per operand slab
   constexpr int LPT = CH / NT;
@@ -535,13 +539,14 @@ size_t chol_need(int m) {
 
 size_t chol_work_doubles(int n) { return std::max<size_t>(chol_need(n), 1); }
 
-// Products of the recursion with min(M, N, K) at least this many run in
-// FP16x3 when the factorisation is given operand bounds (smaller ones are
-// latency-bound, where the split on load does not pay): MGW_CHOL_F16_MIN,
-// read at every factorisation; 0 keeps every product in FP64.
+// Levels of the recursion whose two halves are both at least this wide run
+// their products in FP16x3 on planes when the factorisation is given
+// operand bounds (128-wide tiles from 256, 64-wide below; smaller levels are
+// latency-bound and stay FP64): MGW_CHOL_F16_MIN, read at every
+// factorisation; 0 keeps every product in FP64.
 int chol_f16_min() {
   const char* e = std::getenv("MGW_CHOL_F16_MIN");
-  return e ? std::atoi(e) : 256;
+  return e ? std::atoi(e) : 128;
 }
 
 namespace {
@@ -578,6 +583,7 @@ struct PlaneProduct {
   double alpha = 1.0, beta = 0.0;
   bool lower = false;
   int tri_a = 0, kb_hi = 0, kb_lo = 0;
+  int bm = 128;         // tile: 128 or 64
 };
 
 int plane_gemm(const CholCtx& c, const PlaneDims& da, const _Float16* pa, int abnd, const PlaneDims& db,
@@ -592,15 +598,20 @@ int plane_gemm(const CholCtx& c, const PlaneDims& da, const _Float16* pa, int ab
   g.amax = c.bounds + abnd; g.bmax = c.bounds + bbnd;
   g.alpha = p.alpha; g.beta = p.beta;
   g.C = p.C; g.ldc = p.ldc; g.strideC = p.strideC;
-  g.tiles_m = da.rpad / PL_ROWS;
-  g.tiles_n = db.rpad / PL_ROWS;
+  MGW_REQUIRE(p.bm == 64 || p.bm == 128, MGW_ERR_ARG, "chol: plane product tile");
+  g.tiles_m = (p.M + p.bm - 1) / p.bm;     // tiles end inside the padded rows
+  g.tiles_n = (p.N + p.bm - 1) / p.bm;
   g.tiles = p.lower ? (int64_t)g.tiles_m * (g.tiles_m + 1) / 2 : (int64_t)g.tiles_m * g.tiles_n;
   g.nblocks = g.tiles * c.nbatch;
   MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chol: too many tiles");
-  if (p.lower)
-    hipLaunchKernelGGL(chol_plane_gemm_kernel<true>, dim3((unsigned)g.nblocks), dim3(256), 0, c.stream, g);
-  else
-    hipLaunchKernelGGL(chol_plane_gemm_kernel<false>, dim3((unsigned)g.nblocks), dim3(256), 0, c.stream, g);
+  const dim3 grid((unsigned)g.nblocks);
+  if (p.bm == 64) {
+    if (p.lower) hipLaunchKernelGGL((chol_plane_gemm_kernel<true, 64>), grid, dim3(256), 0, c.stream, g);
+    else hipLaunchKernelGGL((chol_plane_gemm_kernel<false, 64>), grid, dim3(256), 0, c.stream, g);
+  } else {
+    if (p.lower) hipLaunchKernelGGL((chol_plane_gemm_kernel<true, 128>), grid, dim3(256), 0, c.stream, g);
+    else hipLaunchKernelGGL((chol_plane_gemm_kernel<false, 128>), grid, dim3(256), 0, c.stream, g);
+  }
   MGW_HIP(hipGetLastError());
   return MGW_OK;
 }
@@ -634,6 +645,7 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   double* l21 = w;                          // m2 x m1, leading dimension m2
   double* rest = w + (int64_t)m2 * m1;      // A22's work, then P
   const bool lowp = c.bounds && c.f16_min > 0 && std::min(m1, m2) >= c.f16_min;
+  const int pbm = std::min(m1, m2) >= 256 ? 128 : 64;   // plane product tile
   MGW_TRY(chol_rec(c, k0, m1, w));
   // L21 = A21 (L^-1)11^T   ((L^-1)11 has exact zeros above the diagonal)
   if (lowp) {
@@ -642,7 +654,7 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
     MGW_TRY(split_planes(c, a11, n, s2, 0, BND_ONE, db, c.pb));
     PlaneProduct p;
     p.M = m2; p.N = m1; p.C = l21; p.ldc = m2; p.strideC = c.sw;
-    p.kb_hi = 1;                            // (L^-1)11(n, k) = 0 for k > n
+    p.kb_hi = 1; p.bm = pbm;                            // (L^-1)11(n, k) = 0 for k > n
     MGW_TRY(plane_gemm(c, da, c.pa, BND_D, db, c.pb, BND_ONE, p));
   } else {
     GemmF64 p;
@@ -659,7 +671,7 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
     MGW_TRY(split_planes(c, l21, m2, c.sw, 0, BND_SQRT_D, dl, c.pa));
     PlaneProduct u;
     u.M = m2; u.N = m2; u.C = a22; u.ldc = n; u.strideC = s2;
-    u.alpha = -1.0; u.beta = 1.0; u.lower = true;
+    u.alpha = -1.0; u.beta = 1.0; u.lower = true; u.bm = pbm;
     MGW_TRY(plane_gemm(c, dl, c.pa, BND_SQRT_D, dl, c.pa, BND_SQRT_D, u));
   } else {
     GemmF64 u;
@@ -681,14 +693,14 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
     MGW_TRY(split_planes(c, a11, n, s2, 1, BND_ONE, dt, c.pb));      // B^T(n, k) = (L^-1)11(k, n)
     PlaneProduct q;
     q.M = m2; q.N = m1; q.C = rest; q.ldc = m2; q.strideC = c.sw;
-    q.kb_lo = 1;                            // (L^-1)11(k, n) = 0 for k < n
+    q.kb_lo = 1; q.bm = pbm;                            // (L^-1)11(k, n) = 0 for k < n
     MGW_TRY(plane_gemm(c, dl, c.pa, BND_SQRT_D, dt, c.pb, BND_ONE, q));
     const PlaneDims da = plane_dims(m2, m2), dp = plane_dims(m1, m2);
     MGW_TRY(split_planes(c, a22, n, s2, 0, BND_ONE, da, c.pa));
     MGW_TRY(split_planes(c, rest, m2, c.sw, 1, BND_SQRT_D, dp, c.pb));  // B^T(n, k) = P(k, n)
     PlaneProduct t;
     t.M = m2; t.N = m1; t.C = a21; t.ldc = n; t.strideC = s2;
-    t.alpha = -1.0; t.tri_a = 1;
+    t.alpha = -1.0; t.tri_a = 1; t.bm = pbm;
     return plane_gemm(c, da, c.pa, BND_ONE, dp, c.pb, BND_SQRT_D, t);
   }
   GemmF64 q;
