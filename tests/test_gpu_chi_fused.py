@@ -140,7 +140,8 @@ def test_syrk_lds_copies_equal_the_register_prefetch(variant):
             assert e1 == e0
 
 
-def test_fp16x3_cholesky_products_against_fp64():
+@pytest.mark.parametrize('scale', [0.3, 1.0])
+def test_fp16x3_cholesky_products_against_fp64(scale):
     """The batched inverse Cholesky of an FP16x3 solve with its large products
     in FP16x3 (operand bounds from max diag(1 - chi0): |L^-1| <= 1,
     |L21| <= sqrt(d), |A21| <= d), forced down to 64-wide products, against
@@ -148,8 +149,9 @@ def test_fp16x3_cholesky_products_against_fp64():
     input): chi0 = 1 - (1 + chi)^-1 recovered on the host, relative to
     max |chi0|, within the FP16x3 bound of the chi0 build itself (8e-6), so
     the factorisation adds no more than the build's own rounding; E_RPA
-    within 1e-6 relative."""
-    s = synthetic_system(60, 330, 12, seed=43, scale=0.3)
+    within 1e-6 relative. Molecular-magnitude couplings (scale 0.3) and the
+    benchmark recipe's strong coupling (scale 1.0)."""
+    s = synthetic_system(60, 330, 12, seed=43, scale=scale)
     with Engine(0) as eng:
         eng.set_eri_3center(s.eri_3center, s.index_basis, s.nbf)
         eng.set_mo(s.c_matrix, s.energy, s.occupation)
