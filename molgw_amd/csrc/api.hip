@@ -351,6 +351,14 @@ bool rectangular(const std::vector<int32_t>& ia, int& i0, int& i1, int& a0, int&
 // Measured on MI355X over the serving stream (real molecules, one solve at a
 // time): AO->MO 4.5 -> 3.5 ms per solve, 42.0 -> 43.3-44.1 solves/s, the
 // largest QP error against FP64 0.0013 -> 0.0014 meV.
+// The second product of an FP16x3 transform on pre-split planes (default)
+// or splitting on every tile load: MGW_AO2MO_PLANES = 1 / 0, read at every
+// transform.
+bool ao2mo_planes() {
+  const char* e = std::getenv("MGW_AO2MO_PLANES");
+  return e ? std::atoi(e) != 0 : true;
+}
+
 bool ao2mo_fp16x3() {
   const char* e = std::getenv("MGW_AO2MO_FP16X3");
   return e ? std::atoi(e) != 0 : true;
@@ -434,6 +442,27 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int preci
       c->half_spin = sp;
       c->half_prec = prec_key;
     }
+  }
+  if (f16 && !half && ao2mo_planes()) {
+    // the second product on planes: H split once per matrix f, the
+    // transform's columns of C once for all f, instead of splitting both on
+    // every tile load (the same arithmetic in the same order)
+    const Fp16Planes dh = fp16_planes(naux, nbf), dc = fp16_planes(no, nbf);
+    _Float16* pl = nullptr;   // dbuf counts elements of the pointer's type: fp16 here
+    MGW_TRY(dbuf(c, "ao2mo_planes", (size_t)(dh.per_f * nf + dc.per_f), &pl));
+    _Float16* pc = pl + dh.per_f * nf;
+    Fp16PlaneProduct p;
+    p.M = naux; p.N = no; p.nbatch = nf;
+    p.a_f = dh.per_f; p.b_f = 0;
+    p.C = out;
+    if (first_n) { p.ldc = naux; p.strideC = (int64_t)naux * nm; }
+    else { p.ldc = (int64_t)naux * nm; p.strideC = naux; }
+    ProfScope ps(&c->prof, MGW_KCLASS_GEMM_AO2MO, 2.0 * naux * (double)no * nbf * nf,
+                 8.0 * ((double)naux * nbf * nf + (double)nbf * no + (double)naux * no * nf), c->stream);
+    MGW_TRY(fp16_split_planes(H, naux, (int64_t)naux * nbf, 0, dh, nf, mx + 1, pl, c->stream));
+    MGW_TRY(fp16_split_planes(C + (int64_t)(olo - 1) * nbf, nbf, 0, 1, dc, 1, mx, pc, c->stream));
+    MGW_TRY(fp16_plane_gemm(dh, pl, mx + 1, dc, pc, mx, p, c->stream));
+    return scratch_done(c, "ao2mo_planes");
   }
   GemmF64 g2;
   g2.no_tune = half != nullptr;
@@ -1421,6 +1450,9 @@ int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int techn
     if (split_precision(precision))
       stage_w += (int64_t)(mgw::chi0_split_workspace((int)na, (int)npole, nomega_chi_imag, precision) / 8);
   }
+  // FP16x3 transforms: fp16 hi/lo planes of the half-transformed tensor
+  // (4 bytes per element, rows padded to 128, k to 32) for the widest first range
+  if (precision == MGW_FP16X3) stage_w += (na + 128) * (nb + 32) * (no + 16) / 2;
   int64_t stage_s = na * nb * ns + na * ns * ns + na * npole + (int64_t(1) << 27);
   if (technique == MGW_TECH_CONTOUR)   // column-dot partials and v on every frequency
     stage_s += (int64_t)(mgw::gemm_f64_rowblocks((int)na) + 1) * ns * ns * nomega_chi_imag;

@@ -293,22 +293,6 @@ __device__ inline double plane_scale(double bound) {
 // LDS rows of 64 bytes; 16-byte chunk j of row r stored at chunk j ^ ((r >> 1) & 3)
 __device__ inline int pl_chunk(int r, int j) { return j ^ ((r >> 1) & 3); }
 
-struct PlaneDims {
-  int rows, k;           // logical operand: rows x k
-  int rpad, nkt;         // padded rows, k tiles
-  int64_t per_q;         // elements of one plane of one matrix
-  int64_t per_f;         // elements of both planes of one matrix
-};
-inline PlaneDims plane_dims(int rows, int k) {
-  PlaneDims d;
-  d.rows = rows; d.k = k;
-  d.rpad = (rows + PL_ROWS - 1) / PL_ROWS * PL_ROWS;
-  d.nkt = (k + PL_K - 1) / PL_K;
-  d.per_q = (int64_t)d.nkt * d.rpad * PL_K;
-  d.per_f = 2 * d.per_q;
-  return d;
-}
-
 // Split op(X) (rows x k) of every matrix f into the two planes. trans = 0:
 // op(X)(r, k) = X[r + k ld] (rows contiguous: staged through LDS so that a
 // thread writes 8 consecutive k); trans = 1: op(X)(r, k) = X[k + r ld].
@@ -569,51 +553,17 @@ constexpr int BND_D = 0, BND_SQRT_D = 1, BND_ONE = 2;
 
 // Split op(X) of every matrix into the planes at dst (layout d).
 int split_planes(const CholCtx& c, const double* X, int64_t ld, int64_t strideX, int trans, int bnd,
-                 const PlaneDims& d, _Float16* dst) {
+                 const Fp16Planes& d, _Float16* dst) {
   MGW_REQUIRE(d.per_f <= c.plane_elems, MGW_ERR_ARG, "chol: plane workspace too small");
-  hipLaunchKernelGGL(chol_split_kernel, dim3(d.rpad / 64, d.nkt, c.nbatch), dim3(256), 0, c.stream, X, ld, strideX,
-                     trans, d.rows, d.k, d.rpad, d.nkt, c.bounds + bnd, dst);
-  MGW_HIP(hipGetLastError());
-  return MGW_OK;
+  return fp16_split_planes(X, ld, strideX, trans, d, c.nbatch, c.bounds + bnd, dst, c.stream);
 }
 
-struct PlaneProduct {
-  int M, N;
-  double* C; int64_t ldc, strideC;
-  double alpha = 1.0, beta = 0.0;
-  bool lower = false;
-  int tri_a = 0, kb_hi = 0, kb_lo = 0;
-  int bm = 128;         // tile: 128 or 64
-};
-
-int plane_gemm(const CholCtx& c, const PlaneDims& da, const _Float16* pa, int abnd, const PlaneDims& db,
-               const _Float16* pb, int bbnd, const PlaneProduct& p) {
-  MGW_REQUIRE(da.nkt == db.nkt && da.rows == p.M && db.rows == p.N && (!p.lower || p.M == p.N), MGW_ERR_ARG,
-              "chol: plane product shapes disagree");
-  PlaneGemmArgs g;
-  g.a = pa; g.a_q = da.per_q; g.a_f = da.per_f; g.a_rpad = da.rpad;
-  g.b = pb; g.b_q = db.per_q; g.b_f = db.per_f; g.b_rpad = db.rpad;
-  g.M = p.M; g.N = p.N; g.nkt = da.nkt;
-  g.tri_a = p.tri_a; g.kb_hi = p.kb_hi; g.kb_lo = p.kb_lo;
-  g.amax = c.bounds + abnd; g.bmax = c.bounds + bbnd;
-  g.alpha = p.alpha; g.beta = p.beta;
-  g.C = p.C; g.ldc = p.ldc; g.strideC = p.strideC;
-  MGW_REQUIRE(p.bm == 64 || p.bm == 128, MGW_ERR_ARG, "chol: plane product tile");
-  g.tiles_m = (p.M + p.bm - 1) / p.bm;     // tiles end inside the padded rows
-  g.tiles_n = (p.N + p.bm - 1) / p.bm;
-  g.tiles = p.lower ? (int64_t)g.tiles_m * (g.tiles_m + 1) / 2 : (int64_t)g.tiles_m * g.tiles_n;
-  g.nblocks = g.tiles * c.nbatch;
-  MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "chol: too many tiles");
-  const dim3 grid((unsigned)g.nblocks);
-  if (p.bm == 64) {
-    if (p.lower) hipLaunchKernelGGL((chol_plane_gemm_kernel<true, 64>), grid, dim3(256), 0, c.stream, g);
-    else hipLaunchKernelGGL((chol_plane_gemm_kernel<false, 64>), grid, dim3(256), 0, c.stream, g);
-  } else {
-    if (p.lower) hipLaunchKernelGGL((chol_plane_gemm_kernel<true, 128>), grid, dim3(256), 0, c.stream, g);
-    else hipLaunchKernelGGL((chol_plane_gemm_kernel<false, 128>), grid, dim3(256), 0, c.stream, g);
-  }
-  MGW_HIP(hipGetLastError());
-  return MGW_OK;
+int plane_gemm(const CholCtx& c, const Fp16Planes& da, const _Float16* pa, int abnd, const Fp16Planes& db,
+               const _Float16* pb, int bbnd, Fp16PlaneProduct p) {
+  p.nbatch = c.nbatch;
+  p.a_f = da.per_f;
+  p.b_f = db.per_f;
+  return fp16_plane_gemm(da, pa, c.bounds + abnd, db, pb, c.bounds + bbnd, p, c.stream);
 }
 
 // Recursive inverse Cholesky factor of the diagonal block [k0, k0 + m):
@@ -649,10 +599,10 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   MGW_TRY(chol_rec(c, k0, m1, w));
   // L21 = A21 (L^-1)11^T   ((L^-1)11 has exact zeros above the diagonal)
   if (lowp) {
-    const PlaneDims da = plane_dims(m2, m1), db = plane_dims(m1, m1);
+    const Fp16Planes da = fp16_planes(m2, m1), db = fp16_planes(m1, m1);
     MGW_TRY(split_planes(c, a21, n, s2, 0, BND_D, da, c.pa));
     MGW_TRY(split_planes(c, a11, n, s2, 0, BND_ONE, db, c.pb));
-    PlaneProduct p;
+    Fp16PlaneProduct p;
     p.M = m2; p.N = m1; p.C = l21; p.ldc = m2; p.strideC = c.sw;
     p.kb_hi = 1; p.bm = pbm;                            // (L^-1)11(n, k) = 0 for k > n
     MGW_TRY(plane_gemm(c, da, c.pa, BND_D, db, c.pb, BND_ONE, p));
@@ -667,9 +617,9 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   }
   // A22 -= L21 L21^T (lower triangle; the upper stays zero)
   if (lowp) {
-    const PlaneDims dl = plane_dims(m2, m1);
+    const Fp16Planes dl = fp16_planes(m2, m1);
     MGW_TRY(split_planes(c, l21, m2, c.sw, 0, BND_SQRT_D, dl, c.pa));
-    PlaneProduct u;
+    Fp16PlaneProduct u;
     u.M = m2; u.N = m2; u.C = a22; u.ldc = n; u.strideC = s2;
     u.alpha = -1.0; u.beta = 1.0; u.lower = true; u.bm = pbm;
     MGW_TRY(plane_gemm(c, dl, c.pa, BND_SQRT_D, dl, c.pa, BND_SQRT_D, u));
@@ -688,17 +638,17 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   // (L^-1)21 = -(L^-1)22 P,  P = L21 (L^-1)11
   if (lowp) {
     // the recursion on A22 reused the plane regions: split L21 again
-    const PlaneDims dl = plane_dims(m2, m1), dt = plane_dims(m1, m1);
+    const Fp16Planes dl = fp16_planes(m2, m1), dt = fp16_planes(m1, m1);
     MGW_TRY(split_planes(c, l21, m2, c.sw, 0, BND_SQRT_D, dl, c.pa));
     MGW_TRY(split_planes(c, a11, n, s2, 1, BND_ONE, dt, c.pb));      // B^T(n, k) = (L^-1)11(k, n)
-    PlaneProduct q;
+    Fp16PlaneProduct q;
     q.M = m2; q.N = m1; q.C = rest; q.ldc = m2; q.strideC = c.sw;
     q.kb_lo = 1; q.bm = pbm;                            // (L^-1)11(k, n) = 0 for k < n
     MGW_TRY(plane_gemm(c, dl, c.pa, BND_SQRT_D, dt, c.pb, BND_ONE, q));
-    const PlaneDims da = plane_dims(m2, m2), dp = plane_dims(m1, m2);
+    const Fp16Planes da = fp16_planes(m2, m2), dp = fp16_planes(m1, m2);
     MGW_TRY(split_planes(c, a22, n, s2, 0, BND_ONE, da, c.pa));
     MGW_TRY(split_planes(c, rest, m2, c.sw, 1, BND_SQRT_D, dp, c.pb));  // B^T(n, k) = P(k, n)
-    PlaneProduct t;
+    Fp16PlaneProduct t;
     t.M = m2; t.N = m1; t.C = a21; t.ldc = n; t.strideC = s2;
     t.alpha = -1.0; t.tri_a = 1; t.bm = pbm;
     return plane_gemm(c, da, c.pa, BND_ONE, dp, c.pb, BND_SQRT_D, t);
@@ -722,12 +672,63 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
 
 }  // namespace
 
+Fp16Planes fp16_planes(int rows, int k) {
+  Fp16Planes d;
+  d.rows = rows; d.k = k;
+  d.rpad = (rows + PL_ROWS - 1) / PL_ROWS * PL_ROWS;
+  d.nkt = (k + PL_K - 1) / PL_K;
+  d.per_q = (int64_t)d.nkt * d.rpad * PL_K;
+  d.per_f = 2 * d.per_q;
+  return d;
+}
+
+int fp16_split_planes(const double* X, int64_t ld, int64_t strideX, int trans, const Fp16Planes& d, int nbatch,
+                      const double* bound, _Float16* dst, hipStream_t stream) {
+  if (d.rows <= 0 || d.k <= 0 || nbatch <= 0) return MGW_OK;
+  MGW_REQUIRE(nbatch <= 65535 && d.nkt <= 65535, MGW_ERR_ARG, "fp16 planes: grid too large");
+  hipLaunchKernelGGL(chol_split_kernel, dim3(d.rpad / 64, d.nkt, nbatch), dim3(256), 0, stream, X, ld, strideX,
+                     trans, d.rows, d.k, d.rpad, d.nkt, bound, dst);
+  MGW_HIP(hipGetLastError());
+  return MGW_OK;
+}
+
+int fp16_plane_gemm(const Fp16Planes& da, const _Float16* pa, const double* amax, const Fp16Planes& db,
+                    const _Float16* pb, const double* bmax, const Fp16PlaneProduct& p, hipStream_t stream) {
+  if (p.M <= 0 || p.N <= 0 || p.nbatch <= 0) return MGW_OK;
+  MGW_REQUIRE(da.nkt == db.nkt && da.rows == p.M && db.rows == p.N && (!p.lower || p.M == p.N), MGW_ERR_ARG,
+              "fp16 planes: product shapes disagree");
+  MGW_REQUIRE(p.bm == 64 || p.bm == 128, MGW_ERR_ARG, "fp16 planes: product tile");
+  PlaneGemmArgs g;
+  g.a = pa; g.a_q = da.per_q; g.a_f = p.a_f; g.a_rpad = da.rpad;
+  g.b = pb; g.b_q = db.per_q; g.b_f = p.b_f; g.b_rpad = db.rpad;
+  g.M = p.M; g.N = p.N; g.nkt = da.nkt;
+  g.tri_a = p.tri_a; g.kb_hi = p.kb_hi; g.kb_lo = p.kb_lo;
+  g.amax = amax; g.bmax = bmax;
+  g.alpha = p.alpha; g.beta = p.beta;
+  g.C = p.C; g.ldc = p.ldc; g.strideC = p.strideC;
+  g.tiles_m = (p.M + p.bm - 1) / p.bm;     // tiles end inside the padded rows
+  g.tiles_n = (p.N + p.bm - 1) / p.bm;
+  g.tiles = p.lower ? (int64_t)g.tiles_m * (g.tiles_m + 1) / 2 : (int64_t)g.tiles_m * g.tiles_n;
+  g.nblocks = g.tiles * p.nbatch;
+  MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "fp16 planes: too many tiles");
+  const dim3 grid((unsigned)g.nblocks);
+  if (p.bm == 64) {
+    if (p.lower) hipLaunchKernelGGL((chol_plane_gemm_kernel<true, 64>), grid, dim3(256), 0, stream, g);
+    else hipLaunchKernelGGL((chol_plane_gemm_kernel<false, 64>), grid, dim3(256), 0, stream, g);
+  } else {
+    if (p.lower) hipLaunchKernelGGL((chol_plane_gemm_kernel<true, 128>), grid, dim3(256), 0, stream, g);
+    else hipLaunchKernelGGL((chol_plane_gemm_kernel<false, 128>), grid, dim3(256), 0, stream, g);
+  }
+  MGW_HIP(hipGetLastError());
+  return MGW_OK;
+}
+
 // fp16 elements per matrix of each of the two plane regions: the top
 // level's operands are the largest
 int64_t chol_plane_elems(int n) {
   if (n <= NB) return 0;
   const int m1 = chol_split(n), mx = std::max(m1, n - m1);
-  return plane_dims(mx, mx).per_f;
+  return fp16_planes(mx, mx).per_f;
 }
 
 size_t chol_plane_bytes(int n, int nbatch) {

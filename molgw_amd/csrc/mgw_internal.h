@@ -164,6 +164,40 @@ int wb_split_dot(const double* d, int naux, int nomega, const double* b, int64_t
 // planes (planes: chol_plane_bytes(n, nbatch) of device memory) with
 // operand bounds derived from max_i A_f(i, i), which a first kernel writes
 // to bounds.
+// FP16x3 operand planes (chol.hip): op(X) (rows x k) of each matrix of a
+// batch split once into fp16 hi and lo of 2^e x (2^e from a device bound,
+// so that the bound lands in [2^13, 2^14)), in the k-tiled layout
+// plane[matrix][q][kt][row][32] (rows padded to 128, k to 32, zeros in the
+// padding); trans = 0: op(X)(r, k) = X[r + k ld], trans = 1: X[k + r ld].
+struct Fp16Planes {
+  int rows, k;           // logical operand: rows x k
+  int rpad, nkt;         // padded rows, k tiles
+  int64_t per_q;         // elements of one plane of one matrix
+  int64_t per_f;         // elements of both planes of one matrix
+};
+Fp16Planes fp16_planes(int rows, int k);
+int fp16_split_planes(const double* X, int64_t ld, int64_t strideX, int trans, const Fp16Planes& d, int nbatch,
+                      const double* bound, _Float16* dst, hipStream_t stream);
+// C(m, n) = alpha sum_k A(m, k) B^T(n, k) + beta C(m, n) for each matrix of
+// the batch from the planes of A (M rows) and B^T (N rows) with the same k:
+// hi hi + hi lo + lo hi on fp16 MFMA, FP32 accumulation over the tile's K
+// range, FP64 epilogue. a_f / b_f: plane elements between matrices (0 for
+// an operand shared by the batch). lower: lower-triangle tiles of a square
+// C, m < n not written; tri_a: A lower triangular (K stops at the tile's
+// last row); kb_hi / kb_lo: B^T(n, k) zero for k > n / k < n. bm: tile, 128
+// or 64. Tiles of one matrix before the next.
+struct Fp16PlaneProduct {
+  int M = 0, N = 0, nbatch = 1;
+  int64_t a_f = 0, b_f = 0;
+  double* C = nullptr; int64_t ldc = 0, strideC = 0;
+  double alpha = 1.0, beta = 0.0;
+  bool lower = false;
+  int tri_a = 0, kb_hi = 0, kb_lo = 0;
+  int bm = 128;
+};
+int fp16_plane_gemm(const Fp16Planes& da, const _Float16* pa, const double* amax, const Fp16Planes& db,
+                    const _Float16* pb, const double* bmax, const Fp16PlaneProduct& p, hipStream_t stream);
+
 constexpr int CHOL_NB = 64;
 size_t chol_work_doubles(int n);
 size_t chol_plane_bytes(int n, int nbatch);

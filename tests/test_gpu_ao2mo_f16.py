@@ -111,3 +111,28 @@ def test_real_molecules_with_fp16x3_transforms():
         err = np.max(np.abs(b.energy_qp[sl] - ref.energy_qp[sl])) * HA_MEV
         print(f'{s.name}: FP16x3 transforms and chi0 vs FP64, states {rng.nsemin}..{rng.nsemax}: {err:.4f} meV')
         assert err <= 1.0
+
+
+def test_plane_transform_equals_split_on_load():
+    """The second product of the FP16x3 transform on pre-split planes (H
+    split once per matrix, the transform's columns of C once) against
+    splitting both operands on every tile load (MGW_AO2MO_PLANES=0): the same
+    split of each element, the same products in the same order per
+    accumulator, so chi and the QP energies are identical bit for bit
+    (naux 500, nbf 160: several row, column and K tiles)."""
+    s = synthetic_system(160, 500, 20, seed=9, scale=0.3)
+    out = {}
+    with Engine(0) as eng:
+        eng.set_eri_3center(s.eri_3center, s.index_basis, s.nbf)
+        eng.set_mo(s.c_matrix, s.energy, s.occupation)
+        rng = selfenergy_set_state_range(s.occupation)
+        om, wt = imaginary_quad_grid(8)
+        for planes in (1, 0):
+            with _env(MGW_AO2MO_PLANES=planes):
+                e, _, chi = eng.polarizability_grid(rng, om, wt, return_chi=True, precision='fp16x3')
+                qp = eng.g0w0(technique='imaginary_pade', nomega_chi_imag=8, nomega_sigma_calc=8,
+                              precision='fp16x3', return_sigma=False).energy_qp
+            out[planes] = (e, chi, qp)
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    assert out[1][0] == out[0][0]
+    np.testing.assert_array_equal(out[1][2], out[0][2])
