@@ -459,9 +459,10 @@ int ao2mo(mgw_ctx* c, int mlo, int mhi, int nlo, int nhi, double* out, int preci
     else { p.ldc = (int64_t)naux * nm; p.strideC = naux; }
     ProfScope ps(&c->prof, MGW_KCLASS_GEMM_AO2MO, 2.0 * naux * (double)no * nbf * nf,
                  8.0 * ((double)naux * nbf * nf + (double)nbf * no + (double)naux * no * nf), c->stream);
-    MGW_TRY(fp16_split_planes(H, naux, (int64_t)naux * nbf, 0, dh, nf, mx + 1, pl, c->stream));
-    MGW_TRY(fp16_split_planes(C + (int64_t)(olo - 1) * nbf, nbf, 0, 1, dc, 1, mx, pc, c->stream));
-    MGW_TRY(fp16_plane_gemm(dh, pl, mx + 1, dc, pc, mx, p, c->stream));
+    constexpr int K = MGW_KCLASS_GEMM_AO2MO;
+    MGW_TRY(fp16_split_planes(H, naux, (int64_t)naux * nbf, 0, dh, nf, mx + 1, pl, K, c->stream));
+    MGW_TRY(fp16_split_planes(C + (int64_t)(olo - 1) * nbf, nbf, 0, 1, dc, 1, mx, pc, K, c->stream));
+    MGW_TRY(fp16_plane_gemm(dh, pl, mx + 1, dc, pc, mx, p, K, c->stream));
     return scratch_done(c, "ao2mo_planes");
   }
   GemmF64 g2;

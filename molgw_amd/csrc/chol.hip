@@ -296,9 +296,11 @@ __device__ inline int pl_chunk(int r, int j) { return j ^ ((r >> 1) & 3); }
 // Split op(X) (rows x k) of every matrix f into the two planes. trans = 0:
 // op(X)(r, k) = X[r + k ld] (rows contiguous: staged through LDS so that a
 // thread writes 8 consecutive k); trans = 1: op(X)(r, k) = X[k + r ld].
-// Block: 64 rows x 32 k of one matrix.
+// Block: 64 rows x 32 k of one matrix. KLASS tags the caller's kernel class
+// (distinct kernel symbols for the profiler).
+template <int KLASS>
 __global__ void __launch_bounds__(256)
-chol_split_kernel(const double* __restrict__ X, int64_t ld, int64_t strideX, int trans, int rows, int k,
+plane_split_kernel(const double* __restrict__ X, int64_t ld, int64_t strideX, int trans, int rows, int k,
                   int rpad, int nkt, const double* __restrict__ bound, _Float16* __restrict__ planes) {
   __shared__ double tile[PL_K][65];
   const int tid = threadIdx.x;
@@ -362,9 +364,9 @@ struct PlaneGemmArgs {
 // range, unscaled in FP64 in the epilogue. LOWER: lower-triangle tiles
 // (M == N), entries with m < n not written. Tiles of one matrix before the
 // next.
-template <bool LOWER, int BM>
+template <bool LOWER, int BM, int KLASS>
 __global__ void __launch_bounds__(256, 2)
-chol_plane_gemm_kernel(PlaneGemmArgs g) {
+plane_gemm_kernel(PlaneGemmArgs g) {
   constexpr int WM = 2, WN = 2, NT = 256;
   static_assert(BM == 64 || BM == 128, "tile");
   static_assert(PL_ROWS % BM == 0, "plane row padding covers whole tiles");
@@ -555,7 +557,7 @@ constexpr int BND_D = 0, BND_SQRT_D = 1, BND_ONE = 2;
 int split_planes(const CholCtx& c, const double* X, int64_t ld, int64_t strideX, int trans, int bnd,
                  const Fp16Planes& d, _Float16* dst) {
   MGW_REQUIRE(d.per_f <= c.plane_elems, MGW_ERR_ARG, "chol: plane workspace too small");
-  return fp16_split_planes(X, ld, strideX, trans, d, c.nbatch, c.bounds + bnd, dst, c.stream);
+  return fp16_split_planes(X, ld, strideX, trans, d, c.nbatch, c.bounds + bnd, dst, MGW_KCLASS_CHOLESKY, c.stream);
 }
 
 int plane_gemm(const CholCtx& c, const Fp16Planes& da, const _Float16* pa, int abnd, const Fp16Planes& db,
@@ -563,7 +565,7 @@ int plane_gemm(const CholCtx& c, const Fp16Planes& da, const _Float16* pa, int a
   p.nbatch = c.nbatch;
   p.a_f = da.per_f;
   p.b_f = db.per_f;
-  return fp16_plane_gemm(da, pa, c.bounds + abnd, db, pb, c.bounds + bbnd, p, c.stream);
+  return fp16_plane_gemm(da, pa, c.bounds + abnd, db, pb, c.bounds + bbnd, p, MGW_KCLASS_CHOLESKY, c.stream);
 }
 
 // Recursive inverse Cholesky factor of the diagonal block [k0, k0 + m):
@@ -683,17 +685,22 @@ Fp16Planes fp16_planes(int rows, int k) {
 }
 
 int fp16_split_planes(const double* X, int64_t ld, int64_t strideX, int trans, const Fp16Planes& d, int nbatch,
-                      const double* bound, _Float16* dst, hipStream_t stream) {
+                      const double* bound, _Float16* dst, int klass, hipStream_t stream) {
   if (d.rows <= 0 || d.k <= 0 || nbatch <= 0) return MGW_OK;
   MGW_REQUIRE(nbatch <= 65535 && d.nkt <= 65535, MGW_ERR_ARG, "fp16 planes: grid too large");
-  hipLaunchKernelGGL(chol_split_kernel, dim3(d.rpad / 64, d.nkt, nbatch), dim3(256), 0, stream, X, ld, strideX,
-                     trans, d.rows, d.k, d.rpad, d.nkt, bound, dst);
+  const dim3 grid(d.rpad / 64, d.nkt, nbatch);
+  if (klass == MGW_KCLASS_GEMM_AO2MO)
+    hipLaunchKernelGGL(plane_split_kernel<MGW_KCLASS_GEMM_AO2MO>, grid, dim3(256), 0, stream, X, ld, strideX, trans,
+                       d.rows, d.k, d.rpad, d.nkt, bound, dst);
+  else
+    hipLaunchKernelGGL(plane_split_kernel<MGW_KCLASS_CHOLESKY>, grid, dim3(256), 0, stream, X, ld, strideX, trans,
+                       d.rows, d.k, d.rpad, d.nkt, bound, dst);
   MGW_HIP(hipGetLastError());
   return MGW_OK;
 }
 
 int fp16_plane_gemm(const Fp16Planes& da, const _Float16* pa, const double* amax, const Fp16Planes& db,
-                    const _Float16* pb, const double* bmax, const Fp16PlaneProduct& p, hipStream_t stream) {
+                    const _Float16* pb, const double* bmax, const Fp16PlaneProduct& p, int klass, hipStream_t stream) {
   if (p.M <= 0 || p.N <= 0 || p.nbatch <= 0) return MGW_OK;
   MGW_REQUIRE(da.nkt == db.nkt && da.rows == p.M && db.rows == p.N && (!p.lower || p.M == p.N), MGW_ERR_ARG,
               "fp16 planes: product shapes disagree");
@@ -712,12 +719,16 @@ int fp16_plane_gemm(const Fp16Planes& da, const _Float16* pa, const double* amax
   g.nblocks = g.tiles * p.nbatch;
   MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "fp16 planes: too many tiles");
   const dim3 grid((unsigned)g.nblocks);
-  if (p.bm == 64) {
-    if (p.lower) hipLaunchKernelGGL((chol_plane_gemm_kernel<true, 64>), grid, dim3(256), 0, stream, g);
-    else hipLaunchKernelGGL((chol_plane_gemm_kernel<false, 64>), grid, dim3(256), 0, stream, g);
+  constexpr int KC = MGW_KCLASS_CHOLESKY, KA = MGW_KCLASS_GEMM_AO2MO;
+  if (klass == KA) {
+    MGW_REQUIRE(!p.lower && p.bm == 128, MGW_ERR_ARG, "fp16 planes: transform products are full 128-wide tiles");
+    hipLaunchKernelGGL((plane_gemm_kernel<false, 128, KA>), grid, dim3(256), 0, stream, g);
+  } else if (p.bm == 64) {
+    if (p.lower) hipLaunchKernelGGL((plane_gemm_kernel<true, 64, KC>), grid, dim3(256), 0, stream, g);
+    else hipLaunchKernelGGL((plane_gemm_kernel<false, 64, KC>), grid, dim3(256), 0, stream, g);
   } else {
-    if (p.lower) hipLaunchKernelGGL((chol_plane_gemm_kernel<true, 128>), grid, dim3(256), 0, stream, g);
-    else hipLaunchKernelGGL((chol_plane_gemm_kernel<false, 128>), grid, dim3(256), 0, stream, g);
+    if (p.lower) hipLaunchKernelGGL((plane_gemm_kernel<true, 128, KC>), grid, dim3(256), 0, stream, g);
+    else hipLaunchKernelGGL((plane_gemm_kernel<false, 128, KC>), grid, dim3(256), 0, stream, g);
   }
   MGW_HIP(hipGetLastError());
   return MGW_OK;

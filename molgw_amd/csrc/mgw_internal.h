@@ -177,7 +177,7 @@ struct Fp16Planes {
 };
 Fp16Planes fp16_planes(int rows, int k);
 int fp16_split_planes(const double* X, int64_t ld, int64_t strideX, int trans, const Fp16Planes& d, int nbatch,
-                      const double* bound, _Float16* dst, hipStream_t stream);
+                      const double* bound, _Float16* dst, int klass, hipStream_t stream);
 // C(m, n) = alpha sum_k A(m, k) B^T(n, k) + beta C(m, n) for each matrix of
 // the batch from the planes of A (M rows) and B^T (N rows) with the same k:
 // hi hi + hi lo + lo hi on fp16 MFMA, FP32 accumulation over the tile's K
@@ -185,7 +185,9 @@ int fp16_split_planes(const double* X, int64_t ld, int64_t strideX, int trans, c
 // an operand shared by the batch). lower: lower-triangle tiles of a square
 // C, m < n not written; tri_a: A lower triangular (K stops at the tile's
 // last row); kb_hi / kb_lo: B^T(n, k) zero for k > n / k < n. bm: tile, 128
-// or 64. Tiles of one matrix before the next.
+// or 64. Tiles of one matrix before the next. klass (MGW_KCLASS_CHOLESKY or
+// MGW_KCLASS_GEMM_AO2MO) picks the kernel symbol, so profiles separate the
+// callers.
 struct Fp16PlaneProduct {
   int M = 0, N = 0, nbatch = 1;
   int64_t a_f = 0, b_f = 0;
@@ -196,7 +198,8 @@ struct Fp16PlaneProduct {
   int bm = 128;
 };
 int fp16_plane_gemm(const Fp16Planes& da, const _Float16* pa, const double* amax, const Fp16Planes& db,
-                    const _Float16* pb, const double* bmax, const Fp16PlaneProduct& p, hipStream_t stream);
+                    const _Float16* pb, const double* bmax, const Fp16PlaneProduct& p, int klass,
+                    hipStream_t stream);
 
 constexpr int CHOL_NB = 64;
 size_t chol_work_doubles(int n);
