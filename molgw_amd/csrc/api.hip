@@ -703,7 +703,8 @@ int run_chol(mgw_ctx* c, double* A, int n, int nbatch, double* logdet, int32_t* 
              double* bounds, void* planes) {
   if (!c->use_graphs || gemm_autotune_mode(-1) == 1)
     return chol_inverse_batched(A, n, nbatch, logdet, info, work, c->stream, bounds, planes);
-  const mgw_ctx::GraphKey key{n, nbatch, A, logdet, info, work, bounds, planes, bounds ? chol_f16_min() : 0};
+  const mgw_ctx::GraphKey key{n, nbatch, A, logdet, info, work, bounds, planes,
+                             bounds ? chol_f16_min() * 2 + (chol_epi_planes() ? 1 : 0) : 0};
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     MGW_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

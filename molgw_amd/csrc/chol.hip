@@ -352,6 +352,12 @@ struct PlaneGemmArgs {
   const double* amax; const double* bmax;
   double alpha, beta;
   double* C; int64_t ldc, strideC;
+  // optional: the result written as the planes of a later operand instead of
+  // C (op_trans = 0: plane row m, k n; 1: plane row n, k m), scaled by the
+  // power of two of obound; entries of the tile outside M x N are the
+  // planes' zero padding
+  _Float16* op; int64_t op_q, op_f; int op_rpad, op_nkt, op_trans;
+  const double* obound;
   int tiles_m, tiles_n;
   int64_t tiles, nblocks;
 };
@@ -461,6 +467,32 @@ plane_gemm_kernel(PlaneGemmArgs g) {
   }
   // lane l, register r holds C(m = m_base + (l & 15), n = n_base + 4 (l >> 4) + r)
   const double unscale = 1.0 / (plane_scale(g.amax[0]) * plane_scale(g.bmax[0]));   // a power of two
+  if (g.op) {
+    // the same split as plane_split_kernel applied to the FP64 result
+    // alpha part unscale (exact), so the planes are those of C
+    const double so = plane_scale(g.obound[0]);
+    _Float16* __restrict__ O = g.op + (int64_t)f * g.op_f;
+    const int okmax = g.op_nkt * PL_K;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int m = m0 + wm * TM + i * 16 + fr;
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn * TN + j * 16 + 4 * fq + r;
+          const int orow = g.op_trans ? n : m, ok = g.op_trans ? m : n;
+          if (orow >= g.op_rpad || ok >= okmax) continue;
+          const double v = (m < g.M && n < g.N) ? g.alpha * ((double)part[i][j][r] * unscale) : 0.0;
+          const float y = (float)(v * so);
+          const _Float16 h = (_Float16)y;
+          const int64_t idx = ((int64_t)(ok / PL_K) * g.op_rpad + orow) * PL_K + (ok % PL_K);
+          O[idx] = h;
+          O[idx + g.op_q] = (_Float16)(y - (float)h);
+        }
+    }
+    return;
+  }
   double* __restrict__ C = g.C + (int64_t)f * g.strideC;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
@@ -549,7 +581,11 @@ struct CholCtx {
   int f16_min;          // smallest dimension of a level whose products run in FP16x3
   _Float16* pa;         // plane regions of the two operands of a product
   _Float16* pb;
-  int64_t plane_elems;  // capacity of each region, elements per matrix
+  _Float16* pl21;       // stack of the levels' L21 planes (live across the A22
+                        // recursion), each level's P planes above its L21
+  int64_t plane_elems;  // capacity of pa, pb: elements per matrix
+  int64_t l21_elems;    // capacity of the stack: elements per matrix
+  bool epi;             // products write the planes of L21 and P (else: FP64, then split)
 };
 constexpr int BND_D = 0, BND_SQRT_D = 1, BND_ONE = 2;
 
@@ -576,8 +612,9 @@ int plane_gemm(const CholCtx& c, const Fp16Planes& da, const _Float16* pa, int a
 // large levels when the factorisation has operand bounds); the 64 x 64
 // leaves are factored and inverted by chol_leaf_kernel (or
 // chol_diag_kernel). `w` is this block's work (chol_need(m) doubles per
-// matrix, matrix stride c.sw).
-int chol_rec(const CholCtx& c, int k0, int m, double* w) {
+// matrix, matrix stride c.sw); `l21_off` the L21-plane stack used by the
+// enclosing levels (elements per matrix).
+int chol_rec(const CholCtx& c, int k0, int m, double* w, int64_t l21_off) {
   const int n = c.n;
   const int64_t s2 = (int64_t)n * n;
   if (m <= NB) {
@@ -598,7 +635,49 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
   double* rest = w + (int64_t)m2 * m1;      // A22's work, then P
   const bool lowp = c.bounds && c.f16_min > 0 && std::min(m1, m2) >= c.f16_min;
   const int pbm = std::min(m1, m2) >= 256 ? 128 : 64;   // plane product tile
-  MGW_TRY(chol_rec(c, k0, m1, w));
+  MGW_TRY(chol_rec(c, k0, m1, w, l21_off));
+  if (lowp && c.epi) {
+    // L21 and P exist only as planes: the products that make them write
+    // them, and the splits of L21 (twice) and P go away
+    const Fp16Planes dl = fp16_planes(m2, m1);
+    MGW_REQUIRE(l21_off + dl.per_f <= c.l21_elems, MGW_ERR_ARG, "chol: L21 plane stack too small");
+    _Float16* pl = c.pl21 + l21_off * c.nbatch;
+    {  // L21 = A21 (L^-1)11^T  ((L^-1)11 has exact zeros above the diagonal)
+      const Fp16Planes da = fp16_planes(m2, m1), db = fp16_planes(m1, m1);
+      MGW_TRY(split_planes(c, a21, n, s2, 0, BND_D, da, c.pa));
+      MGW_TRY(split_planes(c, a11, n, s2, 0, BND_ONE, db, c.pb));
+      Fp16PlaneProduct p;
+      p.M = m2; p.N = m1; p.kb_hi = 1; p.bm = pbm;        // (L^-1)11(n, k) = 0 for k > n
+      p.out_planes = pl; p.out_dims = dl; p.out_trans = 0; p.out_bound = c.bounds + BND_SQRT_D;
+      MGW_TRY(plane_gemm(c, da, c.pa, BND_D, db, c.pb, BND_ONE, p));
+    }
+    {  // A22 -= L21 L21^T (lower triangle; the upper stays zero)
+      Fp16PlaneProduct u;
+      u.M = m2; u.N = m2; u.C = a22; u.ldc = n; u.strideC = s2;
+      u.alpha = -1.0; u.beta = 1.0; u.lower = true; u.bm = pbm;
+      MGW_TRY(plane_gemm(c, dl, pl, BND_SQRT_D, dl, pl, BND_SQRT_D, u));
+    }
+    MGW_TRY(chol_rec(c, k0 + m1, m2, rest, l21_off + dl.per_f));
+    // the A22 recursion is done: P's planes go above this level's L21
+    const Fp16Planes dp = fp16_planes(m1, m2);
+    MGW_REQUIRE(l21_off + dl.per_f + dp.per_f <= c.l21_elems, MGW_ERR_ARG, "chol: plane stack too small");
+    _Float16* pp = c.pl21 + (l21_off + dl.per_f) * c.nbatch;
+    {  // P = L21 (L^-1)11, written as the planes of P^T (the B^T operand below)
+      const Fp16Planes dt = fp16_planes(m1, m1);
+      MGW_TRY(split_planes(c, a11, n, s2, 1, BND_ONE, dt, c.pb));      // B^T(n, k) = (L^-1)11(k, n)
+      Fp16PlaneProduct q;
+      q.M = m2; q.N = m1; q.kb_lo = 1; q.bm = pbm;        // (L^-1)11(k, n) = 0 for k < n
+      q.out_planes = pp; q.out_dims = dp; q.out_trans = 1; q.out_bound = c.bounds + BND_SQRT_D;
+      MGW_TRY(plane_gemm(c, dl, pl, BND_SQRT_D, dt, c.pb, BND_ONE, q));
+    }
+    // (L^-1)21 = -(L^-1)22 P
+    const Fp16Planes da = fp16_planes(m2, m2);
+    MGW_TRY(split_planes(c, a22, n, s2, 0, BND_ONE, da, c.pa));
+    Fp16PlaneProduct t;
+    t.M = m2; t.N = m1; t.C = a21; t.ldc = n; t.strideC = s2;
+    t.alpha = -1.0; t.tri_a = 1; t.bm = pbm;
+    return plane_gemm(c, da, c.pa, BND_ONE, dp, pp, BND_SQRT_D, t);
+  }
   // L21 = A21 (L^-1)11^T   ((L^-1)11 has exact zeros above the diagonal)
   if (lowp) {
     const Fp16Planes da = fp16_planes(m2, m1), db = fp16_planes(m1, m1);
@@ -636,7 +715,7 @@ int chol_rec(const CholCtx& c, int k0, int m, double* w) {
     u.batch = c.nbatch;
     MGW_TRY(gemm_chol(u, c.stream));
   }
-  MGW_TRY(chol_rec(c, k0 + m1, m2, rest));
+  MGW_TRY(chol_rec(c, k0 + m1, m2, rest, l21_off));
   // (L^-1)21 = -(L^-1)22 P,  P = L21 (L^-1)11
   if (lowp) {
     // the recursion on A22 reused the plane regions: split L21 again
@@ -713,6 +792,18 @@ int fp16_plane_gemm(const Fp16Planes& da, const _Float16* pa, const double* amax
   g.amax = amax; g.bmax = bmax;
   g.alpha = p.alpha; g.beta = p.beta;
   g.C = p.C; g.ldc = p.ldc; g.strideC = p.strideC;
+  g.op = p.out_planes; g.op_q = p.out_dims.per_q; g.op_f = p.out_dims.per_f; g.op_rpad = p.out_dims.rpad;
+  g.op_nkt = p.out_dims.nkt; g.op_trans = p.out_trans; g.obound = p.out_bound;
+  if (p.out_planes) {
+    // the output planes: op(C) is out_dims.rows x out_dims.k, and the tiles
+    // cover every entry its consumers read (the same tile size)
+    const int orows = p.out_trans ? p.N : p.M, ok = p.out_trans ? p.M : p.N;
+    MGW_REQUIRE(!p.lower && p.beta == 0.0 && p.out_bound && p.out_dims.rows == orows && p.out_dims.k == ok,
+                MGW_ERR_ARG,
+                "fp16 planes: output planes disagree with the product");
+  } else {
+    MGW_REQUIRE(p.C != nullptr, MGW_ERR_ARG, "fp16 planes: no output");
+  }
   g.tiles_m = (p.M + p.bm - 1) / p.bm;     // tiles end inside the padded rows
   g.tiles_n = (p.N + p.bm - 1) / p.bm;
   g.tiles = p.lower ? (int64_t)g.tiles_m * (g.tiles_m + 1) / 2 : (int64_t)g.tiles_m * g.tiles_n;
@@ -742,8 +833,21 @@ int64_t chol_plane_elems(int n) {
   return fp16_planes(mx, mx).per_f;
 }
 
+// the regions pa, pb and the plane stack: the L21 planes of the levels on
+// the A22 path (at most 4/3 of the top level's), and one level's P planes
+// above its L21 once its A22 recursion is done (at most the top level's L21
+// and P: two regions)
+constexpr int CHOL_PLANE_REGIONS = 4;
 size_t chol_plane_bytes(int n, int nbatch) {
-  return (size_t)2 * chol_plane_elems(n) * nbatch * sizeof(_Float16);
+  return (size_t)CHOL_PLANE_REGIONS * chol_plane_elems(n) * nbatch * sizeof(_Float16);
+}
+
+// The products of a level write the planes of L21 and P (default), or write
+// them in FP64 and split them (MGW_CHOL_EPI_PLANES=0; the same planes bit
+// for bit). Read at every factorisation.
+bool chol_epi_planes() {
+  const char* e = std::getenv("MGW_CHOL_EPI_PLANES");
+  return e ? std::atoi(e) != 0 : true;
 }
 
 int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* info, double* work,
@@ -759,9 +863,11 @@ int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* 
   }
   const int64_t pe = chol_plane_elems(n);
   _Float16* pa = static_cast<_Float16*>(planes);
+  auto region = [&](int r) { return pa ? pa + (int64_t)r * pe * nbatch : nullptr; };
   const CholCtx c{A, n, nbatch, logdet, info, work, (int64_t)chol_work_doubles(n), stream,
-                  f16_min > 0 ? bounds : nullptr, f16_min, pa, pa ? pa + pe * nbatch : nullptr, pe};
-  return chol_rec(c, 0, n, work);
+                  f16_min > 0 ? bounds : nullptr, f16_min, region(0), region(1), region(2), pe,
+                  (CHOL_PLANE_REGIONS - 2) * pe, chol_epi_planes()};
+  return chol_rec(c, 0, n, work, 0);
 }
 
 int launch_lower_frob2(const double* m, int n, int nbatch, double* partial, hipStream_t s) {

@@ -188,6 +188,10 @@ int fp16_split_planes(const double* X, int64_t ld, int64_t strideX, int trans, c
 // or 64. Tiles of one matrix before the next. klass (MGW_KCLASS_CHOLESKY or
 // MGW_KCLASS_GEMM_AO2MO) picks the kernel symbol, so profiles separate the
 // callers.
+// out_planes (optional, beta = 0 only): the result written as the planes
+// (layout out_dims, scale from the device bound out_bound) of op(C), C itself
+// (out_trans = 0) or C^T (1), instead of C; the padding inside the tiles
+// is written with zeros.
 struct Fp16PlaneProduct {
   int M = 0, N = 0, nbatch = 1;
   int64_t a_f = 0, b_f = 0;
@@ -196,6 +200,10 @@ struct Fp16PlaneProduct {
   bool lower = false;
   int tri_a = 0, kb_hi = 0, kb_lo = 0;
   int bm = 128;
+  _Float16* out_planes = nullptr;
+  Fp16Planes out_dims = {};
+  int out_trans = 0;
+  const double* out_bound = nullptr;
 };
 int fp16_plane_gemm(const Fp16Planes& da, const _Float16* pa, const double* amax, const Fp16Planes& db,
                     const _Float16* pb, const double* bmax, const Fp16PlaneProduct& p, int klass,
@@ -205,6 +213,7 @@ constexpr int CHOL_NB = 64;
 size_t chol_work_doubles(int n);
 size_t chol_plane_bytes(int n, int nbatch);
 int chol_f16_min();
+bool chol_epi_planes();
 int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* info, double* work,
                          hipStream_t stream, double* bounds = nullptr, void* planes = nullptr);
 // tr[f] = sum of squares of the lower triangle of M_f (n x n, stride n^2).

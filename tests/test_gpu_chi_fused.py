@@ -119,6 +119,23 @@ def test_cholesky_leaf_kernels_agree():
     assert abs(e_lds - e_wave) < 1e-12 * abs(e_wave)
 
 
+@pytest.mark.parametrize('naux,f16_min', [(330, '64'), (600, '128')])
+def test_cholesky_epilogue_planes_equal_split_planes(naux, f16_min):
+    """The FP16x3 Cholesky products that write L21 and P straight into the
+    hi/lo planes of their later uses (the same split applied to the FP64
+    value in the epilogue) against the products that write FP64 and split
+    it afterwards: chi identical bit for bit (64-wide products on naux 330;
+    128-wide ones at the 600 level's split)."""
+    s = synthetic_system(60, naux, 12, seed=47, scale=1.0)
+    with Engine(0) as eng:
+        eng.set_eri_3center(s.eri_3center, s.index_basis, s.nbf)
+        eng.set_mo(s.c_matrix, s.energy, s.occupation)
+        e0, chi0 = _chi(eng, s, 'fp16x3', {'MGW_CHOL_F16_MIN': f16_min, 'MGW_CHOL_EPI_PLANES': '0'})
+        e1, chi1 = _chi(eng, s, 'fp16x3', {'MGW_CHOL_F16_MIN': f16_min, 'MGW_CHOL_EPI_PLANES': '1'})
+    np.testing.assert_array_equal(chi1, chi0)
+    assert e1 == e0
+
+
 @pytest.mark.parametrize('variant', ['1', '2', '3', '4', '5', '6', '7', '8'])
 def test_syrk_lds_copies_equal_the_register_prefetch(variant):
     """The SYRK variants that copy slabs straight into LDS (one or two
