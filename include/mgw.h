@@ -120,9 +120,10 @@ int mgw_ctx_trim(mgw_ctx* ctx, int eager);
  * per shape and buffer set and replayed on later solves. No reference
  * counterpart (launch batching for the serving loop). */
 int mgw_ctx_graphs(mgw_ctx* ctx, int enable);
-/* GEMM tile autotuner (process-wide, no reference counterpart): mode 1 times
- * the candidate tile configurations on the first product of each new shape
- * class and keeps the fastest; 0 uses the table or the measured defaults;
+/* Kernel autotuner (process-wide, no reference counterpart): mode 1 times
+ * the candidates (GEMM tile configurations, the chi0 SYRK variants, the W.B
+ * dot pipelines; all give identical results) on the first launch of each
+ * new shape class and keeps the fastest; 0 uses the table or the measured defaults;
  * -1 only queries. The table is read from and appended to the file named by
  * the environment variable MGW_TUNE_FILE, if set. `previous` (nullable)
  * receives the mode before the call. */

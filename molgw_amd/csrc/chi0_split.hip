@@ -1243,9 +1243,21 @@ int split_and_syrk(const double* x, int naux, int npole, const double* sw, int n
       const int var = NS == 2 ? chi_syrk_variant() : 0;
       MGW_REQUIRE((L.bm == 256) == (var == 5 || var == 6), MGW_ERR_ARG,
                   "chi0 split: tile size and SYRK variant disagree");
-      if (var) MGW_TRY(launch_syrk_lds(H, var, grid, g, stream));
-      else hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
-      MGW_HIP(hipGetLastError());
+      auto launch = [&](int v) -> int {
+        if (v) return launch_syrk_lds(H, v, grid, g, stream);
+        hipLaunchKernelGGL((chi_syrk_kernel<NS, H, BM, 2, 2, 4, 1, 1, true>), grid, block, 0, stream, g);
+        MGW_HIP(hipGetLastError());
+        return MGW_OK;
+      };
+      if (NS == 2 && !std::getenv("MGW_CHI_SYRK")) {
+        // the tuner picks among the 128-wide variants (same products in the
+        // same order: identical results) per shape class
+        static constexpr int CAND[7] = {0, 1, 2, 3, 4, 7, 8};
+        MGW_TRY(autotune_select(MGW_TUNE_CHI_SYRK, NS, H ? 1 : 0, log2_floor(naux), log2_floor(L.nkt), log2_floor(nf),
+                                7, 0, [&](int c) { return launch(CAND[c]); }, stream));
+      } else {
+        MGW_TRY(launch(var));
+      }
       if (L.bm == 256)
         hipLaunchKernelGGL((chi_ksplit_reduce_kernel<256>), dim3(256 * 256 / 256, L.tiles, nf), block, 0, stream,
                            part, L.nkc, L.tiles, L.tm, naux, g.unscale, g.C, g.strideC);

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstddef>
 #include <map>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -220,6 +221,21 @@ int chol_inverse_batched(double* A, int n, int nbatch, double* logdet, int32_t* 
 int launch_lower_frob2(const double* m, int n, int nbatch, double* partial, hipStream_t s);
 constexpr int FROB_BLOCKS = 64;
 int gemm_autotune_mode(int mode); // -1 query; returns the previous mode
+// The same tuner for the split-operand kernel families: launch(c) runs
+// candidate c (0 <= c < ncand); every candidate must give the same result.
+// Key: family, planes, fp16 flag and three log2 shape buckets. With tuning
+// on, an unseen key times every candidate on the actual operands (the last
+// run's output stands) and records the fastest; otherwise the recorded
+// choice, or `fallback`, is launched.
+int autotune_select(int family, int ns, int h, int mb, int nb, int kb, int ncand, int fallback,
+                    const std::function<int(int)>& launch, hipStream_t stream);
+constexpr int MGW_TUNE_CHI_SYRK = 105;   // K-split chi0 SYRK variants
+constexpr int MGW_TUNE_WB_DOT = 107;     // W.B dot pipelines
+inline int log2_floor(int64_t v) {
+  int b = 0;
+  while (v > 1) { v >>= 1; ++b; }
+  return b;
+}
 
 // ---------------------------------------------------------------- integrals
 // Device view of a set of contracted shells (struct of arrays): angular

@@ -428,11 +428,25 @@ int split_and_dot(const double* d, int naux, int nomega, const double* b, int64_
     // two planes: FP32 accumulation over the whole K (the products of the
     // 11-bit fp16 / 8-bit bf16 planes are exact in FP32; the FP64 accumulators
     // would cost 128 VGPRs and half the occupancy)
-    if (NS == 2)
-      hipLaunchKernelGGL((wb_dot_kernel<NS, H, BM, BN, 2, 2, 0, 3>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
-    else
+    if (NS == 2) {
+      // two LDS buffers (default) or two prefetch register sets: the same
+      // products in the same order, so the tuner may pick either per shape
+      auto launch = [&](int c) -> int {
+        if (c == 0)
+          hipLaunchKernelGGL((wb_dot_kernel<NS, H, BM, BN, 2, 2, 0, 3>), dim3((unsigned)g.nblocks), dim3(256), 0,
+                             stream, g);
+        else
+          hipLaunchKernelGGL((wb_dot_kernel<NS, H, BM, BN, 2, 2, 0, 2>), dim3((unsigned)g.nblocks), dim3(256), 0,
+                             stream, g);
+        MGW_HIP(hipGetLastError());
+        return MGW_OK;
+      };
+      MGW_TRY(autotune_select(MGW_TUNE_WB_DOT, NS, H ? 1 : 0, log2_floor(naux), log2_floor(ncol), log2_floor(nf), 2, 0,
+                              launch, stream));
+    } else {
       hipLaunchKernelGGL((wb_dot_kernel<NS, H, BM, BN, 2, 2, 4, 2>), dim3((unsigned)g.nblocks), dim3(256), 0, stream, g);
-    MGW_HIP(hipGetLastError());
+      MGW_HIP(hipGetLastError());
+    }
   }
   return MGW_OK;
 }
