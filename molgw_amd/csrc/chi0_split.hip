@@ -723,16 +723,25 @@ chi_ksplit_reduce_kernel(const float* __restrict__ part, int nkc, int tiles, int
   if (e >= BM * BM) return;
   int bm, bn;
   tri_grouped(tile, tm, bm, bn);
-  const int m = bm * BM + e % BM, n = bn * BM + e / BM;
+  const int ml = e % BM, nl = e / BM;
+  const int m = bm * BM + ml, n = bn * BM + nl;
+  double* __restrict__ out = C + (int64_t)f * strideC;
+  // zeros above the diagonal, in the same coalesced column order: the whole
+  // mirror tile (bn, bm) of an off-diagonal tile, the strict upper part of a
+  // diagonal one (every upper element exactly once)
+  if (bm != bn) {
+    const int mu = bn * BM + ml, nu = bm * BM + nl;
+    if (nu < naux) out[mu + (int64_t)nu * naux] = 0.0;
+  } else if (m < n && n < naux) {
+    out[m + (int64_t)n * naux] = 0.0;
+  }
   if (m >= naux || n >= naux || m < n) return;
   const float* __restrict__ p = part + (int64_t)(f * tiles + tile) * nkc * (BM * BM) + e;
   double v = 0.0;
   for (int c = 0; c < nkc; ++c) v += (double)p[(int64_t)c * (BM * BM)];
   if (unscale) v *= unscale[f];
   if (m == n) v += 1.0;
-  double* __restrict__ out = C + (int64_t)f * strideC;
   out[m + (int64_t)n * naux] = v;
-  if (m > n) out[n + (int64_t)m * naux] = 0.0;
 }
 
 // ---------------------------------------------------------------- fused split

@@ -105,7 +105,10 @@ def test_large_strongly_coupled_grid_against_fp64(engine, prec):
     """A (H2O)19-shaped system (nbf 456, naux 1596, nocc 95) of the benchmark
     recipe: chi0 eigenvalues reach ~10^4, where a split-product 1 - chi0 can
     lose positive definiteness. Frequencies that do are rebuilt in FP64; the
-    solve must finish and E_RPA stay close to FP64."""
+    solve must finish and E_RPA stay close to FP64: measured on MI355X
+    (E_RPA -3538.14 Ha) within 1.2e-2 Ha for BF16x3 and BF16x6 (3.3e-6
+    relative) and 2.0e-3 Ha for FP16x3 (5.6e-7), so the bounds are 1e-5
+    and 2e-6 relative."""
     s = synthetic_system(456, 1596, 95, seed=1000 + 97 * 19 + 1, scale=0.3)
     load(engine, s)
     rng = selfenergy_set_state_range(s.occupation)
@@ -115,7 +118,7 @@ def test_large_strongly_coupled_grid_against_fp64(engine, prec):
     e, g, _ = engine.polarizability_grid(rng, om, wt, precision=prec)
     esc = engine.precision_escalations() - before
     print(f'{prec}: E_RPA {e:.10f} vs {e64:.10f} (diff {abs(e - e64):.2e} Ha), {esc} frequencies rebuilt in FP64')
-    assert abs(e - e64) < (1e-2 if prec == 'bf16x3' else 1e-4) * abs(e64)
+    assert abs(e - e64) < (2e-6 if prec == 'fp16x3' else 1e-5) * abs(e64)
 
 
 @pytest.mark.parametrize('prec', ['bf16x3', 'bf16x6', 'fp16x3'])
