@@ -165,6 +165,12 @@ int mgw_precision_escalations(mgw_ctx* ctx, int64_t* count);
  * nomega_chi_imag. */
 int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int technique,
                             int nomega_chi_imag, int precision);
+/* The same for a self-energy range of nse states (nsemax - nsemin + 1): the
+ * buffers of the self-energy stage scale with it, so a solve restricted to a
+ * window around the gap needs far less than the whole range (nse = nstate,
+ * as mgw_footprint_bytes). No reference counterpart (serving admission). */
+int64_t mgw_footprint_bytes_nse(int nbf, int nauxil, int nstate, int nocc, int nse, int technique,
+                                int nomega_chi_imag, int precision);
 
 /* ---------------------------------- integral front end (upstream of A1) */
 /* A set of contracted Gaussian shells: the reference's basis%shell(:)

@@ -62,6 +62,8 @@ SIGNATURES = {
     'mgw_device_memory': (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'mgw_precision_escalations': (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     'mgw_footprint_bytes': (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    'mgw_footprint_bytes_nse': (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.c_int]),
     'mgw_eri_2center_ao': (C.c_int, [_vp, C.POINTER(Shells), C.c_int, _vp]),
     'mgw_eri_3center_ao': (C.c_int, [_vp, C.POINTER(Shells), C.POINTER(Shells), C.c_int64, _ip, C.c_int, _vp]),
     'mgw_hartree_exchange_ri': (C.c_int, [_vp, _dp, C.c_int, C.c_int, C.c_double, _dp, _dp]),

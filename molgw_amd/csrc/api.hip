@@ -1431,7 +1431,13 @@ int mgw_memory_report(mgw_ctx* c, int64_t* current_bytes, int64_t* peak_bytes) {
 
 int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int technique,
                             int nomega_chi_imag, int precision) {
+  return mgw_footprint_bytes_nse(nbf, nauxil, nstate, nocc, nstate, technique, nomega_chi_imag, precision);
+}
+
+int64_t mgw_footprint_bytes_nse(int nbf, int nauxil, int nstate, int nocc, int nse, int technique,
+                                int nomega_chi_imag, int precision) {
   const int64_t nb = nbf, na = nauxil, ns = nstate, no = nocc, nv = nstate - nocc;
+  const int64_t ne = std::max(1, std::min(nse, nstate));   // self-energy states
   const int64_t npole = no * nv;
   int64_t d = na * nb * nb;                       // resident dense RI tensor
   d += nb * ns;                                   // C
@@ -1452,14 +1458,17 @@ int64_t mgw_footprint_bytes(int nbf, int nauxil, int nstate, int nocc, int techn
     if (split_precision(precision))
       stage_w += (int64_t)(mgw::chi0_split_workspace((int)na, (int)npole, nomega_chi_imag, precision) / 8);
   }
-  // FP16x3 transforms: fp16 hi/lo planes of the half-transformed tensor
-  // (4 bytes per element, rows padded to 128, k to 32) for the widest first range
-  if (precision == MGW_FP16X3) stage_w += (na + 128) * (nb + 32) * (no + 16) / 2;
-  int64_t stage_s = na * nb * ns + na * ns * ns + na * npole + (int64_t(1) << 27);
-  if (technique == MGW_TECH_CONTOUR)   // column-dot partials and v on every frequency
-    stage_s += (int64_t)(mgw::gemm_f64_rowblocks((int)na) + 1) * ns * ns * nomega_chi_imag;
+  // FP16x3 transforms: the first product's states run up to the top of the
+  // self-energy range (its half transform is reused there), and the fp16
+  // hi/lo planes of the half-transformed tensor (4 bytes per element, rows
+  // padded to 128, k to 32)
+  if (precision == MGW_FP16X3) stage_w += na * nb * ne + (na + 128) * (nb + 32) * (no + ne + 32) / 2;
+  // self-energy stage: half transform and (P|pm) for the ne states p, all m
+  int64_t stage_s = na * nb * ne + na * ne * ns + na * npole + (int64_t(1) << 27);
+  if (technique != MGW_TECH_ONE_SHOT)   // column-dot partials and v on every frequency
+    stage_s += (int64_t)(mgw::gemm_f64_rowblocks((int)na) + 1) * ns * ne * nomega_chi_imag;
   if (technique != MGW_TECH_ONE_SHOT && split_precision(precision))
-    stage_s += (int64_t)((mgw::wb_split_workspace((int)na, ns * ns, nomega_chi_imag, precision) + 7) / 8);
+    stage_s += (int64_t)((mgw::wb_split_workspace((int)na, ns * ne, nomega_chi_imag, precision) + 7) / 8);
   // the temporaries of the W stage stay cached while the self-energy runs
   return 8 * (d + stage_w + stage_s);
 }

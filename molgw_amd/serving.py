@@ -62,11 +62,21 @@ class Result:
         return self.end - self.arrival
 
 
+def _se_range(s, window: int) -> tuple[int, int]:
+    """The self-energy states (1-based, inclusive) a request's solve computes:
+    HOMO-window+1 .. LUMO+window-1 inside the default range."""
+    rng = selfenergy_set_state_range(s.occupation)
+    return max(rng.nsemin, rng.nhomo_G - window + 1), min(rng.nsemax, rng.nhomo_G + window)
+
+
 def footprint_bytes(req: Request) -> int:
+    """HBM of the request's solve (mgw_footprint_bytes_nse on its
+    self-energy window)."""
     s = req.system
     lib = _lib.load()
-    return int(lib.mgw_footprint_bytes(s.nbf, s.naux, s.nstate, s.nocc, MGW_TECH[req.technique],
-                                       req.nomega_chi_imag, PRECISION[req.precision]))
+    lo, hi = _se_range(s, req.window)
+    return int(lib.mgw_footprint_bytes_nse(s.nbf, s.naux, s.nstate, s.nocc, hi - lo + 1, MGW_TECH[req.technique],
+                                           req.nomega_chi_imag, PRECISION[req.precision]))
 
 
 def device_memory_bytes(device: int = 0) -> int:
@@ -122,8 +132,7 @@ class Server:
             eng.set_eri_3center(s.eri_3center, s.index_basis, s.nbf)
         eng.set_mo(s.c_matrix, s.energy, s.occupation)
         rng = selfenergy_set_state_range(s.occupation)
-        rng.nsemin = max(rng.nsemin, rng.nhomo_G - req.window + 1)
-        rng.nsemax = min(rng.nsemax, rng.nhomo_G + req.window)
+        rng.nsemin, rng.nsemax = _se_range(s, req.window)
         out = eng.g0w0(rng, technique=req.technique, precision=req.precision,
                        nomega_chi_imag=req.nomega_chi_imag, nomega_sigma_calc=req.nomega_sigma_calc,
                        exchange_m_vxc=getattr(s, 'exchange_m_vxc', None), return_sigma=False)
@@ -197,10 +206,19 @@ class Server:
                 free = [k for k in range(len(self.slots)) if not self._busy[k]]
                 if not free:
                     break
-                k = free[0]
+                # the free slot whose cached buffers fit the request best: the
+                # least memory after it grows to the request's footprint, then
+                # the most buffers to reuse
+                cache = {j: self.slots[j].memory()[0] for j in free}
+                k = min(free, key=lambda j: (max(req.footprint, cache[j]), -cache[j]))
                 others = self._others(k)
-                need = max(req.footprint, self.slots[k].memory()[0])
+                need = max(req.footprint, cache[k])
                 if others + need > self.budget:
+                    # freeing idle slots' caches waits for the whole device:
+                    # only when that makes room next to the running solves
+                    busy = sum(self._slot_bytes(j) for j in range(len(self.slots)) if self._busy[j])
+                    if busy + req.footprint > self.budget:
+                        continue
                     self._trim_until(k, req)
                     others = self._others(k)
                     need = max(req.footprint, self.slots[k].memory()[0])
