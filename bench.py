@@ -567,8 +567,9 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=None,
                     help='timed steps: solves (single-system workloads, default 5) or requests of the stream '
-                         '(serving, batch64; default 64)')
-    ap.add_argument('--warmup', type=int, default=None, help='untimed warm-up steps (default 1; streams: slots)')
+                         '(serving: default 20; batch64: default 64)')
+    ap.add_argument('--warmup', type=int, default=None,
+                    help='untimed warm-up steps (default 1; serving: 5 requests; batch64: one per slot)')
     ap.add_argument('--workload', default='serving', choices=sorted(WORKLOADS) + ['serving', 'batch64', 'ri-build'])
     ap.add_argument('--requests', type=int, default=None, help='stream / batch length (same as --steps)')
     ap.add_argument('--rate', type=float, default=20.0, help='Poisson arrival rate, requests/s (serving)')
@@ -603,11 +604,11 @@ def main():
     if args.precision is None:
         args.precision = 'fp16x3' if stream else 'fp64'
     if args.requests is None:
-        args.requests = args.steps if args.steps is not None else 64
+        args.requests = args.steps if args.steps is not None else (64 if args.workload == 'batch64' else 20)
     if args.steps is None:
         args.steps = args.requests if stream else 5
     if args.warmup is None:
-        args.warmup = args.slots if stream else 1
+        args.warmup = (5 if args.workload == 'serving' else args.slots) if stream else 1
     if args.workload == 'ri-build':
         ri_build_bench(args)
         return
