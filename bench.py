@@ -147,7 +147,9 @@ def stream_bench(workload, args):
         for a, b in zip(run, ref):
             sl = slice(a.nsemin - 1, a.nsemax)
             err = max(err, float(np.max(np.abs(a.energy_qp[sl] - b.energy_qp[sl]))))
-    print(f'# serving capacity: {sm["solves_per_s"]:.2f} solves/s; live stream at {args.rate}/s: {lv}; '
+    print(f'# serving capacity: {sm["solves_per_s"]:.2f} solves/s, {sm["busy_slots_mean"]:.1f} slots solving on '
+          f'average, {100 * sm["ingest_share"]:.0f} % of their time loading RI tensors and MO data; '
+          f'live stream at {args.rate}/s: {lv}; '
           f'max |dE_QP| = {err * 27211.38505:.4f} meV', file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:

@@ -56,6 +56,7 @@ class Result:
     slot: int
     nsemin: int = 0
     nsemax: int = 0
+    ingest: float = 0.0            # seconds of the solve spent loading its RI tensor and MO data
 
     @property
     def latency(self) -> float:
@@ -131,6 +132,7 @@ class Server:
         else:
             eng.set_eri_3center(s.eri_3center, s.index_basis, s.nbf)
         eng.set_mo(s.c_matrix, s.energy, s.occupation)
+        ingest = time.perf_counter() - t0 - start
         rng = selfenergy_set_state_range(s.occupation)
         rng.nsemin, rng.nsemax = _se_range(s, req.window)
         out = eng.g0w0(rng, technique=req.technique, precision=req.precision,
@@ -138,7 +140,7 @@ class Server:
                        exchange_m_vxc=getattr(s, 'exchange_m_vxc', None), return_sigma=False)
         end = time.perf_counter() - t0
         return Result(req.rid, out.energy_qp, out.zz, out.en_rpa, out.en_gw, req.arrival, start, end, k,
-                      rng.nsemin, rng.nsemax)
+                      rng.nsemin, rng.nsemax, ingest)
 
     def _slot_bytes(self, k: int) -> int:
         cur, _ = self.slots[k].memory()
@@ -287,10 +289,15 @@ def summarize(results: list[Result]) -> dict:
     lat = np.array([r.latency for r in results])
     t_first = min(r.arrival for r in results)
     t_last = max(r.end for r in results)
-    return {'solves': len(results), 'wall_s': t_last - t_first,
-            'solves_per_s': len(results) / (t_last - t_first),
+    wall = t_last - t_first
+    return {'solves': len(results), 'wall_s': wall,
+            'solves_per_s': len(results) / wall,
             'p50_ms': float(np.percentile(lat, 50) * 1e3), 'p99_ms': float(np.percentile(lat, 99) * 1e3),
-            'mean_ms': float(lat.mean() * 1e3)}
+            'mean_ms': float(lat.mean() * 1e3),
+            # slots solving on average (time-weighted), and the share of that
+            # time spent loading RI tensors and MO data
+            'busy_slots_mean': float(sum(r.end - r.start for r in results) / wall),
+            'ingest_share': float(sum(r.ingest for r in results) / max(1e-12, sum(r.end - r.start for r in results)))}
 
 
 # ---------------------------------------------------------------- streams
