@@ -737,8 +737,18 @@ chi_ksplit_reduce_kernel(const float* __restrict__ part, int nkc, int tiles, int
   }
   if (m >= naux || n >= naux || m < n) return;
   const float* __restrict__ p = part + (int64_t)(f * tiles + tile) * nkc * (BM * BM) + e;
+  // four chunks' loads in flight, summed in chunk order
   double v = 0.0;
-  for (int c = 0; c < nkc; ++c) v += (double)p[(int64_t)c * (BM * BM)];
+  int c = 0;
+  for (; c + 4 <= nkc; c += 4) {
+    const float a0 = p[(int64_t)c * (BM * BM)], a1 = p[(int64_t)(c + 1) * (BM * BM)];
+    const float a2 = p[(int64_t)(c + 2) * (BM * BM)], a3 = p[(int64_t)(c + 3) * (BM * BM)];
+    v += (double)a0;
+    v += (double)a1;
+    v += (double)a2;
+    v += (double)a3;
+  }
+  for (; c < nkc; ++c) v += (double)p[(int64_t)c * (BM * BM)];
   if (unscale) v *= unscale[f];
   if (m == n) v += 1.0;
   out[m + (int64_t)n * naux] = v;
