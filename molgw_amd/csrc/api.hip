@@ -351,9 +351,9 @@ bool rectangular(const std::vector<int32_t>& ia, int& i0, int& i1, int& a0, int&
 // Measured on MI355X over the serving stream (real molecules, one solve at a
 // time): AO->MO 4.5 -> 3.5 ms per solve, 42.0 -> 43.3-44.1 solves/s, the
 // largest QP error against FP64 0.0013 -> 0.0014 meV.
-// The second product of an FP16x3 transform on pre-split planes (default)
-// or splitting on every tile load: MGW_AO2MO_PLANES = 1 / 0, read at every
-// transform.
+// FP16x3 transforms on pre-split planes (default: both operands of the
+// second product, and the columns of C in the first) or splitting on every
+// tile load: MGW_AO2MO_PLANES = 1 / 0, read at every transform.
 bool ao2mo_planes() {
   const char* e = std::getenv("MGW_AO2MO_PLANES");
   return e ? std::atoi(e) != 0 : true;
@@ -384,6 +384,18 @@ int ao2mo_first(mgw_ctx* c, const double* B, const double* C, int flo, int nf1, 
   g1.precision = precision;
   ProfScope ps(&c->prof, MGW_KCLASS_GEMM_AO2MO, 2.0 * g1.M * (double)g1.N * g1.K,
                8.0 * ((double)g1.M * g1.K + (double)g1.K * g1.N + (double)g1.M * g1.N), c->stream);
+  if (f16 && !no_tune && ao2mo_planes()) {
+    // the columns of C split once (scale of max |C|, the kernel's own split
+    // of B) instead of by every row tile of the large operand: the same
+    // planes, so the same products
+    const Fp16Planes dc = fp16_planes(nf1, nbf);
+    _Float16* pc = nullptr;   // dbuf counts elements of the pointer's type: fp16 here
+    MGW_TRY(dbuf(c, "ao2mo_cplanes", (size_t)dc.per_f, &pc));
+    MGW_TRY(fp16_split_planes(g1.B, nbf, 0, 1, dc, 1, mx, pc, MGW_KCLASS_GEMM_AO2MO, c->stream));
+    g1.bplanes = pc;
+    g1.bplane_q = dc.per_q;
+    g1.bplane_rpad = dc.rpad;
+  }
   return gemm_f64(g1, c->stream);
 }
 

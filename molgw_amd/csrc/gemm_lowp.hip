@@ -371,6 +371,7 @@ gemm_f32acc64_kernel(LowpDev g) {
 // scale input of the next transform.
 __device__ inline int lds_chunk16(int r, int j) { return j ^ ((r >> 1) & 3); }
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 
 struct F16Dev {
   int M, N, K;
@@ -383,13 +384,16 @@ struct F16Dev {
   double* cmax;
   int tiles_m, tiles_n, batch;
   int64_t nblocks;
+  const _Float16* bp;  // BPL: planes of op(B)^T (plane[q][kt][row n][32])
+  int64_t bq;
+  int brpad;
 };
 
 __device__ inline double pow2_scale(double mx) {
   return (mx > 0.0 && mx <= 1.0e300) ? ldexp(1.0, 13 - ilogb(mx)) : 1.0;
 }
 
-template <int BM, int BN, int WM, int WN, int KLASS>
+template <int BM, int BN, int WM, int WN, int KLASS, bool BPL>
 __global__ void __launch_bounds__(64 * WM * WN)
 gemm_f16x3_kernel(F16Dev g) {
   constexpr int BK = 32;
@@ -425,7 +429,8 @@ gemm_f16x3_kernel(F16Dev g) {
 
   const int am = tid % BM, ako = tid / BM;    // A: row m, first k octet
   constexpr int A_KO_STEP = NT / BM;
-  double ra[AU][8], rb[BU][8];
+  double ra[AU][8], rb[BPL ? 1 : BU][8];
+  u4v rbp[BPL ? BU : 1][2];        // BPL: the hi and lo 16-byte chunks of B^T
   auto load = [&](int k0) __attribute__((always_inline)) {
     const int gm = m0 + am;
 #pragma unroll
@@ -437,15 +442,32 @@ gemm_f16x3_kernel(F16Dev g) {
         ra[u][e] = (gm < g.M && gk < g.K) ? A[gm + (int64_t)gk * g.lda] : 0.0;
       }
     }
+    if constexpr (BPL) {
+      // rows n < N of the planes (padded with zeros past N and K); rows past
+      // N are not read (their columns are not stored)
+      const int kt = k0 / BK;
 #pragma unroll
-    for (int u = 0; u < BU; ++u) {
-      const int c = tid + u * NT;             // (n, k octet) with the octet fastest
-      const int n = c >> 2, ko = c & 3;
-      const int gn = n0 + n, kb = k0 + 8 * ko;
+      for (int u = 0; u < BU; ++u) {
+        const int c = tid + u * NT;
+        const int n = c >> 2, ko = c & 3;
+        const int gn = n0 + n;
+        const bool in = c < BN * 4 && gn < g.N;
+        const int64_t off = ((int64_t)kt * g.brpad + gn) * BK + 8 * ko;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int gk = kb + e;
-        rb[u][e] = (c < BN * 4 && gn < g.N && gk < g.K) ? B[gk + (int64_t)gn * g.ldb] : 0.0;
+        for (int q = 0; q < 2; ++q)
+          rbp[u][q] = in ? *reinterpret_cast<const u4v*>(g.bp + q * g.bq + off) : u4v{0u, 0u, 0u, 0u};
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < BU; ++u) {
+        const int c = tid + u * NT;             // (n, k octet) with the octet fastest
+        const int n = c >> 2, ko = c & 3;
+        const int gn = n0 + n, kb = k0 + 8 * ko;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int gk = kb + e;
+          rb[u][e] = (c < BN * 4 && gn < g.N && gk < g.K) ? B[gk + (int64_t)gn * g.ldb] : 0.0;
+        }
       }
     }
   };
@@ -469,16 +491,21 @@ gemm_f16x3_kernel(F16Dev g) {
       const int c = tid + u * NT;
       if (c >= BN * 4) break;
       const int n = c >> 2, ko = c & 3;
-      h8v hi, lo;
+      if constexpr (BPL) {
+        *reinterpret_cast<u4v*>(&Bs[0][n][8 * lds_chunk16(n, ko)]) = rbp[u][0];
+        *reinterpret_cast<u4v*>(&Bs[1][n][8 * lds_chunk16(n, ko)]) = rbp[u][1];
+      } else {
+        h8v hi, lo;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const float y = (float)(rb[u][e] * sb);
-        const _Float16 h = (_Float16)y;
-        hi[e] = h;
-        lo[e] = (_Float16)(y - (float)h);
+        for (int e = 0; e < 8; ++e) {
+          const float y = (float)(rb[u][e] * sb);
+          const _Float16 h = (_Float16)y;
+          hi[e] = h;
+          lo[e] = (_Float16)(y - (float)h);
+        }
+        *reinterpret_cast<h8v*>(&Bs[0][n][8 * lds_chunk16(n, ko)]) = hi;
+        *reinterpret_cast<h8v*>(&Bs[1][n][8 * lds_chunk16(n, ko)]) = lo;
       }
-      *reinterpret_cast<h8v*>(&Bs[0][n][8 * lds_chunk16(n, ko)]) = hi;
-      *reinterpret_cast<h8v*>(&Bs[1][n][8 * lds_chunk16(n, ko)]) = lo;
     }
   };
 
@@ -554,6 +581,8 @@ gemm_f16x3_kernel(F16Dev g) {
 
 template <int BM, int BN, int WM, int WN, int KLASS>
 int launch_f16x3(const GemmF64& h, hipStream_t stream) {
+  MGW_REQUIRE(!h.bplanes || (h.batch == 1 && h.bplane_rpad >= h.N), MGW_ERR_ARG,
+              "gemm_lowp: B planes need one matrix and rows covering N");
   F16Dev g;
   g.M = h.M; g.N = h.N; g.K = h.K;
   g.A = h.A; g.lda = h.lda; g.strideA = h.strideA;
@@ -566,8 +595,13 @@ int launch_f16x3(const GemmF64& h, hipStream_t stream) {
   g.batch = h.batch;
   g.nblocks = (int64_t)g.tiles_m * g.tiles_n * h.batch;
   MGW_REQUIRE(g.nblocks < (1ll << 31), MGW_ERR_ARG, "gemm_lowp: too many tiles");
-  hipLaunchKernelGGL((gemm_f16x3_kernel<BM, BN, WM, WN, KLASS>), dim3((unsigned)g.nblocks), dim3(64 * WM * WN), 0,
-                     stream, g);
+  g.bp = h.bplanes; g.bq = h.bplane_q; g.brpad = h.bplane_rpad;
+  if (g.bp)
+    hipLaunchKernelGGL((gemm_f16x3_kernel<BM, BN, WM, WN, KLASS, true>), dim3((unsigned)g.nblocks),
+                       dim3(64 * WM * WN), 0, stream, g);
+  else
+    hipLaunchKernelGGL((gemm_f16x3_kernel<BM, BN, WM, WN, KLASS, false>), dim3((unsigned)g.nblocks),
+                       dim3(64 * WM * WN), 0, stream, g);
   MGW_HIP(hipGetLastError());
   return MGW_OK;
 }

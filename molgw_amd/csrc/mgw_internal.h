@@ -136,6 +136,11 @@ struct GemmF64 {
   const double* amax = nullptr;
   const double* bmax = nullptr;
   double* cmax = nullptr;
+  // MGW_FP16X3, batch 1: op(B)^T already split (fp16_split_planes: rows N,
+  // k K, scale of bmax), read instead of splitting B on every tile load
+  const _Float16* bplanes = nullptr;
+  int64_t bplane_q = 0;   // elements of one plane
+  int bplane_rpad = 0;    // padded rows of a plane
 };
 int gemm_f64(const GemmF64& g, hipStream_t stream);
 int gemm_lowp(const GemmF64& g, int precision, hipStream_t stream);
