@@ -41,6 +41,20 @@ def test_abi_version_and_error_string_without_gpu():
     assert b > 8 * 1188 * 492 * 492
 
 
+def test_windowed_footprint_without_gpu():
+    """mgw_footprint_bytes_nse: the whole range (nse = nstate) is
+    mgw_footprint_bytes; a window needs less, never below the resident dense
+    RI tensor, and more states never need less."""
+    from molgw_amd import _lib
+    lib = _lib.load()
+    for tech, nw, prec in ((1, 32, 4), (1, 32, 0), (0, 0, 0)):   # imaginary+Pade FP16x3 / FP64, one-shot
+        full = lib.mgw_footprint_bytes(768, 2688, 768, 160, tech, nw, prec)
+        assert lib.mgw_footprint_bytes_nse(768, 2688, 768, 160, 768, tech, nw, prec) == full
+        sizes = [lib.mgw_footprint_bytes_nse(768, 2688, 768, 160, ne, tech, nw, prec) for ne in (1, 16, 64, 768)]
+        assert sizes == sorted(sizes) and sizes[1] < full
+        assert sizes[0] > 8 * 2688 * 768 * 768
+
+
 def test_struct_layouts_match_header():
     from molgw_amd._lib import G0W0Params, Ranges
     assert ctypes.sizeof(Ranges) == 32
